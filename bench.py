@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Benchmark: gradient shots per second on the 276^3 (256^3 + CPML) elastic
+FWI workload, one MI355X.
+
+A step is one gradient shot: forward NT=1000 steps with the on-the-fly DFT
+(6 frequencies), residual against observed data, adjoint NT steps with the
+DFT, gradient_F.  Inputs (models, observed data) are resident in HBM before
+the timed region.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad276|toy84]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ifos3d-dev_amd"))
+
+METRIC = "gradient shots/sec at 256³ (batched vs plain); Gcell-updates/s; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other"]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def slab_stats(w, free_surf=0):
+    """cells of the interior box, of the CPML slabs, and the number of
+    (cell, axis) pairs with an active CPML memory variable"""
+    fw = w.fw
+    bi = (w.nx - 2 * fw) * (w.ny - (fw if free_surf else 2 * fw)) * (w.nz - 2 * fw)
+    slab = w.cells - bi
+    axes = 2 * fw * w.ny * w.nz + (fw if free_surf else 2 * fw) * w.nx * w.nz + 2 * fw * w.nx * w.ny
+    return bi, slab, axes
+
+
+def algorithmic_bytes(w, nf, kahan=False):
+    """bytes each kernel category must move per launch (FP32, each array
+    counted once per pass): v 60 B/cell, sigma 80 B/cell, +24 B per active
+    CPML axis, DFT 12 + 48*nf B/cell (x2 accumulators with Kahan),
+    gradient 48*nf + 36 B/cell"""
+    bi, slab, axes = slab_stats(w)
+    acc = 96 if kahan else 48
+    return {
+        "v_interior": 60 * bi, "v_cpml": 60 * slab + 24 * axes, "v_fused": 60 * w.cells + 24 * axes,
+        "s_interior": 80 * bi, "s_cpml": 80 * slab + 24 * axes, "s_fused": 80 * w.cells + 24 * axes,
+        "dft": (12 + acc * nf) * w.cells, "gradient": (48 * nf + 36) * w.cells,
+    }
+
+
+def to_device_layout(G, t):
+    import torch
+    import ifosgpu as ig
+    out = torch.zeros(G.dshape, dtype=torch.float32, device="cuda")
+    out[:, :, ig.KOFF - 2:ig.KOFF - 2 + G.nz + 5] = t
+    return out
+
+
+def run_gpu(w, steps, warmup, precision):
+    import torch
+    import ifosgpu as ig
+    import workloads as wl
+
+    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+    log(f"grid {w.nx}x{w.ny}x{w.nz} pitch {G.pitch} NT {w.nt} dt {w.dt:.4e} nf {len(w.finv)}")
+    if w.name == "toy84":
+        true_np, start_np = wl.sphere_models(w)
+        true = [torch.from_numpy(a).cuda() for a in true_np]
+        start = [torch.from_numpy(a).cuda() for a in start_np]
+    else:
+        true, start = wl.random_models(w)
+    d_true = [to_device_layout(G, a) for a in true]
+    d_start = [to_device_layout(G, a) for a in start]
+    host_start = [a.cpu().numpy() for a in start]
+    del true, start
+    ntr = len(w.receivers)
+    ctx = ig.ShotContext(G, len(w.finv), ntr, precision)
+    ctx.set_cpml(w.vppml, w.fpml)
+    ctx.set_freqs(w.finv, w.tast)
+    nshots = warmup + steps
+    shot_ids = [n % len(w.shots) for n in range(nshots)]
+    # observed data: forward runs on the true model (untimed setup)
+    ctx.set_model(*d_true)
+    obs = []
+    d_syn = torch.zeros((3, ntr, w.nt + 1), device="cuda")
+    for n, s in enumerate(shot_ids):
+        ctx.run_shot(w.shots[s], w.receivers, d_syn, None, want_adjoint=False)
+        obs.append(d_syn.clone())
+        log(f"observed data shot {n + 1}/{nshots}")
+    ctx.set_model(*d_start)
+    grads = [G.empty() for _ in range(3)]
+    torch.cuda.synchronize()
+    for n in range(warmup):
+        ctx.run_shot(w.shots[shot_ids[n]], w.receivers, d_syn, grads, d_obs=obs[n])
+        torch.cuda.synchronize()
+        log(f"warmup shot {n + 1}/{warmup} done")
+    lib = ig.lib()
+    import ctypes as C
+    lib.ifos_prof_reset()
+    lib.ifos_prof_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    l2 = 0.0
+    for n in range(warmup, nshots):
+        l2 += ctx.run_shot(w.shots[shot_ids[n]], w.receivers, d_syn, grads, d_obs=obs[n])
+        log(f"timed shot {n - warmup + 1}/{steps}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.ifos_prof_enable(0)
+    ms = (C.c_double * len(CATS))()
+    cnt = (C.c_long * len(CATS))()
+    ig.check(lib.ifos_prof_read(ms, cnt, len(CATS)), "ifos_prof_read")
+    prof = {CATS[n]: (ms[n], cnt[n]) for n in range(len(CATS))}
+    gnorm = [float(torch.linalg.vector_norm(t.double()).item()) for t in grads]
+    ctx.close()
+    return dict(elapsed=elapsed, prof=prof, l2=l2, gnorm=gnorm, host_start=host_start, pitch=G.pitch)
+
+
+def cpu_baseline(w, host_model, fwd_steps=4, adj_steps=4):
+    """The reference's own routines (oracle/_ref, compiled from its sources)
+    on the host, one core: a bounded sample of forward and adjoint time steps
+    of the same grid, extrapolated to a whole gradient shot."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    g = po.Grid(nx=w.nx, ny=w.ny, nz=w.nz, fw=w.fw, dx=w.dh, dy=w.dh, dz=w.dh, dt=w.dt, nt=w.nt,
+                fdcoeff=w.fdcoeff)
+    ntast = po.Oracle("f32").ntast(max(w.finv), w.tast, w.dt)
+    src = w.shots[0]
+    if po.Ref.available():
+        kind = "reference"
+        r = po.Ref(g, vppml=w.vppml, fpml=w.fpml, nfmax=len(w.finv), nsrc=1, ntr=len(w.receivers))
+        for name, a in zip(("rho", "pi", "u"), host_model):
+            r.set(name, a)
+        r.set_freqs(w.finv)
+        r.set_source(0, *src)
+        r.set_receivers(w.receivers)
+        r.call("matcopy")
+        r.call("av_mat")
+        r.wavelet(1)
+        r.call("zero_wavefield")
+        r.call("zero_invers")
+
+        def fwd(nt):
+            r.call("update_v", nt, 0)
+            r.call("update_v_cpml", nt, 0)
+            r.call("update_s")
+            r.call("update_s_cpml")
+            r.call("psource", nt)
+            if nt % ntast == 0:
+                r.call("discfourier", nt, ntast, 0)
+            r.call("seismo", nt)
+
+        def adj(nt):
+            r.call("update_v", nt, 1)
+            r.call("update_v_cpml", nt, 1)
+            r.call("update_s")
+            r.call("update_s_cpml")
+            if nt % ntast == 0:
+                r.call("discfourier", nt, ntast, 1)
+    else:
+        return None
+    t0 = time.perf_counter()
+    for nt in range(1, fwd_steps + 1):
+        fwd(nt)
+    t1 = time.perf_counter()
+    for nt in range(1, adj_steps + 1):
+        adj(nt)
+    t2 = time.perf_counter()
+    tf = (t1 - t0) / fwd_steps
+    ta = (t2 - t1) / adj_steps
+    shot_s = w.nt * (tf + ta)
+    return {"value": 1.0 / shot_s, "unit": "gradient shots/s", "cores": 1, "kind": kind,
+            "sample": (f"{fwd_steps} forward + {adj_steps} adjoint time steps of the reference routines "
+                       f"(update_v, update_v_CPML, update_s_elastic, update_s_CPML_elastic, psource, seismo, "
+                       f"discfourier every {ntast} steps) on the full {w.nx}x{w.ny}x{w.nz} grid, one core; "
+                       f"{tf:.2f} s/forward step, {ta:.2f} s/adjoint step, extrapolated to {w.nt}+{w.nt} steps "
+                       f"per shot (gradient_F, one pass per shot, left out)")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84"])
+    ap.add_argument("--precision", default="compat", choices=["compat", "kahan"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(shots would be sharded one rank per GPU; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ifosgpu as ig
+    import workloads as wl
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    ig.lib()  # fail loudly if the HIP library is not built
+    w = wl.grad276() if args.config == "grad276" else wl.toy84()
+    prec = ig.DFT_KAHAN if args.precision == "kahan" else ig.DFT_COMPAT
+    res = run_gpu(w, args.steps, args.warmup, prec)
+    el = res["elapsed"]
+    value = args.steps / el
+    nf = len(w.finv)
+    ab = algorithmic_bytes(w, nf, kahan=(prec == ig.DFT_KAHAN))
+    prof = res["prof"]
+    # dominant kernel by total time
+    dom = max((k for k in ab if prof[k][1] > 0), key=lambda k: prof[k][0])
+    avg_ms = prof[dom][0] / prof[dom][1]
+    achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
+    stencil_cats = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused"]
+    st_bytes = sum(ab[k] * prof[k][1] for k in stencil_cats)
+    st_ms = sum(prof[k][0] for k in stencil_cats)
+    hbm_frac_stencils = (st_bytes / (st_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if st_ms > 0 else None
+    gcells = w.cells * 2 * w.nt * args.steps / el / 1e9
+    kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
+                   "avg_ms": round(prof[k][0] / prof[k][1], 5) if prof[k][1] else None,
+                   "GBps": round(ab[k] / (prof[k][0] / prof[k][1] * 1e-3) / 1e9, 1)
+                   if prof[k][1] and k in ab else None}
+               for k in CATS if prof[k][1]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (reference routines, one core, bounded sample)")
+        try:
+            cpu = cpu_baseline(w, res["host_start"])
+        except Exception as e:  # the baseline is reported, never the product
+            log(f"CPU baseline failed: {e}")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "gradient shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * el / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"{w.name}: {w.nx}x{w.ny}x{w.nz} ({w.nx - 2 * w.fw}^3 + {w.fw}-cell CPML), "
+                                f"elastic FP32 4th order, dh {w.dh} m, dt {w.dt:.4e} s, NT {w.nt}, "
+                                f"{len(w.shots)} shot positions, {len(w.receivers)} receivers, "
+                                f"{nf} DFT frequencies {w.finv[0]}-{w.finv[-1]} Hz, TAST {w.tast}"),
+                   "path": "plain (one kernel per reference routine per step, one shot per launch)",
+                   "dft_precision": args.precision, "shots_per_step": 1},
+        "gcell_updates_per_s": round(gcells, 3),
+        "hbm_frac_stencils": round(hbm_frac_stencils, 4) if hbm_frac_stencils else None,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 5)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

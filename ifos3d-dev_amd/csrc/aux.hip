@@ -1,0 +1,516 @@
+// aux.hip -- the per-step companions of the stencils on gfx950: point
+// sources and adjoint injection, receiver gather, free surface, the
+// on-the-fly DFT, the frequency-domain gradient, material averaging and the
+// single-rank halo wrap of the model, the residual, and a fill kernel.
+//
+// Arithmetic follows the reference expression by expression; the DFT and
+// gradient keep the reference's double-precision steps (disc_fourier.c:42-54,
+// gradient_F.c:135-139).
+#include <stdio.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ifos_dev.h"
+
+namespace ifos {
+
+// ---------------------------------------------------------------------------
+// sources (update_v_ssg.c:474-522, psource.c:26-54)
+__global__ void k_add_forces(DevGeom g, float *vx, float *vy, float *vz, const float *rip, const float *rjp,
+                             const float *rkp, const int *src, const float *sig, int nsrc, int nt, int ldsig,
+                             float alpha_rad, float beta_rad)
+{
+    // sources applied in order by one thread: two sources may share a cell
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int l = 0; l < nsrc; ++l) {
+        const int i = src[4 * l], j = src[4 * l + 1], k = src[4 * l + 2], type = src[4 * l + 3];
+        const int64_t c = at(g, j, i, k);
+        const float amp = g.dt * sig[(int64_t)l * ldsig + nt] / (g.dx * g.dy * g.dz);
+        switch (type) {
+        case 2: vx[c] += amp * rip[c]; break;
+        case 3: vz[c] += amp * rjp[c]; break;  // vz with rjp, as the reference pairs them
+        case 4: vy[c] += amp * rkp[c]; break;  // vy with rkp
+        case 5:
+            vx[c] = (float)(vx[c] + cos((double)beta_rad) * cos((double)alpha_rad) * amp);
+            vy[c] = (float)(vy[c] + sin((double)alpha_rad) * amp);
+            vz[c] = (float)(vz[c] + sin((double)beta_rad) * cos((double)alpha_rad) * amp);
+            break;
+        default: break;
+        }
+    }
+}
+
+__global__ void k_add_adjoint(DevGeom g, float *vx, float *vy, float *vz, const int *rec, const float *adj,
+                              int ntr, int nt, int ldadj)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int64_t comp = (int64_t)ntr * ldadj;
+    for (int l = 0; l < ntr; ++l) {
+        const int64_t c = at(g, rec[3 * l + 1], rec[3 * l], rec[3 * l + 2]);
+        const int64_t s = (int64_t)l * ldadj + nt;
+        vx[c] += adj[s];
+        vz[c] += adj[2 * comp + s];
+        vy[c] += adj[comp + s];
+    }
+}
+
+__global__ void k_psource(DevGeom g, float *sxx, float *syy, float *szz, const int *src, const float *sig,
+                          int nsrc, int nt, int ntall, int ldsig)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float amp = 0.0f;
+    const double den = 2.0 * g.dx * g.dy * g.dz;
+    for (int l = 0; l < nsrc; ++l) {
+        if (src[4 * l + 3] != 1) continue;
+        const float *s = sig + (int64_t)l * ldsig;
+        const int64_t c = at(g, src[4 * l + 1], src[4 * l], src[4 * l + 2]);
+        if (nt == 1) amp = (float)(s[nt + 1] / den);
+        if (nt > 1 && nt < ntall) amp = (float)((s[nt + 1] - s[nt - 1]) / den);
+        if (nt == ntall) amp = (float)(-s[nt - 1] / den);
+        sxx[c] += amp;
+        syy[c] += amp;
+        szz[c] += amp;
+    }
+}
+
+// surface_ssg_elastic.c:78-112 (4th order, j = 1)
+__global__ void k_surface(DevGeom g, const float *u, const float *pi, float *sxx, float *syy, float *szz,
+                          float *sxy, float *syz, const float *vx, const float *vy, const float *vz)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, 1, i, k), sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    syy[c] = 0.0f;
+    for (int m = 1; m <= 2; ++m) {
+        syy[c - m * sj] = -syy[c + m * sj];
+        sxy[c - m * sj] = -sxy[c + (m - 1) * sj];
+        syz[c - m * sj] = -syz[c + (m - 1) * sj];
+    }
+    float vxx = (b1 * (vx[c] - vx[c - si]) + b2 * (vx[c + si] - vx[c - 2 * si])) / g.dx;
+    float vyy = (b1 * (vy[c] - vy[c - sj]) + b2 * (vy[c + sj] - vy[c - 2 * sj])) / g.dy;
+    float vzz = (b1 * (vz[c] - vz[c - 1]) + b2 * (vz[c + 1] - vz[c - 2])) / g.dz;
+    float f = u[c] * 2.0f;
+    float gg = pi[c];
+    float h = -(g.dt * (gg - f) * (gg - f) * (vxx + vzz) / gg) - (g.dt * (gg - f) * vyy);
+    sxx[c] += h;
+    szz[c] += h;
+}
+
+// seismo_ssg.c:51-53
+__global__ void k_seismo(DevGeom g, const float *vx, const float *vy, const float *vz, const int *rec, int ntr,
+                         int ins, int ldsec, float *sec)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ntr) return;
+    const int64_t c = at(g, rec[3 * r + 1], rec[3 * r], rec[3 * r + 2]);
+    const int64_t comp = (int64_t)ntr * ldsec, s = (int64_t)r * ldsec + ins;
+    sec[s] = vx[c];
+    sec[comp + s] = vy[c];
+    sec[2 * comp + s] = vz[c];
+}
+
+// ---------------------------------------------------------------------------
+// on-the-fly DFT, disc_fourier.c:42-54.  One thread per cell of [1..N]^3.
+struct Trig {
+    double c[IFOS_MAX_NF], s[IFOS_MAX_NF];
+};
+struct FPtrs {
+    float *p[6];
+};
+
+__global__ __launch_bounds__(256) void k_dft(DevGeom g, const float *vx, const float *vy, const float *vz,
+                                            FPtrs F, int nf, int64_t fstride, Trig T)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k);
+    const double x = vx[c], y = vy[c], z = vz[c];
+    for (int l = 0; l < nf; ++l) {
+        const int64_t o = c + (int64_t)l * fstride;
+        const double t1 = T.c[l], t2 = T.s[l];
+        F.p[0][o] = (float)(F.p[0][o] + x * t1);
+        F.p[1][o] = (float)(F.p[1][o] + y * t1);
+        F.p[2][o] = (float)(F.p[2][o] + z * t1);
+        F.p[3][o] = (float)(F.p[3][o] + x * t2);
+        F.p[4][o] = (float)(F.p[4][o] + y * t2);
+        F.p[5][o] = (float)(F.p[5][o] + z * t2);
+    }
+}
+
+// compensated variant: S, C float; term in double
+__device__ inline void kahan_add(float *S, float *Cc, int64_t o, double term)
+{
+    float s = S[o], cc = Cc[o];
+    float y = (float)(term - (double)cc);
+    float t = s + y;
+    Cc[o] = (t - s) - y;
+    S[o] = t;
+}
+
+__global__ __launch_bounds__(256) void k_dft_kahan(DevGeom g, const float *vx, const float *vy, const float *vz,
+                                                  FPtrs F, FPtrs Cc, int nf, int64_t fstride, Trig T)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k);
+    const double x = vx[c], y = vy[c], z = vz[c];
+    for (int l = 0; l < nf; ++l) {
+        const int64_t o = c + (int64_t)l * fstride;
+        const double t1 = T.c[l], t2 = T.s[l];
+        kahan_add(F.p[0], Cc.p[0], o, x * t1);
+        kahan_add(F.p[1], Cc.p[1], o, y * t1);
+        kahan_add(F.p[2], Cc.p[2], o, z * t1);
+        kahan_add(F.p[3], Cc.p[3], o, x * t2);
+        kahan_add(F.p[4], Cc.p[4], o, y * t2);
+        kahan_add(F.p[5], Cc.p[5], o, z * t2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gradient_F.c:66-139.  One thread per cell, frequencies in order.
+struct G9 {
+    float xx, xy, xz, yx, yy, yz, zx, zy, zz;
+};
+
+__device__ inline G9 grad9(const DevGeom &g, const float *fx, const float *fy, const float *fz, int64_t c)
+{
+    const int64_t sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    G9 o;
+    o.xx = (b1 * (fx[c] - fx[c - si]) + b2 * (fx[c + si] - fx[c - 2 * si])) / g.dx;
+    o.xy = (b1 * (fx[c + sj] - fx[c]) + b2 * (fx[c + 2 * sj] - fx[c - sj])) / g.dy;
+    o.xz = (b1 * (fx[c + 1] - fx[c]) + b2 * (fx[c + 2] - fx[c - 1])) / g.dz;
+    o.yx = (b1 * (fy[c + si] - fy[c]) + b2 * (fy[c + 2 * si] - fy[c - si])) / g.dx;
+    o.yy = (b1 * (fy[c] - fy[c - sj]) + b2 * (fy[c + sj] - fy[c - 2 * sj])) / g.dy;
+    o.yz = (b1 * (fy[c + 1] - fy[c]) + b2 * (fy[c + 2] - fy[c - 1])) / g.dz;
+    o.zx = (b1 * (fz[c + si] - fz[c]) + b2 * (fz[c + 2 * si] - fz[c - si])) / g.dx;
+    o.zy = (b1 * (fz[c + sj] - fz[c]) + b2 * (fz[c + 2 * sj] - fz[c - sj])) / g.dy;
+    o.zz = (b1 * (fz[c] - fz[c - 1]) + b2 * (fz[c + 1] - fz[c - 2])) / g.dz;
+    return o;
+}
+
+struct GradFreq {
+    float fdummy[IFOS_MAX_NF];
+};
+
+__global__ __launch_bounds__(256) void k_gradient(DevGeom g, FPtrs Ff, FPtrs Fb, int nf, int64_t fstride,
+                                                 GradFreq GF, const float *rho, const float *pi, const float *u,
+                                                 float *g1, float *g2, float *g3)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k);
+    const float r = rho[c], p = pi[c], m = u[c];
+    float a1 = g1[c], a2 = g2[c], a3 = g3[c];
+    for (int l = 0; l < nf; ++l) {
+        const int64_t co = c + (int64_t)l * fstride;
+        const float fd = GF.fdummy[l];
+        G9 f = grad9(g, Ff.p[0], Ff.p[1], Ff.p[2], co);
+        G9 b = grad9(g, Fb.p[0], Fb.p[1], Fb.p[2], co);
+        G9 fi = grad9(g, Ff.p[3], Ff.p[4], Ff.p[5], co);
+        G9 bi = grad9(g, Fb.p[3], Fb.p[4], Fb.p[5], co);
+        float gl = (f.xx + f.yy + f.zz) * (b.xx + b.yy + b.zz) + (fi.xx + fi.yy + fi.zz) * (bi.xx + bi.yy + bi.zz);
+        gl = -gl * fd;
+        float gm = 2.0f * f.xx * b.xx + 2.0f * f.yy * b.yy + 2.0f * f.zz * b.zz + 2.0f * fi.xx * bi.xx +
+                   2.0f * fi.yy * bi.yy + 2.0f * fi.zz * bi.zz + (f.xy + f.yx) * (b.xy + b.yx) +
+                   (fi.xy + fi.yx) * (bi.xy + bi.yx) + (f.xz + f.zx) * (b.xz + b.zx) +
+                   (fi.xz + fi.zx) * (bi.xz + bi.zx) + (f.yz + f.zy) * (b.yz + b.zy) +
+                   (fi.yz + fi.zy) * (bi.yz + bi.zy);
+        gm = -gm * fd;
+        float gr = (Ff.p[0][co] * Fb.p[0][co] + Ff.p[1][co] * Fb.p[1][co] + Ff.p[2][co] * Fb.p[2][co]) +
+                   (Ff.p[3][co] * Fb.p[3][co] + Ff.p[4][co] * Fb.p[4][co] + Ff.p[5][co] * Fb.p[5][co]);
+        const double srp = sqrt((double)(r * p)), sru = sqrt((double)(r * m));
+        a1 = (float)(a1 + srp * 2 * gl);
+        a2 = (float)(a2 + (-4 * sru * gl + 2 * sru * gm));
+        a3 += gr + m / r * gm + (p - 2.0f * m) / r * gl;
+    }
+    g1[c] = a1;
+    g2[c] = a2;
+    g3[c] = a3;
+}
+
+// ---------------------------------------------------------------------------
+// matcopy.c:31-307 (single rank): three passes, y faces then x then z
+__global__ void k_wrap_y(DevGeom g, float *m)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz+1
+    const int i = blockIdx.y;                              // 0..nx+1
+    if (k > g.nz + 1 || i > g.nx + 1) return;
+    float top = m[at(g, 1, i, k)], bot = m[at(g, g.ny, i, k)];
+    m[at(g, g.ny + 1, i, k)] = top;
+    m[at(g, 0, i, k)] = bot;
+}
+__global__ void k_wrap_x(DevGeom g, float *m)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k > g.nz + 1 || j > g.ny + 1) return;
+    float lef = m[at(g, j, 1, k)], rig = m[at(g, j, g.nx, k)];
+    m[at(g, j, g.nx + 1, k)] = lef;
+    m[at(g, j, 0, k)] = rig;
+}
+__global__ void k_wrap_z(DevGeom g, float *m)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i > g.nx + 1 || j > g.ny + 1) return;
+    float fro = m[at(g, j, i, 1)], bac = m[at(g, j, i, g.nz)];
+    m[at(g, j, i, g.nz + 1)] = fro;
+    m[at(g, j, i, 0)] = bac;
+}
+
+// av_mat.c:46-64
+__global__ __launch_bounds__(256) void k_av_mat(DevGeom g, const float *rho, const float *u, float *uipjp,
+                                               float *ujpkp, float *uipkp, float *rjp, float *rkp, float *rip)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
+    const float *uc = u + c;
+    uipjp[c] = (float)(4.0 / ((1.0 / uc[0]) + (1.0 / uc[si]) + (1.0 / uc[sj + si]) + (1.0 / uc[sj])));
+    ujpkp[c] = (float)(4.0 / ((1.0 / uc[0]) + (1.0 / uc[1]) + (1.0 / uc[sj + 1]) + (1.0 / uc[sj])));
+    uipkp[c] = (float)(4.0 / ((1.0 / uc[0]) + (1.0 / uc[1]) + (1.0 / uc[si + 1]) + (1.0 / uc[si])));
+    rjp[c] = (float)(0.5 * (rho[c] + rho[c + sj]));
+    rkp[c] = (float)(0.5 * (rho[c] + rho[c + 1]));
+    rip[c] = (float)(0.5 * (rho[c] + rho[c + si]));
+}
+
+// residual.c:39-53: one thread per trace and component
+__global__ void k_residual(const float *obs, const float *syn, float *adj, int ntr, int ns, int ldin, int ldadj,
+                           float dt, double *l2part)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;  // comp*ntr + trace
+    if (t >= 3 * ntr) return;
+    const float *o = obs + (int64_t)t * ldin, *s = syn + (int64_t)t * ldin;
+    float *d = adj + (int64_t)t * ldadj;
+    float vdiff = 0.0f;
+    double acc = 0.0;
+    for (int jj = 1; jj <= ns; ++jj) {
+        vdiff += s[jj] - o[jj];
+        const float dd = dt * vdiff;
+        d[ns - jj + 1] = dd;
+        acc += (double)(dd * dd);
+    }
+    l2part[t] = acc;
+}
+
+__global__ void k_fill(float *d, int64_t n, float v)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) d[t] = v;
+}
+
+static dim3 cell_grid(const ifos_geom *g, dim3 &block)
+{
+    block = dim3(64, 4, 1);
+    return dim3((unsigned)((g->nz + 63) / 64), (unsigned)((g->nx + 3) / 4), (unsigned)g->ny);
+}
+
+}  // namespace ifos
+
+using namespace ifos;
+
+extern "C" int ifos_add_forces(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const int *d_src,
+                               const float *d_sig, int nsrc, int nt, float alpha, float beta, void *stream)
+{
+    if (nsrc <= 0) return IFOS_OK;
+    DevGeom G = make_dev_geom(g);
+    const float ar = (float)(alpha * 3.141592653589793 / 180), br = (float)(beta * 3.141592653589793 / 180);
+    { ProfScope ps_(IFOS_K_SOURCES, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_add_forces, dim3(1), dim3(1), 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, m->rip,
+                       m->rjp, m->rkp, d_src, d_sig, nsrc, nt, g->nt + 2, ar, br);
+    }
+    return hip_check(hipGetLastError(), "k_add_forces");
+}
+
+extern "C" int ifos_add_adjoint(const ifos_geom *g, const ifos_fields *f, const int *d_rec, const float *d_adj,
+                                int ntr, int nt, void *stream)
+{
+    if (ntr <= 0) return IFOS_OK;
+    DevGeom G = make_dev_geom(g);
+    { ProfScope ps_(IFOS_K_SOURCES, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_add_adjoint, dim3(1), dim3(1), 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, d_rec,
+                       d_adj, ntr, nt, g->nt + 2);
+    }
+    return hip_check(hipGetLastError(), "k_add_adjoint");
+}
+
+extern "C" int ifos_psource(const ifos_geom *g, const ifos_fields *f, const int *d_src, const float *d_sig,
+                            int nsrc, int nt, void *stream)
+{
+    if (nsrc <= 0) return IFOS_OK;
+    DevGeom G = make_dev_geom(g);
+    { ProfScope ps_(IFOS_K_SOURCES, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_psource, dim3(1), dim3(1), 0, (hipStream_t)stream, G, f->sxx, f->syy, f->szz, d_src,
+                       d_sig, nsrc, nt, g->nt, g->nt + 2);
+    }
+    return hip_check(hipGetLastError(), "k_psource");
+}
+
+extern "C" int ifos_surface(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, void *stream)
+{
+    DevGeom G = make_dev_geom(g);
+    { ProfScope ps_(IFOS_K_OTHER, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_surface, dim3((unsigned)((g->nz + 63) / 64), (unsigned)g->nx), dim3(64), 0,
+                       (hipStream_t)stream, G, m->u, m->pi, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->vx, f->vy,
+                       f->vz);
+    }
+    return hip_check(hipGetLastError(), "k_surface");
+}
+
+extern "C" int ifos_seismo(const ifos_geom *g, const ifos_fields *f, const int *d_rec, int ntr, int ins, int ns,
+                           float *d_sec, void *stream)
+{
+    if (ntr <= 0) return IFOS_OK;
+    DevGeom G = make_dev_geom(g);
+    { ProfScope ps_(IFOS_K_SOURCES, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_seismo, dim3((unsigned)((ntr + 255) / 256)), dim3(256), 0, (hipStream_t)stream, G, f->vx,
+                       f->vy, f->vz, d_rec, ntr, ins, ns + 1, d_sec);
+    }
+    return hip_check(hipGetLastError(), "k_seismo");
+}
+
+extern "C" int ifos_discfourier(const ifos_geom *g, const ifos_fields *f, float *const d_F[6],
+                                float *const d_C[6], const float *finv, int nf, int nt, int ntast, int back,
+                                int precision, void *stream)
+{
+    if (nf <= 0 || nf > IFOS_MAX_NF) {
+        set_error("ifos_discfourier: nf=%d outside [1,%d]", nf, IFOS_MAX_NF);
+        return IFOS_EINVAL;
+    }
+    if (precision == IFOS_DFT_KAHAN && !d_C) {
+        set_error("ifos_discfourier: Kahan precision needs compensation arrays");
+        return IFOS_EINVAL;
+    }
+    DevGeom G = make_dev_geom(g);
+    // time and trig factors exactly as disc_fourier.c:36-44
+    float t = 0.0f;
+    if (back == 0) t = nt * g->dt;
+    if (back == 1) t = g->time - nt * g->dt;
+    Trig T;
+    for (int l = 0; l < nf; ++l) {
+        T.c[l] = cos(2.0 * t * finv[l] * M_PI) * g->dt * ntast;
+        T.s[l] = sin(2.0 * t * finv[l] * M_PI) * g->dt * ntast;
+    }
+    FPtrs F;
+    for (int n = 0; n < 6; ++n) F.p[n] = d_F[n];
+    dim3 block, grid = cell_grid(g, block);
+    const int64_t fs = (int64_t)ifos_field_elems(g);
+    if (precision == IFOS_DFT_KAHAN) {
+        FPtrs Cc;
+        for (int n = 0; n < 6; ++n) Cc.p[n] = d_C[n];
+        { ProfScope ps_(IFOS_K_DFT, (hipStream_t)stream);
+        hipLaunchKernelGGL(k_dft_kahan, grid, block, 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, F, Cc, nf,
+                           fs, T);
+        }
+    } else {
+        { ProfScope ps_(IFOS_K_DFT, (hipStream_t)stream);
+        hipLaunchKernelGGL(k_dft, grid, block, 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, F, nf, fs, T);
+        }
+    }
+    return hip_check(hipGetLastError(), "k_dft");
+}
+
+extern "C" int ifos_gradient_F(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6],
+                               const ifos_model *m, const float *finv, int nf, float *const d_grad[3],
+                               void *stream)
+{
+    if (nf <= 0 || nf > IFOS_MAX_NF) {
+        set_error("ifos_gradient_F: nf=%d outside [1,%d]", nf, IFOS_MAX_NF);
+        return IFOS_EINVAL;
+    }
+    DevGeom G = make_dev_geom(g);
+    GradFreq GF;
+    for (int l = 0; l < nf; ++l) {
+        // gradient_F.c:70-72: double product stored to float, then 1/x in float
+        float fd = (float)(2.0 * 2.0 * finv[l] * finv[l] * M_PI * M_PI);
+        GF.fdummy[l] = 1 / fd;
+    }
+    FPtrs Ff, Fb;
+    for (int n = 0; n < 6; ++n) {
+        Ff.p[n] = d_Ff[n];
+        Fb.p[n] = d_Fb[n];
+    }
+    dim3 block, grid = cell_grid(g, block);
+    { ProfScope ps_(IFOS_K_GRADIENT, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_gradient, grid, block, 0, (hipStream_t)stream, G, Ff, Fb, nf,
+                       (int64_t)ifos_field_elems(g), GF, m->rho, m->pi, m->u, d_grad[0], d_grad[1], d_grad[2]);
+    }
+    return hip_check(hipGetLastError(), "k_gradient");
+}
+
+extern "C" int ifos_matcopy_wrap(const ifos_geom *g, float *d_m, void *stream)
+{
+    DevGeom G = make_dev_geom(g);
+    hipStream_t s = (hipStream_t)stream;
+    { ProfScope ps_(IFOS_K_OTHER, s);
+    hipLaunchKernelGGL(k_wrap_y, dim3((unsigned)((g->nz + 2 + 63) / 64), (unsigned)(g->nx + 2)), dim3(64), 0, s, G,
+                       d_m);
+    }
+    { ProfScope ps_(IFOS_K_OTHER, s);
+    hipLaunchKernelGGL(k_wrap_x, dim3((unsigned)((g->nz + 2 + 63) / 64), (unsigned)(g->ny + 2)), dim3(64), 0, s, G,
+                       d_m);
+    }
+    { ProfScope ps_(IFOS_K_OTHER, s);
+    hipLaunchKernelGGL(k_wrap_z, dim3((unsigned)((g->nx + 2 + 63) / 64), (unsigned)(g->ny + 2)), dim3(64), 0, s, G,
+                       d_m);
+    }
+    return hip_check(hipGetLastError(), "k_wrap");
+}
+
+extern "C" int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *d_u, float *d_uipjp,
+                           float *d_ujpkp, float *d_uipkp, float *d_rjp, float *d_rkp, float *d_rip, void *stream)
+{
+    DevGeom G = make_dev_geom(g);
+    dim3 block, grid = cell_grid(g, block);
+    { ProfScope ps_(IFOS_K_OTHER, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_av_mat, grid, block, 0, (hipStream_t)stream, G, d_rho, d_u, d_uipjp, d_ujpkp, d_uipkp,
+                       d_rjp, d_rkp, d_rip);
+    }
+    return hip_check(hipGetLastError(), "k_av_mat");
+}
+
+extern "C" int ifos_residual(const ifos_geom *g, const float *d_obs, const float *d_syn, float *d_adj, int ntr,
+                             int ns, double *l2, void *stream)
+{
+    if (ntr <= 0) return IFOS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    double *part = nullptr;
+    if (hip_check(hipMallocAsync((void **)&part, sizeof(double) * 3 * ntr, s), "hipMallocAsync")) return IFOS_EHIP;
+    { ProfScope ps_(IFOS_K_OTHER, s);
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)((3 * ntr + 127) / 128)), dim3(128), 0, s, d_obs, d_syn, d_adj,
+                       ntr, ns, ns + 1, ns + 2, g->dt, part);
+    }
+    std::vector<double> h(3 * (size_t)ntr);
+    int rc = hip_check(hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * ntr, hipMemcpyDeviceToHost, s),
+                       "hipMemcpyAsync");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    hipFreeAsync(part, s);
+    if (rc) return rc;
+    double acc = 0.0;
+    for (double v : h) acc += v;
+    if (l2) *l2 += acc;
+    return IFOS_OK;
+}
+
+extern "C" int ifos_fill(float *d, size_t n, float value, void *stream)
+{
+    if (n == 0) return IFOS_OK;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    { ProfScope ps_(IFOS_K_OTHER, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d, (int64_t)n, value);
+    }
+    return hip_check(hipGetLastError(), "k_fill");
+}

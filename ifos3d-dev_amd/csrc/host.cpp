@@ -1,0 +1,592 @@
+// host.cpp -- host side of libifos3d_hip.so: geometry, error reporting, the
+// CPML profiles and source wavelet (computed once per run on the CPU, with
+// the reference's float/double promotions), and the plain one-shot driver.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ifos_dev.h"
+
+namespace ifos {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return IFOS_OK;
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return IFOS_EHIP;
+}
+
+DevGeom make_dev_geom(const ifos_geom *g)
+{
+    DevGeom d;
+    d.nx = g->nx;
+    d.ny = g->ny;
+    d.nz = g->nz;
+    d.fw = g->fw;
+    d.free_surf = g->free_surf;
+    d.si = g->pitch;
+    d.sj = (int64_t)(g->nx + 5) * g->pitch;
+    d.base = 2 * d.sj + 2 * d.si + IFOS_KOFF;
+    d.dx = g->dx;
+    d.dy = g->dy;
+    d.dz = g->dz;
+    d.dt = g->dt;
+    d.dtdx = g->dt / g->dx;
+    d.dtdy = g->dt / g->dy;
+    d.dtdz = g->dt / g->dz;
+    // update_v_ssg.c:98-100
+    d.b1 = (float)(9.0 / 8.0);
+    d.b2 = (float)(-1.0 / 24.0);
+    if (g->fdcoeff == 2) {
+        d.b1 = (float)1.1382;
+        d.b2 = (float)(-0.046414);
+    }
+    return d;
+}
+
+}  // namespace ifos
+
+using namespace ifos;
+
+extern "C" const char *ifos_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ifos_device_sync(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+extern "C" int ifos_geom_init(ifos_geom *g, int nx, int ny, int nz, int fw, float dx, float dy, float dz, float dt,
+                              int nt, int fdcoeff, int free_surf)
+{
+    if (!g) return IFOS_EINVAL;
+    if (nx < 2 * fw + 1 || ny < 2 * fw + 1 || nz < 2 * fw + 1 || fw < 1) {
+        set_error("ifos_geom_init: grid %dx%dx%d too small for FW=%d", nx, ny, nz, fw);
+        return IFOS_EINVAL;
+    }
+    if (!(dx > 0 && dy > 0 && dz > 0 && dt > 0) || nt < 2) {
+        set_error("ifos_geom_init: need positive spacings/dt and NT >= 2");
+        return IFOS_EINVAL;
+    }
+    if (fdcoeff != 1 && fdcoeff != 2) {
+        set_error("ifos_geom_init: FDCOEFF must be 1 (Taylor) or 2 (Holberg)");
+        return IFOS_EINVAL;
+    }
+    g->nx = nx;
+    g->ny = ny;
+    g->nz = nz;
+    g->fw = fw;
+    g->dx = dx;
+    g->dy = dy;
+    g->dz = dz;
+    g->dt = dt;
+    g->nt = nt;
+    g->time = (nt - 1) * dt;  // ifos3d.c:177
+    g->fdcoeff = fdcoeff;
+    g->free_surf = free_surf ? 1 : 0;
+    g->pitch = ((nz + IFOS_KOFF + 3) + 31) / 32 * 32;  // k in [-2, nz+2] -> [KOFF-2, KOFF+nz+2]
+    return IFOS_OK;
+}
+
+extern "C" size_t ifos_field_elems(const ifos_geom *g) { return (size_t)(g->ny + 5) * (g->nx + 5) * g->pitch; }
+
+extern "C" size_t ifos_offset(const ifos_geom *g, int j, int i, int k)
+{
+    DevGeom d = make_dev_geom(g);
+    return (size_t)at(d, j, i, k);
+}
+
+extern "C" size_t ifos_psi_elems(const ifos_geom *g, int n)
+{
+    const size_t f2 = 2 * (size_t)g->fw;
+    if (n < 3) return (size_t)g->ny * f2 * g->nz;
+    if (n < 6) return f2 * g->nx * g->nz;
+    return (size_t)g->ny * g->nx * f2;
+}
+
+// ---------------------------------------------------------------------------
+// CPML_coeff.c:87-324.  |d| is tested with the integer abs() (the float is
+// truncated), the y pass clears a_x where (int)d_y == 0, and the right-hand
+// side mirrors the left with the integer-grid profile one cell further out.
+// C's libm calls on float arguments promote to double; in C++ the float
+// overloads (expf, powf, logf) would be picked instead, so every call here
+// goes through these double-only wrappers.
+static inline double d_exp(double x) { return ::exp(x); }
+static inline double d_pow(double x, double y) { return ::pow(x, y); }
+static inline double d_log(double x) { return ::log(x); }
+
+static int trunc_nonzero(float d)
+{
+    int t = (int)d;
+    return abs(t) > 1.0e-6;
+}
+
+static void cpml_axis(const ifos_geom *g, float dh, float vppml, float fpml, float *K, float *a, float *b,
+                      float *Kh, float *ah, float *bh, std::vector<float> &d)
+{
+    const int fw = g->fw;
+    const float npower = 3;
+    const float k_max_PML = 2.0;
+    const float alpha_max_PML = (float)(2.0 * 3.141592653589793 * (fpml / 2.0));
+    const float Rcoef = 0.0008f;
+    const float ca = 0.25f, cb = 0.75f, cc = 0.0f;
+    std::vector<float> dh_(2 * fw + 2, 0.0f), alpha(2 * fw + 2, 0.0f), alpha_h(2 * fw + 2, 0.0f);
+    const float d0 = (float)(-(npower + 1) * vppml * d_log(Rcoef) / (2.0 * fw * dh));
+    for (int i = 1; i <= fw + 1; i++) {
+        K[i] = 1.0f;
+        float pos = (float)((fw - i + 1) * dh);
+        float pn = pos / (float)(fw * dh);
+        d[i] = (float)(d0 * (ca * pn + cb * d_pow(pn, npower) + cc * d_pow(pn, 4)));
+        K[i] = (float)(1.0 + (k_max_PML - 1.0) * d_pow(pn, npower));
+        alpha[i] = (float)(alpha_max_PML * (1.0 - pn));
+        b[i] = (float)d_exp(-(d[i] / K[i] + alpha[i]) * g->dt);
+        if (trunc_nonzero(d[i])) a[i] = (float)(d[i] * (b[i] - 1.0) / (K[i] * (d[i] + K[i] * alpha[i])));
+        if (i <= fw) {
+            const int l = 2 * fw - i + 1;
+            pos = (float)((fw - i + 0.5) * dh);
+            pn = pos / (float)(fw * dh);
+            Kh[i] = 1.0f;
+            dh_[i] = (float)(d0 * (ca * pn + cb * d_pow(pn, npower) + cc * d_pow(pn, 4)));
+            Kh[i] = (float)(1.0 + (k_max_PML - 1.0) * d_pow(pn, npower));
+            alpha_h[i] = (float)(alpha_max_PML * (1.0 - pn));
+            bh[i] = (float)d_exp(-(dh_[i] / Kh[i] + alpha_h[i]) * g->dt);
+            if (trunc_nonzero(dh_[i]))
+                ah[i] = (float)(dh_[i] * (bh[i] - 1.0) / (Kh[i] * (dh_[i] + Kh[i] * alpha_h[i])));
+            if (i > 1) {
+                K[l + 1] = K[i];
+                b[l + 1] = b[i];
+                if (trunc_nonzero(d[i])) a[l + 1] = a[i];
+            }
+            Kh[l] = Kh[i];
+            bh[l] = bh[i];
+            if (trunc_nonzero(d[i])) ah[l] = ah[i];
+        }
+    }
+}
+
+extern "C" int ifos_cpml_profiles(const ifos_geom *g, float vppml, float fpml, float *prof)
+{
+    if (!g || !prof) return IFOS_EINVAL;
+    const int len = 2 * g->fw + 1;
+    memset(prof, 0, sizeof(float) * 18 * (size_t)len);
+    const float dh[3] = {g->dx, g->dy, g->dz};
+    std::vector<float> d(2 * g->fw + 2);
+    for (int ax = 0; ax < 3; ++ax) {
+        float *p = prof + 6 * ax * len;
+        std::fill(d.begin(), d.end(), 0.0f);
+        cpml_axis(g, dh[ax], vppml, fpml, p, p + len, p + 2 * len, p + 3 * len, p + 4 * len, p + 5 * len, d);
+        if (ax == 1)
+            for (int i = 1; i <= g->fw + 1; ++i)
+                if (!trunc_nonzero(d[i])) prof[1 * len + i] = 0.0f;  // a_x, CPML_coeff.c:216
+    }
+    return IFOS_OK;
+}
+
+// wavelet.c:47-90
+extern "C" int ifos_wavelet(const ifos_geom *g, int shape, float tshift, float fc, float amp, float *sig)
+{
+    if (!g || !sig) return IFOS_EINVAL;
+    if (shape != 1 && shape != 2 && shape != 4 && shape != 5) {
+        set_error("ifos_wavelet: source shape %d not supported (1, 2, 4, 5)", shape);
+        return IFOS_EINVAL;
+    }
+    const double PI_ = 3.141592653589793;
+    sig[0] = 0.0f;
+    sig[g->nt + 1] = 0.0f;
+    for (int nt = 1; nt <= g->nt; nt++) {
+        float t = (float)nt * g->dt;
+        float ts = (float)(1.0 / fc), tau, a = 0.0f;
+        switch (shape) {
+        case 1:
+            tau = (float)(PI_ * (t - 1.5 * ts - tshift) / (ts));
+            a = (float)((1.0 - 2.0 * tau * tau) * d_exp(-tau * tau));
+            break;
+        case 2:
+            if ((t < tshift) || (t > (tshift + ts)))
+                a = 0.0f;
+            else
+                a = (float)(sin(2.0 * PI_ * (t - tshift) * fc) - 0.5 * sin(4.0 * PI_ * (t - tshift) * fc));
+            break;
+        case 4:
+            if ((t < tshift) || (t > (tshift + ts)))
+                a = 0.0f;
+            else
+                a = (float)((0.75 * PI_ / ts) * d_pow(::sin(PI_ * (t - tshift) / ts), 3.0));
+            break;
+        case 5: a = (t < tshift) ? 0.0f : 1.0f; break;
+        }
+        sig[nt] = a * amp;
+    }
+    return IFOS_OK;
+}
+
+extern "C" int ifos_ntast(float fmax, int tast, float dt)
+{
+    float dummy = (1 / (fmax * tast * dt));
+    int n = (int)dummy;
+    return n ? n : 1;
+}
+
+// ---------------------------------------------------------------------------
+// plain one-shot driver
+struct ifos_ctx {
+    ifos_geom g;
+    int nf_max, ntr_max, prec;
+    size_t fe;
+    float *fields[9];
+    float *psi_s[9], *psi_v[9];
+    float *model[9];       // rho pi u uipjp ujpkp uipkp rjp rkp rip
+    float *prof;
+    float *F[12], *C[12];  // forward Re xyz Im xyz, adjoint ...; compensation
+    float *sig;            // [nt+2]
+    int *src;              // [4]
+    int *rec;              // [ntr][3]
+    float *adj;            // [3][ntr][nt+2]
+    float finv[IFOS_MAX_NF];
+    int nf, tast;
+    std::vector<void *> allocs;
+};
+
+static int dalloc(ifos_ctx *c, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        return IFOS_ENOMEM;
+    }
+    c->allocs.push_back(*p);
+    e = hipMemset(*p, 0, bytes);
+    return hip_check(e, "hipMemset");
+}
+
+extern "C" int ifos_ctx_destroy(ifos_ctx *c)
+{
+    if (!c) return IFOS_OK;
+    hipDeviceSynchronize();
+    for (void *p : c->allocs) hipFree(p);
+    delete c;
+    return IFOS_OK;
+}
+
+extern "C" int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max, int ntr_max, int dft_precision)
+{
+    if (!out || !g || nf_max < 1 || nf_max > IFOS_MAX_NF || ntr_max < 0) {
+        set_error("ifos_ctx_create: bad arguments (nf_max in [1,%d])", IFOS_MAX_NF);
+        return IFOS_EINVAL;
+    }
+    ifos_ctx *c = new ifos_ctx();
+    c->g = *g;
+    c->nf_max = nf_max;
+    c->ntr_max = ntr_max;
+    c->prec = dft_precision;
+    c->fe = ifos_field_elems(g);
+    int rc = 0;
+    for (int n = 0; n < 9 && !rc; ++n) rc = dalloc(c, (void **)&c->fields[n], c->fe * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = dalloc(c, (void **)&c->psi_s[n], ifos_psi_elems(g, n) * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = dalloc(c, (void **)&c->psi_v[n], ifos_psi_elems(g, n) * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = dalloc(c, (void **)&c->model[n], c->fe * 4);
+    if (!rc) rc = dalloc(c, (void **)&c->prof, 18 * (size_t)(2 * g->fw + 1) * 4);
+    for (int n = 0; n < 12 && !rc; ++n) rc = dalloc(c, (void **)&c->F[n], c->fe * 4 * (size_t)nf_max);
+    for (int n = 0; n < 12; ++n) c->C[n] = nullptr;
+    if (dft_precision == IFOS_DFT_KAHAN)
+        for (int n = 0; n < 12 && !rc; ++n) rc = dalloc(c, (void **)&c->C[n], c->fe * 4 * (size_t)nf_max);
+    if (!rc) rc = dalloc(c, (void **)&c->sig, (size_t)(g->nt + 2) * 4);
+    if (!rc) rc = dalloc(c, (void **)&c->src, 4 * sizeof(int));
+    if (!rc) rc = dalloc(c, (void **)&c->rec, 3 * sizeof(int) * (size_t)(ntr_max > 0 ? ntr_max : 1));
+    if (!rc) rc = dalloc(c, (void **)&c->adj, 3 * (size_t)(ntr_max > 0 ? ntr_max : 1) * (g->nt + 2) * 4);
+    if (rc) {
+        ifos_ctx_destroy(c);
+        return rc;
+    }
+    c->nf = 0;
+    c->tast = 100;
+    *out = c;
+    return IFOS_OK;
+}
+
+static ifos_model ctx_model(const ifos_ctx *c)
+{
+    ifos_model m = {c->model[0], c->model[1], c->model[2], c->model[3], c->model[4],
+                    c->model[5], c->model[6], c->model[7], c->model[8]};
+    return m;
+}
+
+static ifos_fields ctx_fields(const ifos_ctx *c)
+{
+    ifos_fields f = {c->fields[0], c->fields[1], c->fields[2], c->fields[3], c->fields[4],
+                     c->fields[5], c->fields[6], c->fields[7], c->fields[8]};
+    return f;
+}
+
+extern "C" int ifos_ctx_fields(ifos_ctx *c, ifos_fields *f)
+{
+    *f = ctx_fields(c);
+    return IFOS_OK;
+}
+
+extern "C" int ifos_ctx_dft(ifos_ctx *c, int back, float *d_F[6])
+{
+    for (int n = 0; n < 6; ++n) d_F[n] = c->F[(back ? 6 : 0) + n];
+    return IFOS_OK;
+}
+
+extern "C" int ifos_ctx_set_model(ifos_ctx *c, const float *d_rho, const float *d_pi, const float *d_u, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const float *in[3] = {d_rho, d_pi, d_u};
+    for (int n = 0; n < 3; ++n) {
+        int rc = hip_check(hipMemcpyAsync(c->model[n], in[n], c->fe * 4, hipMemcpyDeviceToDevice, s), "copy model");
+        if (rc) return rc;
+        rc = ifos_matcopy_wrap(&c->g, c->model[n], stream);
+        if (rc) return rc;
+    }
+    return ifos_av_mat(&c->g, c->model[0], c->model[2], c->model[3], c->model[4], c->model[5], c->model[6],
+                       c->model[7], c->model[8], stream);
+}
+
+extern "C" int ifos_ctx_set_cpml(ifos_ctx *c, float vppml, float fpml)
+{
+    std::vector<float> h(18 * (size_t)(2 * c->g.fw + 1));
+    int rc = ifos_cpml_profiles(&c->g, vppml, fpml, h.data());
+    if (rc) return rc;
+    return hip_check(hipMemcpy(c->prof, h.data(), h.size() * 4, hipMemcpyHostToDevice), "upload CPML profiles");
+}
+
+extern "C" int ifos_ctx_set_freqs(ifos_ctx *c, const float *finv, int nf, int tast)
+{
+    if (nf < 1 || nf > c->nf_max) {
+        set_error("ifos_ctx_set_freqs: nf=%d outside [1,%d]", nf, c->nf_max);
+        return IFOS_EINVAL;
+    }
+    for (int l = 0; l < nf; ++l) c->finv[l] = finv[l];
+    c->nf = nf;
+    c->tast = tast;
+    return IFOS_OK;
+}
+
+// one propagation of NT steps; back = 0 forward (sources), 1 adjoint
+static int propagate(ifos_ctx *c, int back, const ifos_shot *s, int ntast, float *d_syn, hipStream_t st)
+{
+    const ifos_geom *g = &c->g;
+    ifos_fields f = ctx_fields(c);
+    ifos_model m = ctx_model(c);
+    ifos_psi ps, pv;
+    for (int n = 0; n < 9; ++n) {
+        ps.p[n] = c->psi_s[n];
+        pv.p[n] = c->psi_v[n];
+    }
+    float *Fp[6], *Cp[6];
+    for (int n = 0; n < 6; ++n) {
+        Fp[n] = c->F[(back ? 6 : 0) + n];
+        Cp[n] = c->C[(back ? 6 : 0) + n];
+    }
+    const bool dft = s->want_adjoint != 0;
+    const size_t probe = ifos_offset(g, g->ny / 2, g->nx / 2, g->nz / 2);
+    for (int nt = 1; nt <= g->nt; ++nt) {
+        int rc;
+        // NaN probe every 64 steps (ifos3d.c:761 checks every step on the host)
+        if ((nt & 63) == 0) {
+            float v = 0.0f;
+            rc = hip_check(hipMemcpyAsync(&v, c->fields[1] + probe, 4, hipMemcpyDeviceToHost, st), "probe");
+            if (!rc) rc = hip_check(hipStreamSynchronize(st), "probe sync");
+            if (rc) return rc;
+            if (isnan(v)) {
+                set_error("simulation unstable: NaN at the probe cell (step %d)", nt);
+                return IFOS_ENAN;
+            }
+        }
+        rc = ifos_update_v(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &ps, st);
+        if (!rc) {
+            if (!back)
+                rc = ifos_add_forces(g, &f, &m, c->src, c->sig, 1, nt, 0.0f, 0.0f, st);
+            else
+                rc = ifos_add_adjoint(g, &f, c->rec, c->adj, s->ntr, nt, st);
+        }
+        if (!rc) rc = ifos_update_v(g, IFOS_REGION_CPML, &f, &m, c->prof, &ps, st);
+        if (!rc && back) rc = ifos_add_adjoint(g, &f, c->rec, c->adj, s->ntr, nt, st);
+        if (!rc) rc = ifos_update_s(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &pv, st);
+        if (!rc) rc = ifos_update_s(g, IFOS_REGION_CPML, &f, &m, c->prof, &pv, st);
+        if (!rc && !back) rc = ifos_psource(g, &f, c->src, c->sig, 1, nt, st);
+        if (!rc && dft && nt % ntast == 0)
+            rc = ifos_discfourier(g, &f, Fp, c->prec == IFOS_DFT_KAHAN ? Cp : nullptr, c->finv, c->nf, nt, ntast,
+                                  back, c->prec, st);
+        if (!rc && g->free_surf) rc = ifos_surface(g, &f, &m, st);
+        if (!rc && !back && d_syn) rc = ifos_seismo(g, &f, c->rec, s->ntr, nt, g->nt, d_syn, st);
+        if (rc) return rc;
+    }
+    return IFOS_OK;
+}
+
+static int zero_wavefield(ifos_ctx *c, hipStream_t st)
+{
+    int rc = 0;
+    for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->fields[n], c->fe, 0.0f, st);
+    for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->psi_s[n], ifos_psi_elems(&c->g, n), 0.0f, st);
+    for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->psi_v[n], ifos_psi_elems(&c->g, n), 0.0f, st);
+    return rc;
+}
+
+extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, float *const d_grad[3],
+                                 double *l2, void *stream)
+{
+    if (!c || !s) return IFOS_EINVAL;
+    const ifos_geom *g = &c->g;
+    hipStream_t st = (hipStream_t)stream;
+    if (s->ntr > c->ntr_max) {
+        set_error("ifos_ctx_run_shot: ntr=%d > ntr_max=%d", s->ntr, c->ntr_max);
+        return IFOS_EINVAL;
+    }
+    if (s->want_adjoint && c->nf < 1) {
+        set_error("ifos_ctx_run_shot: frequencies not set");
+        return IFOS_EINVAL;
+    }
+    // bounds of source and receivers
+    if (s->src_i < 1 || s->src_i > g->nx || s->src_j < 1 || s->src_j > g->ny || s->src_k < 1 || s->src_k > g->nz) {
+        set_error("ifos_ctx_run_shot: source (%d,%d,%d) outside the grid", s->src_i, s->src_j, s->src_k);
+        return IFOS_EINVAL;
+    }
+    for (int r = 0; r < s->ntr; ++r) {
+        const int *p = s->rec_ijk + 3 * r;
+        if (p[0] < 1 || p[0] > g->nx || p[1] < 1 || p[1] > g->ny || p[2] < 1 || p[2] > g->nz) {
+            set_error("ifos_ctx_run_shot: receiver %d (%d,%d,%d) outside the grid", r, p[0], p[1], p[2]);
+            return IFOS_EINVAL;
+        }
+    }
+    std::vector<float> sig(g->nt + 2);
+    int rc = ifos_wavelet(g, s->source_shape, s->tshift, s->fc, s->amp, sig.data());
+    if (rc) return rc;
+    int hsrc[4] = {s->src_i, s->src_j, s->src_k, s->src_type};
+    rc = hip_check(hipMemcpyAsync(c->sig, sig.data(), sig.size() * 4, hipMemcpyHostToDevice, st), "upload signal");
+    if (!rc) rc = hip_check(hipMemcpyAsync(c->src, hsrc, sizeof hsrc, hipMemcpyHostToDevice, st), "upload source");
+    if (!rc && s->ntr > 0)
+        rc = hip_check(hipMemcpyAsync(c->rec, s->rec_ijk, 3 * sizeof(int) * s->ntr, hipMemcpyHostToDevice, st),
+                       "upload receivers");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "sync uploads");
+    if (rc) return rc;
+
+    const int ntast = s->want_adjoint ? ifos_ntast(c->finv[c->nf - 1], c->tast, g->dt) : 1;
+    // ifos3d.c:738-739: zero wavefield and DFT accumulators
+    rc = zero_wavefield(c, st);
+    for (int n = 0; n < 12 && !rc; ++n) {
+        rc = ifos_fill(c->F[n], c->fe * c->nf_max, 0.0f, st);
+        if (!rc && c->C[n]) rc = ifos_fill(c->C[n], c->fe * c->nf_max, 0.0f, st);
+    }
+    if (!rc && d_syn) rc = ifos_fill(d_syn, 3 * (size_t)s->ntr * (g->nt + 1), 0.0f, st);
+    if (!rc) rc = propagate(c, 0, s, ntast, d_syn, st);
+    if (rc || !s->want_adjoint) return rc;
+
+    // residual -> adjoint sources (ifos3d.c:837-866), zero adjoint traces first
+    rc = ifos_fill(c->adj, 3 * (size_t)s->ntr * (g->nt + 2), 0.0f, st);
+    if (!rc && s->d_obs && d_syn) rc = ifos_residual(g, s->d_obs, d_syn, c->adj, s->ntr, g->nt, l2, st);
+    if (!rc) rc = zero_wavefield(c, st);
+    if (!rc) rc = propagate(c, 1, s, ntast, nullptr, st);
+    if (rc) return rc;
+    ifos_model m = ctx_model(c);
+    float *Ff[6], *Fb[6];
+    for (int n = 0; n < 6; ++n) {
+        Ff[n] = c->F[n];
+        Fb[n] = c->F[6 + n];
+    }
+    return ifos_gradient_F(g, Ff, Fb, &m, c->finv, c->nf, d_grad, st);
+}
+
+// ---------------------------------------------------------------------------
+// launch timer
+namespace ifos {
+struct ProfState {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, size_t>> marks;  // (category, start event index)
+    double ms[IFOS_K_NCAT] = {};
+    long count[IFOS_K_NCAT] = {};
+};
+static ProfState g_prof;
+
+static hipEvent_t prof_event()
+{
+    if (g_prof.used == g_prof.pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        g_prof.pool.push_back(e);
+    }
+    return g_prof.pool[g_prof.used++];
+}
+
+void prof_begin(int cat, hipStream_t s)
+{
+    if (!g_prof.on) return;
+    size_t idx = g_prof.used;
+    hipEvent_t e = prof_event();
+    if (!e) return;
+    (void)hipEventRecord(e, s);
+    g_prof.marks.push_back({cat, idx});
+}
+
+void prof_end(int cat, hipStream_t s)
+{
+    if (!g_prof.on || g_prof.marks.empty()) return;
+    hipEvent_t e = prof_event();
+    if (!e) return;
+    (void)hipEventRecord(e, s);
+    (void)cat;
+}
+
+static void prof_collect()
+{
+    for (auto &m : g_prof.marks) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, g_prof.pool[m.second], g_prof.pool[m.second + 1]) == hipSuccess) {
+            g_prof.ms[m.first] += ms;
+            g_prof.count[m.first] += 1;
+        }
+    }
+    g_prof.marks.clear();
+    g_prof.used = 0;
+}
+}  // namespace ifos
+
+extern "C" int ifos_prof_enable(int on)
+{
+    g_prof.on = on != 0;
+    return IFOS_OK;
+}
+
+extern "C" int ifos_prof_reset(void)
+{
+    (void)hipDeviceSynchronize();
+    g_prof.marks.clear();
+    g_prof.used = 0;
+    for (int n = 0; n < IFOS_K_NCAT; ++n) {
+        g_prof.ms[n] = 0.0;
+        g_prof.count[n] = 0;
+    }
+    return IFOS_OK;
+}
+
+extern "C" int ifos_prof_read(double *ms, long *count, int ncat)
+{
+    int rc = hip_check(hipDeviceSynchronize(), "ifos_prof_read sync");
+    if (rc) return rc;
+    prof_collect();
+    for (int n = 0; n < ncat && n < IFOS_K_NCAT; ++n) {
+        if (ms) ms[n] = g_prof.ms[n];
+        if (count) count[n] = g_prof.count[n];
+    }
+    return IFOS_OK;
+}

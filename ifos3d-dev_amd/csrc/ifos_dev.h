@@ -1,0 +1,100 @@
+// ifos_dev.h -- device-side geometry and helpers shared by the gfx950 kernels.
+//
+// All kernels keep the reference's arithmetic order and are compiled with
+// -ffp-contract=off and IEEE division, so an FP32 update is bit-identical to
+// the reference C code on the same inputs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ifos3d_hip.h"
+
+// largest number of DFT frequencies per call (kernel-argument arrays)
+#define IFOS_MAX_NF 32
+
+namespace ifos {
+
+// Geometry in the form the kernels use, passed by value.
+struct DevGeom {
+    int nx, ny, nz, fw;
+    int free_surf;
+    int64_t sj, si;      // element strides of j and i (k stride is 1)
+    int64_t base;        // offset of (0,0,0)
+    float dx, dy, dz, dt;
+    float dtdx, dtdy, dtdz;  // DT/DX ... in float, update_v_ssg.c:95-97
+    float b1, b2;            // 4th-order coefficients, update_v_ssg.c:98-100
+};
+
+__host__ __device__ inline int64_t at(const DevGeom &g, int j, int i, int k)
+{
+    return g.base + (int64_t)j * g.sj + (int64_t)i * g.si + k;
+}
+
+// Slab index of a cell along one axis: [1..fw] -> h, [n-fw+1..n] -> h in
+// [fw+1..2fw], else 0 (update_v_ssg_CPML.c:70-375 index maps).
+__host__ __device__ inline int slab_h(int idx, int n, int fw, int lo_active)
+{
+    if (lo_active && idx <= fw) return idx;
+    if (idx >= n - fw + 1) return idx - (n - 2 * fw);
+    return 0;
+}
+
+__device__ inline int64_t psi_x_at(const DevGeom &g, int j, int h, int k)
+{
+    return ((int64_t)(j - 1) * (2 * g.fw) + (h - 1)) * g.nz + (k - 1);
+}
+__device__ inline int64_t psi_y_at(const DevGeom &g, int h, int i, int k)
+{
+    return ((int64_t)(h - 1) * g.nx + (i - 1)) * g.nz + (k - 1);
+}
+__device__ inline int64_t psi_z_at(const DevGeom &g, int j, int i, int h)
+{
+    return ((int64_t)(j - 1) * g.nx + (i - 1)) * (2 * g.fw) + (h - 1);
+}
+
+// CPML profile accessor: prof[(6*axis + n)*(2fw+1) + h], n = K a b Kh ah bh
+struct Prof {
+    const float *p;
+    int len;  // 2fw+1
+    __device__ float K(int ax, int h) const { return p[(6 * ax + 0) * len + h]; }
+    __device__ float A(int ax, int h) const { return p[(6 * ax + 1) * len + h]; }
+    __device__ float B(int ax, int h) const { return p[(6 * ax + 2) * len + h]; }
+    __device__ float KH(int ax, int h) const { return p[(6 * ax + 3) * len + h]; }
+    __device__ float AH(int ax, int h) const { return p[(6 * ax + 4) * len + h]; }
+    __device__ float BH(int ax, int h) const { return p[(6 * ax + 5) * len + h]; }
+};
+
+// psi <- b*psi + a*d ;  d <- d/K + psi   (update_v_ssg_CPML.c:85-90)
+__device__ inline void cpml_apply(float *psi, int64_t at_, float &d, float K, float a, float b)
+{
+    float p = b * psi[at_] + a * d;
+    psi[at_] = p;
+    d = d / K + p;
+}
+
+// Index box (inclusive), and a list of them for the slab launches.
+struct Box {
+    int i0, i1, j0, j1, k0, k1;
+};
+struct BoxList {
+    Box b[6];
+    int64_t start[7];  // cumulative cell counts
+    int n;
+};
+
+DevGeom make_dev_geom(const ifos_geom *g);
+void set_error(const char *fmt, ...);
+int hip_check(hipError_t e, const char *what);
+
+// Optional per-kernel timing (ifos_prof_enable): a pair of HIP events is
+// recorded around each launch on the launch's own stream.
+void prof_begin(int cat, hipStream_t s);
+void prof_end(int cat, hipStream_t s);
+struct ProfScope {
+    int cat;
+    hipStream_t s;
+    ProfScope(int c, hipStream_t st) : cat(c), s(st) { prof_begin(cat, s); }
+    ~ProfScope() { prof_end(cat, s); }
+};
+
+}  // namespace ifos

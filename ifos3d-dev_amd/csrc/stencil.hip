@@ -1,0 +1,447 @@
+// stencil.hip -- 4th-order staggered-grid velocity and stress updates for
+// gfx950, elastic, with CPML memory variables updated only in the slabs.
+//
+// Two launch shapes:
+//  * march kernels (k_v_march / k_s_march): a 64-wide wavefront spans k (the
+//    contiguous axis, coalesced 256-B rows), a block spans TI rows of i, and
+//    each thread marches a chunk of j planes keeping the j-direction operands
+//    in a 4-deep register queue, so each j-neighbour is loaded from memory
+//    once per column.  In-plane (i, k) neighbours come through L1/L2.
+//    Blocks are renumbered so that the tiles each XCD receives form one
+//    contiguous stretch of the grid (workgroups are dealt round-robin over
+//    the 8 XCDs), which keeps the i-halo rows of a tile in the same L2.
+//  * slab kernel (k_v_slabs / k_s_slabs): one thread per cell over the six
+//    CPML slabs, the shape of update_v_ssg_CPML.c / update_s_ssg_CPML_elastic.c,
+//    used by the reference-shaped (drop-in) update_*_CPML entry points.
+//
+// Arithmetic follows the reference expression by expression
+// (update_v_ssg.c:106-125, update_v_ssg_CPML.c:73-100,
+//  update_s_ssg_elastic.c:106-138, update_s_ssg_CPML_elastic.c:63-140).
+#include "ifos_dev.h"
+
+namespace ifos {
+
+struct VFields {
+    float *vx, *vy, *vz;
+    const float *sxx, *syy, *szz, *sxy, *syz, *sxz;
+    const float *rip, *rjp, *rkp;
+};
+struct SFields {
+    const float *vx, *vy, *vz;
+    float *sxx, *syy, *szz, *sxy, *syz, *sxz;
+    const float *pi, *u, *uipjp, *ujpkp, *uipkp;
+};
+struct Psi9 {
+    float *p[9];
+};
+
+// 4th-order differences: forward f(c+1)-f(c), backward f(c)-f(c-1)
+#define D4F(b1, b2, m1, c0, p1, p2) ((b1) * ((p1) - (c0)) + (b2) * ((p2) - (m1)))
+#define D4B(b1, b2, m2, m1, c0, p1) ((b1) * ((c0) - (m1)) + (b2) * ((p1) - (m2)))
+
+// ---------------------------------------------------------------------------
+// velocity: CPML corrections of the 9 stress derivatives of one cell
+// (update_v_ssg_CPML.c:84-130: x with K/a/b_x(_half), y, z)
+__device__ inline void v_cpml(const DevGeom &g, const Prof &P, const Psi9 &psi, int j, int i, int k,
+                              int hx, int hy, int hz, float &sxx_x, float &sxy_x, float &sxz_x,
+                              float &sxy_y, float &syy_y, float &syz_y, float &sxz_z, float &syz_z,
+                              float &szz_z)
+{
+    if (hx) {
+        int64_t a = psi_x_at(g, j, hx, k);
+        cpml_apply(psi.p[0], a, sxx_x, P.KH(0, hx), P.AH(0, hx), P.BH(0, hx));
+        cpml_apply(psi.p[1], a, sxy_x, P.K(0, hx), P.A(0, hx), P.B(0, hx));
+        cpml_apply(psi.p[2], a, sxz_x, P.K(0, hx), P.A(0, hx), P.B(0, hx));
+    }
+    if (hy) {
+        int64_t a = psi_y_at(g, hy, i, k);
+        cpml_apply(psi.p[3], a, sxy_y, P.K(1, hy), P.A(1, hy), P.B(1, hy));
+        cpml_apply(psi.p[4], a, syy_y, P.KH(1, hy), P.AH(1, hy), P.BH(1, hy));
+        cpml_apply(psi.p[5], a, syz_y, P.K(1, hy), P.A(1, hy), P.B(1, hy));
+    }
+    if (hz) {
+        int64_t a = psi_z_at(g, j, i, hz);
+        cpml_apply(psi.p[6], a, sxz_z, P.K(2, hz), P.A(2, hz), P.B(2, hz));
+        cpml_apply(psi.p[7], a, syz_z, P.K(2, hz), P.A(2, hz), P.B(2, hz));
+        cpml_apply(psi.p[8], a, szz_z, P.KH(2, hz), P.AH(2, hz), P.BH(2, hz));
+    }
+}
+
+// stress: CPML corrections of the 9 velocity derivatives
+// (update_s_ssg_CPML_elastic.c:72-115; the low-z layer divides by the y-axis
+// K profile, :104-108)
+__device__ inline void s_cpml(const DevGeom &g, const Prof &P, const Psi9 &psi, int j, int i, int k,
+                              int hx, int hy, int hz, float &vxx, float &vyx, float &vzx, float &vxy,
+                              float &vyy, float &vzy, float &vxz, float &vyz, float &vzz)
+{
+    if (hx) {
+        int64_t a = psi_x_at(g, j, hx, k);
+        cpml_apply(psi.p[0], a, vxx, P.K(0, hx), P.A(0, hx), P.B(0, hx));
+        cpml_apply(psi.p[1], a, vyx, P.KH(0, hx), P.AH(0, hx), P.BH(0, hx));
+        cpml_apply(psi.p[2], a, vzx, P.KH(0, hx), P.AH(0, hx), P.BH(0, hx));
+    }
+    if (hy) {
+        int64_t a = psi_y_at(g, hy, i, k);
+        cpml_apply(psi.p[3], a, vxy, P.KH(1, hy), P.AH(1, hy), P.BH(1, hy));
+        cpml_apply(psi.p[4], a, vyy, P.K(1, hy), P.A(1, hy), P.B(1, hy));
+        cpml_apply(psi.p[5], a, vzy, P.KH(1, hy), P.AH(1, hy), P.BH(1, hy));
+    }
+    if (hz) {
+        int64_t a = psi_z_at(g, j, i, hz);
+        const int kax = (k <= g.fw) ? 1 : 2;
+        cpml_apply(psi.p[6], a, vxz, P.KH(kax, hz), P.AH(2, hz), P.BH(2, hz));
+        cpml_apply(psi.p[7], a, vyz, P.KH(kax, hz), P.AH(2, hz), P.BH(2, hz));
+        cpml_apply(psi.p[8], a, vzz, P.K(kax, hz), P.A(2, hz), P.B(2, hz));
+    }
+}
+
+// stress update of one cell from its 9 velocity derivatives
+// (update_s_ssg_elastic.c:117-138)
+__device__ inline void s_apply(const DevGeom &g, const SFields &F, int64_t c, float vxx, float vxy,
+                               float vxz, float vyx, float vyy, float vyz, float vzx, float vzy, float vzz)
+{
+    const float DT = g.dt;
+    float fipjp = F.uipjp[c] * DT;
+    float fjpkp = F.ujpkp[c] * DT;
+    float fipkp = F.uipkp[c] * DT;
+    float gg = F.pi[c];
+    float f = 2.0f * F.u[c];
+    float vxyyx = vxy + vyx;
+    float vyzzy = vyz + vzy;
+    float vxzzx = vxz + vzx;
+    float vxxyyzz = vxx + vyy + vzz;
+    float vyyzz = vyy + vzz;
+    float vxxzz = vxx + vzz;
+    float vxxyy = vxx + vyy;
+    F.sxy[c] += (fipjp * vxyyx);
+    F.syz[c] += (fjpkp * vyzzy);
+    F.sxz[c] += (fipkp * vxzzx);
+    F.sxx[c] += DT * ((gg * vxxyyzz) - (f * vyyzz));
+    F.syy[c] += DT * ((gg * vxxyyzz) - (f * vxxzz));
+    F.szz[c] += DT * ((gg * vxxyyzz) - (f * vxxyy));
+}
+
+// ---------------------------------------------------------------------------
+// tile decode with XCD-aware renumbering: physical block b runs on XCD b % 8;
+// logical tile L = (b % 8) * per + b / 8 gives each XCD a contiguous range.
+struct Tiling {
+    int ntk, nti, ntj, total, per;
+    int jc;
+};
+
+__device__ inline bool decode_tile(const Tiling &T, int &tk, int &ti, int &tj)
+{
+    int b = blockIdx.x;
+    int L = (b & 7) * T.per + (b >> 3);
+    if (L >= T.total) return false;
+    tk = L % T.ntk;
+    int r = L / T.ntk;
+    ti = r % T.nti;
+    tj = r / T.nti;
+    return true;
+}
+
+template <int TK, int TI, bool CPML>
+__global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, VFields F, Prof P, Psi9 psi)
+{
+    int tk, ti, tj;
+    if (!decode_tile(T, tk, ti, tj)) return;
+    const int k = R.k0 + tk * TK + (int)threadIdx.x;
+    const int i = R.i0 + ti * TI + (int)threadIdx.y;
+    if (k > R.k1 || i > R.i1) return;
+    const int jb = R.j0 + tj * T.jc;
+    const int je = min(jb + T.jc - 1, R.j1);
+    const int64_t sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    int64_t c = at(g, jb, i, k);
+    int hx = 0;
+    if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
+    const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
+
+    // j queues: sxy and syz at j-2..j+1, syy at j-1..j+2
+    float sxy_m2 = F.sxy[c - 2 * sj], sxy_m1 = F.sxy[c - sj], sxy_0 = F.sxy[c], sxy_p1 = F.sxy[c + sj];
+    float syz_m2 = F.syz[c - 2 * sj], syz_m1 = F.syz[c - sj], syz_0 = F.syz[c], syz_p1 = F.syz[c + sj];
+    float syy_m1 = F.syy[c - sj], syy_0 = F.syy[c], syy_p1 = F.syy[c + sj], syy_p2 = F.syy[c + 2 * sj];
+
+    for (int j = jb; j <= je; ++j, c += sj) {
+        float sxx_x = g.dtdx * D4F(b1, b2, F.sxx[c - si], F.sxx[c], F.sxx[c + si], F.sxx[c + 2 * si]);
+        float sxy_y = g.dtdy * D4B(b1, b2, sxy_m2, sxy_m1, sxy_0, sxy_p1);
+        float sxz_z = g.dtdz * D4B(b1, b2, F.sxz[c - 2], F.sxz[c - 1], F.sxz[c], F.sxz[c + 1]);
+        float syy_y = g.dtdy * D4F(b1, b2, syy_m1, syy_0, syy_p1, syy_p2);
+        float sxy_x = g.dtdx * D4B(b1, b2, F.sxy[c - 2 * si], F.sxy[c - si], sxy_0, F.sxy[c + si]);
+        float syz_z = g.dtdz * D4B(b1, b2, F.syz[c - 2], F.syz[c - 1], syz_0, F.syz[c + 1]);
+        float szz_z = g.dtdz * D4F(b1, b2, F.szz[c - 1], F.szz[c], F.szz[c + 1], F.szz[c + 2]);
+        float sxz_x = g.dtdx * D4B(b1, b2, F.sxz[c - 2 * si], F.sxz[c - si], F.sxz[c], F.sxz[c + si]);
+        float syz_y = g.dtdy * D4B(b1, b2, syz_m2, syz_m1, syz_0, syz_p1);
+        if (CPML) {
+            const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+            v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z,
+                   szz_z);
+        }
+        F.vx[c] += (sxx_x + sxy_y + sxz_z) / F.rip[c];
+        F.vy[c] += (syy_y + sxy_x + syz_z) / F.rjp[c];
+        F.vz[c] += (szz_z + sxz_x + syz_y) / F.rkp[c];
+        if (j < je) {
+            sxy_m2 = sxy_m1; sxy_m1 = sxy_0; sxy_0 = sxy_p1; sxy_p1 = F.sxy[c + 2 * sj];
+            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1; syz_p1 = F.syz[c + 2 * sj];
+            syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = syy_p2; syy_p2 = F.syy[c + 3 * sj];
+        }
+    }
+}
+
+template <int TK, int TI, bool CPML>
+__global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, SFields F, Prof P, Psi9 psi)
+{
+    int tk, ti, tj;
+    if (!decode_tile(T, tk, ti, tj)) return;
+    const int k = R.k0 + tk * TK + (int)threadIdx.x;
+    const int i = R.i0 + ti * TI + (int)threadIdx.y;
+    if (k > R.k1 || i > R.i1) return;
+    const int jb = R.j0 + tj * T.jc;
+    const int je = min(jb + T.jc - 1, R.j1);
+    const int64_t sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    const float DX = g.dx, DY = g.dy, DZ = g.dz;
+    int64_t c = at(g, jb, i, k);
+    int hx = 0;
+    if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
+    const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
+
+    // j queues: vx, vz at j-1..j+2, vy at j-2..j+1
+    float vx_m1 = F.vx[c - sj], vx_0 = F.vx[c], vx_p1 = F.vx[c + sj], vx_p2 = F.vx[c + 2 * sj];
+    float vz_m1 = F.vz[c - sj], vz_0 = F.vz[c], vz_p1 = F.vz[c + sj], vz_p2 = F.vz[c + 2 * sj];
+    float vy_m2 = F.vy[c - 2 * sj], vy_m1 = F.vy[c - sj], vy_0 = F.vy[c], vy_p1 = F.vy[c + sj];
+
+    for (int j = jb; j <= je; ++j, c += sj) {
+        float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], vx_0, F.vx[c + si]) / DX;
+        float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, vx_p2) / DY;
+        float vxz = D4F(b1, b2, F.vx[c - 1], vx_0, F.vx[c + 1], F.vx[c + 2]) / DZ;
+        float vyx = D4F(b1, b2, F.vy[c - si], vy_0, F.vy[c + si], F.vy[c + 2 * si]) / DX;
+        float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, vy_p1) / DY;
+        float vyz = D4F(b1, b2, F.vy[c - 1], vy_0, F.vy[c + 1], F.vy[c + 2]) / DZ;
+        float vzx = D4F(b1, b2, F.vz[c - si], vz_0, F.vz[c + si], F.vz[c + 2 * si]) / DX;
+        float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, vz_p2) / DY;
+        float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], vz_0, F.vz[c + 1]) / DZ;
+        if (CPML) {
+            const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+            s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
+        }
+        s_apply(g, F, c, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
+        if (j < je) {
+            vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2; vx_p2 = F.vx[c + 3 * sj];
+            vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_p2; vz_p2 = F.vz[c + 3 * sj];
+            vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = F.vy[c + 2 * sj];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// slab kernels: one thread per CPML cell, k fastest within each box
+__device__ inline bool slab_cell(const BoxList &L, int64_t t, int &j, int &i, int &k)
+{
+    if (t >= L.start[L.n]) return false;
+    int b = 0;
+    while (t >= L.start[b + 1]) ++b;
+    const Box &B = L.b[b];
+    int64_t r = t - L.start[b];
+    const int nk = B.k1 - B.k0 + 1, ni = B.i1 - B.i0 + 1;
+    k = B.k0 + (int)(r % nk);
+    r /= nk;
+    i = B.i0 + (int)(r % ni);
+    j = B.j0 + (int)(r / ni);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_v_slabs(DevGeom g, BoxList L, VFields F, Prof P, Psi9 psi)
+{
+    int j, i, k;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!slab_cell(L, t, j, i, k)) return;
+    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    float sxx_x = g.dtdx * D4F(b1, b2, F.sxx[c - si], F.sxx[c], F.sxx[c + si], F.sxx[c + 2 * si]);
+    float sxy_x = g.dtdx * D4B(b1, b2, F.sxy[c - 2 * si], F.sxy[c - si], F.sxy[c], F.sxy[c + si]);
+    float sxz_x = g.dtdx * D4B(b1, b2, F.sxz[c - 2 * si], F.sxz[c - si], F.sxz[c], F.sxz[c + si]);
+    float sxy_y = g.dtdy * D4B(b1, b2, F.sxy[c - 2 * sj], F.sxy[c - sj], F.sxy[c], F.sxy[c + sj]);
+    float syy_y = g.dtdy * D4F(b1, b2, F.syy[c - sj], F.syy[c], F.syy[c + sj], F.syy[c + 2 * sj]);
+    float syz_y = g.dtdy * D4B(b1, b2, F.syz[c - 2 * sj], F.syz[c - sj], F.syz[c], F.syz[c + sj]);
+    float sxz_z = g.dtdz * D4B(b1, b2, F.sxz[c - 2], F.sxz[c - 1], F.sxz[c], F.sxz[c + 1]);
+    float syz_z = g.dtdz * D4B(b1, b2, F.syz[c - 2], F.syz[c - 1], F.syz[c], F.syz[c + 1]);
+    float szz_z = g.dtdz * D4F(b1, b2, F.szz[c - 1], F.szz[c], F.szz[c + 1], F.szz[c + 2]);
+    const int hx = slab_h(i, g.nx, g.fw, 1);
+    const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+    const int hz = slab_h(k, g.nz, g.fw, 1);
+    v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z, szz_z);
+    F.vx[c] += (sxx_x + sxy_y + sxz_z) / F.rip[c];
+    F.vy[c] += (syy_y + sxy_x + syz_z) / F.rjp[c];
+    F.vz[c] += (szz_z + sxz_x + syz_y) / F.rkp[c];
+}
+
+__global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F, Prof P, Psi9 psi)
+{
+    int j, i, k;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!slab_cell(L, t, j, i, k)) return;
+    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    const float DX = g.dx, DY = g.dy, DZ = g.dz;
+    float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], F.vx[c], F.vx[c + si]) / DX;
+    float vxy = D4F(b1, b2, F.vx[c - sj], F.vx[c], F.vx[c + sj], F.vx[c + 2 * sj]) / DY;
+    float vxz = D4F(b1, b2, F.vx[c - 1], F.vx[c], F.vx[c + 1], F.vx[c + 2]) / DZ;
+    float vyx = D4F(b1, b2, F.vy[c - si], F.vy[c], F.vy[c + si], F.vy[c + 2 * si]) / DX;
+    float vyy = D4B(b1, b2, F.vy[c - 2 * sj], F.vy[c - sj], F.vy[c], F.vy[c + sj]) / DY;
+    float vyz = D4F(b1, b2, F.vy[c - 1], F.vy[c], F.vy[c + 1], F.vy[c + 2]) / DZ;
+    float vzx = D4F(b1, b2, F.vz[c - si], F.vz[c], F.vz[c + si], F.vz[c + 2 * si]) / DX;
+    float vzy = D4F(b1, b2, F.vz[c - sj], F.vz[c], F.vz[c + sj], F.vz[c + 2 * sj]) / DY;
+    float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], F.vz[c], F.vz[c + 1]) / DZ;
+    const int hx = slab_h(i, g.nx, g.fw, 1);
+    const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+    const int hz = slab_h(k, g.nz, g.fw, 1);
+    s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
+    s_apply(g, F, c, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+static const int TK = 64, TI = 4;
+
+static Tiling make_tiling(const Box &R, int tk, int ti, int jc)
+{
+    Tiling T;
+    T.ntk = (R.k1 - R.k0 + 1 + tk - 1) / tk;
+    T.nti = (R.i1 - R.i0 + 1 + ti - 1) / ti;
+    T.ntj = (R.j1 - R.j0 + 1 + jc - 1) / jc;
+    T.total = T.ntk * T.nti * T.ntj;
+    T.per = (T.total + 7) / 8;
+    T.jc = jc;
+    return T;
+}
+
+static int pick_jc(const Box &R)
+{
+    // enough blocks to give every SIMD several waves: aim for >= 4096 tiles
+    const int nk = (R.k1 - R.k0 + 1 + TK - 1) / TK, ni = (R.i1 - R.i0 + 1 + TI - 1) / TI;
+    const int ny = R.j1 - R.j0 + 1;
+    int jc = 64;
+    while (jc > 8 && (int64_t)nk * ni * ((ny + jc - 1) / jc) < 4096) jc /= 2;
+    return jc;
+}
+
+static BoxList slab_boxes(const ifos_geom *g)
+{
+    BoxList L;
+    const int fw = g->fw, nx = g->nx, ny = g->ny, nz = g->nz;
+    const int xb0 = fw + 1, xb1 = nx - fw, yb0 = g->free_surf ? 1 : fw + 1, yb1 = ny - fw;
+    Box all[6] = {
+        {1, fw, 1, ny, 1, nz},                 // x low   (update_v_ssg_CPML.c:70)
+        {xb1 + 1, nx, 1, ny, 1, nz},           // x high  (:142)
+        {xb0, xb1, 1, fw, 1, nz},              // y low   (:215, no free surface)
+        {xb0, xb1, yb1 + 1, ny, 1, nz},        // y high  (:267)
+        {xb0, xb1, yb0, yb1, 1, fw},           // z low   (:320)
+        {xb0, xb1, yb0, yb1, nz - fw + 1, nz}, // z high  (:352)
+    };
+    L.n = 0;
+    L.start[0] = 0;
+    for (int b = 0; b < 6; ++b) {
+        if (b == 2 && g->free_surf) continue;
+        const Box &B = all[b];
+        if (B.i1 < B.i0 || B.j1 < B.j0 || B.k1 < B.k0) continue;
+        L.b[L.n] = B;
+        L.start[L.n + 1] = L.start[L.n] + (int64_t)(B.i1 - B.i0 + 1) * (B.j1 - B.j0 + 1) * (B.k1 - B.k0 + 1);
+        ++L.n;
+    }
+    for (int b = L.n + 1; b < 7; ++b) L.start[b] = L.start[L.n];
+    return L;
+}
+
+static Box region_box(const ifos_geom *g, int region)
+{
+    if (region == IFOS_REGION_INTERIOR) {
+        Box B = {g->fw + 1, g->nx - g->fw, g->free_surf ? 1 : g->fw + 1, g->ny - g->fw, g->fw + 1,
+                 g->nz - g->fw};
+        return B;
+    }
+    Box B = {1, g->nx, 1, g->ny, 1, g->nz};
+    return B;
+}
+
+}  // namespace ifos
+
+using namespace ifos;
+
+extern "C" int ifos_update_v(const ifos_geom *g, int region, const ifos_fields *f, const ifos_model *m,
+                             const float *d_prof, const ifos_psi *psi_s, void *stream)
+{
+    if (!g || !f || !m) {
+        set_error("ifos_update_v: null argument");
+        return IFOS_EINVAL;
+    }
+    if (region != IFOS_REGION_INTERIOR && (!d_prof || !psi_s)) {
+        set_error("ifos_update_v: CPML region needs profiles and psi arrays");
+        return IFOS_EINVAL;
+    }
+    DevGeom G = make_dev_geom(g);
+    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
+    Prof P = {d_prof, 2 * g->fw + 1};
+    Psi9 psi = {};
+    if (psi_s)
+        for (int n = 0; n < 9; ++n) psi.p[n] = psi_s->p[n];
+    hipStream_t s = (hipStream_t)stream;
+    if (region == IFOS_REGION_CPML) {
+        BoxList L = slab_boxes(g);
+        int64_t n = L.start[L.n];
+        if (n == 0) return IFOS_OK;
+        ProfScope ps_(IFOS_K_V_CPML, s);
+        hipLaunchKernelGGL(k_v_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, L, F, P, psi);
+        return hip_check(hipGetLastError(), "k_v_slabs");
+    }
+    Box R = region_box(g, region);
+    if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
+    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
+    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    ProfScope ps_(region == IFOS_REGION_ALL ? IFOS_K_V_FUSED : IFOS_K_V_INTERIOR, s);
+    if (region == IFOS_REGION_ALL)
+        hipLaunchKernelGGL((k_v_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi);
+    else
+        hipLaunchKernelGGL((k_v_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi);
+    return hip_check(hipGetLastError(), "k_v_march");
+}
+
+extern "C" int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *f, const ifos_model *m,
+                             const float *d_prof, const ifos_psi *psi_v, void *stream)
+{
+    if (!g || !f || !m) {
+        set_error("ifos_update_s: null argument");
+        return IFOS_EINVAL;
+    }
+    if (region != IFOS_REGION_INTERIOR && (!d_prof || !psi_v)) {
+        set_error("ifos_update_s: CPML region needs profiles and psi arrays");
+        return IFOS_EINVAL;
+    }
+    DevGeom G = make_dev_geom(g);
+    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
+                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
+    Prof P = {d_prof, 2 * g->fw + 1};
+    Psi9 psi = {};
+    if (psi_v)
+        for (int n = 0; n < 9; ++n) psi.p[n] = psi_v->p[n];
+    hipStream_t s = (hipStream_t)stream;
+    if (region == IFOS_REGION_CPML) {
+        BoxList L = slab_boxes(g);
+        int64_t n = L.start[L.n];
+        if (n == 0) return IFOS_OK;
+        ProfScope ps_(IFOS_K_S_CPML, s);
+        hipLaunchKernelGGL(k_s_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, L, F, P, psi);
+        return hip_check(hipGetLastError(), "k_s_slabs");
+    }
+    Box R = region_box(g, region);
+    if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
+    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
+    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    ProfScope ps_(region == IFOS_REGION_ALL ? IFOS_K_S_FUSED : IFOS_K_S_INTERIOR, s);
+    if (region == IFOS_REGION_ALL)
+        hipLaunchKernelGGL((k_s_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi);
+    else
+        hipLaunchKernelGGL((k_s_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi);
+    return hip_check(hipGetLastError(), "k_s_march");
+}

@@ -1,0 +1,137 @@
+"""Synthetic workloads of the benchmark configurations (models, shots,
+receivers, time step), in the canonical/device layouts of ifosgpu.
+
+* toy84:   84^3 (64^3 + 10-cell CPML), dh 2 m, dt 2e-4 s, NT 600, homogeneous
+           start model (vp 4000, vs 2300, rho 2000) and a true model with two
+           +-10% spheres of radius 8 cells, 4 explosive Ricker shots (50 Hz),
+           32 receivers, DFT at 40/60/80 Hz.
+* grad276: 276^3 (256^3 + 10-cell CPML), dh 4 m, dt = 0.9 CFL, NT 1000,
+           seeded Gaussian random field (correlation length 20 cells, vp in
+           2500-5500 m/s, vp/vs in 1.7-2.0, Gardner density) as the true
+           model and its 2-cell Gaussian smoothing as the start model,
+           16 explosive Ricker shots (10 Hz) on a 4x4 grid, 1024 receivers
+           (32x32) near the top, 6 DFT frequencies 4-14 Hz.
+
+The CFL limit is the reference's (checkfd_ssg.c:473-510):
+dt <= dh / (gamma * sqrt(3) * vmax), gamma = 1.184614 at 4th order.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GAMMA4 = 1.184614
+
+
+def cfl_dt(dh: float, vmax: float, frac: float = 0.9) -> float:
+    return frac * dh / (GAMMA4 * math.sqrt(3.0) * vmax)
+
+
+@dataclass
+class Workload:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    fw: int
+    dh: float
+    dt: float
+    nt: int
+    finv: list
+    tast: int
+    vppml: float
+    fpml: float
+    shots: list            # (i, j, k, tshift, fc, amp, type)
+    receivers: np.ndarray  # (ntr, 3) i, j, k
+    fdcoeff: int = 2
+    seed: int = 0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def cells(self):
+        return self.nx * self.ny * self.nz
+
+
+def toy84() -> Workload:
+    fw, n = 10, 84
+    shots = [(fw + i, fw + 12, fw + k, 0.0, 50.0, 1.0, 1) for i, k in ((21, 21), (21, 43), (43, 21), (43, 43))]
+    rec = np.array([[fw + 1 + 2 * m, fw + 4, fw + 32] for m in range(32)], np.int32)
+    return Workload("toy84", n, n, n, fw, 2.0, 2e-4, 600, [40.0, 60.0, 80.0], 100, 4000.0, 50.0, shots, rec)
+
+
+def grad276() -> Workload:
+    fw, n, dh = 10, 276, 4.0
+    dt = cfl_dt(dh, 5500.0)
+    pos = [fw + 32 + 64 * m for m in range(4)]
+    shots = [(i, fw + 10, k, 0.0, 10.0, 1.0, 1) for i in pos for k in pos]
+    rec = np.array([[fw + 1 + 8 * a, fw + 1, fw + 1 + 8 * b] for a in range(32) for b in range(32)], np.int32)
+    finv = [4.0, 6.0, 8.0, 10.0, 12.0, 14.0]
+    return Workload("grad276", n, n, n, fw, dh, float(np.float32(dt)), 1000, finv, 100, 5500.0, 10.0, shots, rec,
+                    seed=5)
+
+
+# ---------------------------------------------------------------------------
+# models
+def _grf(shape, corr_cells, seed, device):
+    """zero-mean Gaussian random field with covariance exp(-r^2/l^2), via FFT"""
+    import torch
+    gen = torch.Generator(device=device).manual_seed(seed)
+    w = torch.randn(shape, generator=gen, device=device, dtype=torch.float32)
+    W = torch.fft.rfftn(w)
+    ks = [torch.fft.fftfreq(s, device=device) * 2 * math.pi for s in shape[:-1]]
+    ks.append(torch.fft.rfftfreq(shape[-1], device=device) * 2 * math.pi)
+    k2 = ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
+    W *= torch.exp(-k2 * corr_cells ** 2 / 8.0)
+    return torch.fft.irfftn(W, s=shape)
+
+
+def _minmax(a, lo, hi):
+    amin, amax = a.min(), a.max()
+    return lo + (hi - lo) * (a - amin) / (amax - amin)
+
+
+def _smooth(a, sigma_cells):
+    import torch
+    A = torch.fft.rfftn(a)
+    shape = a.shape
+    dev = a.device
+    ks = [torch.fft.fftfreq(s, device=dev) * 2 * math.pi for s in shape[:-1]]
+    ks.append(torch.fft.rfftfreq(shape[-1], device=dev) * 2 * math.pi)
+    k2 = ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
+    A *= torch.exp(-0.5 * k2 * sigma_cells ** 2)
+    return torch.fft.irfftn(A, s=shape)
+
+
+def random_models(w: Workload, device="cuda", corr=20.0, smooth_cells=2.0):
+    """(true, start) models as canonical-shape torch tensors (rho, pi, u)"""
+    import torch
+    shape = (w.ny + 5, w.nx + 5, w.nz + 5)
+    vp = _minmax(_grf(shape, corr, w.seed, device), 2500.0, 5500.0)
+    ratio = _minmax(_grf(shape, corr, w.seed + 1000, device), 1.7, 2.0)
+
+    def moduli(vp_):
+        vs_ = vp_ / ratio
+        rho_ = 310.0 * vp_ ** 0.25
+        return [t.to(torch.float32).contiguous() for t in (rho_, rho_ * vp_ * vp_, rho_ * vs_ * vs_)]
+
+    true = moduli(vp)
+    start = moduli(_smooth(vp, smooth_cells))
+    return true, start
+
+
+def sphere_models(w: Workload):
+    """toy84: homogeneous start, two +-10% spheres of radius 8 cells (numpy)"""
+    shape = (w.ny + 5, w.nx + 5, w.nz + 5)
+    J, I, K = np.meshgrid(*(np.arange(s) - 2 for s in shape), indexing="ij")
+    fac = np.ones(shape)
+    for (ci, sgn) in ((22, 1.0), (42, -1.0)):
+        c = (w.fw + 32, w.fw + ci, w.fw + 32)
+        fac += sgn * 0.1 * (((J - c[0]) ** 2 + (I - c[1]) ** 2 + (K - c[2]) ** 2) <= 64)
+
+    def moduli(f):
+        vp, vs, rho = 4000.0 * f, 2300.0 * f, 2000.0 * f
+        return [a.astype(np.float32) for a in (rho, rho * vp * vp, rho * vs * vs)]
+
+    return moduli(fac), moduli(np.ones(shape))

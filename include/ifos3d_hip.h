@@ -1,0 +1,212 @@
+/*
+ * ifos3d_hip.h -- native C ABI of the MI355X (gfx950) IFOS3D hot-path library
+ * (libifos3d_hip.so).
+ *
+ * Device pointers, plain sizes, one HIP stream per call (passed as void*).
+ * The reference-compatible entry points with the IFOS3D prototypes
+ * (update_v, update_s_elastic, ..., gradient_F) live in ifos3d_dropin.h and
+ * are built on top of these.
+ *
+ * Device layout of a 3-D field ("device layout"):
+ *   j (vertical, slowest) in [-2, ny+2], i in [-2, nx+2], k (fastest) in
+ *   [-2, nz+2]; element (j,i,k) at ((j+2)*(nx+5) + (i+2))*pitch + k + IFOS_KOFF.
+ *   pitch is a multiple of 32 floats (128 B) and IFOS_KOFF puts k = 1 on a
+ *   128-B boundary, so every interior row starts a cache line.
+ *   This is the reference's NR layout (util.c:250-288: one block, k fastest,
+ *   then i, then j) with a uniform 2-cell halo and a padded k pitch.
+ * CPML memory arrays keep the reference's compact slab shapes
+ *   (ifos3d.c:421-440): psi_x [ny][2fw][nz], psi_y [2fw][nx][nz],
+ *   psi_z [ny][nx][2fw], dense, 0-based.
+ * CPML profiles: one device array of 18*(2fw+1) floats, per axis
+ *   (x, y, z) the six 1-based vectors K a b K_half a_half b_half
+ *   (CPML_coeff.c:87-324), element 0 of each unused.
+ *
+ * Errors: every function returns 0 on success and a negative IFOS_E* code
+ * otherwise; ifos_last_error() gives the message.  Nothing aborts.
+ */
+#ifndef IFOS3D_HIP_H
+#define IFOS3D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFOS_KOFF 31
+
+enum {
+    IFOS_OK = 0,
+    IFOS_EINVAL = -1,   /* bad argument / unsupported configuration */
+    IFOS_EHIP = -2,     /* a HIP runtime call failed */
+    IFOS_ENAN = -3,     /* simulation unstable: NaN at the probe cell */
+    IFOS_ENOMEM = -4,
+};
+
+/* Regions of the velocity/stress updates.  INTERIOR is the box of
+ * CPML_ini_elastic.c:26-71, CPML its complement in [1..N]^3 (the six
+ * slabs of update_v_ssg_CPML.c / update_s_ssg_CPML_elastic.c), ALL both in
+ * one fused pass. */
+enum { IFOS_REGION_INTERIOR = 1, IFOS_REGION_CPML = 2, IFOS_REGION_ALL = 3 };
+
+/* DFT accumulation precision */
+enum {
+    IFOS_DFT_COMPAT = 0, /* F = float(F + double(v)*trig): disc_fourier.c:48-54 */
+    IFOS_DFT_KAHAN = 1,  /* compensated FP32 sum of the double product */
+};
+
+typedef struct ifos_geom {
+    int nx, ny, nz;      /* local grid including CPML cells (NX, NY, NZ) */
+    int fw;              /* CPML width (FW) */
+    int free_surf;       /* FREE_SURF */
+    int fdcoeff;         /* 1 Taylor, 2 Holberg (FDCOEFF); FDORDER is 4 */
+    float dx, dy, dz, dt;
+    int nt;              /* NT */
+    float time;          /* TIME = (NT-1)*DT, ifos3d.c:177 */
+    int pitch;           /* floats per (j,i) row, set by ifos_geom_init */
+} ifos_geom;
+
+/* Fills g (pitch, time) and validates the configuration. */
+int ifos_geom_init(ifos_geom *g, int nx, int ny, int nz, int fw, float dx,
+                   float dy, float dz, float dt, int nt, int fdcoeff,
+                   int free_surf);
+/* elements (floats) of one device-layout field */
+size_t ifos_field_elems(const ifos_geom *g);
+/* element offset of (j,i,k) */
+size_t ifos_offset(const ifos_geom *g, int j, int i, int k);
+/* elements of psi array n (0-2 x slab, 3-5 y slab, 6-8 z slab) */
+size_t ifos_psi_elems(const ifos_geom *g, int n);
+
+typedef struct ifos_fields {       /* device, device layout */
+    float *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *syz, *sxz;
+} ifos_fields;
+
+typedef struct ifos_model {        /* device, device layout */
+    const float *rho, *pi, *u;     /* cell values, halos [0..N+1] valid */
+    const float *uipjp, *ujpkp, *uipkp, *rjp, *rkp, *rip; /* av_mat output */
+} ifos_model;
+
+typedef struct ifos_psi {          /* device, compact slab layout */
+    float *p[9];
+} ifos_psi;
+
+/* kernel categories of the optional launch timer */
+enum {
+    IFOS_K_V_INTERIOR = 0, IFOS_K_V_CPML, IFOS_K_V_FUSED,
+    IFOS_K_S_INTERIOR, IFOS_K_S_CPML, IFOS_K_S_FUSED,
+    IFOS_K_DFT, IFOS_K_GRADIENT, IFOS_K_SOURCES, IFOS_K_OTHER,
+    IFOS_K_NCAT
+};
+
+/* Launch timer: when enabled, every kernel launch is bracketed by two HIP
+ * events on its stream; ifos_prof_read synchronizes and returns, per
+ * category, the summed kernel milliseconds and the launch count. */
+int ifos_prof_enable(int on);
+int ifos_prof_reset(void);
+int ifos_prof_read(double *ms, long *count, int ncat);
+
+const char *ifos_last_error(void);
+int ifos_device_sync(void);
+
+/* --- host helpers (same arithmetic as the reference, run on the CPU) --- */
+/* CPML_coeff.c:87-324 -> prof[18*(2fw+1)] host array */
+int ifos_cpml_profiles(const ifos_geom *g, float vppml, float fpml,
+                       float *prof);
+/* wavelet.c:47-90 (shapes 1, 2, 4, 5) -> sig[nt+2] indexed by nt */
+int ifos_wavelet(const ifos_geom *g, int shape, float tshift, float fc,
+                 float amp, float *sig);
+/* ifos3d.c:745-750 */
+int ifos_ntast(float fmax, int tast, float dt);
+
+/* --- device kernels (one launch each, asynchronous on `stream`) --- */
+
+/* update_v_ssg.c:27-130 (INTERIOR), update_v_ssg_CPML.c:30-380 (CPML). */
+int ifos_update_v(const ifos_geom *g, int region, const ifos_fields *f,
+                  const ifos_model *m, const float *d_prof,
+                  const ifos_psi *psi_s, void *stream);
+/* update_s_ssg_elastic.c:28-147 / update_s_ssg_CPML_elastic.c:30-480 */
+int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *f,
+                  const ifos_model *m, const float *d_prof,
+                  const ifos_psi *psi_v, void *stream);
+/* body forces (update_v_ssg.c:474-510), d_src [nsrc][4] = i, j, k, type;
+ * d_sig [nsrc][nt+2] by nt; alpha/beta in degrees (ALPHA, BETA) */
+int ifos_add_forces(const ifos_geom *g, const ifos_fields *f,
+                    const ifos_model *m, const int *d_src, const float *d_sig,
+                    int nsrc, int nt, float alpha, float beta, void *stream);
+/* raw adjoint traces (update_v_ssg.c:512-522), d_rec [ntr][3] = i, j, k;
+ * d_adj [3][ntr][nt+2] by nt (x, y, z) */
+int ifos_add_adjoint(const ifos_geom *g, const ifos_fields *f,
+                     const int *d_rec, const float *d_adj, int ntr, int nt,
+                     void *stream);
+/* psource.c:26-54 (type-1 sources only) */
+int ifos_psource(const ifos_geom *g, const ifos_fields *f, const int *d_src,
+                 const float *d_sig, int nsrc, int nt, void *stream);
+/* surface_ssg_elastic.c:26-112 (free surface at j = 1) */
+int ifos_surface(const ifos_geom *g, const ifos_fields *f,
+                 const ifos_model *m, void *stream);
+/* seismo_ssg.c:27-53: d_sec [3][ntr][ns+1] by sample */
+int ifos_seismo(const ifos_geom *g, const ifos_fields *f, const int *d_rec,
+                int ntr, int ins, int ns, float *d_sec, void *stream);
+/* disc_fourier.c:28-62 on [1..N]^3.  d_F[6] = Re x, y, z, Im x, y, z; each
+ * nf consecutive device-layout fields.  d_C[6] = Kahan compensations (same
+ * shape) for IFOS_DFT_KAHAN, else NULL.  t = nt*DT (back 0) or
+ * TIME - nt*DT (back 1); trig factors computed on the host in double. */
+int ifos_discfourier(const ifos_geom *g, const ifos_fields *f, float *const d_F[6],
+                     float *const d_C[6], const float *finv, int nf, int nt,
+                     int ntast, int back, int precision, void *stream);
+/* gradient_F.c:30-170: accumulates d_grad[3] (vp, vs, rho) on [1..N]^3 */
+int ifos_gradient_F(const ifos_geom *g, float *const d_Ff[6],
+                    float *const d_Fb[6], const ifos_model *m,
+                    const float *finv, int nf, float *const d_grad[3],
+                    void *stream);
+/* matcopy.c:31-307 single rank: periodic halos of a model array */
+int ifos_matcopy_wrap(const ifos_geom *g, float *d_m, void *stream);
+/* av_mat.c:28-80: writes the six averaged arrays of *out */
+int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *d_u,
+                float *d_uipjp, float *d_ujpkp, float *d_uipkp, float *d_rjp,
+                float *d_rkp, float *d_rip, void *stream);
+/* residual.c:27-56: d_obs, d_syn [3][ntr][ns+1]; d_adj [3][ntr][ns+2];
+ * *l2 (host) += sum of squares, reduced in FP64 */
+int ifos_residual(const ifos_geom *g, const float *d_obs, const float *d_syn,
+                  float *d_adj, int ntr, int ns, double *l2, void *stream);
+/* fill n floats with a value (graph-capture safe zeroing) */
+int ifos_fill(float *d, size_t n, float value, void *stream);
+
+/* --- one gradient shot, the plain path ---------------------------------
+ * Forward NT steps, residual, adjoint NT steps, gradient_F, in the call
+ * order of ifos3d.c:712-975, one kernel per reference routine per step.
+ * A context owns every device buffer of one shot (fields, CPML memory,
+ * averaged model, DFT accumulators, traces). */
+typedef struct ifos_ctx ifos_ctx;
+
+typedef struct ifos_shot {
+    int src_i, src_j, src_k, src_type;   /* grid indices, srcpos_loc[7] */
+    float tshift, fc, amp;               /* srcpos_loc[4..6] */
+    int source_shape;                    /* SOURCE_SHAPE */
+    int ntr;
+    const int *rec_ijk;                  /* host [ntr][3] */
+    const float *d_obs;                  /* device [3][ntr][nt+1] or NULL */
+    int want_adjoint;                    /* 0: forward only */
+} ifos_shot;
+
+int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max,
+                    int ntr_max, int dft_precision);
+int ifos_ctx_destroy(ifos_ctx *c);
+/* model in device layout (rho, pi, u); copied, halos wrapped, averaged */
+int ifos_ctx_set_model(ifos_ctx *c, const float *d_rho, const float *d_pi,
+                       const float *d_u, void *stream);
+int ifos_ctx_set_cpml(ifos_ctx *c, float vppml, float fpml);
+/* DFT frequencies and TAST (ntast = ifos_ntast(finv[nf-1], tast, dt)) */
+int ifos_ctx_set_freqs(ifos_ctx *c, const float *finv, int nf, int tast);
+/* runs one shot; d_syn [3][ntr][nt+1] (device, written); d_grad[3]
+ * (device layout, accumulated); *l2 (host, accumulated) */
+int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn,
+                      float *const d_grad[3], double *l2, void *stream);
+/* device pointers owned by the context (fields, DFT accumulators) */
+int ifos_ctx_fields(ifos_ctx *c, ifos_fields *f);
+int ifos_ctx_dft(ifos_ctx *c, int back, float *d_F[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

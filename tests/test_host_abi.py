@@ -1,0 +1,77 @@
+"""CPU tests of the HIP library's host side and ABI (no GPU calls).
+
+* the library loads and exports every function declared in include/*.h;
+* the host-computed CPML profiles and source wavelets are bit-identical to
+  the oracle (same float/double promotions as CPML_coeff.c and wavelet.c).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import small_grid
+from pyoracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([a-zA-Z_][a-zA-Z0-9_]*)\s*\([^;{]*\)\s*;", txt)
+    return sorted(set(n for n in names if n not in ("if", "for", "while", "return", "sizeof")))
+
+
+def test_native_library_exports_header():
+    import ifosgpu as ig
+    lib = ig.lib()
+    names = _declared("ifos3d_hip.h")
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("fw,dh,vppml,fpml,dt", [(10, 2.0, 4000.0, 50.0, 2e-4), (8, 5.0, 5500.0, 15.0, 4e-4),
+                                                 (12, 0.8, 6200.0, 200.0, 5e-5), (3, 50.0, 300.0, 1.0, 1e-3)])
+def test_cpml_profiles_match_oracle(fw, dh, vppml, fpml, dt):
+    import ifosgpu as ig
+    g = small_grid(nx=2 * fw + 5, ny=2 * fw + 6, nz=2 * fw + 7, fw=fw, dh=dh, dt=dt)
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt)
+    got = ig.cpml_profiles(G, vppml, fpml).reshape(18, 2 * fw + 1)
+    ref, _ = Oracle("f32").cpml_coeff(g, vppml, fpml)
+    for n in range(18):
+        assert np.array_equal(got[n], ref[n]), n
+
+
+@pytest.mark.parametrize("shape", [1, 2, 4, 5])
+def test_wavelet_matches_oracle(shape):
+    import ifosgpu as ig
+    g = small_grid(nt=700)
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt)
+    for tshift, fc, amp in [(0.0, 50.0, 1.0), (0.003, 17.5, 2.5e3)]:
+        got = ig.wavelet(G, shape, tshift, fc, amp)
+        ref = Oracle("f32").wavelet(g, [(1, 1, 1, tshift, fc, amp, 1)], shape)[0]
+        assert np.array_equal(got[1:g.nt + 1], ref[1:g.nt + 1])
+
+
+def test_ntast_and_geometry():
+    import ifosgpu as ig
+    o = Oracle("f32")
+    for fmax, tast, dt in [(80.0, 100, 2e-4), (14.0, 100, 3.19e-4), (40.0, 10, 4e-4)]:
+        assert ig.ntast(fmax, tast, dt) == o.ntast(fmax, tast, dt)
+    G = ig.Geom(84, 84, 84, 10, 2.0, 2.0, 2.0, 2e-4, 600)
+    assert G.pitch % 32 == 0 and G.pitch >= 84 + 34
+    # k = 1 starts a 128-byte line, k in [-2, nz+2] fits the pitch
+    off = ig.lib().ifos_offset(G.ref, 1, 1, 1)
+    assert off % 32 == 0
+    assert ig.lib().ifos_offset(G.ref, 0, 0, 85) - ig.lib().ifos_offset(G.ref, 0, 0, -2) < G.pitch
+
+
+def test_invalid_configuration_is_rejected():
+    import ifosgpu as ig
+    with pytest.raises(ig.IfosError):
+        ig.Geom(10, 10, 10, 10, 2.0, 2.0, 2.0, 2e-4, 100)  # grid smaller than 2*FW+1
+    with pytest.raises(ig.IfosError):
+        ig.Geom(40, 40, 40, 5, 2.0, 2.0, 2.0, 2e-4, 100, fdcoeff=3)
