@@ -123,6 +123,27 @@ def run_gpu(w, steps, warmup, precision):
     return dict(elapsed=elapsed, prof=prof, l2=l2, gnorm=gnorm, host_start=host_start, pitch=G.pitch)
 
 
+def copy_probe(mib=1024, reps=5):
+    """plain device copy of a known size: the practical HBM ceiling, and the
+    reference kernel the rocprof FETCH_SIZE/WRITE_SIZE counters are scaled by"""
+    import torch
+    import ifosgpu as ig
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    ig.copy(b, a, n)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ig.copy(b, a, n)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+
+
 def cpu_baseline(w, host_model, fwd_steps=4, adj_steps=4):
     """The reference's own routines (oracle/_ref, compiled from its sources)
     on the host, one core: a bounded sample of forward and adjoint time steps
@@ -205,6 +226,8 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     ig.lib()  # fail loudly if the HIP library is not built
+    copy = copy_probe()
+    log(f"copy probe: {copy}")
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
     prec = ig.DFT_KAHAN if args.precision == "kahan" else ig.DFT_COMPAT
     res = run_gpu(w, args.steps, args.warmup, prec)
@@ -259,6 +282,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 5)},
         "kernels": kernels,
+        "copy_probe": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

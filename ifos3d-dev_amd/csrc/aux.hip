@@ -41,18 +41,46 @@ __global__ void k_add_forces(DevGeom g, float *vx, float *vy, float *vz, const f
     }
 }
 
-__global__ void k_add_adjoint(DevGeom g, float *vx, float *vy, float *vz, const int *rec, const float *adj,
-                              int ntr, int nt, int ldadj)
+// One thread per receiver.  Receivers that share a cell are applied by the
+// lowest-numbered one, in receiver order, so each cell sees the adds in the
+// reference's sequence (update_v_ssg.c:512-522).  chain[2r] says whether r is
+// the first receiver of its cell, chain[2r+1] is the next receiver of the
+// same cell (-1 at the end); without a chain the kernel finds both itself.
+__global__ __launch_bounds__(256) void k_add_adjoint(DevGeom g, float *vx, float *vy, float *vz, const int *rec,
+                                                    const int *chain, const float *adj, int ntr, int nt, int ldadj)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const int64_t comp = (int64_t)ntr * ldadj;
-    for (int l = 0; l < ntr; ++l) {
-        const int64_t c = at(g, rec[3 * l + 1], rec[3 * l], rec[3 * l + 2]);
-        const int64_t s = (int64_t)l * ldadj + nt;
-        vx[c] += adj[s];
-        vz[c] += adj[2 * comp + s];
-        vy[c] += adj[comp + s];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ntr) return;
+    const int ri = rec[3 * r], rj = rec[3 * r + 1], rk = rec[3 * r + 2];
+    if (chain) {
+        if (!chain[2 * r]) return;
+    } else {
+        for (int q = 0; q < r; ++q)
+            if (rec[3 * q] == ri && rec[3 * q + 1] == rj && rec[3 * q + 2] == rk) return;
     }
+    const int64_t comp = (int64_t)ntr * ldadj;
+    const int64_t c = at(g, rj, ri, rk);
+    float x = vx[c], y = vy[c], z = vz[c];
+    for (int q = r; q >= 0 && q < ntr;) {
+        const int64_t s = (int64_t)q * ldadj + nt;
+        x += adj[s];
+        z += adj[2 * comp + s];
+        y += adj[comp + s];
+        if (chain) {
+            q = chain[2 * q + 1];
+        } else {
+            int nq = -1;
+            for (int p = q + 1; p < ntr; ++p)
+                if (rec[3 * p] == ri && rec[3 * p + 1] == rj && rec[3 * p + 2] == rk) {
+                    nq = p;
+                    break;
+                }
+            q = nq;
+        }
+    }
+    vx[c] = x;
+    vy[c] = y;
+    vz[c] = z;
 }
 
 __global__ void k_psource(DevGeom g, float *sxx, float *syy, float *szz, const int *src, const float *sig,
@@ -170,6 +198,98 @@ __global__ __launch_bounds__(256) void k_dft_kahan(DevGeom g, const float *vx, c
         kahan_add(F.p[3], Cc.p[3], o, x * t2);
         kahan_add(F.p[4], Cc.p[4], o, y * t2);
         kahan_add(F.p[5], Cc.p[5], o, z * t2);
+    }
+}
+
+// Unrolled variants for nf <= 8: every accumulator load of a cell is in
+// flight at once.  Buffer access: one 32-bit offset per cell, frequency l at
+// scalar offset l * (field bytes).
+template <int NF>
+__global__ __launch_bounds__(256) void k_dft_nf(DevGeom g, const float *vx, const float *vy, const float *vz,
+                                               FPtrs F, uint32_t tot, Trig T)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const uint32_t vo = (uint32_t)at(g, j, i, k) * 4u;
+    const double x = bload(make_rsrc(vx, g.fbytes), vo, 0);
+    const double y = bload(make_rsrc(vy, g.fbytes), vo, 0);
+    const double z = bload(make_rsrc(vz, g.fbytes), vo, 0);
+    rsrc_t r[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) r[n] = make_rsrc(F.p[n], tot);
+    float a[6][NF];
+#pragma unroll
+    for (int l = 0; l < NF; ++l)
+#pragma unroll
+        for (int n = 0; n < 6; ++n) a[n][l] = bload(r[n], vo, (uint32_t)l * g.fbytes);
+#pragma unroll
+    for (int l = 0; l < NF; ++l) {
+        const double t1 = T.c[l], t2 = T.s[l];
+        bstore((float)(a[0][l] + x * t1), r[0], vo, (uint32_t)l * g.fbytes);
+        bstore((float)(a[1][l] + y * t1), r[1], vo, (uint32_t)l * g.fbytes);
+        bstore((float)(a[2][l] + z * t1), r[2], vo, (uint32_t)l * g.fbytes);
+        bstore((float)(a[3][l] + x * t2), r[3], vo, (uint32_t)l * g.fbytes);
+        bstore((float)(a[4][l] + y * t2), r[4], vo, (uint32_t)l * g.fbytes);
+        bstore((float)(a[5][l] + z * t2), r[5], vo, (uint32_t)l * g.fbytes);
+    }
+}
+
+template <int NF>
+__global__ __launch_bounds__(256) void k_dft_kahan_nf(DevGeom g, const float *vx, const float *vy,
+                                                     const float *vz, FPtrs F, FPtrs Cc, uint32_t tot, Trig T)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const uint32_t vo = (uint32_t)at(g, j, i, k) * 4u;
+    const double v[3] = {bload(make_rsrc(vx, g.fbytes), vo, 0), bload(make_rsrc(vy, g.fbytes), vo, 0),
+                         bload(make_rsrc(vz, g.fbytes), vo, 0)};
+    rsrc_t r[6], rc[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+        r[n] = make_rsrc(F.p[n], tot);
+        rc[n] = make_rsrc(Cc.p[n], tot);
+    }
+    float a[6][NF], cc[6][NF];
+#pragma unroll
+    for (int l = 0; l < NF; ++l)
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+            a[n][l] = bload(r[n], vo, (uint32_t)l * g.fbytes);
+            cc[n][l] = bload(rc[n], vo, (uint32_t)l * g.fbytes);
+        }
+#pragma unroll
+    for (int l = 0; l < NF; ++l)
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+            const double term = v[n % 3] * (n < 3 ? T.c[l] : T.s[l]);
+            const float yv = (float)(term - (double)cc[n][l]);
+            const float t = a[n][l] + yv;
+            bstore((t - a[n][l]) - yv, rc[n], vo, (uint32_t)l * g.fbytes);
+            bstore(t, r[n], vo, (uint32_t)l * g.fbytes);
+        }
+}
+
+// plain streaming copy (the known-size reference kernel for memory counters):
+// four 16-byte loads in flight per thread before the stores
+__global__ __launch_bounds__(256) void k_copy(float4 *__restrict__ dst, const float4 *__restrict__ src, int64_t n4)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
+    for (int64_t b = ((int64_t)blockIdx.x * 4) * blockDim.x + threadIdx.x; b < n4; b += stride) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = b + (int64_t)u * blockDim.x;
+            if (t < n4) v[u] = __builtin_nontemporal_load(src + t);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = b + (int64_t)u * blockDim.x;
+            if (t < n4) __builtin_nontemporal_store(v[u], dst + t);
+        }
     }
 }
 
@@ -335,14 +455,14 @@ extern "C" int ifos_add_forces(const ifos_geom *g, const ifos_fields *f, const i
     return hip_check(hipGetLastError(), "k_add_forces");
 }
 
-extern "C" int ifos_add_adjoint(const ifos_geom *g, const ifos_fields *f, const int *d_rec, const float *d_adj,
-                                int ntr, int nt, void *stream)
+extern "C" int ifos_add_adjoint(const ifos_geom *g, const ifos_fields *f, const int *d_rec, const int *d_chain,
+                                const float *d_adj, int ntr, int nt, void *stream)
 {
     if (ntr <= 0) return IFOS_OK;
     DevGeom G = make_dev_geom(g);
     { ProfScope ps_(IFOS_K_SOURCES, (hipStream_t)stream);
-    hipLaunchKernelGGL(k_add_adjoint, dim3(1), dim3(1), 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, d_rec,
-                       d_adj, ntr, nt, g->nt + 2);
+    hipLaunchKernelGGL(k_add_adjoint, dim3((unsigned)((ntr + 255) / 256)), dim3(256), 0, (hipStream_t)stream, G,
+                       f->vx, f->vy, f->vz, d_rec, d_chain, d_adj, ntr, nt, g->nt + 2);
     }
     return hip_check(hipGetLastError(), "k_add_adjoint");
 }
@@ -408,17 +528,36 @@ extern "C" int ifos_discfourier(const ifos_geom *g, const ifos_fields *f, float 
     for (int n = 0; n < 6; ++n) F.p[n] = d_F[n];
     dim3 block, grid = cell_grid(g, block);
     const int64_t fs = (int64_t)ifos_field_elems(g);
-    if (precision == IFOS_DFT_KAHAN) {
-        FPtrs Cc;
+    const double tot = (double)fs * 4.0 * nf;
+    FPtrs Cc = {};
+    if (precision == IFOS_DFT_KAHAN)
         for (int n = 0; n < 6; ++n) Cc.p[n] = d_C[n];
-        { ProfScope ps_(IFOS_K_DFT, (hipStream_t)stream);
-        hipLaunchKernelGGL(k_dft_kahan, grid, block, 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, F, Cc, nf,
-                           fs, T);
+    hipStream_t st = (hipStream_t)stream;
+    ProfScope ps_(IFOS_K_DFT, st);
+    if (nf <= 8 && tot < 4294967295.0) {
+        const uint32_t t32 = (uint32_t)tot;
+#define IFOS_DFT_CASE(N)                                                                                         \
+    case N:                                                                                                      \
+        if (precision == IFOS_DFT_KAHAN)                                                                         \
+            hipLaunchKernelGGL(k_dft_kahan_nf<N>, grid, block, 0, st, G, f->vx, f->vy, f->vz, F, Cc, t32, T);    \
+        else                                                                                                     \
+            hipLaunchKernelGGL(k_dft_nf<N>, grid, block, 0, st, G, f->vx, f->vy, f->vz, F, t32, T);              \
+        break;
+        switch (nf) {
+            IFOS_DFT_CASE(1)
+            IFOS_DFT_CASE(2)
+            IFOS_DFT_CASE(3)
+            IFOS_DFT_CASE(4)
+            IFOS_DFT_CASE(5)
+            IFOS_DFT_CASE(6)
+            IFOS_DFT_CASE(7)
+            IFOS_DFT_CASE(8)
         }
+#undef IFOS_DFT_CASE
+    } else if (precision == IFOS_DFT_KAHAN) {
+        hipLaunchKernelGGL(k_dft_kahan, grid, block, 0, st, G, f->vx, f->vy, f->vz, F, Cc, nf, fs, T);
     } else {
-        { ProfScope ps_(IFOS_K_DFT, (hipStream_t)stream);
-        hipLaunchKernelGGL(k_dft, grid, block, 0, (hipStream_t)stream, G, f->vx, f->vy, f->vz, F, nf, fs, T);
-        }
+        hipLaunchKernelGGL(k_dft, grid, block, 0, st, G, f->vx, f->vy, f->vz, F, nf, fs, T);
     }
     return hip_check(hipGetLastError(), "k_dft");
 }
@@ -513,4 +652,41 @@ extern "C" int ifos_fill(float *d, size_t n, float value, void *stream)
     hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d, (int64_t)n, value);
     }
     return hip_check(hipGetLastError(), "k_fill");
+}
+
+
+extern "C" int ifos_rec_chain(const int *rec_ijk, int ntr, int *chain)
+{
+    if (ntr < 0 || (ntr > 0 && (!rec_ijk || !chain))) return IFOS_EINVAL;
+    for (int r = 0; r < ntr; ++r) {
+        chain[2 * r] = 1;
+        chain[2 * r + 1] = -1;
+    }
+    for (int r = 0; r < ntr; ++r) {
+        if (!chain[2 * r]) continue;
+        int last = r;
+        for (int q = r + 1; q < ntr; ++q)
+            if (rec_ijk[3 * q] == rec_ijk[3 * r] && rec_ijk[3 * q + 1] == rec_ijk[3 * r + 1] &&
+                rec_ijk[3 * q + 2] == rec_ijk[3 * r + 2]) {
+                chain[2 * q] = 0;
+                chain[2 * last + 1] = q;
+                last = q;
+            }
+    }
+    return IFOS_OK;
+}
+
+extern "C" int ifos_copy(float *d_dst, const float *d_src, size_t n, void *stream)
+{
+    if (n == 0) return IFOS_OK;
+    if (n % 4 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16) {
+        set_error("ifos_copy: n must be a multiple of 4 and pointers 16-byte aligned");
+        return IFOS_EINVAL;
+    }
+    const int64_t n4 = (int64_t)(n / 4);
+    const unsigned blocks = (unsigned)std::min<int64_t>((n4 + 1023) / 1024, 256 * 32);
+    ProfScope ps_(IFOS_K_OTHER, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (float4 *)d_dst,
+                       (const float4 *)d_src, n4);
+    return hip_check(hipGetLastError(), "k_copy");
 }

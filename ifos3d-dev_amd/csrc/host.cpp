@@ -49,6 +49,7 @@ DevGeom make_dev_geom(const ifos_geom *g)
     d.dy = g->dy;
     d.dz = g->dz;
     d.dt = g->dt;
+    d.fbytes = (uint32_t)(ifos_field_elems(g) * 4);
     d.dtdx = g->dt / g->dx;
     d.dtdy = g->dt / g->dy;
     d.dtdz = g->dt / g->dz;
@@ -99,6 +100,10 @@ extern "C" int ifos_geom_init(ifos_geom *g, int nx, int ny, int nz, int fw, floa
     g->fdcoeff = fdcoeff;
     g->free_surf = free_surf ? 1 : 0;
     g->pitch = ((nz + IFOS_KOFF + 3) + 31) / 32 * 32;  // k in [-2, nz+2] -> [KOFF-2, KOFF+nz+2]
+    if ((double)(ny + 5) * (nx + 5) * g->pitch * 4.0 >= 4294967295.0) {
+        set_error("ifos_geom_init: one field of %dx%dx%d exceeds 4 GiB (32-bit buffer offsets)", nx, ny, nz);
+        return IFOS_EINVAL;
+    }
     return IFOS_OK;
 }
 
@@ -255,6 +260,7 @@ struct ifos_ctx {
     float *sig;            // [nt+2]
     int *src;              // [4]
     int *rec;              // [ntr][3]
+    int *chain;            // [ntr][2] same-cell chain of the receivers
     float *adj;            // [3][ntr][nt+2]
     float finv[IFOS_MAX_NF];
     int nf, tast;
@@ -308,6 +314,7 @@ extern "C" int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max, i
     if (!rc) rc = dalloc(c, (void **)&c->sig, (size_t)(g->nt + 2) * 4);
     if (!rc) rc = dalloc(c, (void **)&c->src, 4 * sizeof(int));
     if (!rc) rc = dalloc(c, (void **)&c->rec, 3 * sizeof(int) * (size_t)(ntr_max > 0 ? ntr_max : 1));
+    if (!rc) rc = dalloc(c, (void **)&c->chain, 2 * sizeof(int) * (size_t)(ntr_max > 0 ? ntr_max : 1));
     if (!rc) rc = dalloc(c, (void **)&c->adj, 3 * (size_t)(ntr_max > 0 ? ntr_max : 1) * (g->nt + 2) * 4);
     if (rc) {
         ifos_ctx_destroy(c);
@@ -415,10 +422,10 @@ static int propagate(ifos_ctx *c, int back, const ifos_shot *s, int ntast, float
             if (!back)
                 rc = ifos_add_forces(g, &f, &m, c->src, c->sig, 1, nt, 0.0f, 0.0f, st);
             else
-                rc = ifos_add_adjoint(g, &f, c->rec, c->adj, s->ntr, nt, st);
+                rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
         }
         if (!rc) rc = ifos_update_v(g, IFOS_REGION_CPML, &f, &m, c->prof, &ps, st);
-        if (!rc && back) rc = ifos_add_adjoint(g, &f, c->rec, c->adj, s->ntr, nt, st);
+        if (!rc && back) rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
         if (!rc) rc = ifos_update_s(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &pv, st);
         if (!rc) rc = ifos_update_s(g, IFOS_REGION_CPML, &f, &m, c->prof, &pv, st);
         if (!rc && !back) rc = ifos_psource(g, &f, c->src, c->sig, 1, nt, st);
@@ -473,9 +480,14 @@ extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, 
     int hsrc[4] = {s->src_i, s->src_j, s->src_k, s->src_type};
     rc = hip_check(hipMemcpyAsync(c->sig, sig.data(), sig.size() * 4, hipMemcpyHostToDevice, st), "upload signal");
     if (!rc) rc = hip_check(hipMemcpyAsync(c->src, hsrc, sizeof hsrc, hipMemcpyHostToDevice, st), "upload source");
+    std::vector<int> chain(2 * (size_t)(s->ntr > 0 ? s->ntr : 1));
+    if (!rc && s->ntr > 0) rc = ifos_rec_chain(s->rec_ijk, s->ntr, chain.data());
     if (!rc && s->ntr > 0)
         rc = hip_check(hipMemcpyAsync(c->rec, s->rec_ijk, 3 * sizeof(int) * s->ntr, hipMemcpyHostToDevice, st),
                        "upload receivers");
+    if (!rc && s->ntr > 0)
+        rc = hip_check(hipMemcpyAsync(c->chain, chain.data(), 2 * sizeof(int) * s->ntr, hipMemcpyHostToDevice, st),
+                       "upload receiver chain");
     if (!rc) rc = hip_check(hipStreamSynchronize(st), "sync uploads");
     if (rc) return rc;
 

@@ -23,6 +23,7 @@ struct DevGeom {
     float dx, dy, dz, dt;
     float dtdx, dtdy, dtdz;  // DT/DX ... in float, update_v_ssg.c:95-97
     float b1, b2;            // 4th-order coefficients, update_v_ssg.c:98-100
+    uint32_t fbytes;         // bytes of one device-layout field
 };
 
 __host__ __device__ inline int64_t at(const DevGeom &g, int j, int i, int k)
@@ -81,6 +82,25 @@ struct BoxList {
     int64_t start[7];  // cumulative cell counts
     int n;
 };
+
+// Raw buffer access: one 32-bit byte offset per thread, the neighbour
+// displacements folded into the scalar offset and the instruction's
+// immediate offset, so a load costs no vector address arithmetic.
+// Word 3 of the descriptor is the gfx9-family value (raw, dword format).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ inline rsrc_t make_rsrc(const void *p, uint32_t nbytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)nbytes, 0x00020000);
+}
+// the _b32 builtins move raw 32-bit words: bit casts, never value conversions
+__device__ inline float bload(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ inline void bstore(float v, rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
 
 DevGeom make_dev_geom(const ifos_geom *g);
 void set_error(const char *fmt, ...);

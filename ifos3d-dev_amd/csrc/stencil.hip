@@ -141,6 +141,13 @@ __device__ inline bool decode_tile(const Tiling &T, int &tk, int &ti, int &tj)
     return true;
 }
 
+// Byte offsets for the march kernels: vo is the offset of (j-2, i-2, k-2)
+// of the thread's column at the current plane j, so every operand
+// (j+dj, i+di, k+dk), |d| <= 2 (dj <= 3), lies at vo + (dk+2)*4 (immediate)
+// + (dj+2)*SJ + (di+2)*SI (scalar): all non-negative, all uniform but vo.
+#define LD(r, dj, di, dk) bload(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
+#define ST(v, r) bstore(v, r, vo + 8, 2 * SJ + 2 * SI)
+
 template <int TK, int TI, bool CPML>
 __global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, VFields F, Prof P, Psi9 psi)
 {
@@ -151,41 +158,64 @@ __global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, 
     if (k > R.k1 || i > R.i1) return;
     const int jb = R.j0 + tj * T.jc;
     const int je = min(jb + T.jc - 1, R.j1);
-    const int64_t sj = g.sj, si = g.si;
+    const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
-    int64_t c = at(g, jb, i, k);
+    const rsrc_t rsxx = make_rsrc(F.sxx, g.fbytes), rsyy = make_rsrc(F.syy, g.fbytes),
+                 rszz = make_rsrc(F.szz, g.fbytes), rsxy = make_rsrc(F.sxy, g.fbytes),
+                 rsyz = make_rsrc(F.syz, g.fbytes), rsxz = make_rsrc(F.sxz, g.fbytes),
+                 rvx = make_rsrc(F.vx, g.fbytes), rvy = make_rsrc(F.vy, g.fbytes), rvz = make_rsrc(F.vz, g.fbytes),
+                 rrip = make_rsrc(F.rip, g.fbytes), rrjp = make_rsrc(F.rjp, g.fbytes),
+                 rrkp = make_rsrc(F.rkp, g.fbytes);
+    uint32_t vo = (uint32_t)((int64_t)jb * g.sj + (int64_t)i * g.si + k + IFOS_KOFF - 2) * 4u;
     int hx = 0;
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
     const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
 
     // j queues: sxy and syz at j-2..j+1, syy at j-1..j+2
-    float sxy_m2 = F.sxy[c - 2 * sj], sxy_m1 = F.sxy[c - sj], sxy_0 = F.sxy[c], sxy_p1 = F.sxy[c + sj];
-    float syz_m2 = F.syz[c - 2 * sj], syz_m1 = F.syz[c - sj], syz_0 = F.syz[c], syz_p1 = F.syz[c + sj];
-    float syy_m1 = F.syy[c - sj], syy_0 = F.syy[c], syy_p1 = F.syy[c + sj], syy_p2 = F.syy[c + 2 * sj];
+    float sxy_m2 = LD(rsxy, -2, 0, 0), sxy_m1 = LD(rsxy, -1, 0, 0), sxy_0 = LD(rsxy, 0, 0, 0),
+          sxy_p1 = LD(rsxy, 1, 0, 0);
+    float syz_m2 = LD(rsyz, -2, 0, 0), syz_m1 = LD(rsyz, -1, 0, 0), syz_0 = LD(rsyz, 0, 0, 0),
+          syz_p1 = LD(rsyz, 1, 0, 0);
+    float syy_m1 = LD(rsyy, -1, 0, 0), syy_0 = LD(rsyy, 0, 0, 0), syy_p1 = LD(rsyy, 1, 0, 0),
+          syy_p2 = LD(rsyy, 2, 0, 0);
 
-    for (int j = jb; j <= je; ++j, c += sj) {
-        float sxx_x = g.dtdx * D4F(b1, b2, F.sxx[c - si], F.sxx[c], F.sxx[c + si], F.sxx[c + 2 * si]);
+    for (int j = jb; j <= je; ++j, vo += SJ) {
+        const float sxx_a = LD(rsxx, 0, -1, 0), sxx_b = LD(rsxx, 0, 0, 0), sxx_c = LD(rsxx, 0, 1, 0),
+                    sxx_d = LD(rsxx, 0, 2, 0);
+        const float sxy_a = LD(rsxy, 0, -2, 0), sxy_b = LD(rsxy, 0, -1, 0), sxy_d = LD(rsxy, 0, 1, 0);
+        const float sxz_xa = LD(rsxz, 0, -2, 0), sxz_xb = LD(rsxz, 0, -1, 0), sxz_0 = LD(rsxz, 0, 0, 0),
+                    sxz_xd = LD(rsxz, 0, 1, 0);
+        const float sxz_za = LD(rsxz, 0, 0, -2), sxz_zb = LD(rsxz, 0, 0, -1), sxz_zd = LD(rsxz, 0, 0, 1);
+        const float syz_za = LD(rsyz, 0, 0, -2), syz_zb = LD(rsyz, 0, 0, -1), syz_zd = LD(rsyz, 0, 0, 1);
+        const float szz_a = LD(rszz, 0, 0, -1), szz_b = LD(rszz, 0, 0, 0), szz_c = LD(rszz, 0, 0, 1),
+                    szz_d = LD(rszz, 0, 0, 2);
+        const float rip = LD(rrip, 0, 0, 0), rjp = LD(rrjp, 0, 0, 0), rkp = LD(rrkp, 0, 0, 0);
+        const float vx0 = LD(rvx, 0, 0, 0), vy0 = LD(rvy, 0, 0, 0), vz0 = LD(rvz, 0, 0, 0);
+        float sxx_x = g.dtdx * D4F(b1, b2, sxx_a, sxx_b, sxx_c, sxx_d);
         float sxy_y = g.dtdy * D4B(b1, b2, sxy_m2, sxy_m1, sxy_0, sxy_p1);
-        float sxz_z = g.dtdz * D4B(b1, b2, F.sxz[c - 2], F.sxz[c - 1], F.sxz[c], F.sxz[c + 1]);
+        float sxz_z = g.dtdz * D4B(b1, b2, sxz_za, sxz_zb, sxz_0, sxz_zd);
         float syy_y = g.dtdy * D4F(b1, b2, syy_m1, syy_0, syy_p1, syy_p2);
-        float sxy_x = g.dtdx * D4B(b1, b2, F.sxy[c - 2 * si], F.sxy[c - si], sxy_0, F.sxy[c + si]);
-        float syz_z = g.dtdz * D4B(b1, b2, F.syz[c - 2], F.syz[c - 1], syz_0, F.syz[c + 1]);
-        float szz_z = g.dtdz * D4F(b1, b2, F.szz[c - 1], F.szz[c], F.szz[c + 1], F.szz[c + 2]);
-        float sxz_x = g.dtdx * D4B(b1, b2, F.sxz[c - 2 * si], F.sxz[c - si], F.sxz[c], F.sxz[c + si]);
+        float sxy_x = g.dtdx * D4B(b1, b2, sxy_a, sxy_b, sxy_0, sxy_d);
+        float syz_z = g.dtdz * D4B(b1, b2, syz_za, syz_zb, syz_0, syz_zd);
+        float szz_z = g.dtdz * D4F(b1, b2, szz_a, szz_b, szz_c, szz_d);
+        float sxz_x = g.dtdx * D4B(b1, b2, sxz_xa, sxz_xb, sxz_0, sxz_xd);
         float syz_y = g.dtdy * D4B(b1, b2, syz_m2, syz_m1, syz_0, syz_p1);
         if (CPML) {
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z,
                    szz_z);
         }
-        F.vx[c] += (sxx_x + sxy_y + sxz_z) / F.rip[c];
-        F.vy[c] += (syy_y + sxy_x + syz_z) / F.rjp[c];
-        F.vz[c] += (szz_z + sxz_x + syz_y) / F.rkp[c];
-        if (j < je) {
-            sxy_m2 = sxy_m1; sxy_m1 = sxy_0; sxy_0 = sxy_p1; sxy_p1 = F.sxy[c + 2 * sj];
-            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1; syz_p1 = F.syz[c + 2 * sj];
-            syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = syy_p2; syy_p2 = F.syy[c + 3 * sj];
+        const float nvx = vx0 + (sxx_x + sxy_y + sxz_z) / rip;
+        const float nvy = vy0 + (syy_y + sxy_x + syz_z) / rjp;
+        const float nvz = vz0 + (szz_z + sxz_x + syz_y) / rkp;
+        if (j < je) {  // next plane's queue operands, issued before this plane's stores
+            sxy_m2 = sxy_m1; sxy_m1 = sxy_0; sxy_0 = sxy_p1; sxy_p1 = LD(rsxy, 2, 0, 0);
+            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1; syz_p1 = LD(rsyz, 2, 0, 0);
+            syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = syy_p2; syy_p2 = LD(rsyy, 3, 0, 0);
         }
+        ST(nvx, rvx);
+        ST(nvy, rvy);
+        ST(nvz, rvz);
     }
 }
 
@@ -199,41 +229,77 @@ __global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, 
     if (k > R.k1 || i > R.i1) return;
     const int jb = R.j0 + tj * T.jc;
     const int je = min(jb + T.jc - 1, R.j1);
-    const int64_t sj = g.sj, si = g.si;
+    const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
-    const float DX = g.dx, DY = g.dy, DZ = g.dz;
-    int64_t c = at(g, jb, i, k);
+    const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
+    const rsrc_t rvx = make_rsrc(F.vx, g.fbytes), rvy = make_rsrc(F.vy, g.fbytes), rvz = make_rsrc(F.vz, g.fbytes),
+                 rsxx = make_rsrc(F.sxx, g.fbytes), rsyy = make_rsrc(F.syy, g.fbytes),
+                 rszz = make_rsrc(F.szz, g.fbytes), rsxy = make_rsrc(F.sxy, g.fbytes),
+                 rsyz = make_rsrc(F.syz, g.fbytes), rsxz = make_rsrc(F.sxz, g.fbytes),
+                 rpi = make_rsrc(F.pi, g.fbytes), ru = make_rsrc(F.u, g.fbytes),
+                 ruipjp = make_rsrc(F.uipjp, g.fbytes), rujpkp = make_rsrc(F.ujpkp, g.fbytes),
+                 ruipkp = make_rsrc(F.uipkp, g.fbytes);
+    uint32_t vo = (uint32_t)((int64_t)jb * g.sj + (int64_t)i * g.si + k + IFOS_KOFF - 2) * 4u;
     int hx = 0;
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
     const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
 
     // j queues: vx, vz at j-1..j+2, vy at j-2..j+1
-    float vx_m1 = F.vx[c - sj], vx_0 = F.vx[c], vx_p1 = F.vx[c + sj], vx_p2 = F.vx[c + 2 * sj];
-    float vz_m1 = F.vz[c - sj], vz_0 = F.vz[c], vz_p1 = F.vz[c + sj], vz_p2 = F.vz[c + 2 * sj];
-    float vy_m2 = F.vy[c - 2 * sj], vy_m1 = F.vy[c - sj], vy_0 = F.vy[c], vy_p1 = F.vy[c + sj];
+    float vx_m1 = LD(rvx, -1, 0, 0), vx_0 = LD(rvx, 0, 0, 0), vx_p1 = LD(rvx, 1, 0, 0), vx_p2 = LD(rvx, 2, 0, 0);
+    float vz_m1 = LD(rvz, -1, 0, 0), vz_0 = LD(rvz, 0, 0, 0), vz_p1 = LD(rvz, 1, 0, 0), vz_p2 = LD(rvz, 2, 0, 0);
+    float vy_m2 = LD(rvy, -2, 0, 0), vy_m1 = LD(rvy, -1, 0, 0), vy_0 = LD(rvy, 0, 0, 0), vy_p1 = LD(rvy, 1, 0, 0);
 
-    for (int j = jb; j <= je; ++j, c += sj) {
-        float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], vx_0, F.vx[c + si]) / DX;
+    for (int j = jb; j <= je; ++j, vo += SJ) {
+        const float vx_xa = LD(rvx, 0, -2, 0), vx_xb = LD(rvx, 0, -1, 0), vx_xd = LD(rvx, 0, 1, 0);
+        const float vx_zb = LD(rvx, 0, 0, -1), vx_zc = LD(rvx, 0, 0, 1), vx_zd = LD(rvx, 0, 0, 2);
+        const float vy_xb = LD(rvy, 0, -1, 0), vy_xc = LD(rvy, 0, 1, 0), vy_xd = LD(rvy, 0, 2, 0);
+        const float vy_zb = LD(rvy, 0, 0, -1), vy_zc = LD(rvy, 0, 0, 1), vy_zd = LD(rvy, 0, 0, 2);
+        const float vz_xb = LD(rvz, 0, -1, 0), vz_xc = LD(rvz, 0, 1, 0), vz_xd = LD(rvz, 0, 2, 0);
+        const float vz_za = LD(rvz, 0, 0, -2), vz_zb = LD(rvz, 0, 0, -1), vz_zd = LD(rvz, 0, 0, 1);
+        const float uipjp = LD(ruipjp, 0, 0, 0), ujpkp = LD(rujpkp, 0, 0, 0), uipkp = LD(ruipkp, 0, 0, 0);
+        const float pi = LD(rpi, 0, 0, 0), u = LD(ru, 0, 0, 0);
+        const float sxx0 = LD(rsxx, 0, 0, 0), syy0 = LD(rsyy, 0, 0, 0), szz0 = LD(rszz, 0, 0, 0);
+        const float sxy0 = LD(rsxy, 0, 0, 0), syz0 = LD(rsyz, 0, 0, 0), sxz0 = LD(rsxz, 0, 0, 0);
+        float vxx = D4B(b1, b2, vx_xa, vx_xb, vx_0, vx_xd) / DX;
         float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, vx_p2) / DY;
-        float vxz = D4F(b1, b2, F.vx[c - 1], vx_0, F.vx[c + 1], F.vx[c + 2]) / DZ;
-        float vyx = D4F(b1, b2, F.vy[c - si], vy_0, F.vy[c + si], F.vy[c + 2 * si]) / DX;
+        float vxz = D4F(b1, b2, vx_zb, vx_0, vx_zc, vx_zd) / DZ;
+        float vyx = D4F(b1, b2, vy_xb, vy_0, vy_xc, vy_xd) / DX;
         float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, vy_p1) / DY;
-        float vyz = D4F(b1, b2, F.vy[c - 1], vy_0, F.vy[c + 1], F.vy[c + 2]) / DZ;
-        float vzx = D4F(b1, b2, F.vz[c - si], vz_0, F.vz[c + si], F.vz[c + 2 * si]) / DX;
+        float vyz = D4F(b1, b2, vy_zb, vy_0, vy_zc, vy_zd) / DZ;
+        float vzx = D4F(b1, b2, vz_xb, vz_0, vz_xc, vz_xd) / DX;
         float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, vz_p2) / DY;
-        float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], vz_0, F.vz[c + 1]) / DZ;
+        float vzz = D4B(b1, b2, vz_za, vz_zb, vz_0, vz_zd) / DZ;
         if (CPML) {
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
         }
-        s_apply(g, F, c, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
+        // update_s_ssg_elastic.c:117-138
+        const float fipjp = uipjp * DT, fjpkp = ujpkp * DT, fipkp = uipkp * DT;
+        const float f = 2.0f * u;
+        const float vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+        const float vxxyyzz = vxx + vyy + vzz;
+        const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
+        const float nsxy = sxy0 + (fipjp * vxyyx);
+        const float nsyz = syz0 + (fjpkp * vyzzy);
+        const float nsxz = sxz0 + (fipkp * vxzzx);
+        const float nsxx = sxx0 + DT * ((pi * vxxyyzz) - (f * vyyzz));
+        const float nsyy = syy0 + DT * ((pi * vxxyyzz) - (f * vxxzz));
+        const float nszz = szz0 + DT * ((pi * vxxyyzz) - (f * vxxyy));
         if (j < je) {
-            vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2; vx_p2 = F.vx[c + 3 * sj];
-            vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_p2; vz_p2 = F.vz[c + 3 * sj];
-            vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = F.vy[c + 2 * sj];
+            vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2; vx_p2 = LD(rvx, 3, 0, 0);
+            vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_p2; vz_p2 = LD(rvz, 3, 0, 0);
+            vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = LD(rvy, 2, 0, 0);
         }
+        ST(nsxy, rsxy);
+        ST(nsyz, rsyz);
+        ST(nsxz, rsxz);
+        ST(nsxx, rsxx);
+        ST(nsyy, rsyy);
+        ST(nszz, rszz);
     }
 }
+#undef LD
+#undef ST
 
 // ---------------------------------------------------------------------------
 // slab kernels: one thread per CPML cell, k fastest within each box

@@ -189,9 +189,21 @@ def add_forces(g, f, m, d_src, d_sig, nsrc, nt, alpha=0.0, beta=0.0, stream=None
                                 stream or stream_ptr()), "ifos_add_forces")
 
 
-def add_adjoint(g, f, d_rec, d_adj, ntr, nt, stream=None):
-    check(lib().ifos_add_adjoint(g.ref, C.byref(fields_struct(f)), _ptr(d_rec), _ptr(d_adj), ntr, nt,
-                                 stream or stream_ptr()), "ifos_add_adjoint")
+def add_adjoint(g, f, d_rec, d_adj, ntr, nt, d_chain=None, stream=None):
+    check(lib().ifos_add_adjoint(g.ref, C.byref(fields_struct(f)), _ptr(d_rec), _ptr(d_chain), _ptr(d_adj), ntr,
+                                 nt, stream or stream_ptr()), "ifos_add_adjoint")
+
+
+def rec_chain(rec_ijk) -> np.ndarray:
+    rec = np.ascontiguousarray(rec_ijk, np.int32)
+    out = np.zeros((len(rec), 2), np.int32)
+    check(lib().ifos_rec_chain(rec.ctypes.data_as(C.c_void_p), len(rec), out.ctypes.data_as(C.c_void_p)),
+          "ifos_rec_chain")
+    return out
+
+
+def copy(d_dst, d_src, n, stream=None):
+    check(lib().ifos_copy(_ptr(d_dst), _ptr(d_src), C.c_size_t(n), stream or stream_ptr()), "ifos_copy")
 
 
 def psource(g, f, d_src, d_sig, nsrc, nt, stream=None):

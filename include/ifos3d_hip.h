@@ -134,10 +134,15 @@ int ifos_add_forces(const ifos_geom *g, const ifos_fields *f,
                     const ifos_model *m, const int *d_src, const float *d_sig,
                     int nsrc, int nt, float alpha, float beta, void *stream);
 /* raw adjoint traces (update_v_ssg.c:512-522), d_rec [ntr][3] = i, j, k;
- * d_adj [3][ntr][nt+2] by nt (x, y, z) */
+ * d_adj [3][ntr][nt+2] by nt (x, y, z).  d_chain [ntr][2] from
+ * ifos_rec_chain (or NULL: found on the device each call) orders the adds
+ * of receivers sharing a cell as the reference does. */
 int ifos_add_adjoint(const ifos_geom *g, const ifos_fields *f,
-                     const int *d_rec, const float *d_adj, int ntr, int nt,
-                     void *stream);
+                     const int *d_rec, const int *d_chain, const float *d_adj,
+                     int ntr, int nt, void *stream);
+/* host: same-cell chain of a receiver list, chain [ntr][2] =
+ * (first of its cell?, next receiver of the cell or -1) */
+int ifos_rec_chain(const int *rec_ijk, int ntr, int *chain);
 /* psource.c:26-54 (type-1 sources only) */
 int ifos_psource(const ifos_geom *g, const ifos_fields *f, const int *d_src,
                  const float *d_sig, int nsrc, int nt, void *stream);
@@ -171,6 +176,8 @@ int ifos_residual(const ifos_geom *g, const float *d_obs, const float *d_syn,
                   float *d_adj, int ntr, int ns, double *l2, void *stream);
 /* fill n floats with a value (graph-capture safe zeroing) */
 int ifos_fill(float *d, size_t n, float value, void *stream);
+/* plain streaming copy of n floats (n % 4 == 0, 16-byte aligned) */
+int ifos_copy(float *d_dst, const float *d_src, size_t n, void *stream);
 
 /* --- one gradient shot, the plain path ---------------------------------
  * Forward NT steps, residual, adjoint NT steps, gradient_F, in the call

@@ -164,6 +164,13 @@ def test_sources_and_receivers(env):
     torch.cuda.synchronize()
     for n in range(3):
         assert max_ulp(g.interior(G.from_dev(df[n])), g.interior(zero2[n])) == 0, "adjoint " + NAMES[n]
+    # same with the precomputed same-cell chain
+    dmodel_c, df_c, _, _ = _upload(torch, G, rho, pi, u, avg, f, psi, prof)
+    dchain = torch.from_numpy(ig.rec_chain(rec)).cuda()
+    ig.add_adjoint(G, df_c, drec, dadj, len(rec), nt, d_chain=dchain)
+    torch.cuda.synchronize()
+    for n in range(3):
+        assert max_ulp(g.interior(G.from_dev(df_c[n])), g.interior(zero2[n])) == 0, "chained adjoint " + NAMES[n]
     # seismo
     ns = g.nt
     dsec = torch.zeros((3, len(rec), ns + 1), dtype=torch.float32, device="cuda")
@@ -183,14 +190,15 @@ def test_sources_and_receivers(env):
     assert np.array_equal(sec[0], sx) and np.array_equal(sec[1], sy) and np.array_equal(sec[2], sz)
 
 
-def test_dft_compat_and_kahan(env):
+@pytest.mark.parametrize("nf", [3, 9])
+def test_dft_compat_and_kahan(env, nf):
     torch, ig = env
     g = small_grid()
-    _note("discfourier compat and Kahan")
+    _note(f"discfourier compat and Kahan nf={nf}")
     o = Oracle("f32")
     rng = np.random.default_rng(8)
     f = random_fields(g, rng, n=3)
-    finv = [40.0, 60.0, 80.0]
+    finv = [40.0, 60.0, 80.0, 20.0, 30.0, 50.0, 70.0, 90.0, 100.0][:nf]
     nf = len(finv)
     Fr = [(rng.standard_normal((nf,) + g.shape)).astype(np.float32) for _ in range(3)]
     Fi = [(rng.standard_normal((nf,) + g.shape)).astype(np.float32) for _ in range(3)]

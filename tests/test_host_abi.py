@@ -75,3 +75,12 @@ def test_invalid_configuration_is_rejected():
         ig.Geom(10, 10, 10, 10, 2.0, 2.0, 2.0, 2e-4, 100)  # grid smaller than 2*FW+1
     with pytest.raises(ig.IfosError):
         ig.Geom(40, 40, 40, 5, 2.0, 2.0, 2.0, 2e-4, 100, fdcoeff=3)
+
+
+def test_receiver_chain():
+    import ifosgpu as ig
+    rec = np.array([[1, 2, 3], [4, 5, 6], [1, 2, 3], [7, 7, 7], [1, 2, 3], [4, 5, 6]], np.int32)
+    ch = ig.rec_chain(rec)
+    assert ch[:, 0].tolist() == [1, 1, 0, 1, 0, 0]
+    assert ch[:, 1].tolist() == [2, 5, 4, -1, -1, -1]
+    assert ig.rec_chain(np.zeros((0, 3), np.int32)).shape == (0, 2)
