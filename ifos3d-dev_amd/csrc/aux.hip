@@ -275,11 +275,12 @@ __global__ __launch_bounds__(256) void k_dft_kahan_nf(DevGeom g, const float *vx
 
 // plain streaming copy (the known-size reference kernel for memory counters):
 // four 16-byte loads in flight per thread before the stores
-__global__ __launch_bounds__(256) void k_copy(float4 *__restrict__ dst, const float4 *__restrict__ src, int64_t n4)
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(v4f *__restrict__ dst, const v4f *__restrict__ src, int64_t n4)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
     for (int64_t b = ((int64_t)blockIdx.x * 4) * blockDim.x + threadIdx.x; b < n4; b += stride) {
-        float4 v[4];
+        v4f v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t t = b + (int64_t)u * blockDim.x;
@@ -686,7 +687,7 @@ extern "C" int ifos_copy(float *d_dst, const float *d_src, size_t n, void *strea
     const int64_t n4 = (int64_t)(n / 4);
     const unsigned blocks = (unsigned)std::min<int64_t>((n4 + 1023) / 1024, 256 * 32);
     ProfScope ps_(IFOS_K_OTHER, (hipStream_t)stream);
-    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (float4 *)d_dst,
-                       (const float4 *)d_src, n4);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (v4f *)d_dst,
+                       (const v4f *)d_src, n4);
     return hip_check(hipGetLastError(), "k_copy");
 }
