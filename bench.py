@@ -55,31 +55,17 @@ def algorithmic_bytes(w, nf, kahan=False):
     }
 
 
-def to_device_layout(G, t):
-    import torch
-    import ifosgpu as ig
-    out = torch.zeros(G.dshape, dtype=torch.float32, device="cuda")
-    out[:, :, ig.KOFF - 2:ig.KOFF - 2 + G.nz + 5] = t
-    return out
-
-
 def run_gpu(w, steps, warmup, precision):
-    import torch
     import ifosgpu as ig
     import workloads as wl
 
     G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
     log(f"grid {w.nx}x{w.ny}x{w.nz} pitch {G.pitch} NT {w.nt} dt {w.dt:.4e} nf {len(w.finv)}")
-    if w.name == "toy84":
-        true_np, start_np = wl.sphere_models(w)
-        true = [torch.from_numpy(a).cuda() for a in true_np]
-        start = [torch.from_numpy(a).cuda() for a in start_np]
-    else:
-        true, start = wl.random_models(w)
-    d_true = [to_device_layout(G, a) for a in true]
-    d_start = [to_device_layout(G, a) for a in start]
-    host_start = [a.cpu().numpy() for a in start]
-    del true, start
+    true, start = wl.sphere_models(w) if w.name == "toy84" else wl.random_models(w)
+    d_true = [G.to_dev(a) for a in true]
+    d_start = [G.to_dev(a) for a in start]
+    host_start = start
+    del true
     ntr = len(w.receivers)
     ctx = ig.ShotContext(G, len(w.finv), ntr, precision)
     ctx.set_cpml(w.vppml, w.fpml)
@@ -89,36 +75,36 @@ def run_gpu(w, steps, warmup, precision):
     # observed data: forward runs on the true model (untimed setup)
     ctx.set_model(*d_true)
     obs = []
-    d_syn = torch.zeros((3, ntr, w.nt + 1), device="cuda")
+    d_syn = ig.DBuf((3, ntr, w.nt + 1))
     for n, s in enumerate(shot_ids):
         ctx.run_shot(w.shots[s], w.receivers, d_syn, None, want_adjoint=False)
         obs.append(d_syn.clone())
         log(f"observed data shot {n + 1}/{nshots}")
     ctx.set_model(*d_start)
     grads = [G.empty() for _ in range(3)]
-    torch.cuda.synchronize()
+    ig.sync()
     for n in range(warmup):
         ctx.run_shot(w.shots[shot_ids[n]], w.receivers, d_syn, grads, d_obs=obs[n])
-        torch.cuda.synchronize()
+        ig.sync()
         log(f"warmup shot {n + 1}/{warmup} done")
     lib = ig.lib()
     import ctypes as C
     lib.ifos_prof_reset()
     lib.ifos_prof_enable(1)
-    torch.cuda.synchronize()
+    ig.sync()
     t0 = time.perf_counter()
     l2 = 0.0
     for n in range(warmup, nshots):
         l2 += ctx.run_shot(w.shots[shot_ids[n]], w.receivers, d_syn, grads, d_obs=obs[n])
         log(f"timed shot {n - warmup + 1}/{steps}")
-    torch.cuda.synchronize()
+    ig.sync()
     elapsed = time.perf_counter() - t0
     lib.ifos_prof_enable(0)
     ms = (C.c_double * len(CATS))()
     cnt = (C.c_long * len(CATS))()
     ig.check(lib.ifos_prof_read(ms, cnt, len(CATS)), "ifos_prof_read")
     prof = {CATS[n]: (ms[n], cnt[n]) for n in range(len(CATS))}
-    gnorm = [float(torch.linalg.vector_norm(t.double()).item()) for t in grads]
+    gnorm = [float(np.linalg.norm(G.from_dev(t).astype(np.float64))) for t in grads]
     ctx.close()
     return dict(elapsed=elapsed, prof=prof, l2=l2, gnorm=gnorm, host_start=host_start, pitch=G.pitch)
 
@@ -126,21 +112,26 @@ def run_gpu(w, steps, warmup, precision):
 def copy_probe(mib=1024, reps=5):
     """plain device copy of a known size: the practical HBM ceiling, and the
     reference kernel the rocprof FETCH_SIZE/WRITE_SIZE counters are scaled by"""
-    import torch
+    import ctypes as C
     import ifosgpu as ig
     n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    a = ig.DBuf((n,))
+    b = ig.DBuf((n,))
     ig.copy(b, a, n)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+    lib = ig.lib()
+    lib.ifos_prof_reset()
+    lib.ifos_prof_enable(1)
     for _ in range(reps):
         ig.copy(b, a, n)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del a, b
+    lib.ifos_prof_enable(0)
+    ms_c = (C.c_double * len(CATS))()
+    cnt = (C.c_long * len(CATS))()
+    ig.check(lib.ifos_prof_read(ms_c, cnt, len(CATS)), "ifos_prof_read")
+    other = CATS.index("other")
+    ms = ms_c[other] / max(cnt[other], 1)
+    lib.ifos_prof_reset()
+    a.free()
+    b.free()
     return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
 
 
@@ -219,13 +210,13 @@ def main():
               "(shots would be sharded one rank per GPU; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import ifosgpu as ig
     import workloads as wl
-    if not torch.cuda.is_available():
+    ig.lib()  # fail loudly if the HIP library is not built
+    if ig.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
-    ig.lib()  # fail loudly if the HIP library is not built
+    log(f"device: {ig.device_name()}")
     copy = copy_probe()
     log(f"copy probe: {copy}")
     w = wl.grad276() if args.config == "grad276" else wl.toy84()

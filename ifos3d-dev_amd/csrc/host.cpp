@@ -69,6 +69,60 @@ using namespace ifos;
 
 extern "C" const char *ifos_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// device memory for callers without a HIP runtime of their own (the tests and
+// benchmark use these, so that one HIP runtime serves the whole process)
+extern "C" int ifos_dmalloc(void **p, size_t bytes)
+{
+    if (!p) return IFOS_EINVAL;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        return IFOS_ENOMEM;
+    }
+    return IFOS_OK;
+}
+
+extern "C" int ifos_dfree(void *p) { return p ? hip_check(hipFree(p), "hipFree") : IFOS_OK; }
+
+extern "C" int ifos_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+extern "C" int ifos_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+extern "C" int ifos_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+}
+
+extern "C" int ifos_dmemset(void *p, int value, size_t bytes)
+{
+    return hip_check(hipMemset(p, value, bytes), "hipMemset");
+}
+
+extern "C" int ifos_device_count(int *n)
+{
+    if (!n) return IFOS_EINVAL;
+    *n = 0;
+    hipError_t e = hipGetDeviceCount(n);
+    if (e != hipSuccess) *n = 0;
+    return IFOS_OK;
+}
+
+extern "C" int ifos_device_name(char *name, int len)
+{
+    hipDeviceProp_t prop;
+    int rc = hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+    if (rc) return rc;
+    snprintf(name, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return IFOS_OK;
+}
+
 extern "C" int ifos_device_sync(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
 extern "C" int ifos_geom_init(ifos_geom *g, int nx, int ny, int nz, int fw, float dx, float dy, float dz, float dt,
@@ -141,7 +195,7 @@ static int trunc_nonzero(float d)
 }
 
 static void cpml_axis(const ifos_geom *g, float dh, float vppml, float fpml, float *K, float *a, float *b,
-                      float *Kh, float *ah, float *bh, std::vector<float> &d)
+                      float *Kh, float *ah, float *bh, std::vector<float> &d, float *alpha, float *alpha_h)
 {
     const int fw = g->fw;
     const float npower = 3;
@@ -149,7 +203,7 @@ static void cpml_axis(const ifos_geom *g, float dh, float vppml, float fpml, flo
     const float alpha_max_PML = (float)(2.0 * 3.141592653589793 * (fpml / 2.0));
     const float Rcoef = 0.0008f;
     const float ca = 0.25f, cb = 0.75f, cc = 0.0f;
-    std::vector<float> dh_(2 * fw + 2, 0.0f), alpha(2 * fw + 2, 0.0f), alpha_h(2 * fw + 2, 0.0f);
+    std::vector<float> dh_(2 * fw + 2, 0.0f);
     const float d0 = (float)(-(npower + 1) * vppml * d_log(Rcoef) / (2.0 * fw * dh));
     for (int i = 1; i <= fw + 1; i++) {
         K[i] = 1.0f;
@@ -189,14 +243,40 @@ extern "C" int ifos_cpml_profiles(const ifos_geom *g, float vppml, float fpml, f
     const int len = 2 * g->fw + 1;
     memset(prof, 0, sizeof(float) * 18 * (size_t)len);
     const float dh[3] = {g->dx, g->dy, g->dz};
-    std::vector<float> d(2 * g->fw + 2);
+    std::vector<float> d(2 * g->fw + 2), al(2 * g->fw + 2), alh(2 * g->fw + 2);
     for (int ax = 0; ax < 3; ++ax) {
         float *p = prof + 6 * ax * len;
         std::fill(d.begin(), d.end(), 0.0f);
-        cpml_axis(g, dh[ax], vppml, fpml, p, p + len, p + 2 * len, p + 3 * len, p + 4 * len, p + 5 * len, d);
+        cpml_axis(g, dh[ax], vppml, fpml, p, p + len, p + 2 * len, p + 3 * len, p + 4 * len, p + 5 * len, d,
+                  al.data(), alh.data());
         if (ax == 1)
             for (int i = 1; i <= g->fw + 1; ++i)
                 if (!trunc_nonzero(d[i])) prof[1 * len + i] = 0.0f;  // a_x, CPML_coeff.c:216
+    }
+    return IFOS_OK;
+}
+
+// alpha_prime profiles of CPML_coeff.c (written for the caller, not used by
+// the kernels): out[6*(2fw+1)] = x, x_half, y, y_half, z, z_half
+extern "C" int ifos_cpml_alpha(const ifos_geom *g, float fpml, float *out)
+{
+    if (!g || !out) return IFOS_EINVAL;
+    const int fw = g->fw, len = 2 * fw + 1;
+    const float alpha_max_PML = (float)(2.0 * 3.141592653589793 * (fpml / 2.0));
+    const float dh[3] = {g->dx, g->dy, g->dz};
+    memset(out, 0, sizeof(float) * 6 * (size_t)len);
+    for (int ax = 0; ax < 3; ++ax) {
+        float *al = out + 2 * ax * len, *alh = out + (2 * ax + 1) * len;
+        for (int i = 1; i <= fw + 1; i++) {
+            float pos = (float)((fw - i + 1) * dh[ax]);
+            float pn = pos / (float)(fw * dh[ax]);
+            al[i] = (float)(alpha_max_PML * (1.0 - pn));
+            if (i <= fw) {
+                pos = (float)((fw - i + 0.5) * dh[ax]);
+                pn = pos / (float)(fw * dh[ax]);
+                alh[i] = (float)(alpha_max_PML * (1.0 - pn));
+            }
+        }
     }
     return IFOS_OK;
 }

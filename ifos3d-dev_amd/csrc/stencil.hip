@@ -511,3 +511,51 @@ extern "C" int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *
         hipLaunchKernelGGL((k_s_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi);
     return hip_check(hipGetLastError(), "k_s_march");
 }
+
+
+// box-restricted interior updates (the reference's update_v / update_s_elastic
+// take the box as arguments, update_v_ssg.c:27-30)
+extern "C" int ifos_update_v_box(const ifos_geom *g, const int box[6], const ifos_fields *f, const ifos_model *m,
+                                 void *stream)
+{
+    if (!g || !f || !m || !box) return IFOS_EINVAL;
+    Box R = {box[0], box[1], box[2], box[3], box[4], box[5]};
+    if (R.i0 < 1 || R.i1 > g->nx || R.j0 < 1 || R.j1 > g->ny || R.k0 < 1 || R.k1 > g->nz) {
+        set_error("ifos_update_v_box: box outside [1..N]^3");
+        return IFOS_EINVAL;
+    }
+    if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
+    DevGeom G = make_dev_geom(g);
+    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
+    Prof P = {nullptr, 2 * g->fw + 1};
+    Psi9 psi = {};
+    hipStream_t s = (hipStream_t)stream;
+    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
+    ProfScope ps_(IFOS_K_V_INTERIOR, s);
+    hipLaunchKernelGGL((k_v_march<TK, TI, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T, F, P,
+                       psi);
+    return hip_check(hipGetLastError(), "k_v_march");
+}
+
+extern "C" int ifos_update_s_box(const ifos_geom *g, const int box[6], const ifos_fields *f, const ifos_model *m,
+                                 void *stream)
+{
+    if (!g || !f || !m || !box) return IFOS_EINVAL;
+    Box R = {box[0], box[1], box[2], box[3], box[4], box[5]};
+    if (R.i0 < 1 || R.i1 > g->nx || R.j0 < 1 || R.j1 > g->ny || R.k0 < 1 || R.k1 > g->nz) {
+        set_error("ifos_update_s_box: box outside [1..N]^3");
+        return IFOS_EINVAL;
+    }
+    if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
+    DevGeom G = make_dev_geom(g);
+    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
+                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
+    Prof P = {nullptr, 2 * g->fw + 1};
+    Psi9 psi = {};
+    hipStream_t s = (hipStream_t)stream;
+    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
+    ProfScope ps_(IFOS_K_S_INTERIOR, s);
+    hipLaunchKernelGGL((k_s_march<TK, TI, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T, F, P,
+                       psi);
+    return hip_check(hipGetLastError(), "k_s_march");
+}

@@ -1,9 +1,10 @@
 """Python host side of libifos3d_hip.so (ctypes), used by the tests and bench.
 
-Mirrors the native C ABI of include/ifos3d_hip.h.  Device memory comes from
-torch (HIP allocations); the library is called with raw device pointers and
-the current torch stream.  The library must be built (make -C ifos3d-dev_amd);
-there is no fallback: a missing library raises.
+Mirrors the native C ABI of include/ifos3d_hip.h.  Device memory is
+allocated through the library itself (DBuf), so the process runs one HIP
+runtime and every launch and copy is ordered on the library's (null) stream.
+The library must be built (make -C ifos3d-dev_amd); there is no fallback: a
+missing library raises.
 
 Canonical arrays (numpy, shape (ny+5, nx+5, nz+5), index [j+2, i+2, k+2])
 are converted to and from the device layout (shape (ny+5, nx+5, pitch), k at
@@ -77,13 +78,84 @@ def check(rc: int, what: str = ""):
         raise IfosError(f"{what}: error {rc}: {msg}")
 
 
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().ifos_device_count(C.byref(n))
+    return n.value
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().ifos_device_name(buf, 256), "ifos_device_name")
+    return buf.value.decode()
+
+
+def sync():
+    check(lib().ifos_device_sync(), "ifos_device_sync")
+
+
+class DBuf:
+    """A device array owned by the library's HIP runtime (synchronous copies)."""
+
+    def __init__(self, shape, dtype=np.float32, zero=True):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().ifos_dmalloc(C.byref(p), C.c_size_t(self.nbytes)), "ifos_dmalloc")
+        self._p = p.value
+        if zero:
+            check(lib().ifos_dmemset(C.c_void_p(self._p), 0, C.c_size_t(self.nbytes)), "ifos_dmemset")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.shape, a.dtype, zero=False)
+        check(lib().ifos_memcpy_h2d(C.c_void_p(b._p), a.ctypes.data_as(C.c_void_p), C.c_size_t(b.nbytes)),
+              "ifos_memcpy_h2d")
+        return b
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().ifos_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self._p), C.c_size_t(self.nbytes)),
+              "ifos_memcpy_d2h")
+        return out
+
+    def copy_from(self, other: "DBuf"):
+        assert other.nbytes == self.nbytes
+        check(lib().ifos_memcpy_d2d(C.c_void_p(self._p), C.c_void_p(other._p), C.c_size_t(self.nbytes)),
+              "ifos_memcpy_d2d")
+        return self
+
+    def clone(self):
+        return DBuf(self.shape, self.dtype, zero=False).copy_from(self)
+
+    def zero_(self):
+        check(lib().ifos_dmemset(C.c_void_p(self._p), 0, C.c_size_t(self.nbytes)), "ifos_dmemset")
+        return self
+
+    def data_ptr(self) -> int:
+        return self._p
+
+    def ptr(self, offset_elems=0) -> int:
+        return self._p + offset_elems * self.dtype.itemsize
+
+    def free(self):
+        if getattr(self, "_p", None):
+            try:
+                lib().ifos_dfree(C.c_void_p(self._p))
+            finally:
+                self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
-
-
-def stream_ptr():
-    import torch
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 class Geom:
@@ -116,34 +188,31 @@ class Geom:
         f2 = 2 * self.fw
         return [(self.ny, f2, self.nz), (f2, self.nx, self.nz), (self.ny, self.nx, f2)][n // 3]
 
-    # ---- layout conversion (canonical numpy <-> device torch) ----
-    def empty(self, n=None, dtype=None):
-        import torch
-        shp = self.dshape if n is None else (n,) + self.dshape
-        return torch.zeros(shp, dtype=dtype or torch.float32, device="cuda")
+    # ---- layout conversion (canonical numpy <-> device) ----
+    def empty(self, n=None):
+        return DBuf(self.dshape if n is None else (n,) + self.dshape)
 
-    def to_dev(self, a, out=None):
-        import torch
+    def to_host_layout(self, a):
+        """canonical (..., ny+5, nx+5, nz+5) -> device layout numpy array"""
         a = np.asarray(a, dtype=np.float32)
-        host = np.zeros(self.dshape, np.float32)
-        host[:, :, KOFF - 2:KOFF - 2 + self.nz + 5] = a
-        t = torch.from_numpy(host).cuda()
-        if out is not None:
-            out.copy_(t)
-            return out
-        return t
+        host = np.zeros(a.shape[:-1] + (self.pitch,), np.float32)
+        host[..., KOFF - 2:KOFF - 2 + self.nz + 5] = a
+        return host
+
+    def to_dev(self, a):
+        return DBuf.from_numpy(self.to_host_layout(a))
 
     def from_dev(self, t):
-        h = t.detach().cpu().numpy()
+        h = t.numpy() if isinstance(t, DBuf) else np.asarray(t)
         return np.ascontiguousarray(h[..., KOFF - 2:KOFF - 2 + self.nz + 5])
 
 
 def fields_struct(ts):
-    return IfosFields(*[t.data_ptr() for t in ts])
+    return IfosFields(*[t.data_ptr() if t is not None else None for t in ts])
 
 
 def model_struct(ts):
-    return IfosModel(*[t.data_ptr() for t in ts])
+    return IfosModel(*[t.data_ptr() if t is not None else None for t in ts])
 
 
 def psi_struct(ts):
@@ -173,25 +242,25 @@ def ntast(fmax, tast, dt) -> int:
 
 def update_v(g, region, f, m, prof, psi, stream=None):
     check(lib().ifos_update_v(g.ref, region, C.byref(fields_struct(f)), C.byref(model_struct(m)),
-                              _ptr(prof), C.byref(psi_struct(psi)) if psi is not None else None,
-                              stream or stream_ptr()), "ifos_update_v")
+                              _ptr(prof), C.byref(psi_struct(psi)) if psi is not None else None, stream),
+          "ifos_update_v")
 
 
 def update_s(g, region, f, m, prof, psi, stream=None):
     check(lib().ifos_update_s(g.ref, region, C.byref(fields_struct(f)), C.byref(model_struct(m)),
-                              _ptr(prof), C.byref(psi_struct(psi)) if psi is not None else None,
-                              stream or stream_ptr()), "ifos_update_s")
+                              _ptr(prof), C.byref(psi_struct(psi)) if psi is not None else None, stream),
+          "ifos_update_s")
 
 
 def add_forces(g, f, m, d_src, d_sig, nsrc, nt, alpha=0.0, beta=0.0, stream=None):
     check(lib().ifos_add_forces(g.ref, C.byref(fields_struct(f)), C.byref(model_struct(m)), _ptr(d_src),
-                                _ptr(d_sig), nsrc, nt, C.c_float(alpha), C.c_float(beta),
-                                stream or stream_ptr()), "ifos_add_forces")
+                                _ptr(d_sig), nsrc, nt, C.c_float(alpha), C.c_float(beta), stream),
+          "ifos_add_forces")
 
 
 def add_adjoint(g, f, d_rec, d_adj, ntr, nt, d_chain=None, stream=None):
     check(lib().ifos_add_adjoint(g.ref, C.byref(fields_struct(f)), _ptr(d_rec), _ptr(d_chain), _ptr(d_adj), ntr,
-                                 nt, stream or stream_ptr()), "ifos_add_adjoint")
+                                 nt, stream), "ifos_add_adjoint")
 
 
 def rec_chain(rec_ijk) -> np.ndarray:
@@ -203,22 +272,21 @@ def rec_chain(rec_ijk) -> np.ndarray:
 
 
 def copy(d_dst, d_src, n, stream=None):
-    check(lib().ifos_copy(_ptr(d_dst), _ptr(d_src), C.c_size_t(n), stream or stream_ptr()), "ifos_copy")
+    check(lib().ifos_copy(_ptr(d_dst), _ptr(d_src), C.c_size_t(n), stream), "ifos_copy")
 
 
 def psource(g, f, d_src, d_sig, nsrc, nt, stream=None):
-    check(lib().ifos_psource(g.ref, C.byref(fields_struct(f)), _ptr(d_src), _ptr(d_sig), nsrc, nt,
-                             stream or stream_ptr()), "ifos_psource")
+    check(lib().ifos_psource(g.ref, C.byref(fields_struct(f)), _ptr(d_src), _ptr(d_sig), nsrc, nt, stream),
+          "ifos_psource")
 
 
 def surface(g, f, m, stream=None):
-    check(lib().ifos_surface(g.ref, C.byref(fields_struct(f)), C.byref(model_struct(m)),
-                             stream or stream_ptr()), "ifos_surface")
+    check(lib().ifos_surface(g.ref, C.byref(fields_struct(f)), C.byref(model_struct(m)), stream), "ifos_surface")
 
 
 def seismo(g, f, d_rec, ntr, ins, ns, d_sec, stream=None):
-    check(lib().ifos_seismo(g.ref, C.byref(fields_struct(f)), _ptr(d_rec), ntr, ins, ns, _ptr(d_sec),
-                            stream or stream_ptr()), "ifos_seismo")
+    check(lib().ifos_seismo(g.ref, C.byref(fields_struct(f)), _ptr(d_rec), ntr, ins, ns, _ptr(d_sec), stream),
+          "ifos_seismo")
 
 
 def discfourier(g, f, F6, C6, finv, nt, ntast_, back, precision=DFT_COMPAT, stream=None):
@@ -226,8 +294,7 @@ def discfourier(g, f, F6, C6, finv, nt, ntast_, back, precision=DFT_COMPAT, stre
     Fp = (C.c_void_p * 6)(*[t.data_ptr() for t in F6])
     Cp = (C.c_void_p * 6)(*[t.data_ptr() for t in C6]) if C6 is not None else None
     check(lib().ifos_discfourier(g.ref, C.byref(fields_struct(f)), Fp, Cp, fv.ctypes.data_as(C.c_void_p),
-                                 len(fv), nt, ntast_, back, precision, stream or stream_ptr()),
-          "ifos_discfourier")
+                                 len(fv), nt, ntast_, back, precision, stream), "ifos_discfourier")
 
 
 def gradient_F(g, Ff6, Fb6, m, finv, grads, stream=None):
@@ -236,22 +303,21 @@ def gradient_F(g, Ff6, Fb6, m, finv, grads, stream=None):
     Bp = (C.c_void_p * 6)(*[t.data_ptr() for t in Fb6])
     Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads])
     check(lib().ifos_gradient_F(g.ref, Fp, Bp, C.byref(model_struct(m)), fv.ctypes.data_as(C.c_void_p),
-                                len(fv), Gp, stream or stream_ptr()), "ifos_gradient_F")
+                                len(fv), Gp, stream), "ifos_gradient_F")
 
 
 def matcopy_wrap(g, d_m, stream=None):
-    check(lib().ifos_matcopy_wrap(g.ref, _ptr(d_m), stream or stream_ptr()), "ifos_matcopy_wrap")
+    check(lib().ifos_matcopy_wrap(g.ref, _ptr(d_m), stream), "ifos_matcopy_wrap")
 
 
 def av_mat(g, rho, u, out6, stream=None):
-    check(lib().ifos_av_mat(g.ref, _ptr(rho), _ptr(u), *[_ptr(t) for t in out6], stream or stream_ptr()),
-          "ifos_av_mat")
+    check(lib().ifos_av_mat(g.ref, _ptr(rho), _ptr(u), *[_ptr(t) for t in out6], stream), "ifos_av_mat")
 
 
 def residual(g, d_obs, d_syn, d_adj, ntr, ns, stream=None) -> float:
     l2 = C.c_double(0.0)
-    check(lib().ifos_residual(g.ref, _ptr(d_obs), _ptr(d_syn), _ptr(d_adj), ntr, ns, C.byref(l2),
-                              stream or stream_ptr()), "ifos_residual")
+    check(lib().ifos_residual(g.ref, _ptr(d_obs), _ptr(d_syn), _ptr(d_adj), ntr, ns, C.byref(l2), stream),
+          "ifos_residual")
     return l2.value
 
 
@@ -275,8 +341,7 @@ class ShotContext:
             pass
 
     def set_model(self, d_rho, d_pi, d_u, stream=None):
-        check(lib().ifos_ctx_set_model(self.h, _ptr(d_rho), _ptr(d_pi), _ptr(d_u), stream or stream_ptr()),
-              "ifos_ctx_set_model")
+        check(lib().ifos_ctx_set_model(self.h, _ptr(d_rho), _ptr(d_pi), _ptr(d_u), stream), "ifos_ctx_set_model")
 
     def set_cpml(self, vppml, fpml):
         check(lib().ifos_ctx_set_cpml(self.h, C.c_float(vppml), C.c_float(fpml)), "ifos_ctx_set_cpml")
@@ -293,8 +358,8 @@ class ShotContext:
                      rec.ctypes.data, d_obs.data_ptr() if d_obs is not None else None, int(want_adjoint))
         l2 = C.c_double(0.0)
         Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads]) if grads is not None else None
-        check(lib().ifos_ctx_run_shot(self.h, C.byref(s), _ptr(d_syn), Gp, C.byref(l2),
-                                      stream or stream_ptr()), "ifos_ctx_run_shot")
+        check(lib().ifos_ctx_run_shot(self.h, C.byref(s), _ptr(d_syn), Gp, C.byref(l2), stream),
+              "ifos_ctx_run_shot")
         return l2.value
 
     def fields(self):

@@ -73,18 +73,20 @@ def grad276() -> Workload:
 
 
 # ---------------------------------------------------------------------------
-# models
-def _grf(shape, corr_cells, seed, device):
+# models (host, numpy/scipy FFT)
+def _kgrid(shape):
+    ks = [np.fft.fftfreq(s) * 2 * np.pi for s in shape[:-1]]
+    ks.append(np.fft.rfftfreq(shape[-1]) * 2 * np.pi)
+    return ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
+
+
+def _grf(shape, corr_cells, seed):
     """zero-mean Gaussian random field with covariance exp(-r^2/l^2), via FFT"""
-    import torch
-    gen = torch.Generator(device=device).manual_seed(seed)
-    w = torch.randn(shape, generator=gen, device=device, dtype=torch.float32)
-    W = torch.fft.rfftn(w)
-    ks = [torch.fft.fftfreq(s, device=device) * 2 * math.pi for s in shape[:-1]]
-    ks.append(torch.fft.rfftfreq(shape[-1], device=device) * 2 * math.pi)
-    k2 = ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
-    W *= torch.exp(-k2 * corr_cells ** 2 / 8.0)
-    return torch.fft.irfftn(W, s=shape)
+    import scipy.fft as sf
+    w = np.random.default_rng(seed).standard_normal(shape).astype(np.float32)
+    W = sf.rfftn(w, workers=-1)
+    W *= np.exp(-_kgrid(shape) * corr_cells ** 2 / 8.0).astype(np.float32)
+    return sf.irfftn(W, s=shape, workers=-1).astype(np.float64)
 
 
 def _minmax(a, lo, hi):
@@ -93,32 +95,24 @@ def _minmax(a, lo, hi):
 
 
 def _smooth(a, sigma_cells):
-    import torch
-    A = torch.fft.rfftn(a)
-    shape = a.shape
-    dev = a.device
-    ks = [torch.fft.fftfreq(s, device=dev) * 2 * math.pi for s in shape[:-1]]
-    ks.append(torch.fft.rfftfreq(shape[-1], device=dev) * 2 * math.pi)
-    k2 = ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
-    A *= torch.exp(-0.5 * k2 * sigma_cells ** 2)
-    return torch.fft.irfftn(A, s=shape)
+    import scipy.fft as sf
+    A = sf.rfftn(a, workers=-1)
+    A *= np.exp(-0.5 * _kgrid(a.shape) * sigma_cells ** 2)
+    return sf.irfftn(A, s=a.shape, workers=-1)
 
 
-def random_models(w: Workload, device="cuda", corr=20.0, smooth_cells=2.0):
-    """(true, start) models as canonical-shape torch tensors (rho, pi, u)"""
-    import torch
+def random_models(w: Workload, corr=20.0, smooth_cells=2.0):
+    """(true, start) models as canonical-shape numpy arrays (rho, pi, u)"""
     shape = (w.ny + 5, w.nx + 5, w.nz + 5)
-    vp = _minmax(_grf(shape, corr, w.seed, device), 2500.0, 5500.0)
-    ratio = _minmax(_grf(shape, corr, w.seed + 1000, device), 1.7, 2.0)
+    vp = _minmax(_grf(shape, corr, w.seed), 2500.0, 5500.0)
+    ratio = _minmax(_grf(shape, corr, w.seed + 1000), 1.7, 2.0)
 
     def moduli(vp_):
         vs_ = vp_ / ratio
         rho_ = 310.0 * vp_ ** 0.25
-        return [t.to(torch.float32).contiguous() for t in (rho_, rho_ * vp_ * vp_, rho_ * vs_ * vs_)]
+        return [np.ascontiguousarray(t, np.float32) for t in (rho_, rho_ * vp_ * vp_, rho_ * vs_ * vs_)]
 
-    true = moduli(vp)
-    start = moduli(_smooth(vp, smooth_cells))
-    return true, start
+    return moduli(vp), moduli(_smooth(vp, smooth_cells))
 
 
 def sphere_models(w: Workload):

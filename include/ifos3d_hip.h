@@ -108,10 +108,24 @@ int ifos_prof_read(double *ms, long *count, int ncat);
 const char *ifos_last_error(void);
 int ifos_device_sync(void);
 
+/* device memory and copies (synchronous), for callers without a HIP
+ * runtime of their own */
+int ifos_dmalloc(void **p, size_t bytes);
+int ifos_dfree(void *p);
+int ifos_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ifos_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int ifos_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int ifos_dmemset(void *p, int value, size_t bytes);
+int ifos_device_count(int *n);
+int ifos_device_name(char *name, int len);
+
 /* --- host helpers (same arithmetic as the reference, run on the CPU) --- */
 /* CPML_coeff.c:87-324 -> prof[18*(2fw+1)] host array */
 int ifos_cpml_profiles(const ifos_geom *g, float vppml, float fpml,
                        float *prof);
+/* the alpha_prime profiles CPML_coeff also returns (not used by the
+ * kernels): out[6*(2fw+1)] = x, x_half, y, y_half, z, z_half, 1-based */
+int ifos_cpml_alpha(const ifos_geom *g, float fpml, float *out);
 /* wavelet.c:47-90 (shapes 1, 2, 4, 5) -> sig[nt+2] indexed by nt */
 int ifos_wavelet(const ifos_geom *g, int shape, float tshift, float fc,
                  float amp, float *sig);
@@ -128,6 +142,13 @@ int ifos_update_v(const ifos_geom *g, int region, const ifos_fields *f,
 int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *f,
                   const ifos_model *m, const float *d_prof,
                   const ifos_psi *psi_v, void *stream);
+/* the same interior updates over an explicit box {i0,i1,j0,j1,k0,k1}
+ * inside [1..N]^3 (update_v and update_s_elastic take the box as
+ * arguments), without CPML */
+int ifos_update_v_box(const ifos_geom *g, const int box[6],
+                      const ifos_fields *f, const ifos_model *m, void *stream);
+int ifos_update_s_box(const ifos_geom *g, const int box[6],
+                      const ifos_fields *f, const ifos_model *m, void *stream);
 /* body forces (update_v_ssg.c:474-510), d_src [nsrc][4] = i, j, k, type;
  * d_sig [nsrc][nt+2] by nt; alpha/beta in degrees (ALPHA, BETA) */
 int ifos_add_forces(const ifos_geom *g, const ifos_fields *f,

@@ -328,6 +328,15 @@ class Ref:
         self.lib.rg_get_sections(*[_p(x) for x in s])
         return s
 
+    def set_sections(self, syn):
+        s = [np.ascontiguousarray(x, np.float32) for x in syn]
+        self.lib.rg_set_sections(*[_p(x) for x in s])
+
+    def get_adjoint(self):
+        a = [np.zeros((self.ntr, self.g.nt + 2), np.float32) for _ in range(3)]
+        self.lib.rg_get_adjoint(*[_p(x) for x in a])
+        return a
+
     def residual(self, obs):
         o = [np.ascontiguousarray(x, np.float32) for x in obs]
         return float(self.lib.rg_residual(*[_p(x) for x in o], C.c_float(0.0)))

@@ -306,6 +306,29 @@ void rg_get_sections(float *sx, float *sy, float *sz)
         }
 }
 
+void rg_set_sections(const float *sx, const float *sy, const float *sz)
+{
+    int r, n;
+    for (r = 1; r <= S.ntr; r++)
+        for (n = 1; n <= NT; n++) {
+            S.secx[r][n] = sx[(size_t)(r - 1) * (NT + 1) + n];
+            S.secy[r][n] = sy[(size_t)(r - 1) * (NT + 1) + n];
+            S.secz[r][n] = sz[(size_t)(r - 1) * (NT + 1) + n];
+        }
+}
+
+/* adjoint traces as [ntr][nt+2] by step */
+void rg_get_adjoint(float *ax, float *ay, float *az)
+{
+    int r, n;
+    for (r = 1; r <= S.ntr; r++)
+        for (n = 1; n <= NT; n++) {
+            ax[(size_t)(r - 1) * (NT + 2) + n] = S.adjx[r][n];
+            ay[(size_t)(r - 1) * (NT + 2) + n] = S.adjy[r][n];
+            az[(size_t)(r - 1) * (NT + 2) + n] = S.adjz[r][n];
+        }
+}
+
 /* --- single reference calls ------------------------------------------ */
 #define W S.f
 void rg_update_v(int nt, int back)

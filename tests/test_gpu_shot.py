@@ -44,7 +44,6 @@ def _models(g, seed):
 
 
 def _run_case(g, src, rec, finv, tast, vppml, fpml, precision="compat"):
-    import torch
     import ifosgpu as ig
     o = Oracle("f32")
     start, true = _models(g, 1)
@@ -56,7 +55,7 @@ def _run_case(g, src, rec, finv, tast, vppml, fpml, precision="compat"):
     ctx.set_cpml(vppml, fpml)
     ctx.set_freqs(finv, tast)
     ntr = len(rec)
-    d_syn = torch.zeros((3, ntr, g.nt + 1), device="cuda")
+    d_syn = ig.DBuf((3, ntr, g.nt + 1))
     grads = [G.empty() for _ in range(3)]
     # observed data through the device path too (forward only on the true model)
     ctx.set_model(*[G.to_dev(a) for a in true])
@@ -64,8 +63,8 @@ def _run_case(g, src, rec, finv, tast, vppml, fpml, precision="compat"):
     d_obs = d_syn.clone()
     ctx.set_model(*[G.to_dev(a) for a in start])
     l2 = ctx.run_shot(src, rec, d_syn, grads, d_obs=d_obs)
-    torch.cuda.synchronize()
-    out = dict(obs=d_obs.cpu().numpy(), syn=d_syn.cpu().numpy(), l2=l2,
+    ig.sync()
+    out = dict(obs=d_obs.numpy(), syn=d_syn.numpy(), l2=l2,
                grads=[g.interior(G.from_dev(t)) for t in grads])
     ctx.close()
     return ref, obs, out
@@ -94,7 +93,6 @@ def test_gradient_shot_matches_oracle(fdcoeff, free_surf):
 
 def test_force_source_shot():
     """single force (type 4, vertical) and non-cubic grid, forward only"""
-    import torch
     import ifosgpu as ig
     g = small_grid(nx=40, ny=30, nz=44, fw=7, nt=120)
     _note("force-source forward shot")
@@ -108,9 +106,9 @@ def test_force_source_shot():
     ctx.set_cpml(4000.0, 50.0)
     ctx.set_freqs([50.0])
     ctx.set_model(*[G.to_dev(a) for a in start])
-    d_syn = torch.zeros((3, len(rec), g.nt + 1), device="cuda")
+    d_syn = ig.DBuf((3, len(rec), g.nt + 1))
     ctx.run_shot(src, rec, d_syn, None, want_adjoint=False)
-    syn = d_syn.cpu().numpy()
+    syn = d_syn.numpy()
     ctx.close()
     for c in range(3):
         assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0

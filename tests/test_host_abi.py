@@ -33,6 +33,19 @@ def test_native_library_exports_header():
     assert not missing, missing
 
 
+def test_dropin_library_exports_header():
+    import ctypes as C
+    drv = os.path.join(ROOT, "tests", "dropin", "lib", "libnr_driver.so")
+    lib = C.CDLL(drv)  # defines the globals, pulls in libifos3d_dropin.so
+    dropin = C.CDLL(os.path.join(ROOT, "ifos3d-dev_amd", "lib", "libifos3d_dropin.so"))
+    names = _declared("ifos3d_dropin.h")
+    assert {"update_v", "update_v_CPML", "update_s_elastic", "update_s_CPML_elastic", "discfourier",
+            "gradient_F"} <= set(names)
+    missing = [n for n in names if not hasattr(dropin, n)]
+    assert not missing, missing
+    assert hasattr(lib, "dd_run_shot")
+
+
 @pytest.mark.parametrize("fw,dh,vppml,fpml,dt", [(10, 2.0, 4000.0, 50.0, 2e-4), (8, 5.0, 5500.0, 15.0, 4e-4),
                                                  (12, 0.8, 6200.0, 200.0, 5e-5), (3, 50.0, 300.0, 1.0, 1e-3)])
 def test_cpml_profiles_match_oracle(fw, dh, vppml, fpml, dt):
