@@ -24,6 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "ifos3d-dev_amd"))
 
 METRIC = "gradient shots/sec at 256³ (batched vs plain); Gcell-updates/s; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+# kernel (as rocprofv3 names it) behind each launch-timer category
+CAT_KERNEL = {"v_interior": "ifos::k_v_march<64, 4, false>", "s_interior": "ifos::k_s_march<64, 4, false>",
+              "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs", "dft": "ifos::k_dft_nf<{nf}>",
+              "dft_kahan": "ifos::k_dft_kahan_nf<{nf}>", "gradient": "ifos::k_gradient"}
 CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other"]
 
 
@@ -231,6 +235,14 @@ def main():
     dom = max((k for k in ab if prof[k][1] > 0), key=lambda k: prof[k][0])
     avg_ms = prof[dom][0] / prof[dom][1]
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
+    # HBM traffic of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE summaries (scaled by the copy kernel), when they cover it
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{w.name}_plain_traffic.txt")
+    kname = CAT_KERNEL.get("dft_kahan" if dom == "dft" and prec == ig.DFT_KAHAN else dom, "").format(nf=nf)
+    if os.path.exists(tpath) and kname:
+        import profsum
+        traffic = profsum.read_traffic(tpath).get(kname)
     stencil_cats = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused"]
     st_bytes = sum(ab[k] * prof[k][1] for k in stencil_cats)
     st_ms = sum(prof[k][0] for k in stencil_cats)
@@ -270,8 +282,10 @@ def main():
         "gcell_updates_per_s": round(gcells, 3),
         "hbm_frac_stencils": round(hbm_frac_stencils, 4) if hbm_frac_stencils else None,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 5)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "traffic_source": os.path.relpath(tpath, ROOT) if traffic else None,
+                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 5),
+                     "rocprof_kernel": kname or None},
         "kernels": kernels,
         "copy_probe": copy,
         "cpu_baseline": cpu,
