@@ -321,17 +321,12 @@ struct GradFreq {
     float fdummy[IFOS_MAX_NF];
 };
 
-__global__ __launch_bounds__(256) void k_gradient(DevGeom g, FPtrs Ff, FPtrs Fb, int nf, int64_t fstride,
-                                                 GradFreq GF, const float *rho, const float *pi, const float *u,
-                                                 float *g1, float *g2, float *g3)
+// one shot's contribution to the three gradients of cell c, frequencies in
+// order (gradient_F.c:79-139)
+__device__ inline void grad_accum(const DevGeom &g, const FPtrs &Ff, const FPtrs &Fb, int nf, int64_t fstride,
+                                  const GradFreq &GF, float r, float p, float m, int64_t c, float &a1, float &a2,
+                                  float &a3)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int j = blockIdx.z + 1;
-    if (k > g.nz || i > g.nx) return;
-    const int64_t c = at(g, j, i, k);
-    const float r = rho[c], p = pi[c], m = u[c];
-    float a1 = g1[c], a2 = g2[c], a3 = g3[c];
     for (int l = 0; l < nf; ++l) {
         const int64_t co = c + (int64_t)l * fstride;
         const float fd = GF.fdummy[l];
@@ -353,6 +348,47 @@ __global__ __launch_bounds__(256) void k_gradient(DevGeom g, FPtrs Ff, FPtrs Fb,
         a1 = (float)(a1 + srp * 2 * gl);
         a2 = (float)(a2 + (-4 * sru * gl + 2 * sru * gm));
         a3 += gr + m / r * gm + (p - 2.0f * m) / r * gl;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gradient(DevGeom g, FPtrs Ff, FPtrs Fb, int nf, int64_t fstride,
+                                                 GradFreq GF, const float *rho, const float *pi, const float *u,
+                                                 float *g1, float *g2, float *g3)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k);
+    float a1 = g1[c], a2 = g2[c], a3 = g3[c];
+    grad_accum(g, Ff, Fb, nf, fstride, GF, rho[c], pi[c], u[c], c, a1, a2, a3);
+    g1[c] = a1;
+    g2[c] = a2;
+    g3[c] = a3;
+}
+
+// several shots accumulated in shot order, as consecutive gradient_F calls
+// would (shot s's DFT fields at +s*fshot elements)
+__global__ __launch_bounds__(256) void k_gradient_shots(DevGeom g, FPtrs Ff, FPtrs Fb, int nf, int64_t fstride,
+                                                       int64_t fshot, int ns, GradFreq GF, const float *rho,
+                                                       const float *pi, const float *u, float *g1, float *g2,
+                                                       float *g3)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k);
+    const float r = rho[c], p = pi[c], m = u[c];
+    float a1 = g1[c], a2 = g2[c], a3 = g3[c];
+    for (int s = 0; s < ns; ++s) {
+        FPtrs fs = Ff, bs = Fb;
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+            fs.p[n] += (int64_t)s * fshot;
+            bs.p[n] += (int64_t)s * fshot;
+        }
+        grad_accum(g, fs, bs, nf, fstride, GF, r, p, m, c, a1, a2, a3);
     }
     g1[c] = a1;
     g2[c] = a2;
@@ -590,6 +626,29 @@ extern "C" int ifos_gradient_F(const ifos_geom *g, float *const d_Ff[6], float *
     }
     return hip_check(hipGetLastError(), "k_gradient");
 }
+
+namespace ifos {
+int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,
+                          const ifos_model *m, const float *finv, int nf, float *const d_grad[3], hipStream_t st)
+{
+    DevGeom G = make_dev_geom(g);
+    GradFreq GF;
+    for (int l = 0; l < nf; ++l) {
+        float fd = (float)(2.0 * 2.0 * finv[l] * finv[l] * M_PI * M_PI);
+        GF.fdummy[l] = 1 / fd;
+    }
+    FPtrs Ff, Fb;
+    for (int n = 0; n < 6; ++n) {
+        Ff.p[n] = d_Ff[n];
+        Fb.p[n] = d_Fb[n];
+    }
+    dim3 block, grid = cell_grid(g, block);
+    ProfScope ps_(IFOS_K_GRADIENT, st);
+    hipLaunchKernelGGL(k_gradient_shots, grid, block, 0, st, G, Ff, Fb, nf, (int64_t)ifos_field_elems(g), fshot, ns,
+                       GF, m->rho, m->pi, m->u, d_grad[0], d_grad[1], d_grad[2]);
+    return hip_check(hipGetLastError(), "k_gradient_shots");
+}
+}  // namespace ifos
 
 extern "C" int ifos_matcopy_wrap(const ifos_geom *g, float *d_m, void *stream)
 {

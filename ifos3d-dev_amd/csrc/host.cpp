@@ -620,9 +620,16 @@ static hipEvent_t prof_event()
     return g_prof.pool[g_prof.used++];
 }
 
+// launches captured into a graph are not timed one by one
+static bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 void prof_begin(int cat, hipStream_t s)
 {
-    if (!g_prof.on) return;
+    if (!g_prof.on || capturing(s)) return;
     size_t idx = g_prof.used;
     hipEvent_t e = prof_event();
     if (!e) return;
@@ -632,7 +639,7 @@ void prof_begin(int cat, hipStream_t s)
 
 void prof_end(int cat, hipStream_t s)
 {
-    if (!g_prof.on || g_prof.marks.empty()) return;
+    if (!g_prof.on || g_prof.marks.empty() || capturing(s)) return;
     hipEvent_t e = prof_event();
     if (!e) return;
     (void)hipEventRecord(e, s);

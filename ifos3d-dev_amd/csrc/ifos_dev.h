@@ -73,6 +73,25 @@ __device__ inline void cpml_apply(float *psi, int64_t at_, float &d, float K, fl
     d = d / K + p;
 }
 
+struct Psi9 {
+    float *p[9];
+};
+// shots of one launch: shot s uses wavefield arrays at +s*fstride and CPML
+// memory arrays at +s*pstride[axis] (elements); the model is shared
+struct Shots {
+    int n;
+    int64_t fstride;
+    int64_t pstride[3];
+};
+// velocity snapshot ring of the batched path: shot s, component n of the
+// current slot at byte (s*rstride)*4 + soff + n*fbytes + cell offset
+struct Snap {
+    float *ring;
+    int64_t rstride;
+    uint32_t rbytes;
+    uint32_t soff;
+};
+
 // Index box (inclusive), and a list of them for the slab launches.
 struct Box {
     int i0, i1, j0, j1, k0, k1;
@@ -103,6 +122,17 @@ __device__ inline void bstore(float v, rsrc_t r, uint32_t voff, uint32_t soff)
 }
 
 DevGeom make_dev_geom(const ifos_geom *g);
+
+// multi-shot fused (interior + CPML) updates over [1..N]^3 for the batched
+// path (stencil.hip): f and psi point at shot 0; snap may be null
+int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
+                   const ifos_psi *psi, const Shots &S, const Snap *snap, hipStream_t s);
+int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
+                   const ifos_psi *psi, const Shots &S, hipStream_t s);
+// gradient_F of ns shots in shot order into d_grad (aux.hip); shot s's DFT
+// fields at d_F*[n] + s*fshot
+int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,
+                          const ifos_model *m, const float *finv, int nf, float *const d_grad[3], hipStream_t st);
 void set_error(const char *fmt, ...);
 int hip_check(hipError_t e, const char *what);
 

@@ -14,6 +14,13 @@
 //    CPML slabs, the shape of update_v_ssg_CPML.c / update_s_ssg_CPML_elastic.c,
 //    used by the reference-shaped (drop-in) update_*_CPML entry points.
 //
+// The march kernels also run several shots of one model in one launch (the
+// batched path): logical tile L -> (tile, shot) with the shot fastest, so the
+// workgroups of one XCD sweep the same tile for every shot back to back and
+// the shared model arrays are read from HBM once per tile.  At the steps the
+// DFT samples, the velocity kernel also writes the new velocities into a
+// snapshot ring (the chunked DFT reads them later).
+//
 // Arithmetic follows the reference expression by expression
 // (update_v_ssg.c:106-125, update_v_ssg_CPML.c:73-100,
 //  update_s_ssg_elastic.c:106-138, update_s_ssg_CPML_elastic.c:63-140).
@@ -31,10 +38,6 @@ struct SFields {
     float *sxx, *syy, *szz, *sxy, *syz, *sxz;
     const float *pi, *u, *uipjp, *ujpkp, *uipkp;
 };
-struct Psi9 {
-    float *p[9];
-};
-
 // 4th-order differences: forward f(c+1)-f(c), backward f(c)-f(c-1)
 #define D4F(b1, b2, m1, c0, p1, p2) ((b1) * ((p1) - (c0)) + (b2) * ((p2) - (m1)))
 #define D4B(b1, b2, m2, m1, c0, p1) ((b1) * ((c0) - (m1)) + (b2) * ((p1) - (m2)))
@@ -127,18 +130,27 @@ __device__ inline void s_apply(const DevGeom &g, const SFields &F, int64_t c, fl
 struct Tiling {
     int ntk, nti, ntj, total, per;
     int jc;
+    int ns;  // shots: total = ntk * nti * ntj * ns
 };
 
-__device__ inline bool decode_tile(const Tiling &T, int &tk, int &ti, int &tj)
+__device__ inline bool decode_tile(const Tiling &T, int &tk, int &ti, int &tj, int &shot)
 {
     int b = blockIdx.x;
     int L = (b & 7) * T.per + (b >> 3);
     if (L >= T.total) return false;
+    shot = L % T.ns;
+    L /= T.ns;
     tk = L % T.ntk;
     int r = L / T.ntk;
     ti = r % T.nti;
     tj = r / T.nti;
     return true;
+}
+
+__device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
+{
+#pragma unroll
+    for (int n = 0; n < 9; ++n) psi.p[n] += (int64_t)shot * S.pstride[n / 3];
 }
 
 // Byte offsets for the march kernels: vo is the offset of (j-2, i-2, k-2)
@@ -148,11 +160,12 @@ __device__ inline bool decode_tile(const Tiling &T, int &tk, int &ti, int &tj)
 #define LD(r, dj, di, dk) bload(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
 #define ST(v, r) bstore(v, r, vo + 8, 2 * SJ + 2 * SI)
 
-template <int TK, int TI, bool CPML>
-__global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, VFields F, Prof P, Psi9 psi)
+template <int TK, int TI, bool CPML, bool SNAP>
+__global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, VFields F, Prof P, Psi9 psi,
+                                                   Shots S, Snap N)
 {
-    int tk, ti, tj;
-    if (!decode_tile(T, tk, ti, tj)) return;
+    int tk, ti, tj, shot;
+    if (!decode_tile(T, tk, ti, tj, shot)) return;
     const int k = R.k0 + tk * TK + (int)threadIdx.x;
     const int i = R.i0 + ti * TI + (int)threadIdx.y;
     if (k > R.k1 || i > R.i1) return;
@@ -160,12 +173,16 @@ __global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, 
     const int je = min(jb + T.jc - 1, R.j1);
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
-    const rsrc_t rsxx = make_rsrc(F.sxx, g.fbytes), rsyy = make_rsrc(F.syy, g.fbytes),
-                 rszz = make_rsrc(F.szz, g.fbytes), rsxy = make_rsrc(F.sxy, g.fbytes),
-                 rsyz = make_rsrc(F.syz, g.fbytes), rsxz = make_rsrc(F.sxz, g.fbytes),
-                 rvx = make_rsrc(F.vx, g.fbytes), rvy = make_rsrc(F.vy, g.fbytes), rvz = make_rsrc(F.vz, g.fbytes),
-                 rrip = make_rsrc(F.rip, g.fbytes), rrjp = make_rsrc(F.rjp, g.fbytes),
-                 rrkp = make_rsrc(F.rkp, g.fbytes);
+    const int64_t so = (int64_t)shot * S.fstride;
+    const rsrc_t rsxx = make_rsrc(F.sxx + so, g.fbytes), rsyy = make_rsrc(F.syy + so, g.fbytes),
+                 rszz = make_rsrc(F.szz + so, g.fbytes), rsxy = make_rsrc(F.sxy + so, g.fbytes),
+                 rsyz = make_rsrc(F.syz + so, g.fbytes), rsxz = make_rsrc(F.sxz + so, g.fbytes),
+                 rvx = make_rsrc(F.vx + so, g.fbytes), rvy = make_rsrc(F.vy + so, g.fbytes),
+                 rvz = make_rsrc(F.vz + so, g.fbytes), rrip = make_rsrc(F.rip, g.fbytes),
+                 rrjp = make_rsrc(F.rjp, g.fbytes), rrkp = make_rsrc(F.rkp, g.fbytes);
+    rsrc_t rring;
+    if (SNAP) rring = make_rsrc(N.ring + (int64_t)shot * N.rstride, N.rbytes);
+    if (CPML) shift_psi(psi, S, shot);
     uint32_t vo = (uint32_t)((int64_t)jb * g.sj + (int64_t)i * g.si + k + IFOS_KOFF - 2) * 4u;
     int hx = 0;
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
@@ -216,14 +233,20 @@ __global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, 
         ST(nvx, rvx);
         ST(nvy, rvy);
         ST(nvz, rvz);
+        if (SNAP) {
+            bstore(nvx, rring, vo + 8, 2 * SJ + 2 * SI + N.soff);
+            bstore(nvy, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + g.fbytes);
+            bstore(nvz, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + 2 * g.fbytes);
+        }
     }
 }
 
 template <int TK, int TI, bool CPML>
-__global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, SFields F, Prof P, Psi9 psi)
+__global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, SFields F, Prof P, Psi9 psi,
+                                                   Shots S)
 {
-    int tk, ti, tj;
-    if (!decode_tile(T, tk, ti, tj)) return;
+    int tk, ti, tj, shot;
+    if (!decode_tile(T, tk, ti, tj, shot)) return;
     const int k = R.k0 + tk * TK + (int)threadIdx.x;
     const int i = R.i0 + ti * TI + (int)threadIdx.y;
     if (k > R.k1 || i > R.i1) return;
@@ -232,13 +255,15 @@ __global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, 
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
-    const rsrc_t rvx = make_rsrc(F.vx, g.fbytes), rvy = make_rsrc(F.vy, g.fbytes), rvz = make_rsrc(F.vz, g.fbytes),
-                 rsxx = make_rsrc(F.sxx, g.fbytes), rsyy = make_rsrc(F.syy, g.fbytes),
-                 rszz = make_rsrc(F.szz, g.fbytes), rsxy = make_rsrc(F.sxy, g.fbytes),
-                 rsyz = make_rsrc(F.syz, g.fbytes), rsxz = make_rsrc(F.sxz, g.fbytes),
-                 rpi = make_rsrc(F.pi, g.fbytes), ru = make_rsrc(F.u, g.fbytes),
+    const int64_t so = (int64_t)shot * S.fstride;
+    const rsrc_t rvx = make_rsrc(F.vx + so, g.fbytes), rvy = make_rsrc(F.vy + so, g.fbytes),
+                 rvz = make_rsrc(F.vz + so, g.fbytes), rsxx = make_rsrc(F.sxx + so, g.fbytes),
+                 rsyy = make_rsrc(F.syy + so, g.fbytes), rszz = make_rsrc(F.szz + so, g.fbytes),
+                 rsxy = make_rsrc(F.sxy + so, g.fbytes), rsyz = make_rsrc(F.syz + so, g.fbytes),
+                 rsxz = make_rsrc(F.sxz + so, g.fbytes), rpi = make_rsrc(F.pi, g.fbytes), ru = make_rsrc(F.u, g.fbytes),
                  ruipjp = make_rsrc(F.uipjp, g.fbytes), rujpkp = make_rsrc(F.ujpkp, g.fbytes),
                  ruipkp = make_rsrc(F.uipkp, g.fbytes);
+    if (CPML) shift_psi(psi, S, shot);
     uint32_t vo = (uint32_t)((int64_t)jb * g.sj + (int64_t)i * g.si + k + IFOS_KOFF - 2) * 4u;
     int hx = 0;
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
@@ -371,25 +396,26 @@ __global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F
 // host launchers
 static const int TK = 64, TI = 4;
 
-static Tiling make_tiling(const Box &R, int tk, int ti, int jc)
+static Tiling make_tiling(const Box &R, int tk, int ti, int jc, int ns = 1)
 {
     Tiling T;
     T.ntk = (R.k1 - R.k0 + 1 + tk - 1) / tk;
     T.nti = (R.i1 - R.i0 + 1 + ti - 1) / ti;
     T.ntj = (R.j1 - R.j0 + 1 + jc - 1) / jc;
-    T.total = T.ntk * T.nti * T.ntj;
+    T.ns = ns;
+    T.total = T.ntk * T.nti * T.ntj * ns;
     T.per = (T.total + 7) / 8;
     T.jc = jc;
     return T;
 }
 
-static int pick_jc(const Box &R)
+static int pick_jc(const Box &R, int ns = 1)
 {
     // enough blocks to give every SIMD several waves: aim for >= 4096 tiles
     const int nk = (R.k1 - R.k0 + 1 + TK - 1) / TK, ni = (R.i1 - R.i0 + 1 + TI - 1) / TI;
     const int ny = R.j1 - R.j0 + 1;
     int jc = 64;
-    while (jc > 8 && (int64_t)nk * ni * ((ny + jc - 1) / jc) < 4096) jc /= 2;
+    while (jc > 8 && (int64_t)nk * ni * ((ny + jc - 1) / jc) * ns < 4096) jc /= 2;
     return jc;
 }
 
@@ -465,11 +491,13 @@ extern "C" int ifos_update_v(const ifos_geom *g, int region, const ifos_fields *
     if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
     Tiling T = make_tiling(R, TK, TI, pick_jc(R));
     dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    const Shots one = {1, 0, {0, 0, 0}};
+    const Snap none = {nullptr, 0, 0, 0};
     ProfScope ps_(region == IFOS_REGION_ALL ? IFOS_K_V_FUSED : IFOS_K_V_INTERIOR, s);
     if (region == IFOS_REGION_ALL)
-        hipLaunchKernelGGL((k_v_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi);
+        hipLaunchKernelGGL((k_v_march<TK, TI, true, false>), grid, block, 0, s, G, R, T, F, P, psi, one, none);
     else
-        hipLaunchKernelGGL((k_v_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi);
+        hipLaunchKernelGGL((k_v_march<TK, TI, false, false>), grid, block, 0, s, G, R, T, F, P, psi, one, none);
     return hip_check(hipGetLastError(), "k_v_march");
 }
 
@@ -504,11 +532,12 @@ extern "C" int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *
     if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
     Tiling T = make_tiling(R, TK, TI, pick_jc(R));
     dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    const Shots one = {1, 0, {0, 0, 0}};
     ProfScope ps_(region == IFOS_REGION_ALL ? IFOS_K_S_FUSED : IFOS_K_S_INTERIOR, s);
     if (region == IFOS_REGION_ALL)
-        hipLaunchKernelGGL((k_s_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi);
+        hipLaunchKernelGGL((k_s_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi, one);
     else
-        hipLaunchKernelGGL((k_s_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi);
+        hipLaunchKernelGGL((k_s_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi, one);
     return hip_check(hipGetLastError(), "k_s_march");
 }
 
@@ -531,9 +560,11 @@ extern "C" int ifos_update_v_box(const ifos_geom *g, const int box[6], const ifo
     Psi9 psi = {};
     hipStream_t s = (hipStream_t)stream;
     Tiling T = make_tiling(R, TK, TI, pick_jc(R));
+    const Shots one = {1, 0, {0, 0, 0}};
+    const Snap none = {nullptr, 0, 0, 0};
     ProfScope ps_(IFOS_K_V_INTERIOR, s);
-    hipLaunchKernelGGL((k_v_march<TK, TI, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T, F, P,
-                       psi);
+    hipLaunchKernelGGL((k_v_march<TK, TI, false, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T,
+                       F, P, psi, one, none);
     return hip_check(hipGetLastError(), "k_v_march");
 }
 
@@ -554,8 +585,52 @@ extern "C" int ifos_update_s_box(const ifos_geom *g, const int box[6], const ifo
     Psi9 psi = {};
     hipStream_t s = (hipStream_t)stream;
     Tiling T = make_tiling(R, TK, TI, pick_jc(R));
+    const Shots one = {1, 0, {0, 0, 0}};
     ProfScope ps_(IFOS_K_S_INTERIOR, s);
     hipLaunchKernelGGL((k_s_march<TK, TI, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T, F, P,
-                       psi);
+                       psi, one);
     return hip_check(hipGetLastError(), "k_s_march");
 }
+
+// ---------------------------------------------------------------------------
+// multi-shot fused launches of the batched path
+namespace ifos {
+
+int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
+                   const ifos_psi *psi_s, const Shots &S, const Snap *snap, hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
+    Prof P = {d_prof, 2 * g->fw + 1};
+    Psi9 psi;
+    for (int n = 0; n < 9; ++n) psi.p[n] = psi_s->p[n];
+    Box R = region_box(g, IFOS_REGION_ALL);
+    Tiling T = make_tiling(R, TK, TI, pick_jc(R, S.n), S.n);
+    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    ProfScope ps_(IFOS_K_V_FUSED, s);
+    if (snap)
+        hipLaunchKernelGGL((k_v_march<TK, TI, true, true>), grid, block, 0, s, G, R, T, F, P, psi, S, *snap);
+    else
+        hipLaunchKernelGGL((k_v_march<TK, TI, true, false>), grid, block, 0, s, G, R, T, F, P, psi, S,
+                           Snap{nullptr, 0, 0, 0});
+    return hip_check(hipGetLastError(), "k_v_march (shots)");
+}
+
+int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
+                   const ifos_psi *psi_v, const Shots &S, hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
+                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
+    Prof P = {d_prof, 2 * g->fw + 1};
+    Psi9 psi;
+    for (int n = 0; n < 9; ++n) psi.p[n] = psi_v->p[n];
+    Box R = region_box(g, IFOS_REGION_ALL);
+    Tiling T = make_tiling(R, TK, TI, pick_jc(R, S.n), S.n);
+    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    ProfScope ps_(IFOS_K_S_FUSED, s);
+    hipLaunchKernelGGL((k_s_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi, S);
+    return hip_check(hipGetLastError(), "k_s_march (shots)");
+}
+
+}  // namespace ifos

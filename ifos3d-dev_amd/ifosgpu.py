@@ -69,6 +69,7 @@ def lib() -> C.CDLL:
         _lib.ifos_field_elems.restype = C.c_size_t
         _lib.ifos_offset.restype = C.c_size_t
         _lib.ifos_psi_elems.restype = C.c_size_t
+        _lib.ifos_batch_bytes.restype = C.c_size_t
     return _lib
 
 
@@ -108,6 +109,17 @@ class DBuf:
             check(lib().ifos_dmemset(C.c_void_p(self._p), 0, C.c_size_t(self.nbytes)), "ifos_dmemset")
 
     @classmethod
+    def wrap(cls, ptr, shape, dtype=np.float32):
+        """a view of device memory owned elsewhere (never freed through it)"""
+        b = cls.__new__(cls)
+        b.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        b.dtype = np.dtype(dtype)
+        b.nbytes = int(np.prod(b.shape)) * b.dtype.itemsize
+        b._p = int(ptr)
+        b._owned = False
+        return b
+
+    @classmethod
     def from_numpy(cls, a):
         a = np.ascontiguousarray(a)
         b = cls(a.shape, a.dtype, zero=False)
@@ -141,6 +153,9 @@ class DBuf:
         return self._p + offset_elems * self.dtype.itemsize
 
     def free(self):
+        if not getattr(self, "_owned", True):
+            self._p = None
+            return
         if getattr(self, "_p", None):
             try:
                 lib().ifos_dfree(C.c_void_p(self._p))
@@ -370,4 +385,65 @@ class ShotContext:
     def dft_ptrs(self, back):
         arr = (C.c_void_p * 6)()
         check(lib().ifos_ctx_dft(self.h, back, arr), "ifos_ctx_dft")
+        return list(arr)
+
+
+class BatchContext:
+    """The batched path: several shots of one model per launch (ifos_batch_*)."""
+
+    def __init__(self, g: Geom, nshots: int, nf_max: int, ntr_max: int, precision=DFT_COMPAT, chunk=8):
+        self.g = g
+        self.nshots = nshots
+        self.h = C.c_void_p()
+        check(lib().ifos_batch_create(C.byref(self.h), g.ref, nshots, nf_max, ntr_max, precision, chunk),
+              "ifos_batch_create")
+
+    @staticmethod
+    def bytes_needed(g: Geom, nshots, nf_max, ntr_max, precision=DFT_COMPAT, chunk=8):
+        return int(lib().ifos_batch_bytes(g.ref, nshots, nf_max, ntr_max, precision, chunk))
+
+    def close(self):
+        if self.h:
+            lib().ifos_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, d_rho, d_pi, d_u):
+        check(lib().ifos_batch_set_model(self.h, _ptr(d_rho), _ptr(d_pi), _ptr(d_u)), "ifos_batch_set_model")
+
+    def set_cpml(self, vppml, fpml):
+        check(lib().ifos_batch_set_cpml(self.h, C.c_float(vppml), C.c_float(fpml)), "ifos_batch_set_cpml")
+
+    def set_freqs(self, finv, tast=100):
+        fv = np.asarray(finv, np.float32)
+        check(lib().ifos_batch_set_freqs(self.h, fv.ctypes.data_as(C.c_void_p), len(fv), tast),
+              "ifos_batch_set_freqs")
+
+    def run(self, srcs, rec_ijk, d_syn=None, grads=None, d_obs=None, shape=1, want_adjoint=True, use_graph=False):
+        """srcs: list of (i, j, k, tshift, fc, amp, type), one per shot; rec_ijk:
+        receivers shared by the shots, or a list with one array per shot;
+        d_obs: list of device buffers [3][ntr][nt+1] (one per shot);
+        d_syn: device buffer [ns][3][ntr][nt+1] or None.
+        Returns the per-shot L2 misfits (FP64)."""
+        ns = len(srcs)
+        recs = rec_ijk if isinstance(rec_ijk, (list, tuple)) else [rec_ijk] * ns
+        recs = [np.ascontiguousarray(r, np.int32) for r in recs]
+        shots = (IfosShot * ns)()
+        for n, src in enumerate(srcs):
+            shots[n] = IfosShot(src[0], src[1], src[2], src[6], src[3], src[4], src[5], shape, recs[n].shape[0],
+                                recs[n].ctypes.data, d_obs[n].data_ptr() if d_obs is not None else None,
+                                int(want_adjoint))
+        l2 = (C.c_double * ns)()
+        Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads]) if grads is not None else None
+        check(lib().ifos_batch_run(self.h, shots, ns, _ptr(d_syn), Gp, l2, int(use_graph)), "ifos_batch_run")
+        return list(l2)
+
+    def dft_ptrs(self, shot, back):
+        arr = (C.c_void_p * 6)()
+        check(lib().ifos_batch_dft(self.h, shot, back, arr), "ifos_batch_dft")
         return list(arr)

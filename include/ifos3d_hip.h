@@ -234,6 +234,37 @@ int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn,
 int ifos_ctx_fields(ifos_ctx *c, ifos_fields *f);
 int ifos_ctx_dft(ifos_ctx *c, int back, float *d_F[6]);
 
+/* ---- batched gradient path (no counterpart in the reference: it runs the
+ * ifos3d.c:712-975 shot sequence for several shots of one model at once).
+ * Per step one fused velocity launch and one fused stress launch cover all
+ * shots of the batch; the DFT is accumulated in chunks of `chunk` samples
+ * from a velocity snapshot ring.  Results are bit-identical to
+ * ifos_ctx_run_shot on the same shots in the same order. */
+typedef struct ifos_batch ifos_batch;
+/* device bytes a batch of these dimensions allocates */
+size_t ifos_batch_bytes(const ifos_geom *g, int nshots, int nf_max,
+                        int ntr_max, int dft_precision, int chunk);
+/* nshots: shots per batch; nf_max <= 8; chunk in [1, 16] */
+int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshots,
+                      int nf_max, int ntr_max, int dft_precision, int chunk);
+int ifos_batch_destroy(ifos_batch *b);
+/* model in device layout (rho, pi, u): copied, halos wrapped, averaged */
+int ifos_batch_set_model(ifos_batch *b, const float *d_rho,
+                         const float *d_pi, const float *d_u);
+int ifos_batch_set_cpml(ifos_batch *b, float vppml, float fpml);
+int ifos_batch_set_freqs(ifos_batch *b, const float *finv, int nf, int tast);
+/* runs ns (<= nshots) shots with equal ntr and want_adjoint, synchronously.
+ * d_syn: device [ns][3][ntr][nt+1] or NULL; d_grad[3]: device layout,
+ * accumulated over the shots in order (NULL: no gradient); l2: host [ns]
+ * (per-shot misfit, FP64) or NULL; use_graph: capture the time loop once
+ * into a hipGraph and replay it */
+int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns,
+                   float *d_syn, float *const d_grad[3], double *l2,
+                   int use_graph);
+/* DFT accumulators of one shot of the last batch: frequency l of
+ * component n (Re x y z, Im x y z) at d_F[n] + l * ifos_field_elems(g) */
+int ifos_batch_dft(ifos_batch *b, int shot, int back, float *d_F[6]);
+
 #ifdef __cplusplus
 }
 #endif

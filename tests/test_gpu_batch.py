@@ -1,0 +1,159 @@
+"""The batched path against the plain path, shot for shot.
+
+ifos_batch_run propagates several shots of one model together (fused
+interior+CPML updates over all shots, chunked DFT from a snapshot ring,
+optionally replayed from a hipGraph).  Every number it produces must be
+bit-identical to running the same shots one by one through
+ifos_ctx_run_shot, which is itself bit-identical to the oracle
+(test_gpu_shot.py): seismograms, per-shot misfits, DFT accumulators and the
+gradient accumulated over the shots in order.
+
+The cases put sources and receivers both inside the interior box and inside
+CPML slabs, with two receivers sharing a cell, because the batched path
+reorders the point injections around the fused update (slab points before,
+interior points after) to keep the reference's rounding order.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import max_ulp, small_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _models(g):
+    shp = g.shape
+    J, I, K = np.meshgrid(*(np.arange(s) - 2 for s in shp), indexing="ij")
+    fac = 1.0 + 0.08 * np.exp(-((J - g.ny / 2) ** 2 + (I - g.nx / 3) ** 2 + (K - g.nz / 2) ** 2) / 20.0)
+    vp, vs, rho = 4000.0 * fac, 2300.0 * fac, 2000.0 * fac
+    true = [a.astype(np.float32) for a in (rho, rho * vp * vp, rho * vs * vs)]
+    one = np.ones(shp)
+    start = [a.astype(np.float32) for a in (2000.0 * one, 2000.0 * 4000.0 ** 2 * one, 2000.0 * 2300.0 ** 2 * one)]
+    return true, start
+
+
+def _setup(ig, g, finv, tast, prec):
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt, g.fdcoeff, g.free_surf)
+    return G
+
+
+def _plain(ig, G, g, true, start, srcs, rec, finv, tast, prec):
+    """observed data, seismograms, misfits and accumulated gradient, one shot at a time"""
+    ctx = ig.ShotContext(G, len(finv), len(rec), prec)
+    ctx.set_cpml(4000.0, 50.0)
+    ctx.set_freqs(finv, tast)
+    ntr = len(rec)
+    obs, syn, l2 = [], [], []
+    d_syn = ig.DBuf((3, ntr, g.nt + 1))
+    ctx.set_model(*[G.to_dev(a) for a in true])
+    for src in srcs:
+        ctx.run_shot(src, rec, d_syn, None, want_adjoint=False)
+        obs.append(d_syn.clone())
+    ctx.set_model(*[G.to_dev(a) for a in start])
+    grads = [G.empty() for _ in range(3)]
+    dft_last = None
+    for n, src in enumerate(srcs):
+        l2.append(ctx.run_shot(src, rec, d_syn, grads, d_obs=obs[n]))
+        syn.append(d_syn.numpy())
+    ig.sync()
+    fe = G.field_elems()
+    dft_last = [ig.DBuf.wrap(p, (len(finv) * fe,)).numpy().copy() for p in ctx.dft_ptrs(1)]
+    out = dict(obs=obs, syn=syn, l2=l2, grads=[G.from_dev(t) for t in grads], dft_last=dft_last)
+    ctx.close()
+    return out
+
+
+def _batched(ig, G, g, true, start, srcs, rec, finv, tast, prec, chunk, use_graph, obs=None):
+    ns, ntr = len(srcs), len(rec)
+    b = ig.BatchContext(G, ns, len(finv), ntr, prec, chunk)
+    b.set_cpml(4000.0, 50.0)
+    b.set_freqs(finv, tast)
+    d_syn = ig.DBuf((ns, 3, ntr, g.nt + 1))
+    b.set_model(*[G.to_dev(a) for a in true])
+    b.run(srcs, rec, d_syn=d_syn, want_adjoint=False, use_graph=use_graph)
+    obs_b = d_syn.numpy().copy()
+    if obs is None:
+        obs = [ig.DBuf.from_numpy(obs_b[n]) for n in range(ns)]
+    b.set_model(*[G.to_dev(a) for a in start])
+    grads = [G.empty() for _ in range(3)]
+    l2 = b.run(srcs, rec, d_syn=d_syn, grads=grads, d_obs=obs, use_graph=use_graph)
+    fe = G.field_elems()
+    dft_last = [ig.DBuf.wrap(p, (len(finv) * fe,)).numpy().copy() for p in b.dft_ptrs(ns - 1, 1)]
+    out = dict(obs=obs_b, syn=d_syn.numpy(), l2=l2, grads=[G.from_dev(t) for t in grads], dft_last=dft_last, ctx=b)
+    return out
+
+
+CASES = {
+    # explosive sources in the interior, receivers in the interior and in the y slab, two sharing a cell
+    "interior": dict(free_surf=0, srcs=[(18, 12, 19, 0.0, 50.0, 1.0, 1), (12, 14, 22, 0.0, 45.0, 1.0, 1),
+                                        (24, 13, 15, 0.001, 55.0, 1.0, 1)]),
+    # a vertical force inside the x slab, an explosive source in the z slab
+    "slab_sources": dict(free_surf=0, srcs=[(4, 14, 20, 0.0, 50.0, 1e6, 4), (18, 12, 35, 0.0, 50.0, 1.0, 1)]),
+    "free_surface": dict(free_surf=1, srcs=[(18, 6, 19, 0.0, 50.0, 1.0, 1), (14, 8, 16, 0.0, 40.0, 1.0, 1)]),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("prec_name,chunk,use_graph", [("compat", 4, False), ("compat", 3, True),
+                                                       ("kahan", 5, False)])
+def test_batch_matches_plain(case, prec_name, chunk, use_graph):
+    import ifosgpu as ig
+    c = CASES[case]
+    g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=150, free_surf=c["free_surf"])
+    rec = np.array([[9 + 2 * n, 10, 19] for n in range(8)] + [[13, 10, 19], [20, 5, 21], [30, 4, 33]], np.int32)
+    finv, tast = [40.0, 60.0, 80.0], 100
+    prec = ig.DFT_KAHAN if prec_name == "kahan" else ig.DFT_COMPAT
+    _note(f"batch vs plain: case={case} prec={prec_name} chunk={chunk} graph={use_graph}")
+    true, start = _models(g)
+    G = _setup(ig, g, finv, tast, prec)
+    ref = _plain(ig, G, g, true, start, c["srcs"], rec, finv, tast, prec)
+    out = _batched(ig, G, g, true, start, c["srcs"], rec, finv, tast, prec, chunk, use_graph)
+    for n in range(len(c["srcs"])):
+        assert max_ulp(out["obs"][n], ref["obs"][n].numpy()) == 0, f"observed data, shot {n}"
+        assert max_ulp(out["syn"][n], ref["syn"][n]) == 0, f"synthetics, shot {n}"
+        assert out["l2"][n] == ref["l2"][n], f"misfit, shot {n}"
+    for n in range(6):
+        assert max_ulp(out["dft_last"][n], ref["dft_last"][n]) == 0, f"adjoint DFT component {n}, last shot"
+    for n in range(3):
+        assert max_ulp(out["grads"][n], ref["grads"][n]) == 0, f"gradient {n}"
+    assert np.abs(ref["grads"][0]).max() > 0
+    if use_graph:
+        # replays: the captured loops run again with eager work in between
+        b = out["ctx"]
+        for rep in range(2):
+            ig.DBuf((1 << 20,)).zero_()
+            grads = [G.empty() for _ in range(3)]
+            obs = [ig.DBuf.from_numpy(out["obs"][n]) for n in range(len(c["srcs"]))]
+            l2 = b.run(c["srcs"], rec, grads=grads, d_obs=obs, use_graph=True)
+            assert l2 == out["l2"], f"replay {rep}: misfits"
+            for n in range(3):
+                assert max_ulp(G.from_dev(grads[n]), ref["grads"][n]) == 0, f"replay {rep}: gradient {n}"
+    out["ctx"].close()
+
+
+def test_batch_partial_and_rejects():
+    """a batch may run fewer shots than it was sized for; bad input is refused"""
+    import ifosgpu as ig
+    g = small_grid(nx=30, ny=28, nz=32, fw=6, nt=60)
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt, g.fdcoeff, g.free_surf)
+    rec = np.array([[10, 9, 12], [12, 9, 14]], np.int32)
+    b = ig.BatchContext(G, 4, 2, 2, ig.DFT_COMPAT, 4)
+    b.set_cpml(4000.0, 50.0)
+    b.set_freqs([40.0, 60.0])
+    _, start = _models(g)
+    b.set_model(*[G.to_dev(a) for a in start])
+    d_syn = ig.DBuf((2, 3, 2, g.nt + 1))
+    b.run([(15, 10, 16, 0.0, 50.0, 1.0, 1), (12, 11, 14, 0.0, 50.0, 1.0, 1)], rec, d_syn=d_syn, want_adjoint=False)
+    assert np.abs(d_syn.numpy()).max() > 0
+    with pytest.raises(ig.IfosError):
+        b.run([(15, 10, 16, 0.0, 50.0, 1.0, 1)] * 5, rec, want_adjoint=False)
+    with pytest.raises(ig.IfosError):
+        b.run([(0, 10, 16, 0.0, 50.0, 1.0, 1)], rec, want_adjoint=False)
+    b.close()
