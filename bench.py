@@ -2,12 +2,16 @@
 """Benchmark: gradient shots per second on the 276^3 (256^3 + CPML) elastic
 FWI workload, one MI355X.
 
-A step is one gradient shot: forward NT=1000 steps with the on-the-fly DFT
-(6 frequencies), residual against observed data, adjoint NT steps with the
-DFT, gradient_F.  Inputs (models, observed data) are resident in HBM before
+A gradient shot is: forward NT=1000 steps with the on-the-fly DFT (6
+frequencies), residual against observed data, adjoint NT steps with the
+DFT, gradient_F.  A step of this benchmark is one batch: all 16 shot
+positions propagated together by the batched path (ifos_batch_run).  The
+plain one-shot path (ifos_ctx_run_shot, the drop-in's shape) is timed on one
+shot beside it.  Inputs (models, observed data) are resident in HBM before
 the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad276|toy84]
+                  [--path batched|plain] [--shots-per-batch S] [--chunk C] [--graph]
 """
 from __future__ import annotations
 
@@ -28,6 +32,9 @@ HBM_PEAK_GBS = 8000.0
 CAT_KERNEL = {"v_interior": "ifos::k_v_march<64, 4, false>", "s_interior": "ifos::k_s_march<64, 4, false>",
               "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs", "dft": "ifos::k_dft_nf<{nf}>",
               "dft_kahan": "ifos::k_dft_kahan_nf<{nf}>", "gradient": "ifos::k_gradient"}
+BATCH_KERNEL = {"v_fused": "ifos::k_v_march<64, 4, true, true>", "s_fused": "ifos::k_s_march<64, 4, true>",
+                "dft": "ifos::k_dft_chunk<{nf}, false>", "dft_kahan": "ifos::k_dft_chunk<{nf}, true>",
+                "gradient": "ifos::k_gradient_shots"}
 CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other"]
 
 
@@ -59,81 +66,153 @@ def algorithmic_bytes(w, nf, kahan=False):
     }
 
 
-def run_gpu(w, steps, warmup, precision):
-    import ifosgpu as ig
-    import workloads as wl
+def algorithmic_bytes_batched(w, nf, S, chunk, ntast, kahan=False):
+    """average bytes per launch of the batched path's kernels for a batch of
+    S shots: the fused velocity march moves 48 B per cell and shot plus the
+    shared model (12 B per cell) plus 24 B per active CPML axis and shot, and
+    12 B per cell and shot more at the DFT sampling steps (the snapshot); the
+    fused stress march 60 B per cell and shot plus 20 B of shared model plus
+    the CPML term; a DFT chunk of c snapshots 12*c + 48*nf B per cell and shot
+    (x2 accumulators with Kahan); the gradient 48*nf*S + 36 B per cell"""
+    _, _, axes = slab_stats(w)
+    cells = w.cells
+    nsamp = w.nt // ntast
+    nchunks = (nsamp + chunk - 1) // chunk
+    acc = 96 if kahan else 48
+    v_base = (48 * cells + 24 * axes) * S + 12 * cells
+    v_avg = v_base + 12 * cells * S * nsamp / w.nt
+    s_avg = (60 * cells + 24 * axes) * S + 20 * cells
+    dft_avg = (12 * nsamp + acc * nf * nchunks) * cells * S / nchunks
+    return {"v_fused": v_avg, "s_fused": s_avg, "dft": dft_avg, "gradient": (48 * nf * S + 36) * cells}
 
-    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
-    log(f"grid {w.nx}x{w.ny}x{w.nz} pitch {G.pitch} NT {w.nt} dt {w.dt:.4e} nf {len(w.finv)}")
-    true, start = wl.sphere_models(w) if w.name == "toy84" else wl.random_models(w)
-    d_true = [G.to_dev(a) for a in true]
-    d_start = [G.to_dev(a) for a in start]
-    host_start = start
-    del true
-    ntr = len(w.receivers)
-    ctx = ig.ShotContext(G, len(w.finv), ntr, precision)
-    ctx.set_cpml(w.vppml, w.fpml)
-    ctx.set_freqs(w.finv, w.tast)
-    nshots = warmup + steps
-    shot_ids = [n % len(w.shots) for n in range(nshots)]
-    # observed data: forward runs on the true model (untimed setup)
-    ctx.set_model(*d_true)
-    obs = []
-    d_syn = ig.DBuf((3, ntr, w.nt + 1))
-    for n, s in enumerate(shot_ids):
-        ctx.run_shot(w.shots[s], w.receivers, d_syn, None, want_adjoint=False)
-        obs.append(d_syn.clone())
-        log(f"observed data shot {n + 1}/{nshots}")
-    ctx.set_model(*d_start)
-    grads = [G.empty() for _ in range(3)]
-    ig.sync()
-    for n in range(warmup):
-        ctx.run_shot(w.shots[shot_ids[n]], w.receivers, d_syn, grads, d_obs=obs[n])
-        ig.sync()
-        log(f"warmup shot {n + 1}/{warmup} done")
+
+def _prof_start(ig):
     lib = ig.lib()
-    import ctypes as C
     lib.ifos_prof_reset()
     lib.ifos_prof_enable(1)
-    ig.sync()
-    t0 = time.perf_counter()
-    l2 = 0.0
-    for n in range(warmup, nshots):
-        l2 += ctx.run_shot(w.shots[shot_ids[n]], w.receivers, d_syn, grads, d_obs=obs[n])
-        log(f"timed shot {n - warmup + 1}/{steps}")
-    ig.sync()
-    elapsed = time.perf_counter() - t0
+
+
+def _prof_stop(ig):
+    import ctypes as C
+    lib = ig.lib()
     lib.ifos_prof_enable(0)
     ms = (C.c_double * len(CATS))()
     cnt = (C.c_long * len(CATS))()
     ig.check(lib.ifos_prof_read(ms, cnt, len(CATS)), "ifos_prof_read")
-    prof = {CATS[n]: (ms[n], cnt[n]) for n in range(len(CATS))}
+    lib.ifos_prof_reset()
+    return {CATS[n]: (ms[n], cnt[n]) for n in range(len(CATS))}
+
+
+def setup(w):
+    import ifosgpu as ig
+    import workloads as wl
+    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+    log(f"grid {w.nx}x{w.ny}x{w.nz} pitch {G.pitch} NT {w.nt} dt {w.dt:.4e} nf {len(w.finv)}")
+    true, start = wl.sphere_models(w) if w.name == "toy84" else wl.random_models(w)
+    return G, [G.to_dev(a) for a in true], [G.to_dev(a) for a in start], start
+
+
+def observed_data(w, G, d_true, S, chunk):
+    """forward runs of every shot position on the true model (untimed setup),
+    through the batched path; one device buffer [3][ntr][nt+1] per shot"""
+    import ifosgpu as ig
+    ntr = len(w.receivers)
+    b = ig.BatchContext(G, S, len(w.finv), ntr, ig.DFT_COMPAT, chunk)
+    b.set_cpml(w.vppml, w.fpml)
+    b.set_freqs(w.finv, w.tast)
+    b.set_model(*d_true)
+    obs = []
+    for b0 in range(0, len(w.shots), S):
+        srcs = w.shots[b0:b0 + S]
+        d_syn = ig.DBuf((len(srcs), 3, ntr, w.nt + 1))
+        b.run(srcs, w.receivers, d_syn=d_syn, want_adjoint=False)
+        h = d_syn.numpy()
+        obs += [ig.DBuf.from_numpy(h[n]) for n in range(len(srcs))]
+        d_syn.free()
+        log(f"observed data: shots {b0 + 1}-{b0 + len(srcs)} of {len(w.shots)}")
+    b.close()
+    return obs
+
+
+def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_graph, autotune=True):
+    import ifosgpu as ig
+    ntr = len(w.receivers)
+    log(f"batched path: {S} shots per batch, DFT chunk {chunk}, graph {use_graph}, "
+        f"{ig.BatchContext.bytes_needed(G, S, len(w.finv), ntr, precision, chunk) / 2**30:.1f} GiB")
+    b = ig.BatchContext(G, S, len(w.finv), ntr, precision, chunk)
+    b.set_cpml(w.vppml, w.fpml)
+    b.set_freqs(w.finv, w.tast)
+    b.set_model(*d_start)
+    tune = None
+    if autotune:
+        bv, bs, table = b.autotune(reps=2)
+        tune = {"v": bv, "s": bs, "table": table}
+        log(f"autotune: velocity march {bv}, stress march {bs}")
+    grads = [G.empty() for _ in range(3)]
+    batches = [list(range(b0, min(b0 + S, len(w.shots)))) for b0 in range(0, len(w.shots), S)]
+
+    def one(n):
+        ids = batches[n % len(batches)]
+        return b.run([w.shots[i] for i in ids], w.receivers, grads=grads, d_obs=[obs[i] for i in ids],
+                     use_graph=use_graph), len(ids)
+
+    for n in range(warmup):
+        one(n)
+        log(f"warmup batch {n + 1}/{warmup} done")
+    ig.sync()
+    if not use_graph:
+        _prof_start(ig)
+    t0 = time.perf_counter()
+    l2, nshots = 0.0, 0
+    for n in range(warmup, warmup + steps):
+        l2s, k = one(n)
+        l2 += sum(l2s)
+        nshots += k
+        log(f"timed batch {n - warmup + 1}/{steps}: {k} shots")
+    ig.sync()
+    elapsed = time.perf_counter() - t0
+    prof = _prof_stop(ig) if not use_graph else None
     gnorm = [float(np.linalg.norm(G.from_dev(t).astype(np.float64))) for t in grads]
+    b.close()
+    return dict(elapsed=elapsed, shots=nshots, prof=prof, l2=l2, gnorm=gnorm, tune=tune)
+
+
+def run_plain(w, G, d_start, obs, shots, precision):
+    """the one-shot-per-launch path on the first `shots` shot positions"""
+    import ifosgpu as ig
+    ntr = len(w.receivers)
+    ctx = ig.ShotContext(G, len(w.finv), ntr, precision)
+    ctx.set_cpml(w.vppml, w.fpml)
+    ctx.set_freqs(w.finv, w.tast)
+    ctx.set_model(*d_start)
+    grads = [G.empty() for _ in range(3)]
+    d_syn = ig.DBuf((3, ntr, w.nt + 1))
+    ig.sync()
+    _prof_start(ig)
+    t0 = time.perf_counter()
+    for n in range(shots):
+        ctx.run_shot(w.shots[n], w.receivers, d_syn, grads, d_obs=obs[n])
+        log(f"plain shot {n + 1}/{shots}")
+    ig.sync()
+    elapsed = time.perf_counter() - t0
+    prof = _prof_stop(ig)
     ctx.close()
-    return dict(elapsed=elapsed, prof=prof, l2=l2, gnorm=gnorm, host_start=host_start, pitch=G.pitch)
+    return dict(elapsed=elapsed, shots=shots, prof=prof)
 
 
 def copy_probe(mib=1024, reps=5):
     """plain device copy of a known size: the practical HBM ceiling, and the
     reference kernel the rocprof FETCH_SIZE/WRITE_SIZE counters are scaled by"""
-    import ctypes as C
     import ifosgpu as ig
     n = mib * (1 << 20) // 4
     a = ig.DBuf((n,))
     b = ig.DBuf((n,))
     ig.copy(b, a, n)
-    lib = ig.lib()
-    lib.ifos_prof_reset()
-    lib.ifos_prof_enable(1)
+    _prof_start(ig)
     for _ in range(reps):
         ig.copy(b, a, n)
-    lib.ifos_prof_enable(0)
-    ms_c = (C.c_double * len(CATS))()
-    cnt = (C.c_long * len(CATS))()
-    ig.check(lib.ifos_prof_read(ms_c, cnt, len(CATS)), "ifos_prof_read")
-    other = CATS.index("other")
-    ms = ms_c[other] / max(cnt[other], 1)
-    lib.ifos_prof_reset()
+    prof = _prof_stop(ig)
+    ms = prof["other"][0] / max(prof["other"][1], 1)
     a.free()
     b.free()
     return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
@@ -200,12 +279,34 @@ def cpu_baseline(w, host_model, fwd_steps=4, adj_steps=4):
                        f"per shot (gradient_F, one pass per shot, left out)")}
 
 
+def roofline_of(prof, ab):
+    """the dominant kernel category by total time, its average launch and
+    achieved algorithmic GB/s"""
+    dom = max((k for k in ab if prof[k][1] > 0), key=lambda k: prof[k][0])
+    avg_ms = prof[dom][0] / prof[dom][1]
+    return dom, avg_ms, ab[dom] / (avg_ms * 1e-3) / 1e9
+
+
+def kernel_table(prof, ab):
+    return {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
+                "avg_ms": round(prof[k][0] / prof[k][1], 5) if prof[k][1] else None,
+                "GBps": round(ab[k] / (prof[k][0] / prof[k][1] * 1e-3) / 1e9, 1)
+                if prof[k][1] and k in ab else None}
+            for k in CATS if prof[k][1]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1, help="timed batches (batched) or shots (plain)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="grad276", choices=["grad276", "toy84"])
+    ap.add_argument("--path", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--shots-per-batch", type=int, default=16)
+    ap.add_argument("--chunk", type=int, default=8, help="DFT samples per accumulator pass")
+    ap.add_argument("--graph", action="store_true", help="replay the time loops from hipGraphs")
+    ap.add_argument("--no-autotune", action="store_true", help="default tile shapes instead of the autotuner's")
+    ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
     ap.add_argument("--precision", default="compat", choices=["compat", "kahan"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -225,41 +326,69 @@ def main():
     log(f"copy probe: {copy}")
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
     prec = ig.DFT_KAHAN if args.precision == "kahan" else ig.DFT_COMPAT
-    res = run_gpu(w, args.steps, args.warmup, prec)
-    el = res["elapsed"]
-    value = args.steps / el
+    kahan = prec == ig.DFT_KAHAN
     nf = len(w.finv)
-    ab = algorithmic_bytes(w, nf, kahan=(prec == ig.DFT_KAHAN))
-    prof = res["prof"]
-    # dominant kernel by total time
-    dom = max((k for k in ab if prof[k][1] > 0), key=lambda k: prof[k][0])
-    avg_ms = prof[dom][0] / prof[dom][1]
-    achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
-    # HBM traffic of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE summaries (scaled by the copy kernel), when they cover it
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{w.name}_plain_traffic.txt")
-    kname = CAT_KERNEL.get("dft_kahan" if dom == "dft" and prec == ig.DFT_KAHAN else dom, "").format(nf=nf)
-    if os.path.exists(tpath) and kname:
-        import profsum
-        traffic = profsum.read_traffic(tpath).get(kname)
-    stencil_cats = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused"]
-    st_bytes = sum(ab[k] * prof[k][1] for k in stencil_cats)
-    st_ms = sum(prof[k][0] for k in stencil_cats)
-    hbm_frac_stencils = (st_bytes / (st_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if st_ms > 0 else None
-    gcells = w.cells * 2 * w.nt * args.steps / el / 1e9
-    kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
-                   "avg_ms": round(prof[k][0] / prof[k][1], 5) if prof[k][1] else None,
-                   "GBps": round(ab[k] / (prof[k][0] / prof[k][1] * 1e-3) / 1e9, 1)
-                   if prof[k][1] and k in ab else None}
-               for k in CATS if prof[k][1]}
+    S = min(args.shots_per_batch, len(w.shots))
+    ntast = ig.ntast(w.finv[-1], w.tast, w.dt)
+    G, d_true, d_start, host_start = setup(w)
+    obs = observed_data(w, G, d_true, S, args.chunk)
+    del d_true
+
+    plain = None
+    if args.path == "plain" or args.plain_shots > 0:
+        n_plain = args.steps if args.path == "plain" else args.plain_shots
+        plain = run_plain(w, G, d_start, obs, n_plain, prec)
+        plain["value"] = plain["shots"] / plain["elapsed"]
+        log(f"plain path: {plain['value']:.4f} shots/s")
+    batched = None
+    if args.path == "batched":
+        batched = run_batched(w, G, d_start, obs, args.steps, args.warmup, prec, S, args.chunk, args.graph,
+                              not args.no_autotune)
+        batched["value"] = batched["shots"] / batched["elapsed"]
+        log(f"batched path: {batched['value']:.4f} shots/s")
+
+    res = batched if args.path == "batched" else plain
+    el, value = res["elapsed"], res["value"]
+    if args.path == "batched":
+        ab = algorithmic_bytes_batched(w, nf, S, args.chunk, ntast, kahan)
+    else:
+        ab = algorithmic_bytes(w, nf, kahan)
+    roof = None
+    kernels = None
+    if res["prof"] is not None:
+        dom, avg_ms, achieved = roofline_of(res["prof"], ab)
+        kernels = kernel_table(res["prof"], ab)
+        path_tag = "batched" if args.path == "batched" else "plain"
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"{w.name}_{path_tag}_traffic.txt")
+        kname = (BATCH_KERNEL if args.path == "batched" else CAT_KERNEL).get(
+            "dft_kahan" if dom == "dft" and kahan else dom, "").format(nf=nf)
+        if os.path.exists(tpath) and kname:
+            import profsum
+            traffic = profsum.read_traffic(tpath).get(kname)
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": os.path.relpath(tpath, ROOT) if traffic else None,
+                "algorithmic_bytes_per_launch": int(ab[dom]), "avg_launch_ms": round(avg_ms, 5),
+                "rocprof_kernel": kname or None}
+        st_cats = [k for k in ("v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused") if k in ab]
+        st_bytes = sum(ab[k] * res["prof"][k][1] for k in st_cats)
+        st_ms = sum(res["prof"][k][0] for k in st_cats)
+        hbm_frac_stencils = (st_bytes / (st_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if st_ms > 0 else None
+    else:
+        hbm_frac_stencils = None
+    gcells = w.cells * 2 * w.nt * res["shots"] / el / 1e9
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (reference routines, one core, bounded sample)")
         try:
-            cpu = cpu_baseline(w, res["host_start"])
+            cpu = cpu_baseline(w, host_start)
         except Exception as e:  # the baseline is reported, never the product
             log(f"CPU baseline failed: {e}")
+    path_desc = (f"batched: {S} shots per launch, fused interior+CPML marches, DFT in chunks of {args.chunk} "
+                 f"samples, {'hipGraph replay' if args.graph else 'direct launches'}"
+                 if args.path == "batched" else
+                 "plain (one kernel per reference routine per step, one shot per launch)")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -277,16 +406,17 @@ def main():
                                 f"elastic FP32 4th order, dh {w.dh} m, dt {w.dt:.4e} s, NT {w.nt}, "
                                 f"{len(w.shots)} shot positions, {len(w.receivers)} receivers, "
                                 f"{nf} DFT frequencies {w.finv[0]}-{w.finv[-1]} Hz, TAST {w.tast}"),
-                   "path": "plain (one kernel per reference routine per step, one shot per launch)",
-                   "dft_precision": args.precision, "shots_per_step": 1},
+                   "path": path_desc, "dft_precision": args.precision,
+                   "shots_per_step": res["shots"] // args.steps},
         "gcell_updates_per_s": round(gcells, 3),
         "hbm_frac_stencils": round(hbm_frac_stencils, 4) if hbm_frac_stencils else None,
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": os.path.relpath(tpath, ROOT) if traffic else None,
-                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 5),
-                     "rocprof_kernel": kname or None},
+        "plain_shots_per_s": round(plain["value"], 4) if plain else None,
+        "batched_over_plain": round(batched["value"] / plain["value"], 3) if (batched and plain) else None,
+        "roofline": roof,
         "kernels": kernels,
+        "plain_kernels": kernel_table(plain["prof"], algorithmic_bytes(w, nf, kahan))
+        if (plain and args.path == "batched") else None,
+        "autotune": batched.get("tune") if batched else None,
         "copy_probe": copy,
         "cpu_baseline": cpu,
     }

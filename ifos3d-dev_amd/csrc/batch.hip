@@ -285,6 +285,7 @@ struct ifos_batch {
     int nf, tast, ntast, nsamp;
     int ntr;                   // receivers of the current batch
     bool any_slab_src, any_slab_rec;
+    MarchCfg cfg_v, cfg_s;     // tile shapes of the two march launches
     hipStream_t st;
     std::vector<void *> allocs;
     std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
@@ -394,6 +395,7 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->nf = 0;
     b->tast = 100;
     b->ntr = 0;
+    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 1, 0};
     *out = b;
     return IFOS_OK;
 }
@@ -562,7 +564,7 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
                                nt, nt2, 1, nosnap);
             rc = hip_check(hipGetLastError(), "k_badjoint");
         }
-        if (!rc) rc = launch_v_shots(g, &f, &m, b->prof, &ps, S, samp ? &N : nullptr, st);
+        if (!rc) rc = launch_v_shots(g, &f, &m, b->prof, &ps, S, samp ? &N : nullptr, b->cfg_v, st);
         if (!rc && !back) {
             ProfScope ps_(IFOS_K_SOURCES, st);
             hipLaunchKernelGGL(k_bforces, dim3(ns), dim3(64), 0, st, G, BF, m.rip, m.rjp, m.rkp, b->src, b->srccls,
@@ -577,7 +579,7 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
                                nt, nt2, 2, N);
             rc = hip_check(hipGetLastError(), "k_badjoint");
         }
-        if (!rc) rc = launch_s_shots(g, &f, &m, b->prof, &pv, S, st);
+        if (!rc) rc = launch_s_shots(g, &f, &m, b->prof, &pv, S, b->cfg_s, st);
         if (!rc && !back) {
             ProfScope ps_(IFOS_K_SOURCES, st);
             hipLaunchKernelGGL(k_bpsource, dim3(ns), dim3(64), 0, st, G, BF, b->src, b->sig, 1, nt, g->nt, nt2);
@@ -786,6 +788,94 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
     if (d_grad) rc = launch_gradient_shots(g, Ff, Fb, (int64_t)fshot, ns, &m, b->finv, b->nf, d_grad, st);
     if (!rc) rc = hip_check(hipStreamSynchronize(st), "batch gradient");
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: times each candidate tile shape of the two march launches on
+// the batch's own buffers (ns shots, `reps` launches each) and keeps the
+// fastest.  The wavefields are zeroed afterwards.
+static const MarchCfg k_candidates[] = {
+    {4, 0, 1, 0},  {4, 0, 1, 1},  {4, 0, 4, 1},  {4, 0, 16, 1}, {4, 0, 0, 1},  {2, 0, 1, 1},
+    {2, 0, 8, 1},  {8, 0, 1, 1},  {8, 0, 4, 1},  {4, 32, 1, 1}, {4, 16, 4, 1}, {1, 0, 16, 1},
+};
+
+extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v, int *best_s, float *ms_v,
+                                   float *ms_s)
+{
+    if (!b || ns < 1 || ns > b->S || reps < 1) return IFOS_EINVAL;
+    const int nc = (int)(sizeof(k_candidates) / sizeof(k_candidates[0]));
+    hipStream_t st = b->st;
+    const ifos_geom *g = &b->g;
+    ifos_fields f = bfields(b);
+    ifos_model m = bmodel(b);
+    ifos_psi ps, pv;
+    for (int n = 0; n < 9; ++n) {
+        ps.p[n] = b->psi_s[n];
+        pv.p[n] = b->psi_v[n];
+    }
+    const Shots S = {ns, (int64_t)b->fe, {b->pe[0], b->pe[1], b->pe[2]}};
+    hipEvent_t e0, e1;
+    int rc = hip_check(hipEventCreate(&e0), "event");
+    if (!rc) rc = hip_check(hipEventCreate(&e1), "event");
+    if (rc) return rc;
+    float bv = 1e30f, bs = 1e30f;
+    int iv = 0, is = 0;
+    for (int c = 0; c < nc && !rc; ++c) {
+        for (int kind = 0; kind < 2 && !rc; ++kind) {
+            // one untimed launch, then reps timed
+            for (int r = 0; r <= reps && !rc; ++r) {
+                if (r == 1) rc = hip_check(hipEventRecord(e0, st), "record");
+                if (!rc)
+                    rc = kind == 0 ? launch_v_shots(g, &f, &m, b->prof, &ps, S, nullptr, k_candidates[c], st)
+                                   : launch_s_shots(g, &f, &m, b->prof, &pv, S, k_candidates[c], st);
+            }
+            if (!rc) rc = hip_check(hipEventRecord(e1, st), "record");
+            if (!rc) rc = hip_check(hipEventSynchronize(e1), "sync");
+            float ms = 0.0f;
+            if (!rc) rc = hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            ms /= reps;
+            if (ms_v && kind == 0) ms_v[c] = ms;
+            if (ms_s && kind == 1) ms_s[c] = ms;
+            if (kind == 0 && ms < bv) { bv = ms; iv = c; }
+            if (kind == 1 && ms < bs) { bs = ms; is = c; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->cfg_v = k_candidates[iv];
+    b->cfg_s = k_candidates[is];
+    if (best_v) *best_v = iv;
+    if (best_s) *best_s = is;
+    rc = zero_state(b, b->S, st);
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "autotune");
+    return rc;
+}
+
+extern "C" int ifos_batch_tune_candidates(int *cfg, int max)
+{
+    const int nc = (int)(sizeof(k_candidates) / sizeof(k_candidates[0]));
+    for (int c = 0; c < nc && c < max; ++c) {
+        cfg[4 * c] = k_candidates[c].ti;
+        cfg[4 * c + 1] = k_candidates[c].jc;
+        cfg[4 * c + 2] = k_candidates[c].group;
+        cfg[4 * c + 3] = k_candidates[c].split;
+    }
+    return nc;
+}
+
+extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[4], const int cfg_s[4])
+{
+    if (!b) return IFOS_EINVAL;
+    for (const int *c : {cfg_v, cfg_s})
+        if (c && c[0] != 1 && c[0] != 2 && c[0] != 4 && c[0] != 8) {
+            set_error("ifos_batch_set_tiles: TI must be 1, 2, 4 or 8");
+            return IFOS_EINVAL;
+        }
+    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3]};
+    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3]};
+    drop_graphs(b);
+    return IFOS_OK;
 }
 
 // DFT accumulators of one shot of the last batch (device pointers into the

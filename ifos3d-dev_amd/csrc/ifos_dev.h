@@ -92,14 +92,18 @@ struct Snap {
     uint32_t soff;
 };
 
-// Index box (inclusive), and a list of them for the slab launches.
+// Index box (inclusive).
 struct Box {
     int i0, i1, j0, j1, k0, k1;
 };
-struct BoxList {
-    Box b[6];
-    int64_t start[7];  // cumulative cell counts
-    int n;
+
+// Tile shape of a march launch: TI i-rows per block (1, 2, 4, 8), jc
+// j-planes per tile (<= 0: automatic), `group` consecutive tiles per shot
+// before the next shot (1: shots fastest, <= 0: one shot after the other),
+// split: interior box and CPML slabs in separate launches.
+struct MarchCfg {
+    int ti, jc, group;
+    int split;
 };
 
 // Raw buffer access: one 32-bit byte offset per thread, the neighbour
@@ -126,9 +130,9 @@ DevGeom make_dev_geom(const ifos_geom *g);
 // multi-shot fused (interior + CPML) updates over [1..N]^3 for the batched
 // path (stencil.hip): f and psi point at shot 0; snap may be null
 int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
-                   const ifos_psi *psi, const Shots &S, const Snap *snap, hipStream_t s);
+                   const ifos_psi *psi, const Shots &S, const Snap *snap, const MarchCfg &c, hipStream_t s);
 int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
-                   const ifos_psi *psi, const Shots &S, hipStream_t s);
+                   const ifos_psi *psi, const Shots &S, const MarchCfg &c, hipStream_t s);
 // gradient_F of ns shots in shot order into d_grad (aux.hip); shot s's DFT
 // fields at d_F*[n] + s*fshot
 int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,

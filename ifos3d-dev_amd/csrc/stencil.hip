@@ -24,6 +24,8 @@
 // Arithmetic follows the reference expression by expression
 // (update_v_ssg.c:106-125, update_v_ssg_CPML.c:73-100,
 //  update_s_ssg_elastic.c:106-138, update_s_ssg_CPML_elastic.c:63-140).
+#include <algorithm>
+
 #include "ifos_dev.h"
 
 namespace ifos {
@@ -125,25 +127,46 @@ __device__ inline void s_apply(const DevGeom &g, const SFields &F, int64_t c, fl
 }
 
 // ---------------------------------------------------------------------------
-// tile decode with XCD-aware renumbering: physical block b runs on XCD b % 8;
-// logical tile L = (b % 8) * per + b / 8 gives each XCD a contiguous range.
-struct Tiling {
-    int ntk, nti, ntj, total, per;
-    int jc;
-    int ns;  // shots: total = ntk * nti * ntj * ns
+// Tiles.  A launch covers up to six boxes (the interior box, the CPML slabs,
+// or all of [1..N]^3), each cut into tiles of kw k-columns x (TI * rpw)
+// i-rows x jc j-planes: a wavefront holds rpw = 64 / kw rows of kw lanes, so
+// a thin box (a 10-cell z slab) still fills most lanes.  Logical tile L,
+// for ns shots in groups of `group` consecutive tiles per shot, is
+// renumbered XCD-aware: physical block b runs on XCD b % 8, and
+// L = (b % 8) * per + b / 8 gives each XCD a contiguous range.
+struct MTiles {
+    Box b[6];
+    int kw[6], rpw[6], ntk[6], nti[6];
+    int start[7];  // cumulative tile counts per box
+    int nb, jc, ns, group, total, per;
 };
 
-__device__ inline bool decode_tile(const Tiling &T, int &tk, int &ti, int &tj, int &shot)
+__device__ inline bool decode_tile(const MTiles &T, int TI, int &k, int &i, int &jb, int &je, int &shot,
+                                   Box &R)
 {
-    int b = blockIdx.x;
-    int L = (b & 7) * T.per + (b >> 3);
+    const int b = blockIdx.x;
+    const int L = (b & 7) * T.per + (b >> 3);
     if (L >= T.total) return false;
-    shot = L % T.ns;
-    L /= T.ns;
-    tk = L % T.ntk;
-    int r = L / T.ntk;
-    ti = r % T.nti;
-    tj = r / T.nti;
+    const int gs = T.group * T.ns;
+    const int rem = L % gs;
+    shot = rem / T.group;
+    int t = (L / gs) * T.group + rem % T.group;
+    if (t >= T.start[T.nb]) return false;
+    int q = 0;
+    while (q + 1 < T.nb && t >= T.start[q + 1]) ++q;
+    t -= T.start[q];
+    R = T.b[q];
+    const int kw = T.kw[q], rpw = T.rpw[q];
+    const int dk = (int)threadIdx.x % kw, di = (int)threadIdx.x / kw;
+    if (di >= rpw) return false;
+    const int tk = t % T.ntk[q];
+    t /= T.ntk[q];
+    const int ti = t % T.nti[q], tj = t / T.nti[q];
+    k = R.k0 + tk * kw + dk;
+    i = R.i0 + (ti * TI + (int)threadIdx.y) * rpw + di;
+    if (k > R.k1 || i > R.i1) return false;
+    jb = R.j0 + tj * T.jc;
+    je = min(jb + T.jc - 1, R.j1);
     return true;
 }
 
@@ -160,17 +183,13 @@ __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
 #define LD(r, dj, di, dk) bload(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
 #define ST(v, r) bstore(v, r, vo + 8, 2 * SJ + 2 * SI)
 
-template <int TK, int TI, bool CPML, bool SNAP>
-__global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, VFields F, Prof P, Psi9 psi,
-                                                   Shots S, Snap N)
+template <int TI, bool CPML, bool SNAP>
+__global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VFields F, Prof P, Psi9 psi, Shots S,
+                                                    Snap N)
 {
-    int tk, ti, tj, shot;
-    if (!decode_tile(T, tk, ti, tj, shot)) return;
-    const int k = R.k0 + tk * TK + (int)threadIdx.x;
-    const int i = R.i0 + ti * TI + (int)threadIdx.y;
-    if (k > R.k1 || i > R.i1) return;
-    const int jb = R.j0 + tj * T.jc;
-    const int je = min(jb + T.jc - 1, R.j1);
+    int k, i, jb, je, shot;
+    Box R;
+    if (!decode_tile(T, TI, k, i, jb, je, shot, R)) return;
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const int64_t so = (int64_t)shot * S.fstride;
@@ -241,17 +260,12 @@ __global__ __launch_bounds__(TK *TI) void k_v_march(DevGeom g, Box R, Tiling T, 
     }
 }
 
-template <int TK, int TI, bool CPML>
-__global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, SFields F, Prof P, Psi9 psi,
-                                                   Shots S)
+template <int TI, bool CPML>
+__global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S)
 {
-    int tk, ti, tj, shot;
-    if (!decode_tile(T, tk, ti, tj, shot)) return;
-    const int k = R.k0 + tk * TK + (int)threadIdx.x;
-    const int i = R.i0 + ti * TI + (int)threadIdx.y;
-    if (k > R.k1 || i > R.i1) return;
-    const int jb = R.j0 + tj * T.jc;
-    const int je = min(jb + T.jc - 1, R.j1);
+    int k, i, jb, je, shot;
+    Box R;
+    if (!decode_tile(T, TI, k, i, jb, je, shot, R)) return;
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
@@ -327,104 +341,15 @@ __global__ __launch_bounds__(TK *TI) void k_s_march(DevGeom g, Box R, Tiling T, 
 #undef ST
 
 // ---------------------------------------------------------------------------
-// slab kernels: one thread per CPML cell, k fastest within each box
-__device__ inline bool slab_cell(const BoxList &L, int64_t t, int &j, int &i, int &k)
-{
-    if (t >= L.start[L.n]) return false;
-    int b = 0;
-    while (t >= L.start[b + 1]) ++b;
-    const Box &B = L.b[b];
-    int64_t r = t - L.start[b];
-    const int nk = B.k1 - B.k0 + 1, ni = B.i1 - B.i0 + 1;
-    k = B.k0 + (int)(r % nk);
-    r /= nk;
-    i = B.i0 + (int)(r % ni);
-    j = B.j0 + (int)(r / ni);
-    return true;
-}
-
-__global__ __launch_bounds__(256) void k_v_slabs(DevGeom g, BoxList L, VFields F, Prof P, Psi9 psi)
-{
-    int j, i, k;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!slab_cell(L, t, j, i, k)) return;
-    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
-    const float b1 = g.b1, b2 = g.b2;
-    float sxx_x = g.dtdx * D4F(b1, b2, F.sxx[c - si], F.sxx[c], F.sxx[c + si], F.sxx[c + 2 * si]);
-    float sxy_x = g.dtdx * D4B(b1, b2, F.sxy[c - 2 * si], F.sxy[c - si], F.sxy[c], F.sxy[c + si]);
-    float sxz_x = g.dtdx * D4B(b1, b2, F.sxz[c - 2 * si], F.sxz[c - si], F.sxz[c], F.sxz[c + si]);
-    float sxy_y = g.dtdy * D4B(b1, b2, F.sxy[c - 2 * sj], F.sxy[c - sj], F.sxy[c], F.sxy[c + sj]);
-    float syy_y = g.dtdy * D4F(b1, b2, F.syy[c - sj], F.syy[c], F.syy[c + sj], F.syy[c + 2 * sj]);
-    float syz_y = g.dtdy * D4B(b1, b2, F.syz[c - 2 * sj], F.syz[c - sj], F.syz[c], F.syz[c + sj]);
-    float sxz_z = g.dtdz * D4B(b1, b2, F.sxz[c - 2], F.sxz[c - 1], F.sxz[c], F.sxz[c + 1]);
-    float syz_z = g.dtdz * D4B(b1, b2, F.syz[c - 2], F.syz[c - 1], F.syz[c], F.syz[c + 1]);
-    float szz_z = g.dtdz * D4F(b1, b2, F.szz[c - 1], F.szz[c], F.szz[c + 1], F.szz[c + 2]);
-    const int hx = slab_h(i, g.nx, g.fw, 1);
-    const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
-    const int hz = slab_h(k, g.nz, g.fw, 1);
-    v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z, szz_z);
-    F.vx[c] += (sxx_x + sxy_y + sxz_z) / F.rip[c];
-    F.vy[c] += (syy_y + sxy_x + syz_z) / F.rjp[c];
-    F.vz[c] += (szz_z + sxz_x + syz_y) / F.rkp[c];
-}
-
-__global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F, Prof P, Psi9 psi)
-{
-    int j, i, k;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!slab_cell(L, t, j, i, k)) return;
-    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
-    const float b1 = g.b1, b2 = g.b2;
-    const float DX = g.dx, DY = g.dy, DZ = g.dz;
-    float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], F.vx[c], F.vx[c + si]) / DX;
-    float vxy = D4F(b1, b2, F.vx[c - sj], F.vx[c], F.vx[c + sj], F.vx[c + 2 * sj]) / DY;
-    float vxz = D4F(b1, b2, F.vx[c - 1], F.vx[c], F.vx[c + 1], F.vx[c + 2]) / DZ;
-    float vyx = D4F(b1, b2, F.vy[c - si], F.vy[c], F.vy[c + si], F.vy[c + 2 * si]) / DX;
-    float vyy = D4B(b1, b2, F.vy[c - 2 * sj], F.vy[c - sj], F.vy[c], F.vy[c + sj]) / DY;
-    float vyz = D4F(b1, b2, F.vy[c - 1], F.vy[c], F.vy[c + 1], F.vy[c + 2]) / DZ;
-    float vzx = D4F(b1, b2, F.vz[c - si], F.vz[c], F.vz[c + si], F.vz[c + 2 * si]) / DX;
-    float vzy = D4F(b1, b2, F.vz[c - sj], F.vz[c], F.vz[c + sj], F.vz[c + 2 * sj]) / DY;
-    float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], F.vz[c], F.vz[c + 1]) / DZ;
-    const int hx = slab_h(i, g.nx, g.fw, 1);
-    const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
-    const int hz = slab_h(k, g.nz, g.fw, 1);
-    s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
-    s_apply(g, F, c, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
-}
-
-// ---------------------------------------------------------------------------
 // host launchers
-static const int TK = 64, TI = 4;
 
-static Tiling make_tiling(const Box &R, int tk, int ti, int jc, int ns = 1)
+// the CPML slabs as boxes (update_v_ssg_CPML.c:70-375 loop bounds); the
+// y-low slab is absent with a free surface
+static int slab_boxes(const ifos_geom *g, Box *out)
 {
-    Tiling T;
-    T.ntk = (R.k1 - R.k0 + 1 + tk - 1) / tk;
-    T.nti = (R.i1 - R.i0 + 1 + ti - 1) / ti;
-    T.ntj = (R.j1 - R.j0 + 1 + jc - 1) / jc;
-    T.ns = ns;
-    T.total = T.ntk * T.nti * T.ntj * ns;
-    T.per = (T.total + 7) / 8;
-    T.jc = jc;
-    return T;
-}
-
-static int pick_jc(const Box &R, int ns = 1)
-{
-    // enough blocks to give every SIMD several waves: aim for >= 4096 tiles
-    const int nk = (R.k1 - R.k0 + 1 + TK - 1) / TK, ni = (R.i1 - R.i0 + 1 + TI - 1) / TI;
-    const int ny = R.j1 - R.j0 + 1;
-    int jc = 64;
-    while (jc > 8 && (int64_t)nk * ni * ((ny + jc - 1) / jc) * ns < 4096) jc /= 2;
-    return jc;
-}
-
-static BoxList slab_boxes(const ifos_geom *g)
-{
-    BoxList L;
     const int fw = g->fw, nx = g->nx, ny = g->ny, nz = g->nz;
     const int xb0 = fw + 1, xb1 = nx - fw, yb0 = g->free_surf ? 1 : fw + 1, yb1 = ny - fw;
-    Box all[6] = {
+    const Box all[6] = {
         {1, fw, 1, ny, 1, nz},                 // x low   (update_v_ssg_CPML.c:70)
         {xb1 + 1, nx, 1, ny, 1, nz},           // x high  (:142)
         {xb0, xb1, 1, fw, 1, nz},              // y low   (:215, no free surface)
@@ -432,18 +357,14 @@ static BoxList slab_boxes(const ifos_geom *g)
         {xb0, xb1, yb0, yb1, 1, fw},           // z low   (:320)
         {xb0, xb1, yb0, yb1, nz - fw + 1, nz}, // z high  (:352)
     };
-    L.n = 0;
-    L.start[0] = 0;
+    int n = 0;
     for (int b = 0; b < 6; ++b) {
         if (b == 2 && g->free_surf) continue;
         const Box &B = all[b];
         if (B.i1 < B.i0 || B.j1 < B.j0 || B.k1 < B.k0) continue;
-        L.b[L.n] = B;
-        L.start[L.n + 1] = L.start[L.n] + (int64_t)(B.i1 - B.i0 + 1) * (B.j1 - B.j0 + 1) * (B.k1 - B.k0 + 1);
-        ++L.n;
+        out[n++] = B;
     }
-    for (int b = L.n + 1; b < 7; ++b) L.start[b] = L.start[L.n];
-    return L;
+    return n;
 }
 
 static Box region_box(const ifos_geom *g, int region)
@@ -455,6 +376,150 @@ static Box region_box(const ifos_geom *g, int region)
     }
     Box B = {1, g->nx, 1, g->ny, 1, g->nz};
     return B;
+}
+
+// tiles of boxes for TI i-rows per block; jc <= 0 picks the j chunk so a
+// launch has >= 4096 workgroups (16 per CU) when the boxes allow it
+static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int group)
+{
+    MTiles T = {};
+    T.nb = 0;
+    T.ns = ns;
+    T.start[0] = 0;
+    for (int q = 0; q < nb; ++q) {
+        const Box &B = boxes[q];
+        if (B.i1 < B.i0 || B.j1 < B.j0 || B.k1 < B.k0) continue;
+        const int nk = B.k1 - B.k0 + 1, ni = B.i1 - B.i0 + 1;
+        const int kw = nk >= 64 ? 64 : nk, rpw = 64 / kw;
+        T.b[T.nb] = B;
+        T.kw[T.nb] = kw;
+        T.rpw[T.nb] = rpw;
+        T.ntk[T.nb] = (nk + kw - 1) / kw;
+        T.nti[T.nb] = (ni + ti * rpw - 1) / (ti * rpw);
+        ++T.nb;
+    }
+    auto count = [&](int c) {
+        int64_t tot = 0;
+        for (int q = 0; q < T.nb; ++q)
+            tot += (int64_t)T.ntk[q] * T.nti[q] * ((T.b[q].j1 - T.b[q].j0 + 1 + c - 1) / c);
+        return tot;
+    };
+    if (jc <= 0) {
+        jc = 64;
+        while (jc > 8 && count(jc) * ns < 4096) jc /= 2;
+    }
+    T.jc = jc;
+    for (int q = 0; q < T.nb; ++q)
+        T.start[q + 1] = T.start[q] + T.ntk[q] * T.nti[q] * ((T.b[q].j1 - T.b[q].j0 + 1 + jc - 1) / jc);
+    for (int q = T.nb + 1; q < 7; ++q) T.start[q] = T.start[T.nb];
+    const int tiles = T.start[T.nb];
+    T.group = group <= 0 ? tiles : std::min(group, tiles);
+    if (T.group < 1) T.group = 1;
+    T.total = ((tiles + T.group - 1) / T.group) * T.group * ns;
+    T.per = (T.total + 7) / 8;
+    return T;
+}
+
+// default tile shape of the march kernels (i-rows per block, j chunk,
+// shot grouping); the batched path's autotuner overrides it per kernel
+static MarchCfg g_default_cfg = {4, 0, 0, 0};
+
+template <bool CPML, bool SNAP>
+static int launch_v(const MarchCfg &c, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
+                    const Psi9 &psi, const Shots &S, const Snap &N, hipStream_t s)
+{
+    if (T.total == 0) return IFOS_OK;
+    const dim3 grid((unsigned)(T.per * 8));
+    switch (c.ti) {
+    case 1: hipLaunchKernelGGL((k_v_march<1, CPML, SNAP>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, N); break;
+    case 2: hipLaunchKernelGGL((k_v_march<2, CPML, SNAP>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, N); break;
+    case 4: hipLaunchKernelGGL((k_v_march<4, CPML, SNAP>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, N); break;
+    case 8: hipLaunchKernelGGL((k_v_march<8, CPML, SNAP>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, N); break;
+    default: set_error("march: TI=%d not in {1,2,4,8}", c.ti); return IFOS_EINVAL;
+    }
+    return hip_check(hipGetLastError(), "k_v_march");
+}
+
+template <bool CPML>
+static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
+                    const Psi9 &psi, const Shots &S, hipStream_t s)
+{
+    if (T.total == 0) return IFOS_OK;
+    const dim3 grid((unsigned)(T.per * 8));
+    switch (c.ti) {
+    case 1: hipLaunchKernelGGL((k_s_march<1, CPML>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S); break;
+    case 2: hipLaunchKernelGGL((k_s_march<2, CPML>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S); break;
+    case 4: hipLaunchKernelGGL((k_s_march<4, CPML>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S); break;
+    case 8: hipLaunchKernelGGL((k_s_march<8, CPML>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S); break;
+    default: set_error("march: TI=%d not in {1,2,4,8}", c.ti); return IFOS_EINVAL;
+    }
+    return hip_check(hipGetLastError(), "k_s_march");
+}
+
+static VFields vfields(const ifos_fields *f, const ifos_model *m)
+{
+    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
+    return F;
+}
+static SFields sfields(const ifos_fields *f, const ifos_model *m)
+{
+    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
+                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
+    return F;
+}
+static Psi9 psi9(const ifos_psi *p)
+{
+    Psi9 q = {};
+    if (p)
+        for (int n = 0; n < 9; ++n) q.p[n] = p->p[n];
+    return q;
+}
+
+// one shot, one region: INTERIOR = the interior box, CPML = the six slabs,
+// ALL = [1..N]^3 in one launch
+static int update_v_region(const ifos_geom *g, int region, const Box *box, const ifos_fields *f,
+                           const ifos_model *m, const float *d_prof, const ifos_psi *psi_s, hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    const VFields F = vfields(f, m);
+    const Prof P = {d_prof, 2 * g->fw + 1};
+    const Psi9 psi = psi9(psi_s);
+    const Shots one = {1, 0, {0, 0, 0}};
+    const Snap none = {nullptr, 0, 0, 0};
+    const MarchCfg &c = g_default_cfg;
+    Box boxes[6];
+    int nb = 1;
+    if (box) boxes[0] = *box;
+    else if (region == IFOS_REGION_CPML) nb = slab_boxes(g, boxes);
+    else boxes[0] = region_box(g, region);
+    const MTiles T = make_tiles(boxes, nb, c.ti, c.jc, 1, 0);
+    const int cat = region == IFOS_REGION_ALL ? IFOS_K_V_FUSED
+                    : region == IFOS_REGION_CPML ? IFOS_K_V_CPML : IFOS_K_V_INTERIOR;
+    ProfScope ps_(cat, s);
+    if (region == IFOS_REGION_INTERIOR) return launch_v<false, false>(c, T, G, F, P, psi, one, none, s);
+    return launch_v<true, false>(c, T, G, F, P, psi, one, none, s);
+}
+
+static int update_s_region(const ifos_geom *g, int region, const Box *box, const ifos_fields *f,
+                           const ifos_model *m, const float *d_prof, const ifos_psi *psi_v, hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    const SFields F = sfields(f, m);
+    const Prof P = {d_prof, 2 * g->fw + 1};
+    const Psi9 psi = psi9(psi_v);
+    const Shots one = {1, 0, {0, 0, 0}};
+    const MarchCfg &c = g_default_cfg;
+    Box boxes[6];
+    int nb = 1;
+    if (box) boxes[0] = *box;
+    else if (region == IFOS_REGION_CPML) nb = slab_boxes(g, boxes);
+    else boxes[0] = region_box(g, region);
+    const MTiles T = make_tiles(boxes, nb, c.ti, c.jc, 1, 0);
+    const int cat = region == IFOS_REGION_ALL ? IFOS_K_S_FUSED
+                    : region == IFOS_REGION_CPML ? IFOS_K_S_CPML : IFOS_K_S_INTERIOR;
+    ProfScope ps_(cat, s);
+    if (region == IFOS_REGION_INTERIOR) return launch_s<false>(c, T, G, F, P, psi, one, s);
+    return launch_s<true>(c, T, G, F, P, psi, one, s);
 }
 
 }  // namespace ifos
@@ -472,33 +537,7 @@ extern "C" int ifos_update_v(const ifos_geom *g, int region, const ifos_fields *
         set_error("ifos_update_v: CPML region needs profiles and psi arrays");
         return IFOS_EINVAL;
     }
-    DevGeom G = make_dev_geom(g);
-    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
-    Prof P = {d_prof, 2 * g->fw + 1};
-    Psi9 psi = {};
-    if (psi_s)
-        for (int n = 0; n < 9; ++n) psi.p[n] = psi_s->p[n];
-    hipStream_t s = (hipStream_t)stream;
-    if (region == IFOS_REGION_CPML) {
-        BoxList L = slab_boxes(g);
-        int64_t n = L.start[L.n];
-        if (n == 0) return IFOS_OK;
-        ProfScope ps_(IFOS_K_V_CPML, s);
-        hipLaunchKernelGGL(k_v_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, L, F, P, psi);
-        return hip_check(hipGetLastError(), "k_v_slabs");
-    }
-    Box R = region_box(g, region);
-    if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
-    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
-    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
-    const Shots one = {1, 0, {0, 0, 0}};
-    const Snap none = {nullptr, 0, 0, 0};
-    ProfScope ps_(region == IFOS_REGION_ALL ? IFOS_K_V_FUSED : IFOS_K_V_INTERIOR, s);
-    if (region == IFOS_REGION_ALL)
-        hipLaunchKernelGGL((k_v_march<TK, TI, true, false>), grid, block, 0, s, G, R, T, F, P, psi, one, none);
-    else
-        hipLaunchKernelGGL((k_v_march<TK, TI, false, false>), grid, block, 0, s, G, R, T, F, P, psi, one, none);
-    return hip_check(hipGetLastError(), "k_v_march");
+    return update_v_region(g, region, nullptr, f, m, d_prof, psi_s, (hipStream_t)stream);
 }
 
 extern "C" int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *f, const ifos_model *m,
@@ -512,35 +551,8 @@ extern "C" int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *
         set_error("ifos_update_s: CPML region needs profiles and psi arrays");
         return IFOS_EINVAL;
     }
-    DevGeom G = make_dev_geom(g);
-    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
-                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
-    Prof P = {d_prof, 2 * g->fw + 1};
-    Psi9 psi = {};
-    if (psi_v)
-        for (int n = 0; n < 9; ++n) psi.p[n] = psi_v->p[n];
-    hipStream_t s = (hipStream_t)stream;
-    if (region == IFOS_REGION_CPML) {
-        BoxList L = slab_boxes(g);
-        int64_t n = L.start[L.n];
-        if (n == 0) return IFOS_OK;
-        ProfScope ps_(IFOS_K_S_CPML, s);
-        hipLaunchKernelGGL(k_s_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, L, F, P, psi);
-        return hip_check(hipGetLastError(), "k_s_slabs");
-    }
-    Box R = region_box(g, region);
-    if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
-    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
-    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
-    const Shots one = {1, 0, {0, 0, 0}};
-    ProfScope ps_(region == IFOS_REGION_ALL ? IFOS_K_S_FUSED : IFOS_K_S_INTERIOR, s);
-    if (region == IFOS_REGION_ALL)
-        hipLaunchKernelGGL((k_s_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi, one);
-    else
-        hipLaunchKernelGGL((k_s_march<TK, TI, false>), grid, block, 0, s, G, R, T, F, P, psi, one);
-    return hip_check(hipGetLastError(), "k_s_march");
+    return update_s_region(g, region, nullptr, f, m, d_prof, psi_v, (hipStream_t)stream);
 }
-
 
 // box-restricted interior updates (the reference's update_v / update_s_elastic
 // take the box as arguments, update_v_ssg.c:27-30)
@@ -554,18 +566,7 @@ extern "C" int ifos_update_v_box(const ifos_geom *g, const int box[6], const ifo
         return IFOS_EINVAL;
     }
     if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
-    DevGeom G = make_dev_geom(g);
-    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
-    Prof P = {nullptr, 2 * g->fw + 1};
-    Psi9 psi = {};
-    hipStream_t s = (hipStream_t)stream;
-    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
-    const Shots one = {1, 0, {0, 0, 0}};
-    const Snap none = {nullptr, 0, 0, 0};
-    ProfScope ps_(IFOS_K_V_INTERIOR, s);
-    hipLaunchKernelGGL((k_v_march<TK, TI, false, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T,
-                       F, P, psi, one, none);
-    return hip_check(hipGetLastError(), "k_v_march");
+    return update_v_region(g, IFOS_REGION_INTERIOR, &R, f, m, nullptr, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int ifos_update_s_box(const ifos_geom *g, const int box[6], const ifos_fields *f, const ifos_model *m,
@@ -578,59 +579,64 @@ extern "C" int ifos_update_s_box(const ifos_geom *g, const int box[6], const ifo
         return IFOS_EINVAL;
     }
     if (R.i1 < R.i0 || R.j1 < R.j0 || R.k1 < R.k0) return IFOS_OK;
-    DevGeom G = make_dev_geom(g);
-    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
-                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
-    Prof P = {nullptr, 2 * g->fw + 1};
-    Psi9 psi = {};
-    hipStream_t s = (hipStream_t)stream;
-    Tiling T = make_tiling(R, TK, TI, pick_jc(R));
-    const Shots one = {1, 0, {0, 0, 0}};
-    ProfScope ps_(IFOS_K_S_INTERIOR, s);
-    hipLaunchKernelGGL((k_s_march<TK, TI, false>), dim3((unsigned)(T.per * 8)), dim3(TK, TI), 0, s, G, R, T, F, P,
-                       psi, one);
-    return hip_check(hipGetLastError(), "k_s_march");
+    return update_s_region(g, IFOS_REGION_INTERIOR, &R, f, m, nullptr, nullptr, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------
-// multi-shot fused launches of the batched path
+// multi-shot launches of the batched path: split = 0 one CPML-aware launch
+// over [1..N]^3; split = 1 the interior box without CPML code plus the six
+// slabs with it (two launches)
 namespace ifos {
 
 int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
-                   const ifos_psi *psi_s, const Shots &S, const Snap *snap, hipStream_t s)
+                   const ifos_psi *psi_s, const Shots &S, const Snap *snap, const MarchCfg &c, hipStream_t s)
 {
     DevGeom G = make_dev_geom(g);
-    VFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz, m->rip, m->rjp, m->rkp};
-    Prof P = {d_prof, 2 * g->fw + 1};
-    Psi9 psi;
-    for (int n = 0; n < 9; ++n) psi.p[n] = psi_s->p[n];
-    Box R = region_box(g, IFOS_REGION_ALL);
-    Tiling T = make_tiling(R, TK, TI, pick_jc(R, S.n), S.n);
-    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    const VFields F = vfields(f, m);
+    const Prof P = {d_prof, 2 * g->fw + 1};
+    const Psi9 psi = psi9(psi_s);
+    const Snap N = snap ? *snap : Snap{nullptr, 0, 0, 0};
     ProfScope ps_(IFOS_K_V_FUSED, s);
-    if (snap)
-        hipLaunchKernelGGL((k_v_march<TK, TI, true, true>), grid, block, 0, s, G, R, T, F, P, psi, S, *snap);
-    else
-        hipLaunchKernelGGL((k_v_march<TK, TI, true, false>), grid, block, 0, s, G, R, T, F, P, psi, S,
-                           Snap{nullptr, 0, 0, 0});
-    return hip_check(hipGetLastError(), "k_v_march (shots)");
+    if (!c.split) {
+        const Box all = region_box(g, IFOS_REGION_ALL);
+        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group);
+        return snap ? launch_v<true, true>(c, T, G, F, P, psi, S, N, s)
+                    : launch_v<true, false>(c, T, G, F, P, psi, S, N, s);
+    }
+    const Box in = region_box(g, IFOS_REGION_INTERIOR);
+    Box sl[6];
+    const int nb = slab_boxes(g, sl);
+    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group);
+    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group);
+    int rc = snap ? launch_v<false, true>(c, Ti, G, F, P, psi, S, N, s)
+                  : launch_v<false, false>(c, Ti, G, F, P, psi, S, N, s);
+    if (!rc)
+        rc = snap ? launch_v<true, true>(c, Ts, G, F, P, psi, S, N, s)
+                  : launch_v<true, false>(c, Ts, G, F, P, psi, S, N, s);
+    return rc;
 }
 
 int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
-                   const ifos_psi *psi_v, const Shots &S, hipStream_t s)
+                   const ifos_psi *psi_v, const Shots &S, const MarchCfg &c, hipStream_t s)
 {
     DevGeom G = make_dev_geom(g);
-    SFields F = {f->vx, f->vy, f->vz, f->sxx, f->syy, f->szz, f->sxy, f->syz, f->sxz,
-                 m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
-    Prof P = {d_prof, 2 * g->fw + 1};
-    Psi9 psi;
-    for (int n = 0; n < 9; ++n) psi.p[n] = psi_v->p[n];
-    Box R = region_box(g, IFOS_REGION_ALL);
-    Tiling T = make_tiling(R, TK, TI, pick_jc(R, S.n), S.n);
-    dim3 grid((unsigned)(T.per * 8)), block(TK, TI);
+    const SFields F = sfields(f, m);
+    const Prof P = {d_prof, 2 * g->fw + 1};
+    const Psi9 psi = psi9(psi_v);
     ProfScope ps_(IFOS_K_S_FUSED, s);
-    hipLaunchKernelGGL((k_s_march<TK, TI, true>), grid, block, 0, s, G, R, T, F, P, psi, S);
-    return hip_check(hipGetLastError(), "k_s_march (shots)");
+    if (!c.split) {
+        const Box all = region_box(g, IFOS_REGION_ALL);
+        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group);
+        return launch_s<true>(c, T, G, F, P, psi, S, s);
+    }
+    const Box in = region_box(g, IFOS_REGION_INTERIOR);
+    Box sl[6];
+    const int nb = slab_boxes(g, sl);
+    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group);
+    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group);
+    int rc = launch_s<false>(c, Ti, G, F, P, psi, S, s);
+    if (!rc) rc = launch_s<true>(c, Ts, G, F, P, psi, S, s);
+    return rc;
 }
 
 }  // namespace ifos

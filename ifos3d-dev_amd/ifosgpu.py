@@ -447,3 +447,26 @@ class BatchContext:
         arr = (C.c_void_p * 6)()
         check(lib().ifos_batch_dft(self.h, shot, back, arr), "ifos_batch_dft")
         return list(arr)
+
+    def set_tiles(self, cfg_v=None, cfg_s=None):
+        """tile shapes {TI, jc, group, split} of the velocity / stress marches"""
+        cv = (C.c_int * 4)(*cfg_v) if cfg_v is not None else None
+        cs = (C.c_int * 4)(*cfg_s) if cfg_s is not None else None
+        check(lib().ifos_batch_set_tiles(self.h, cv, cs), "ifos_batch_set_tiles")
+
+    @staticmethod
+    def tune_candidates():
+        buf = (C.c_int * 256)()
+        n = lib().ifos_batch_tune_candidates(buf, 64)
+        return [tuple(buf[4 * c:4 * c + 4]) for c in range(n)]
+
+    def autotune(self, ns=None, reps=3):
+        """times every candidate tile shape; returns (best_v, best_s, table)"""
+        cands = self.tune_candidates()
+        mv = (C.c_float * len(cands))()
+        ms = (C.c_float * len(cands))()
+        bv, bs = C.c_int(), C.c_int()
+        check(lib().ifos_batch_autotune(self.h, ns or self.nshots, reps, C.byref(bv), C.byref(bs), mv, ms),
+              "ifos_batch_autotune")
+        table = [dict(cfg=c, v_ms=round(mv[n], 4), s_ms=round(ms[n], 4)) for n, c in enumerate(cands)]
+        return cands[bv.value], cands[bs.value], table

@@ -261,6 +261,22 @@ int ifos_batch_set_freqs(ifos_batch *b, const float *finv, int nf, int tast);
 int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns,
                    float *d_syn, float *const d_grad[3], double *l2,
                    int use_graph);
+/* tile shapes of the velocity and stress march launches, each
+ * {TI, jc, group, split}: TI i-rows per workgroup (1, 2, 4, 8); jc j-planes
+ * per tile (<= 0: automatic); group: tiles of one shot before the next shot
+ * (1: shots fastest, <= 0: one shot after the other); split: 1 = interior
+ * box and CPML slabs in separate launches, 0 = one CPML-aware launch.
+ * NULL leaves a shape unchanged. */
+int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[4], const int cfg_s[4]);
+/* the autotuner's candidate shapes ({TI, jc, group, split} each, at most
+ * max); returns their number */
+int ifos_batch_tune_candidates(int *cfg, int max);
+/* times every candidate on the batch's buffers for ns shots (reps launches
+ * each), keeps the fastest per launch kind, returns their indices and the
+ * per-candidate average ms (arrays of the candidates' length, or NULL);
+ * clears the wavefields */
+int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
+                        int *best_s, float *ms_v, float *ms_s);
 /* DFT accumulators of one shot of the last batch: frequency l of
  * component n (Re x y z, Im x y z) at d_F[n] + l * ifos_field_elems(g) */
 int ifos_batch_dft(ifos_batch *b, int shot, int back, float *d_F[6]);

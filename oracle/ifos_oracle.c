@@ -516,6 +516,38 @@ void og_update_s_elastic(const og_grid *g, const int box[6], const oreal *vx,
             }
 }
 
+/* CPML corrections of the 9 velocity derivatives of a slab cell
+ * (update_s_ssg_CPML_elastic.c:63-115, update_s_ssg_CPML.c:80-125) */
+static void s_cpml(const og_grid *g, int j, int i, int k, vdiff *d, oreal *const prof[18], oreal *psi[9])
+{
+    const int fw = g->fw;
+    const int ytop = !g->free_surf;
+    oreal *p_vxx = psi[0], *p_vyx = psi[1], *p_vzx = psi[2];
+    oreal *p_vxy = psi[3], *p_vyy = psi[4], *p_vzy = psi[5];
+    oreal *p_vxz = psi[6], *p_vyz = psi[7], *p_vzz = psi[8];
+    const int hx = slab_h(i, g->nx, fw, 1);
+    const int hy = slab_h(j, g->ny, fw, ytop);
+    const int hz = slab_h(k, g->nz, fw, 1);
+    if (hx) {
+        CPML_APPLY(p_vxx, psi_x_at(g, j, hx, k), d->vxx, PK(0, hx), PA(0, hx), PB(0, hx));
+        CPML_APPLY(p_vyx, psi_x_at(g, j, hx, k), d->vyx, PKH(0, hx), PAH(0, hx), PBH(0, hx));
+        CPML_APPLY(p_vzx, psi_x_at(g, j, hx, k), d->vzx, PKH(0, hx), PAH(0, hx), PBH(0, hx));
+    }
+    if (hy) {
+        CPML_APPLY(p_vxy, psi_y_at(g, hy, i, k), d->vxy, PKH(1, hy), PAH(1, hy), PBH(1, hy));
+        CPML_APPLY(p_vyy, psi_y_at(g, hy, i, k), d->vyy, PK(1, hy), PA(1, hy), PB(1, hy));
+        CPML_APPLY(p_vzy, psi_y_at(g, hy, i, k), d->vzy, PKH(1, hy), PAH(1, hy), PBH(1, hy));
+    }
+    if (hz) {
+        /* the low-z layer divides by the y-axis K profile
+         * (update_s_ssg_CPML_elastic.c:104-108, update_s_ssg_CPML.c:110-116) */
+        const int kax = (k <= fw) ? 1 : 2;
+        CPML_APPLY(p_vxz, psi_z_at(g, j, i, hz), d->vxz, PKH(kax, hz), PAH(2, hz), PBH(2, hz));
+        CPML_APPLY(p_vyz, psi_z_at(g, j, i, hz), d->vyz, PKH(kax, hz), PAH(2, hz), PBH(2, hz));
+        CPML_APPLY(p_vzz, psi_z_at(g, j, i, hz), d->vzz, PK(kax, hz), PA(2, hz), PB(2, hz));
+    }
+}
+
 void og_update_s_cpml_elastic(const og_grid *g, const int box[6],
                               const oreal *vx, const oreal *vy,
                               const oreal *vz, oreal *sxx, oreal *syy,
@@ -527,43 +559,115 @@ void og_update_s_cpml_elastic(const og_grid *g, const int box[6],
 {
     oreal b1, b2;
     int i, j, k;
-    const int fw = g->fw;
-    const int ytop = !g->free_surf;
     vdiff d;
-    oreal *p_vxx = psi[0], *p_vyx = psi[1], *p_vzx = psi[2];
-    oreal *p_vxy = psi[3], *p_vyy = psi[4], *p_vzy = psi[5];
-    oreal *p_vxz = psi[6], *p_vyz = psi[7], *p_vzz = psi[8];
     fd4_coeffs(g->fdcoeff, &b1, &b2);
     for (j = 1; j <= g->ny; j++)
         for (i = 1; i <= g->nx; i++)
             for (k = 1; k <= g->nz; k++) {
-                int hx, hy, hz;
                 size_t c;
                 if (in_box(box, j, i, k)) continue;
                 c = og_idx(g, j, i, k);
                 velocity_diffs(g, c, vx, vy, vz, b1, b2, &d);
-                hx = slab_h(i, g->nx, fw, 1);
-                hy = slab_h(j, g->ny, fw, ytop);
-                hz = slab_h(k, g->nz, fw, 1);
-                if (hx) {
-                    CPML_APPLY(p_vxx, psi_x_at(g, j, hx, k), d.vxx, PK(0, hx), PA(0, hx), PB(0, hx));
-                    CPML_APPLY(p_vyx, psi_x_at(g, j, hx, k), d.vyx, PKH(0, hx), PAH(0, hx), PBH(0, hx));
-                    CPML_APPLY(p_vzx, psi_x_at(g, j, hx, k), d.vzx, PKH(0, hx), PAH(0, hx), PBH(0, hx));
-                }
-                if (hy) {
-                    CPML_APPLY(p_vxy, psi_y_at(g, hy, i, k), d.vxy, PKH(1, hy), PAH(1, hy), PBH(1, hy));
-                    CPML_APPLY(p_vyy, psi_y_at(g, hy, i, k), d.vyy, PK(1, hy), PA(1, hy), PB(1, hy));
-                    CPML_APPLY(p_vzy, psi_y_at(g, hy, i, k), d.vzy, PKH(1, hy), PAH(1, hy), PBH(1, hy));
-                }
-                if (hz) {
-                    /* the low-z layer divides by the y-axis K profile
-                     * (update_s_ssg_CPML_elastic.c:104-108) */
-                    const int kax = (k <= fw) ? 1 : 2;
-                    CPML_APPLY(p_vxz, psi_z_at(g, j, i, hz), d.vxz, PKH(kax, hz), PAH(2, hz), PBH(2, hz));
-                    CPML_APPLY(p_vyz, psi_z_at(g, j, i, hz), d.vyz, PKH(kax, hz), PAH(2, hz), PBH(2, hz));
-                    CPML_APPLY(p_vzz, psi_z_at(g, j, i, hz), d.vzz, PK(kax, hz), PA(2, hz), PB(2, hz));
-                }
+                s_cpml(g, j, i, k, &d, prof, psi);
                 stress_apply(g, c, &d, sxx, syy, szz, sxy, syz, sxz, pi, u, uipjp, ujpkp, uipkp);
+            }
+}
+
+/* update_s_ssg.c:174-238: the relaxed stress step, the memory-variable
+ * recursion, and the second half of the memory-variable term.  Written with
+ * the reference's own float/double mix (1.0 + L*tau, 2.0*u, eta*0.5 are
+ * double expressions rounded to float on assignment). */
+static inline void stress_apply_visco(const og_grid *g, size_t c, const vdiff *d,
+                                      oreal *sxx, oreal *syy, oreal *szz,
+                                      oreal *sxy, oreal *syz, oreal *sxz,
+                                      const oreal *pi, const oreal *u,
+                                      const oreal *uipjp, const oreal *ujpkp,
+                                      const oreal *uipkp, const og_visco *V)
+{
+    const oreal DT = g->dt;
+    const oreal dthalbe = DT / 2.0;
+    const int L = V->L;
+    oreal *rxx = V->r[0], *ryy = V->r[1], *rzz = V->r[2];
+    oreal *rxy = V->r[3], *ryz = V->r[4], *rxz = V->r[5];
+    const oreal eta = V->eta;
+    oreal fipjp, fjpkp, fipkp, gg, f, b, cc, dipjp, djpkp, dipkp, dd, e;
+    oreal vxyyx = d->vxy + d->vyx, vyzzy = d->vyz + d->vzy, vxzzx = d->vxz + d->vzx;
+    oreal vxxyyzz = d->vxx + d->vyy + d->vzz;
+    oreal vyyzz = d->vyy + d->vzz, vxxzz = d->vxx + d->vzz, vxxyy = d->vxx + d->vyy;
+    fipjp = uipjp[c] * DT * (1.0 + L * V->tausipjp[c]);
+    fjpkp = ujpkp[c] * DT * (1.0 + L * V->tausjpkp[c]);
+    fipkp = uipkp[c] * DT * (1.0 + L * V->tausipkp[c]);
+    gg = pi[c] * (1.0 + L * V->taup[c]);
+    f = 2.0 * u[c] * (1.0 + L * V->taus[c]);
+    sxy[c] += (fipjp * vxyyx) + (dthalbe * rxy[c]);
+    syz[c] += (fjpkp * vyzzy) + (dthalbe * ryz[c]);
+    sxz[c] += (fipkp * vxzzx) + (dthalbe * rxz[c]);
+    sxx[c] += DT * ((gg * vxxyyzz) - (f * vyyzz)) + (dthalbe * rxx[c]);
+    syy[c] += DT * ((gg * vxxyyzz) - (f * vxxzz)) + (dthalbe * ryy[c]);
+    szz[c] += DT * ((gg * vxxyyzz) - (f * vxxyy)) + (dthalbe * rzz[c]);
+    b = 1.0 / (1.0 + (eta * 0.5));
+    cc = 1.0 - (eta * 0.5);
+    dipjp = uipjp[c] * eta * V->tausipjp[c];
+    djpkp = ujpkp[c] * eta * V->tausjpkp[c];
+    dipkp = uipkp[c] * eta * V->tausipkp[c];
+    dd = 2.0 * u[c] * eta * V->taus[c];
+    e = pi[c] * eta * V->taup[c];
+    rxy[c] = b * (rxy[c] * cc - (dipjp * vxyyx));
+    ryz[c] = b * (ryz[c] * cc - (djpkp * vyzzy));
+    rxz[c] = b * (rxz[c] * cc - (dipkp * vxzzx));
+    rxx[c] = b * (rxx[c] * cc - (e * vxxyyzz) + (dd * vyyzz));
+    ryy[c] = b * (ryy[c] * cc - (e * vxxyyzz) + (dd * vxxzz));
+    rzz[c] = b * (rzz[c] * cc - (e * vxxyyzz) + (dd * vxxyy));
+    sxy[c] += (dthalbe * rxy[c]);
+    syz[c] += (dthalbe * ryz[c]);
+    sxz[c] += (dthalbe * rxz[c]);
+    sxx[c] += (dthalbe * rxx[c]);
+    syy[c] += (dthalbe * ryy[c]);
+    szz[c] += (dthalbe * rzz[c]);
+}
+
+void og_update_s_visco(const og_grid *g, const int box[6], const oreal *vx,
+                       const oreal *vy, const oreal *vz, oreal *sxx,
+                       oreal *syy, oreal *szz, oreal *sxy, oreal *syz,
+                       oreal *sxz, const oreal *pi, const oreal *u,
+                       const oreal *uipjp, const oreal *ujpkp,
+                       const oreal *uipkp, const og_visco *V)
+{
+    oreal b1, b2;
+    int i, j, k;
+    vdiff d;
+    fd4_coeffs(g->fdcoeff, &b1, &b2);
+    for (j = box[2]; j <= box[3]; j++)
+        for (i = box[0]; i <= box[1]; i++)
+            for (k = box[4]; k <= box[5]; k++) {
+                size_t c = og_idx(g, j, i, k);
+                velocity_diffs(g, c, vx, vy, vz, b1, b2, &d);
+                stress_apply_visco(g, c, &d, sxx, syy, szz, sxy, syz, sxz, pi, u, uipjp, ujpkp, uipkp, V);
+            }
+}
+
+void og_update_s_cpml_visco(const og_grid *g, const int box[6],
+                            const oreal *vx, const oreal *vy,
+                            const oreal *vz, oreal *sxx, oreal *syy,
+                            oreal *szz, oreal *sxy, oreal *syz, oreal *sxz,
+                            const oreal *pi, const oreal *u,
+                            const oreal *uipjp, const oreal *ujpkp,
+                            const oreal *uipkp, oreal *const prof[18],
+                            oreal *psi[9], const og_visco *V)
+{
+    oreal b1, b2;
+    int i, j, k;
+    vdiff d;
+    fd4_coeffs(g->fdcoeff, &b1, &b2);
+    for (j = 1; j <= g->ny; j++)
+        for (i = 1; i <= g->nx; i++)
+            for (k = 1; k <= g->nz; k++) {
+                size_t c;
+                if (in_box(box, j, i, k)) continue;
+                c = og_idx(g, j, i, k);
+                velocity_diffs(g, c, vx, vy, vz, b1, b2, &d);
+                s_cpml(g, j, i, k, &d, prof, psi);
+                stress_apply_visco(g, c, &d, sxx, syy, szz, sxy, syz, sxz, pi, u, uipjp, ujpkp, uipkp, V);
             }
 }
 

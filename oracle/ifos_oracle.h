@@ -131,6 +131,34 @@ void og_update_s_cpml_elastic(const og_grid *g, const int box[6],
                               const oreal *uipkp, oreal *const prof[18],
                               oreal *psi[9]);
 
+/* Viscoelastic stress update with one relaxation mechanism, 4th order
+ * (update_s_ssg.c:152-245 interior box, update_s_ssg_CPML.c:30-772 slabs;
+ * the reference hard-codes the mechanism l = 1, update_s_ssg.c:110,211).
+ * r[6] = rxx ryy rzz rxy ryz rxz (memory variables, [1..N]^3 used);
+ * taus, taup cell values; tausipjp/tausjpkp/tausipkp from og_av_mat;
+ * eta = eta[1] of the reference (DT / relaxation time); L the global L. */
+typedef struct og_visco {
+    oreal *r[6];
+    const oreal *taus, *taup, *tausipjp, *tausjpkp, *tausipkp;
+    oreal eta;
+    int L;
+} og_visco;
+
+void og_update_s_visco(const og_grid *g, const int box[6], const oreal *vx,
+                       const oreal *vy, const oreal *vz, oreal *sxx,
+                       oreal *syy, oreal *szz, oreal *sxy, oreal *syz,
+                       oreal *sxz, const oreal *pi, const oreal *u,
+                       const oreal *uipjp, const oreal *ujpkp,
+                       const oreal *uipkp, const og_visco *V);
+void og_update_s_cpml_visco(const og_grid *g, const int box[6],
+                            const oreal *vx, const oreal *vy,
+                            const oreal *vz, oreal *sxx, oreal *syy,
+                            oreal *szz, oreal *sxy, oreal *syz, oreal *sxz,
+                            const oreal *pi, const oreal *u,
+                            const oreal *uipjp, const oreal *ujpkp,
+                            const oreal *uipkp, oreal *const prof[18],
+                            oreal *psi[9], const og_visco *V);
+
 /* psource.c:26-54 (explosive sources, type 1) */
 void og_psource(const og_grid *g, int nt, oreal *sxx, oreal *syy, oreal *szz,
                 const og_source *srcs, const float *signals, int nsrc);

@@ -28,6 +28,18 @@ class OgGrid(C.Structure):
                 ("nt", C.c_int), ("time", C.c_float)]
 
 
+def _visco_struct(real):
+    class OgVisco(C.Structure):
+        _fields_ = [("r", C.c_void_p * 6), ("taus", C.c_void_p), ("taup", C.c_void_p),
+                    ("tausipjp", C.c_void_p), ("tausjpkp", C.c_void_p), ("tausipkp", C.c_void_p),
+                    ("eta", real), ("L", C.c_int)]
+    return OgVisco
+
+
+OgVisco32 = _visco_struct(C.c_float)
+OgVisco64 = _visco_struct(C.c_double)
+
+
 class OgSource(C.Structure):
     _fields_ = [("i", C.c_int), ("j", C.c_int), ("k", C.c_int),
                 ("tshift", C.c_float), ("fc", C.c_float), ("amp", C.c_float), ("type", C.c_int)]
@@ -115,6 +127,37 @@ class Oracle:
         self.lib.og_av_mat(C.byref(og), _p(rho), _p(u), None, _p(out[0]), _p(out[1]), _p(out[2]),
                            None, None, None, _p(out[3]), _p(out[4]), _p(out[5]))
         return out
+
+    def av_tau(self, g: Grid, rho, u, taus):
+        """the arithmetic averages of taus av_mat forms when L > 0 (av_mat.c:56-60)"""
+        tmp = [self.zeros(g) for _ in range(9)]
+        og = g.og()
+        self.lib.og_av_mat(C.byref(og), _p(rho), _p(u), _p(taus), *[_p(a) for a in tmp])
+        return tmp[3:6]
+
+    def visco(self, r, taus, taup, tausipjp, tausjpkp, tausipkp, eta, L=1):
+        """an og_visco for update_s_visco / update_s_cpml_visco (keeps the arrays alive)"""
+        v = (OgVisco32 if self.precision == "f32" else OgVisco64)()
+        for n in range(6):
+            v.r[n] = r[n].ctypes.data
+        v.taus, v.taup = taus.ctypes.data, taup.ctypes.data
+        v.tausipjp, v.tausjpkp, v.tausipkp = tausipjp.ctypes.data, tausjpkp.ctypes.data, tausipkp.ctypes.data
+        v.eta = eta
+        v.L = L
+        v._keep = (r, taus, taup, tausipjp, tausjpkp, tausipkp)
+        return v
+
+    def update_s_visco(self, g, box, f, pi, u, uipjp, ujpkp, uipkp, V):
+        og = g.og()
+        b = (C.c_int * 6)(*box)
+        self.lib.og_update_s_visco(C.byref(og), b, *[_p(a) for a in f], _p(pi), _p(u), _p(uipjp), _p(ujpkp),
+                                   _p(uipkp), C.byref(V))
+
+    def update_s_cpml_visco(self, g, box, f, pi, u, uipjp, ujpkp, uipkp, prof, psi, V):
+        og = g.og()
+        b = (C.c_int * 6)(*box)
+        self.lib.og_update_s_cpml_visco(C.byref(og), b, *[_p(a) for a in f], _p(pi), _p(u), _p(uipjp),
+                                        _p(ujpkp), _p(uipkp), _ptrs(prof), _ptrs(psi), C.byref(V))
 
     @staticmethod
     def _sources(srcs):
@@ -235,7 +278,8 @@ class Ref:
     """The reference's own routines, compiled from its sources (oracle/_ref)."""
 
     FIELDS = ["vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "syz", "sxz", "rho", "pi", "u",
-              "uipjp", "ujpkp", "uipkp", "rjp", "rkp", "rip", "grad1", "grad2", "grad3"]
+              "uipjp", "ujpkp", "uipkp", "rjp", "rkp", "rip", "grad1", "grad2", "grad3",
+              "rxx", "ryy", "rzz", "rxy", "ryz", "rxz", "taus", "taup", "tausipjp", "tausjpkp", "tausipkp"]
 
     @staticmethod
     def available() -> bool:
@@ -343,6 +387,9 @@ class Ref:
 
     def call(self, fn, *args):
         getattr(self.lib, "rg_" + fn)(*args)
+
+    def set_visco(self, L, eta):
+        self.lib.rg_set_visco(L, C.c_float(eta))
 
     def run_shot(self, shape=1, tast=100, obs=None, want_adjoint=True):
         o = [np.ascontiguousarray(x, np.float32) for x in obs] if obs is not None else None

@@ -19,6 +19,8 @@ enum {
     RG_RHO, RG_PI, RG_U,                                                /* 9-11 */
     RG_UIPJP, RG_UJPKP, RG_UIPKP, RG_RJP, RG_RKP, RG_RIP,               /* 12-17 */
     RG_GRAD1, RG_GRAD2, RG_GRAD3,                                       /* 18-20 */
+    RG_RXX, RG_RYY, RG_RZZ, RG_RXY, RG_RYZ, RG_RXZ,                     /* 21-26 */
+    RG_TAUS, RG_TAUP, RG_TAUSIPJP, RG_TAUSJPKP, RG_TAUSIPKP,            /* 27-31 */
     RG_NFIELDS
 };
 
@@ -41,7 +43,7 @@ static struct {
     int **recpos;             /* [1..3][1..ntr] */
     float **secx, **secy, **secz; /* [1..ntr][1..NT] */
     float ***absorb;
-    float ***taus, ***taup;   /* unused (L = 0) but passed through */
+    float *eta;               /* [1..1] relaxation parameter of the mechanism */
     float *finv;
     int nf;
 } S;
@@ -113,8 +115,12 @@ int rg_setup(int nx, int ny, int nz, float dx, float dy, float dz, float dt,
     alloc3(RG_PI, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
     alloc3(RG_U, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
     for (n = RG_UIPJP; n <= RG_GRAD3; n++) alloc3(n, 1, NY, 1, NX, 1, NZ);
-    S.taus = f3tensor(0, NY + 1, 0, NX + 1, 0, NZ + 1);
-    S.taup = f3tensor(0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    /* viscoelastic arrays, ifos3d.c:445-457 */
+    for (n = RG_RXX; n <= RG_RXZ; n++) alloc3(n, 1, NY, 1, NX, 1, NZ);
+    alloc3(RG_TAUS, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    alloc3(RG_TAUP, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    for (n = RG_TAUSIPJP; n <= RG_TAUSIPKP; n++) alloc3(n, 1, NY, 1, NX, 1, NZ);
+    S.eta = vector(1, 1);
     S.absorb = f3tensor(1, NY, 1, NX, 1, NZ);
 
     /* CPML memory, ifos3d.c:421-440 */
@@ -409,19 +415,48 @@ void rg_gradient_F(void)
 
 void rg_matcopy(void)
 {
-    matcopy(W[RG_RHO], W[RG_PI], W[RG_U], S.taus, S.taup);
+    matcopy(W[RG_RHO], W[RG_PI], W[RG_U], W[RG_TAUS], W[RG_TAUP]);
 }
 
 void rg_av_mat(void)
 {
-    av_mat(W[RG_RHO], W[RG_PI], W[RG_U], S.taus, S.taup, W[RG_UIPJP], W[RG_UJPKP], W[RG_UIPKP], NULL, NULL,
-           NULL, W[RG_RJP], W[RG_RKP], W[RG_RIP]);
+    av_mat(W[RG_RHO], W[RG_PI], W[RG_U], W[RG_TAUS], W[RG_TAUP], W[RG_UIPJP], W[RG_UJPKP], W[RG_UIPKP],
+           W[RG_TAUSIPJP], W[RG_TAUSJPKP], W[RG_TAUSIPKP], W[RG_RJP], W[RG_RKP], W[RG_RIP]);
+}
+
+/* the global L (number of relaxation mechanisms; the reference supports
+ * 0 and 1) and eta[1] */
+void rg_set_visco(int l, float eta)
+{
+    L = l;
+    S.eta[1] = eta;
+}
+
+void rg_update_s_visco(void)
+{
+    update_s(S.xb[0], S.xb[1], S.yb[0], S.yb[1], S.zb[0], S.zb[1], W[RG_VX], W[RG_VY], W[RG_VZ], W[RG_SXX],
+             W[RG_SYY], W[RG_SZZ], W[RG_SXY], W[RG_SYZ], W[RG_SXZ], W[RG_RXX], W[RG_RYY], W[RG_RZZ], W[RG_RXY],
+             W[RG_RYZ], W[RG_RXZ], W[RG_PI], W[RG_U], W[RG_UIPJP], W[RG_UJPKP], W[RG_UIPKP], W[RG_TAUS],
+             W[RG_TAUSIPJP], W[RG_TAUSJPKP], W[RG_TAUSIPKP], W[RG_TAUP], S.eta);
+}
+
+void rg_update_s_cpml_visco(void)
+{
+    float **P = S.prof;
+    update_s_CPML(S.xb[0], S.xb[1], S.yb[0], S.yb[1], S.zb[0], S.zb[1], W[RG_VX], W[RG_VY], W[RG_VZ],
+                  W[RG_SXX], W[RG_SYY], W[RG_SZZ], W[RG_SXY], W[RG_SYZ], W[RG_SXZ], W[RG_RXX], W[RG_RYY],
+                  W[RG_RZZ], W[RG_RXY], W[RG_RYZ], W[RG_RXZ], W[RG_PI], W[RG_U], W[RG_UIPJP], W[RG_UJPKP],
+                  W[RG_UIPKP], W[RG_TAUS], W[RG_TAUSIPJP], W[RG_TAUSJPKP], W[RG_TAUSIPKP], W[RG_TAUP], S.eta,
+                  P[0], P[1], P[2], P[3], P[4], P[5], P[8], P[9], P[10], P[11], P[12], P[13],
+                  P[16], P[17], P[18], P[19], P[20], P[21],
+                  S.psi[9], S.psi[10], S.psi[11], S.psi[12], S.psi[13], S.psi[14], S.psi[15],
+                  S.psi[16], S.psi[17]);
 }
 
 void rg_zero_wavefield(void)
 {
     zero_wavefield(NX, NY, NZ, W[RG_VX], W[RG_VY], W[RG_VZ], W[RG_SXX], W[RG_SYY], W[RG_SZZ], W[RG_SXY],
-                   W[RG_SYZ], W[RG_SXZ], NULL, NULL, NULL, NULL, NULL, NULL, S.psi[0], S.psi[1], S.psi[2],
+                   W[RG_SYZ], W[RG_SXZ], W[RG_RXX], W[RG_RYY], W[RG_RZZ], W[RG_RXY], W[RG_RYZ], W[RG_RXZ], S.psi[0], S.psi[1], S.psi[2],
                    S.psi[3], S.psi[4], S.psi[5], S.psi[6], S.psi[7], S.psi[8], S.psi[9], S.psi[10], S.psi[11],
                    S.psi[12], S.psi[13], S.psi[14], S.psi[15], S.psi[16], S.psi[17]);
 }

@@ -102,3 +102,18 @@ def shot_models(g):
         vp, vs, rho = 4000.0 * f, 2300.0 * f, 2000.0 * f
         out.append([a.astype(np.float32) for a in (rho, rho * vp * vp, rho * vs * vs)])
     return out  # start, true
+
+
+# g8: the viscoelastic stress update with one relaxation mechanism
+VISCO_CASES = [(2, 0), (1, 1)]  # (fdcoeff, free_surf)
+VISCO_ETA = 0.0125
+
+
+def visco_inputs(g, seed):
+    """wavefields, model, psi as step_inputs, plus memory variables r (6),
+    taus and taup (relaxation-time ratios, positive)"""
+    f, model, psi, _ = step_inputs(g, seed)
+    r = [det_uniform(g.shape, seed + 400 + n, -1e-2, 1e-2) for n in range(6)]
+    taus = det_uniform(g.shape, seed + 410, 0.01, 0.06)
+    taup = det_uniform(g.shape, seed + 411, 0.01, 0.06)
+    return f, model, psi, r, taus, taup

@@ -149,6 +149,33 @@ def main():
             res[f"grad{n + 1}"] = rec(g.interior(r.get(f"grad{n + 1}")))
         C[f"shot_fs{fs}"] = res
 
+    # g8: viscoelastic stress update, interior box and CPML slabs (update_s_ssg.c,
+    # update_s_ssg_CPML.c), two calls so the memory variables carry over
+    for fdc, fs in cs.VISCO_CASES:
+        g = cs.step_grid(fdc, fs)
+        f, (rho, pi, u), psi, r6, taus, taup = cs.visco_inputs(g, 50 + 7 * fdc + fs)
+        r = Ref(g, vppml=4500.0, fpml=40.0)
+        for name, a in zip(Ref.FIELDS[:9], f):
+            r.set(name, a)
+        for name, a in zip(("rho", "pi", "u", "taus", "taup"), (rho, pi, u, taus, taup)):
+            r.set(name, a)
+        for name, a in zip(("rxx", "ryy", "rzz", "rxy", "ryz", "rxz"), r6):
+            r.set(name, a)
+        for n in range(18):
+            r.psi_set(n, psi[n])
+        r.set_visco(1, cs.VISCO_ETA)
+        r.call("matcopy")
+        r.call("av_mat")
+        res = {name: rec(g.interior(r.get(name))) for name in ("tausipjp", "tausjpkp", "tausipkp")}
+        for _ in range(2):
+            r.call("update_s_visco")
+            r.call("update_s_cpml_visco")
+        for name in ("sxx", "syy", "szz", "sxy", "syz", "sxz", "rxx", "ryy", "rzz", "rxy", "ryz", "rxz"):
+            res[name] = rec(g.interior(r.get(name)))
+        for n in range(9):
+            res[f"psi{n}"] = rec(r.psi_get(9 + n))
+        C[f"visco_fd{fdc}_fs{fs}"] = res
+
     path = os.path.join(HERE, "golden.json")
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)

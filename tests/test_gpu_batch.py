@@ -70,9 +70,11 @@ def _plain(ig, G, g, true, start, srcs, rec, finv, tast, prec):
     return out
 
 
-def _batched(ig, G, g, true, start, srcs, rec, finv, tast, prec, chunk, use_graph, obs=None):
+def _batched(ig, G, g, true, start, srcs, rec, finv, tast, prec, chunk, use_graph, obs=None, tiles=None):
     ns, ntr = len(srcs), len(rec)
     b = ig.BatchContext(G, ns, len(finv), ntr, prec, chunk)
+    if tiles is not None:
+        b.set_tiles(*tiles)
     b.set_cpml(4000.0, 50.0)
     b.set_freqs(finv, tast)
     d_syn = ig.DBuf((ns, 3, ntr, g.nt + 1))
@@ -156,4 +158,49 @@ def test_batch_partial_and_rejects():
         b.run([(15, 10, 16, 0.0, 50.0, 1.0, 1)] * 5, rec, want_adjoint=False)
     with pytest.raises(ig.IfosError):
         b.run([(0, 10, 16, 0.0, 50.0, 1.0, 1)], rec, want_adjoint=False)
+    b.close()
+
+
+@pytest.mark.parametrize("tiles", [((4, 0, 1, 0), (2, 0, 4, 1)), ((8, 5, 0, 1), (1, 0, 16, 0)),
+                                   ((1, 3, 2, 1), (8, 0, 1, 1))])
+def test_batch_tile_shapes(tiles):
+    """every tile shape the autotuner may pick gives the same bits"""
+    import ifosgpu as ig
+    c = CASES["slab_sources"]
+    g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=90)
+    rec = np.array([[9 + 2 * n, 10, 19] for n in range(8)] + [[20, 5, 21], [30, 4, 33]], np.int32)
+    finv, tast = [40.0, 60.0], 100
+    _note(f"batch tile shapes v={tiles[0]} s={tiles[1]}")
+    true, start = _models(g)
+    G = _setup(ig, g, finv, tast, ig.DFT_COMPAT)
+    ref = _plain(ig, G, g, true, start, c["srcs"], rec, finv, tast, ig.DFT_COMPAT)
+    out = _batched(ig, G, g, true, start, c["srcs"], rec, finv, tast, ig.DFT_COMPAT, 4, False, tiles=tiles)
+    for n in range(len(c["srcs"])):
+        assert max_ulp(out["syn"][n], ref["syn"][n]) == 0, f"synthetics, shot {n}"
+    for n in range(3):
+        assert max_ulp(out["grads"][n], ref["grads"][n]) == 0, f"gradient {n}"
+    out["ctx"].close()
+
+
+def test_batch_autotune_keeps_results():
+    import ifosgpu as ig
+    c = CASES["interior"]
+    g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=60)
+    rec = np.array([[9 + 2 * n, 10, 19] for n in range(6)], np.int32)
+    finv, tast = [40.0, 60.0], 100
+    true, start = _models(g)
+    G = _setup(ig, g, finv, tast, ig.DFT_COMPAT)
+    ref = _plain(ig, G, g, true, start, c["srcs"], rec, finv, tast, ig.DFT_COMPAT)
+    ns = len(c["srcs"])
+    b = ig.BatchContext(G, ns, len(finv), len(rec), ig.DFT_COMPAT, 4)
+    b.set_cpml(4000.0, 50.0)
+    b.set_freqs(finv, tast)
+    b.set_model(*[G.to_dev(a) for a in start])
+    bv, bs, table = b.autotune(reps=1)
+    _note(f"autotune picked v={bv} s={bs}")
+    assert len(table) == len(ig.BatchContext.tune_candidates()) and all(t["v_ms"] > 0 for t in table)
+    grads = [G.empty() for _ in range(3)]
+    b.run(c["srcs"], rec, grads=grads, d_obs=ref["obs"])
+    for n in range(3):
+        assert max_ulp(G.from_dev(grads[n]), ref["grads"][n]) == 0, f"gradient {n}"
     b.close()

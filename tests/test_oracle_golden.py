@@ -143,3 +143,31 @@ def test_gradient_shot(gold, fs):
     assert f32hex([res["l2"]])[0] == want["l2"]
     for n in range(3):
         check(want[f"grad{n + 1}"], g.interior(res["grads"][n]), f"grad{n + 1}")
+
+
+@pytest.mark.parametrize("fdc,fs", cs.VISCO_CASES)
+def test_visco_stress(gold, fdc, fs):
+    """update_s + update_s_CPML with one relaxation mechanism, twice"""
+    g = cs.step_grid(fdc, fs)
+    want = gold[f"visco_fd{fdc}_fs{fs}"]
+    o = Oracle("f32")
+    f, (rho, pi, u), psi, r6, taus, taup = cs.visco_inputs(g, 50 + 7 * fdc + fs)
+    for a in (rho, pi, u, taus, taup):
+        o.matcopy_wrap(g, a)
+    uipjp, ujpkp, uipkp, _, _, _ = o.av_mat(g, rho, u)
+    tsa = o.av_tau(g, rho, u, taus)
+    for name, a in zip(("tausipjp", "tausjpkp", "tausipkp"), tsa):
+        check(want[name], g.interior(a), name)
+    prof, _ = o.cpml_coeff(g, 4500.0, 40.0)
+    psv = [p.copy() for p in psi[9:]]
+    V = o.visco(r6, taus, taup, *tsa, np.float32(cs.VISCO_ETA))
+    box = o.box(g)
+    for _ in range(2):
+        o.update_s_visco(g, box, f, pi, u, uipjp, ujpkp, uipkp, V)
+        o.update_s_cpml_visco(g, box, f, pi, u, uipjp, ujpkp, uipkp, prof, psv, V)
+    for n, name in enumerate(("sxx", "syy", "szz", "sxy", "syz", "sxz")):
+        check(want[name], g.interior(f[3 + n]), name)
+    for n, name in enumerate(("rxx", "ryy", "rzz", "rxy", "ryz", "rxz")):
+        check(want[name], g.interior(r6[n]), name)
+    for n in range(9):
+        check(want[f"psi{n}"], psv[n], f"psi{n}")

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from helpers import det_uniform, small_grid
-from pyoracle import Oracle
+from pyoracle import Grid, Oracle
 
 
 def _poly(x, c):
@@ -168,3 +168,59 @@ def test_residual_is_reversed_running_integral():
     run = np.float32(g.dt) * np.cumsum((syn[:, 1:] - obs[:, 1:]).astype(np.float32), axis=1, dtype=np.float32)
     assert np.array_equal(diff[:, 1:], run[:, ::-1])
     assert np.isclose(l2, float(np.sum(run.astype(np.float64) ** 2)), rtol=1e-5)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_visco_without_relaxation_is_elastic(prec):
+    """taus = taup = 0 and r = 0: update_s_ssg.c reduces to the elastic update
+    (1 + L*tau = 1, the memory variables stay 0), bit for bit"""
+    o = Oracle(prec)
+    g = Grid(nx=12, ny=11, nz=13, fw=3, dx=2.0, dy=2.5, dz=3.0, dt=2e-4, nt=5)
+    rng = np.random.default_rng(4)
+    f = [rng.standard_normal(g.shape).astype(o.dtype) for _ in range(9)]
+    rho = (2000 + 100 * rng.random(g.shape)).astype(o.dtype)
+    pi = (rho * 1.6e7).astype(o.dtype)
+    u = (rho * 5e6 * (1 + 0.1 * rng.random(g.shape))).astype(o.dtype)
+    uipjp, ujpkp, uipkp, _, _, _ = o.av_mat(g, rho, u)
+    zero = o.zeros(g)
+    r6 = [o.zeros(g) for _ in range(6)]
+    V = o.visco(r6, zero, zero, zero, zero, zero, 0.02)
+    box = [1, g.nx, 1, g.ny, 1, g.nz]
+    fe = [a.copy() for a in f]
+    o.update_s(g, box, fe, pi, u, uipjp, ujpkp, uipkp)
+    o.update_s_visco(g, box, f, pi, u, uipjp, ujpkp, uipkp, V)
+    for n in range(9):
+        assert np.array_equal(f[n], fe[n]), n
+    assert all(not r.any() for r in r6)
+
+
+def test_memory_variables_decay_geometrically():
+    """with no strain the memory variables follow r <- b*c*r with
+    b = 1/(1 + eta/2), c = 1 - eta/2 (update_s_ssg.c:211-224): after n steps
+    r_n = r_0 * ((1 - eta/2)/(1 + eta/2))^n (FP64 oracle, to rounding)"""
+    o = Oracle("f64")
+    g = Grid(nx=10, ny=10, nz=10, fw=2, dx=2.0, dy=2.0, dz=2.0, dt=2e-4, nt=5)
+    rng = np.random.default_rng(5)
+    f = [o.zeros(g) for _ in range(9)]  # zero velocities: no strain
+    rho = np.full(g.shape, 2000.0)
+    pi = np.full(g.shape, 3.2e10)
+    u = np.full(g.shape, 1.0e10)
+    uipjp, ujpkp, uipkp, _, _, _ = o.av_mat(g, rho, u)
+    taus = np.full(g.shape, 0.04)
+    taup = np.full(g.shape, 0.03)
+    tsa = o.av_tau(g, rho, u, taus)
+    r0 = [rng.standard_normal(g.shape) for _ in range(6)]
+    r6 = [a.copy() for a in r0]
+    eta = 0.05
+    V = o.visco(r6, taus, taup, *tsa, eta)
+    box = [1, g.nx, 1, g.ny, 1, g.nz]
+    n = 7
+    for _ in range(n):
+        o.update_s_visco(g, box, f, pi, u, uipjp, ujpkp, uipkp, V)
+    q = ((1 - eta / 2) / (1 + eta / 2)) ** n
+    for a, b in zip(r6, r0):
+        np.testing.assert_allclose(g.interior(a), g.interior(b) * q, rtol=1e-13)
+    # the stress picks up dt/2 * (r_old + r_new) per step: sxx = dt/2 * sum_k (r_k + r_{k+1})
+    qq = (1 - eta / 2) / (1 + eta / 2)
+    s = sum(qq ** m + qq ** (m + 1) for m in range(n)) * float(np.float32(g.dt)) / 2  # DT is a float global
+    np.testing.assert_allclose(g.interior(f[3]), g.interior(r0[0]) * s, rtol=1e-12)
