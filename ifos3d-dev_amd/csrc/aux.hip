@@ -443,6 +443,21 @@ __global__ __launch_bounds__(256) void k_av_mat(DevGeom g, const float *rho, con
     rip[c] = (float)(0.5 * (rho[c] + rho[c + si]));
 }
 
+// av_mat.c:56-60 (L > 0): arithmetic averages of taus
+__global__ __launch_bounds__(256) void k_av_tau(DevGeom g, const float *taus, float *tausipjp, float *tausjpkp,
+                                               float *tausipkp)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
+    const float *tc = taus + c;
+    tausipjp[c] = (float)(0.25 * (tc[0] + tc[si] + tc[sj + si] + tc[sj]));
+    tausjpkp[c] = (float)(0.25 * (tc[0] + tc[sj] + tc[sj + 1] + tc[1]));
+    tausipkp[c] = (float)(0.25 * (tc[0] + tc[si] + tc[si + 1] + tc[1]));
+}
+
 // residual.c:39-53: one thread per trace and component
 __global__ void k_residual(const float *obs, const float *syn, float *adj, int ntr, int ns, int ldin, int ldadj,
                            float dt, double *l2part)
@@ -649,6 +664,19 @@ int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const
     return hip_check(hipGetLastError(), "k_gradient_shots");
 }
 }  // namespace ifos
+
+extern "C" int ifos_av_tau(const ifos_geom *g, const float *d_taus, float *d_tausipjp, float *d_tausjpkp,
+                           float *d_tausipkp, void *stream)
+{
+    if (!g || !d_taus || !d_tausipjp || !d_tausjpkp || !d_tausipkp) return IFOS_EINVAL;
+    DevGeom G = make_dev_geom(g);
+    dim3 block, grid = cell_grid(g, block);
+    { ProfScope ps_(IFOS_K_OTHER, (hipStream_t)stream);
+    hipLaunchKernelGGL(k_av_tau, grid, block, 0, (hipStream_t)stream, G, d_taus, d_tausipjp, d_tausjpkp,
+                       d_tausipkp);
+    }
+    return hip_check(hipGetLastError(), "k_av_tau");
+}
 
 extern "C" int ifos_matcopy_wrap(const ifos_geom *g, float *d_m, void *stream)
 {

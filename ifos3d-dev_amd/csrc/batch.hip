@@ -286,6 +286,10 @@ struct ifos_batch {
     int ntr;                   // receivers of the current batch
     bool any_slab_src, any_slab_rec;
     MarchCfg cfg_v, cfg_s;     // tile shapes of the two march launches
+    // viscoelastic (L = 1): memory variables [S][fe] x 6, shared ratios
+    int L;
+    float eta;
+    float *r[6], *taus, *taup, *tavg[3];
     hipStream_t st;
     std::vector<void *> allocs;
     std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
@@ -326,7 +330,7 @@ extern "C" size_t ifos_batch_bytes(const ifos_geom *g, int nshots, int nf_max, i
                                    int chunk)
 {
     const size_t fe = ifos_field_elems(g);
-    size_t psi = 0;
+    size_t psi = 0;  // (a viscoelastic batch adds 6 * nshots + 5 fields)
     for (int n = 0; n < 9; ++n) psi += ifos_psi_elems(g, n);
     size_t per_shot = 9 * fe + 2 * psi + 12 * (size_t)nf_max * fe * (dft_precision == IFOS_DFT_KAHAN ? 2 : 1) +
                       (size_t)chunk * 3 * fe + (size_t)(g->nt + 2) * (1 + 6 * (size_t)ntr_max);
@@ -396,6 +400,11 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->tast = 100;
     b->ntr = 0;
     b->cfg_v = b->cfg_s = MarchCfg{4, 0, 1, 0};
+    b->L = 0;
+    b->eta = 0.0f;
+    for (int n = 0; n < 6; ++n) b->r[n] = nullptr;
+    b->taus = b->taup = nullptr;
+    for (int n = 0; n < 3; ++n) b->tavg[n] = nullptr;
     *out = b;
     return IFOS_OK;
 }
@@ -412,6 +421,35 @@ extern "C" int ifos_batch_set_model(ifos_batch *b, const float *d_rho, const flo
     int rc = ifos_av_mat(&b->g, b->model[0], b->model[2], b->model[3], b->model[4], b->model[5], b->model[6],
                          b->model[7], b->model[8], b->st);
     if (!rc) rc = hip_check(hipStreamSynchronize(b->st), "set_model");
+    return rc;
+}
+
+// the viscoelastic update with one relaxation mechanism (update_s_ssg.c)
+// for every shot of the batch; L = 0 switches back to the elastic one
+extern "C" int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const float *d_taup, float eta, int L)
+{
+    if (!b || L < 0 || L > 1 || (L == 1 && (!d_taus || !d_taup))) {
+        set_error("ifos_batch_set_visco: L must be 0 or 1 (with taus and taup)");
+        return IFOS_EINVAL;
+    }
+    drop_graphs(b);
+    b->L = L;
+    b->eta = eta;
+    if (!L) return IFOS_OK;
+    int rc = 0;
+    if (!b->taus) {
+        for (int n = 0; n < 6 && !rc; ++n) rc = balloc(b, (void **)&b->r[n], (size_t)b->S * b->fe * 4);
+        if (!rc) rc = balloc(b, (void **)&b->taus, b->fe * 4);
+        if (!rc) rc = balloc(b, (void **)&b->taup, b->fe * 4);
+        for (int n = 0; n < 3 && !rc; ++n) rc = balloc(b, (void **)&b->tavg[n], b->fe * 4);
+        if (rc) return rc;
+    }
+    rc = hip_check(hipMemcpyAsync(b->taus, d_taus, b->fe * 4, hipMemcpyDeviceToDevice, b->st), "copy taus");
+    if (!rc) rc = hip_check(hipMemcpyAsync(b->taup, d_taup, b->fe * 4, hipMemcpyDeviceToDevice, b->st), "copy taup");
+    if (!rc) rc = ifos_matcopy_wrap(&b->g, b->taus, b->st);
+    if (!rc) rc = ifos_matcopy_wrap(&b->g, b->taup, b->st);
+    if (!rc) rc = ifos_av_tau(&b->g, b->taus, b->tavg[0], b->tavg[1], b->tavg[2], b->st);
+    if (!rc) rc = hip_check(hipStreamSynchronize(b->st), "set_visco");
     return rc;
 }
 
@@ -547,6 +585,17 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
     const int ntr = b->ntr, nt2 = g->nt + 2;
     const dim3 rgrid((unsigned)((ntr + 255) / 256), (unsigned)ns);
     const Snap nosnap = {nullptr, 0, 0, 0};
+    ViscoF V = {};
+    if (b->L) {
+        for (int n = 0; n < 6; ++n) V.r[n] = b->r[n];
+        V.taus = b->taus;
+        V.taup = b->taup;
+        V.tausipjp = b->tavg[0];
+        V.tausjpkp = b->tavg[1];
+        V.tausipkp = b->tavg[2];
+        V.eta = b->eta;
+        V.L = b->L;
+    }
     int nsnap = 0, q0 = 0;
     for (int nt = 1; nt <= g->nt; ++nt) {
         const bool samp = dft && nt % b->ntast == 0 && (nt / b->ntast) <= b->nsamp;
@@ -579,7 +628,7 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
                                nt, nt2, 2, N);
             rc = hip_check(hipGetLastError(), "k_badjoint");
         }
-        if (!rc) rc = launch_s_shots(g, &f, &m, b->prof, &pv, S, b->cfg_s, st);
+        if (!rc) rc = launch_s_shots(g, &f, &m, b->prof, &pv, S, b->cfg_s, st, b->L ? &V : nullptr);
         if (!rc && !back) {
             ProfScope ps_(IFOS_K_SOURCES, st);
             hipLaunchKernelGGL(k_bpsource, dim3(ns), dim3(64), 0, st, G, BF, b->src, b->sig, 1, nt, g->nt, nt2);
@@ -615,6 +664,8 @@ static int zero_state(ifos_batch *b, int ns, hipStream_t st)
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->fields[n], (size_t)ns * b->fe, st);
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->psi_s[n], (size_t)ns * b->pe[n / 3], st);
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->psi_v[n], (size_t)ns * b->pe[n / 3], st);
+    if (b->L)  // zero_wavefield.c:80-95
+        for (int n = 0; n < 6 && !rc; ++n) rc = fill(b->r[n], (size_t)ns * b->fe, st);
     return rc;
 }
 
@@ -795,8 +846,9 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
 // the batch's own buffers (ns shots, `reps` launches each) and keeps the
 // fastest.  The wavefields are zeroed afterwards.
 static const MarchCfg k_candidates[] = {
-    {4, 0, 1, 0},  {4, 0, 1, 1},  {4, 0, 4, 1},  {4, 0, 16, 1}, {4, 0, 0, 1},  {2, 0, 1, 1},
-    {2, 0, 8, 1},  {8, 0, 1, 1},  {8, 0, 4, 1},  {4, 32, 1, 1}, {4, 16, 4, 1}, {1, 0, 16, 1},
+    {4, 0, 1, 0},  {4, 0, 0, 0},  {4, 0, 16, 0}, {4, 0, 64, 0}, {4, 0, 256, 0}, {8, 0, 0, 0},
+    {8, 0, 64, 0}, {2, 0, 0, 0},  {2, 0, 64, 0}, {4, 32, 0, 0}, {4, 64, 0, 0},  {4, 16, 0, 0},
+    {8, 32, 0, 0}, {1, 0, 0, 0},  {4, 0, 0, 1},  {8, 0, 0, 1},  {4, 32, 0, 1},
 };
 
 extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v, int *best_s, float *ms_v,
@@ -814,6 +866,17 @@ extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
         pv.p[n] = b->psi_v[n];
     }
     const Shots S = {ns, (int64_t)b->fe, {b->pe[0], b->pe[1], b->pe[2]}};
+    ViscoF V = {};
+    if (b->L) {
+        for (int n = 0; n < 6; ++n) V.r[n] = b->r[n];
+        V.taus = b->taus;
+        V.taup = b->taup;
+        V.tausipjp = b->tavg[0];
+        V.tausjpkp = b->tavg[1];
+        V.tausipkp = b->tavg[2];
+        V.eta = b->eta;
+        V.L = b->L;
+    }
     hipEvent_t e0, e1;
     int rc = hip_check(hipEventCreate(&e0), "event");
     if (!rc) rc = hip_check(hipEventCreate(&e1), "event");
@@ -827,7 +890,8 @@ extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
                 if (r == 1) rc = hip_check(hipEventRecord(e0, st), "record");
                 if (!rc)
                     rc = kind == 0 ? launch_v_shots(g, &f, &m, b->prof, &ps, S, nullptr, k_candidates[c], st)
-                                   : launch_s_shots(g, &f, &m, b->prof, &pv, S, k_candidates[c], st);
+                                   : launch_s_shots(g, &f, &m, b->prof, &pv, S, k_candidates[c], st,
+                                                    b->L ? &V : nullptr);
             }
             if (!rc) rc = hip_check(hipEventRecord(e1, st), "record");
             if (!rc) rc = hip_check(hipEventSynchronize(e1), "sync");

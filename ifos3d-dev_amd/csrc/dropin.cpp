@@ -88,7 +88,7 @@ const ifos_geom &geom()
     if (NPROCX * NPROCY * NPROCZ != 1 || POS[1] || POS[2] || POS[3])
         fatal("one MPI rank only (NPROCX*NPROCY*NPROCZ = %d)", NPROCX * NPROCY * NPROCZ);
     if (FDORDER != 4) fatal("FDORDER %d not supported (4 only)", FDORDER);
-    if (L != 0) fatal("viscoelastic runs (L = %d) not supported", L);
+    if (L != 0 && L != 1) fatal("L = %d relaxation mechanisms not supported (0 or 1)", L);
     const ifos_geom &g = S.g;
     if (S.have_geom && g.nx == NX && g.ny == NY && g.nz == NZ && g.fw == FW && g.dx == DX && g.dy == DY &&
         g.dz == DZ && g.dt == DT && g.nt == NT && g.fdcoeff == FDCOEFF && g.free_surf == FREE_SURF)
@@ -407,6 +407,77 @@ double update_s_CPML_elastic(int nx1, int nx2, int ny1, int ny2, int nz1, int nz
     return 0.0;
 }
 
+namespace {
+ifos_visco visco_args(float ***rxx, float ***ryy, float ***rzz, float ***rxy, float ***ryz, float ***rxz,
+                      float ***taus, float ***tausipjp, float ***tausjpkp, float ***tausipkp, float ***taup,
+                      float *eta)
+{
+    if (L != 1) fatal("update_s / update_s_CPML need L = 1 (L = %d)", L);
+    ifos_visco V;
+    float ***r[6] = {rxx, ryy, rzz, rxy, ryz, rxz};
+    for (int n = 0; n < 6; ++n) V.r[n] = m3(r[n], AVG);
+    V.taus = m3(taus, MODEL);
+    V.taup = m3(taup, MODEL);
+    V.tausipjp = m3(tausipjp, AVG);
+    V.tausjpkp = m3(tausjpkp, AVG);
+    V.tausipkp = m3(tausipkp, AVG);
+    V.eta = eta[1];
+    V.L = L;
+    return V;
+}
+}  // namespace
+
+double update_s(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2, float ***vx, float ***vy, float ***vz,
+                float ***sxx, float ***syy, float ***szz, float ***sxy, float ***syz, float ***sxz, float ***rxx,
+                float ***ryy, float ***rzz, float ***rxy, float ***ryz, float ***rxz, float ***pi, float ***u,
+                float ***uipjp, float ***ujpkp, float ***uipkp, float ***taus, float ***tausipjp, float ***tausjpkp,
+                float ***tausipkp, float ***taup, float *eta)
+{
+    const ifos_geom &g = geom();
+    check_box(nx1, nx2, ny1, ny2, nz1, nz2);
+    ifos_fields f = fields(vx, vy, vz, sxx, syy, szz, sxy, syz, sxz);
+    ifos_model m = {};
+    m.pi = m3(pi, MODEL);
+    m.u = m3(u, MODEL);
+    m.uipjp = m3(uipjp, AVG);
+    m.ujpkp = m3(ujpkp, AVG);
+    m.uipkp = m3(uipkp, AVG);
+    const ifos_visco V = visco_args(rxx, ryy, rzz, rxy, ryz, rxz, taus, tausipjp, tausjpkp, tausipkp, taup, eta);
+    ck(ifos_update_s_visco(&g, IFOS_REGION_INTERIOR, &f, &m, &V, nullptr, nullptr, nullptr), "update_s");
+    return 0.0;
+}
+
+double update_s_CPML(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2, float ***vx, float ***vy, float ***vz,
+                     float ***sxx, float ***syy, float ***szz, float ***sxy, float ***syz, float ***sxz, float ***rxx,
+                     float ***ryy, float ***rzz, float ***rxy, float ***ryz, float ***rxz, float ***pi, float ***u,
+                     float ***uipjp, float ***ujpkp, float ***uipkp, float ***taus, float ***tausipjp,
+                     float ***tausjpkp, float ***tausipkp, float ***taup, float *eta, float *K_x, float *a_x,
+                     float *b_x, float *K_x_half, float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y,
+                     float *K_y_half, float *a_y_half, float *b_y_half, float *K_z, float *a_z, float *b_z,
+                     float *K_z_half, float *a_z_half, float *b_z_half, float ***psi_vxx, float ***psi_vyx,
+                     float ***psi_vzx, float ***psi_vxy, float ***psi_vyy, float ***psi_vzy, float ***psi_vxz,
+                     float ***psi_vyz, float ***psi_vzz)
+{
+    const ifos_geom &g = geom();
+    check_box(nx1, nx2, ny1, ny2, nz1, nz2);
+    ifos_fields f = fields(vx, vy, vz, sxx, syy, szz, sxy, syz, sxz);
+    ifos_model m = {};
+    m.pi = m3(pi, MODEL);
+    m.u = m3(u, MODEL);
+    m.uipjp = m3(uipjp, AVG);
+    m.ujpkp = m3(ujpkp, AVG);
+    m.uipkp = m3(uipkp, AVG);
+    float *const prof[18] = {K_x, a_x, b_x, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y,
+                             K_y_half, a_y_half, b_y_half, K_z, a_z, b_z, K_z_half, a_z_half, b_z_half};
+    ensure_profiles(prof);
+    ifos_psi psi;
+    float ***p[9] = {psi_vxx, psi_vyx, psi_vzx, psi_vxy, psi_vyy, psi_vzy, psi_vxz, psi_vyz, psi_vzz};
+    for (int n = 0; n < 9; ++n) psi.p[n] = m3(p[n], n < 3 ? PSI_X : n < 6 ? PSI_Y : PSI_Z);
+    const ifos_visco V = visco_args(rxx, ryy, rzz, rxy, ryz, rxz, taus, tausipjp, tausjpkp, tausipkp, taup, eta);
+    ck(ifos_update_s_visco(&g, IFOS_REGION_CPML, &f, &m, &V, S.d_prof, &psi, nullptr), "update_s_CPML");
+    return 0.0;
+}
+
 void psource(int nt, float ***sxx, float ***syy, float ***szz, float **srcpos_loc, float **signals, int nsrc)
 {
     const ifos_geom &g = geom();
@@ -525,7 +596,6 @@ void av_mat(float ***rho, float ***pi, float ***u, float ***taus, float ***taup,
             float ***uipkp, float ***tausipjp, float ***tausjpkp, float ***tausipkp, float ***rjp, float ***rkp,
             float ***rip)
 {
-    (void)taus, (void)taup, (void)tausipjp, (void)tausjpkp, (void)tausipkp;
     const ifos_geom &g = geom();
     // the model (and its halos from the caller's matcopy) is re-read every call
     float ***in[3] = {rho, pi, u};
@@ -539,6 +609,16 @@ void av_mat(float ***rho, float ***pi, float ***u, float ***taus, float ***taup,
     for (int n = 0; n < 6; ++n) o[n] = m3(out[n], AVG, false);
     ck(ifos_av_mat(&g, d[0], d[2], o[0], o[1], o[2], o[3], o[4], o[5], nullptr), "av_mat");
     for (int n = 0; n < 6; ++n) down3(out[n], AVG, o[n]);
+    if (L) {  // av_mat.c:56-60, and the relaxation ratios are (re)read like the model
+        float *dts = m3(taus, MODEL, false), *dtp = m3(taup, MODEL, false);
+        up3(taus, MODEL, dts);
+        up3(taup, MODEL, dtp);
+        float ***tout[3] = {tausipjp, tausjpkp, tausipkp};
+        float *t[3];
+        for (int n = 0; n < 3; ++n) t[n] = m3(tout[n], AVG, false);
+        ck(ifos_av_tau(&g, dts, t[0], t[1], t[2], nullptr), "av_mat");
+        for (int n = 0; n < 3; ++n) down3(tout[n], AVG, t[n]);
+    }
 }
 
 void CPML_coeff(float *K_x, float *alpha_prime_x, float *a_x, float *b_x, float *K_x_half, float *alpha_prime_x_half,
@@ -580,8 +660,12 @@ void zero_wavefield(int nx, int ny, int nz, float ***vx, float ***vy, float ***v
                     float ***psi_vzx, float ***psi_vxy, float ***psi_vyy, float ***psi_vzy, float ***psi_vxz,
                     float ***psi_vyz, float ***psi_vzz)
 {
-    (void)nx, (void)ny, (void)nz, (void)rxx, (void)ryy, (void)rzz, (void)rxy, (void)ryz, (void)rxz;
+    (void)nx, (void)ny, (void)nz;
     geom();
+    if (L) {  // zero_wavefield.c:80-95
+        float ***r[6] = {rxx, ryy, rzz, rxy, ryz, rxz};
+        for (int n = 0; n < 6; ++n) ck(ifos_fill(m3(r[n], AVG, false), S.fe, 0.0f, nullptr), "zero_wavefield");
+    }
     float ***w[9] = {vx, vy, vz, sxx, syy, szz, sxy, syz, sxz};
     const Kind wk[9] = {WAVE_TOP, WAVE_TOP, WAVE_TOP, WAVE, WAVE, WAVE, WAVE_TOP, WAVE_TOP, WAVE};
     for (int n = 0; n < 9; ++n) ck(ifos_fill(m3(w[n], wk[n], false), S.fe, 0.0f, nullptr), "zero_wavefield");

@@ -344,6 +344,10 @@ struct ifos_ctx {
     float *adj;            // [3][ntr][nt+2]
     float finv[IFOS_MAX_NF];
     int nf, tast;
+    // viscoelastic (L = 1): memory variables, relaxation ratios, averages
+    int L;
+    float eta;
+    float *r[6], *taus, *taup, *tavg[3];
     std::vector<void *> allocs;
 };
 
@@ -402,8 +406,43 @@ extern "C" int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max, i
     }
     c->nf = 0;
     c->tast = 100;
+    c->L = 0;
+    c->eta = 0.0f;
+    for (int n = 0; n < 6; ++n) c->r[n] = nullptr;
+    c->taus = c->taup = nullptr;
+    for (int n = 0; n < 3; ++n) c->tavg[n] = nullptr;
     *out = c;
     return IFOS_OK;
+}
+
+// the viscoelastic update with one relaxation mechanism (update_s_ssg.c)
+// instead of the elastic one; L = 0 switches back.  taus/taup in device
+// layout (cell values), copied and halo-wrapped here.
+extern "C" int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float *d_taup, float eta, int L,
+                                  void *stream)
+{
+    if (!c || L < 0 || L > 1 || (L == 1 && (!d_taus || !d_taup))) {
+        set_error("ifos_ctx_set_visco: L must be 0 or 1 (with taus and taup)");
+        return IFOS_EINVAL;
+    }
+    c->L = L;
+    c->eta = eta;
+    if (!L) return IFOS_OK;
+    int rc = 0;
+    if (!c->taus) {
+        for (int n = 0; n < 6 && !rc; ++n) rc = dalloc(c, (void **)&c->r[n], c->fe * 4);
+        if (!rc) rc = dalloc(c, (void **)&c->taus, c->fe * 4);
+        if (!rc) rc = dalloc(c, (void **)&c->taup, c->fe * 4);
+        for (int n = 0; n < 3 && !rc; ++n) rc = dalloc(c, (void **)&c->tavg[n], c->fe * 4);
+        if (rc) return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    rc = hip_check(hipMemcpyAsync(c->taus, d_taus, c->fe * 4, hipMemcpyDeviceToDevice, s), "copy taus");
+    if (!rc) rc = hip_check(hipMemcpyAsync(c->taup, d_taup, c->fe * 4, hipMemcpyDeviceToDevice, s), "copy taup");
+    if (!rc) rc = ifos_matcopy_wrap(&c->g, c->taus, stream);
+    if (!rc) rc = ifos_matcopy_wrap(&c->g, c->taup, stream);
+    if (!rc) rc = ifos_av_tau(&c->g, c->taus, c->tavg[0], c->tavg[1], c->tavg[2], stream);
+    return rc;
 }
 
 static ifos_model ctx_model(const ifos_ctx *c)
@@ -506,8 +545,22 @@ static int propagate(ifos_ctx *c, int back, const ifos_shot *s, int ntast, float
         }
         if (!rc) rc = ifos_update_v(g, IFOS_REGION_CPML, &f, &m, c->prof, &ps, st);
         if (!rc && back) rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
-        if (!rc) rc = ifos_update_s(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &pv, st);
-        if (!rc) rc = ifos_update_s(g, IFOS_REGION_CPML, &f, &m, c->prof, &pv, st);
+        if (c->L) {
+            ifos_visco V;
+            for (int n = 0; n < 6; ++n) V.r[n] = c->r[n];
+            V.taus = c->taus;
+            V.taup = c->taup;
+            V.tausipjp = c->tavg[0];
+            V.tausjpkp = c->tavg[1];
+            V.tausipkp = c->tavg[2];
+            V.eta = c->eta;
+            V.L = c->L;
+            if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_INTERIOR, &f, &m, &V, c->prof, &pv, st);
+            if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_CPML, &f, &m, &V, c->prof, &pv, st);
+        } else {
+            if (!rc) rc = ifos_update_s(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &pv, st);
+            if (!rc) rc = ifos_update_s(g, IFOS_REGION_CPML, &f, &m, c->prof, &pv, st);
+        }
         if (!rc && !back) rc = ifos_psource(g, &f, c->src, c->sig, 1, nt, st);
         if (!rc && dft && nt % ntast == 0)
             rc = ifos_discfourier(g, &f, Fp, c->prec == IFOS_DFT_KAHAN ? Cp : nullptr, c->finv, c->nf, nt, ntast,
@@ -525,6 +578,8 @@ static int zero_wavefield(ifos_ctx *c, hipStream_t st)
     for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->fields[n], c->fe, 0.0f, st);
     for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->psi_s[n], ifos_psi_elems(&c->g, n), 0.0f, st);
     for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->psi_v[n], ifos_psi_elems(&c->g, n), 0.0f, st);
+    if (c->L)  // zero_wavefield.c:80-95
+        for (int n = 0; n < 6 && !rc; ++n) rc = ifos_fill(c->r[n], c->fe, 0.0f, st);
     return rc;
 }
 

@@ -92,9 +92,23 @@ struct Snap {
     uint32_t soff;
 };
 
-// Index box (inclusive).
+// viscoelastic arrays of the stress march: memory variables r (per shot,
+// wavefield layout), relaxation-time ratios (shared model), eta, L
+struct ViscoF {
+    float *r[6];
+    const float *taus, *taup, *tausipjp, *tausjpkp, *tausipkp;
+    float eta;
+    int L;
+};
+
+// Index box (inclusive), and a list of them for the slab launches.
 struct Box {
     int i0, i1, j0, j1, k0, k1;
+};
+struct BoxList {
+    Box b[6];
+    int64_t start[7];  // cumulative cell counts
+    int n;
 };
 
 // Tile shape of a march launch: TI i-rows per block (1, 2, 4, 8), jc
@@ -132,7 +146,8 @@ DevGeom make_dev_geom(const ifos_geom *g);
 int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
                    const ifos_psi *psi, const Shots &S, const Snap *snap, const MarchCfg &c, hipStream_t s);
 int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
-                   const ifos_psi *psi, const Shots &S, const MarchCfg &c, hipStream_t s);
+                   const ifos_psi *psi, const Shots &S, const MarchCfg &c, hipStream_t s,
+                   const ViscoF *V = nullptr);
 // gradient_F of ns shots in shot order into d_grad (aux.hip); shot s's DFT
 // fields at d_F*[n] + s*fshot
 int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,

@@ -126,6 +126,44 @@ __device__ inline void s_apply(const DevGeom &g, const SFields &F, int64_t c, fl
     F.szz[c] += DT * ((gg * vxxyyzz) - (f * vxxyy));
 }
 
+// update_s_ssg.c:174-238: the viscoelastic stress step with one relaxation
+// mechanism.  s = sxx syy szz sxy syz sxz and r = rxx ryy rzz rxy ryz rxz
+// (old values in, new values out).  The double subexpressions of the
+// reference (1.0 + L*tau, 2.0*u, eta*0.5) are evaluated in double and
+// rounded to float on assignment, as the C code does.
+__device__ inline void stress_visco(float DT, float eta, int L, float pi, float u, float uipjp, float ujpkp,
+                                    float uipkp, float ts, float tp, float tsipjp, float tsjpkp, float tsipkp,
+                                    float vxyyx, float vyzzy, float vxzzx, float vxxyyzz, float vyyzz,
+                                    float vxxzz, float vxxyy, float s[6], float r[6])
+{
+    const float Lf = (float)L;
+    const float dthalbe = (float)((double)DT / 2.0);
+    const float fipjp = (float)((double)(uipjp * DT) * (1.0 + (double)(Lf * tsipjp)));
+    const float fjpkp = (float)((double)(ujpkp * DT) * (1.0 + (double)(Lf * tsjpkp)));
+    const float fipkp = (float)((double)(uipkp * DT) * (1.0 + (double)(Lf * tsipkp)));
+    const float gg = (float)((double)pi * (1.0 + (double)(Lf * tp)));
+    const float f = (float)((2.0 * (double)u) * (1.0 + (double)(Lf * ts)));
+    s[3] = s[3] + ((fipjp * vxyyx) + (dthalbe * r[3]));
+    s[4] = s[4] + ((fjpkp * vyzzy) + (dthalbe * r[4]));
+    s[5] = s[5] + ((fipkp * vxzzx) + (dthalbe * r[5]));
+    s[0] = s[0] + (DT * ((gg * vxxyyzz) - (f * vyyzz)) + (dthalbe * r[0]));
+    s[1] = s[1] + (DT * ((gg * vxxyyzz) - (f * vxxzz)) + (dthalbe * r[1]));
+    s[2] = s[2] + (DT * ((gg * vxxyyzz) - (f * vxxyy)) + (dthalbe * r[2]));
+    const float b = (float)(1.0 / (1.0 + ((double)eta * 0.5)));
+    const float cc = (float)(1.0 - ((double)eta * 0.5));
+    const float dipjp = uipjp * eta * tsipjp, djpkp = ujpkp * eta * tsjpkp, dipkp = uipkp * eta * tsipkp;
+    const float dd = (float)(((2.0 * (double)u) * (double)eta) * (double)ts);
+    const float e = pi * eta * tp;
+    r[3] = b * ((r[3] * cc) - (dipjp * vxyyx));
+    r[4] = b * ((r[4] * cc) - (djpkp * vyzzy));
+    r[5] = b * ((r[5] * cc) - (dipkp * vxzzx));
+    r[0] = b * (((r[0] * cc) - (e * vxxyyzz)) + (dd * vyyzz));
+    r[1] = b * (((r[1] * cc) - (e * vxxyyzz)) + (dd * vxxzz));
+    r[2] = b * (((r[2] * cc) - (e * vxxyyzz)) + (dd * vxxyy));
+#pragma unroll
+    for (int n = 0; n < 6; ++n) s[n] = s[n] + (dthalbe * r[n]);
+}
+
 // ---------------------------------------------------------------------------
 // Tiles.  A launch covers up to six boxes (the interior box, the CPML slabs,
 // or all of [1..N]^3), each cut into tiles of kw k-columns x (TI * rpw)
@@ -260,8 +298,9 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
     }
 }
 
-template <int TI, bool CPML>
-__global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S)
+template <int TI, bool CPML, bool VISCO>
+__global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S,
+                                                    ViscoF V)
 {
     int k, i, jb, je, shot;
     Box R;
@@ -278,6 +317,16 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
                  ruipjp = make_rsrc(F.uipjp, g.fbytes), rujpkp = make_rsrc(F.ujpkp, g.fbytes),
                  ruipkp = make_rsrc(F.uipkp, g.fbytes);
     if (CPML) shift_psi(psi, S, shot);
+    rsrc_t rr[6], rtaus, rtaup, rtsipjp, rtsjpkp, rtsipkp;
+    if (VISCO) {
+#pragma unroll
+        for (int n = 0; n < 6; ++n) rr[n] = make_rsrc(V.r[n] + so, g.fbytes);
+        rtaus = make_rsrc(V.taus, g.fbytes);
+        rtaup = make_rsrc(V.taup, g.fbytes);
+        rtsipjp = make_rsrc(V.tausipjp, g.fbytes);
+        rtsjpkp = make_rsrc(V.tausjpkp, g.fbytes);
+        rtsipkp = make_rsrc(V.tausipkp, g.fbytes);
+    }
     uint32_t vo = (uint32_t)((int64_t)jb * g.sj + (int64_t)i * g.si + k + IFOS_KOFF - 2) * 4u;
     int hx = 0;
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
@@ -312,18 +361,38 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
         }
-        // update_s_ssg_elastic.c:117-138
-        const float fipjp = uipjp * DT, fjpkp = ujpkp * DT, fipkp = uipkp * DT;
-        const float f = 2.0f * u;
         const float vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
         const float vxxyyzz = vxx + vyy + vzz;
         const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
-        const float nsxy = sxy0 + (fipjp * vxyyx);
-        const float nsyz = syz0 + (fjpkp * vyzzy);
-        const float nsxz = sxz0 + (fipkp * vxzzx);
-        const float nsxx = sxx0 + DT * ((pi * vxxyyzz) - (f * vyyzz));
-        const float nsyy = syy0 + DT * ((pi * vxxyyzz) - (f * vxxzz));
-        const float nszz = szz0 + DT * ((pi * vxxyyzz) - (f * vxxyy));
+        float nsxy, nsyz, nsxz, nsxx, nsyy, nszz;
+        if (!VISCO) {
+            // update_s_ssg_elastic.c:117-138
+            const float fipjp = uipjp * DT, fjpkp = ujpkp * DT, fipkp = uipkp * DT;
+            const float f = 2.0f * u;
+            nsxy = sxy0 + (fipjp * vxyyx);
+            nsyz = syz0 + (fjpkp * vyzzy);
+            nsxz = sxz0 + (fipkp * vxzzx);
+            nsxx = sxx0 + DT * ((pi * vxxyyzz) - (f * vyyzz));
+            nsyy = syy0 + DT * ((pi * vxxyyzz) - (f * vxxzz));
+            nszz = szz0 + DT * ((pi * vxxyyzz) - (f * vxxyy));
+        } else {
+            const float ts = LD(rtaus, 0, 0, 0), tp = LD(rtaup, 0, 0, 0);
+            const float tsipjp = LD(rtsipjp, 0, 0, 0), tsjpkp = LD(rtsjpkp, 0, 0, 0),
+                        tsipkp = LD(rtsipkp, 0, 0, 0);
+            float rn[6] = {LD(rr[0], 0, 0, 0), LD(rr[1], 0, 0, 0), LD(rr[2], 0, 0, 0),
+                           LD(rr[3], 0, 0, 0), LD(rr[4], 0, 0, 0), LD(rr[5], 0, 0, 0)};
+            float sn[6] = {sxx0, syy0, szz0, sxy0, syz0, sxz0};
+            stress_visco(DT, V.eta, V.L, pi, u, uipjp, ujpkp, uipkp, ts, tp, tsipjp, tsjpkp, tsipkp, vxyyx, vyzzy,
+                         vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn, rn);
+            nsxx = sn[0]; nsyy = sn[1]; nszz = sn[2]; nsxy = sn[3]; nsyz = sn[4]; nsxz = sn[5];
+            const float nrxx = rn[0], nryy = rn[1], nrzz = rn[2], nrxy = rn[3], nryz = rn[4], nrxz = rn[5];
+            ST(nrxx, rr[0]);
+            ST(nryy, rr[1]);
+            ST(nrzz, rr[2]);
+            ST(nrxy, rr[3]);
+            ST(nryz, rr[4]);
+            ST(nrxz, rr[5]);
+        }
         if (j < je) {
             vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2; vx_p2 = LD(rvx, 3, 0, 0);
             vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_p2; vz_p2 = LD(rvz, 3, 0, 0);
@@ -339,6 +408,91 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 }
 #undef LD
 #undef ST
+
+// ---------------------------------------------------------------------------
+// slab kernels: one thread per CPML cell, k fastest within each box
+__device__ inline bool slab_cell(const BoxList &L, int64_t t, int &j, int &i, int &k)
+{
+    if (t >= L.start[L.n]) return false;
+    int b = 0;
+    while (t >= L.start[b + 1]) ++b;
+    const Box &B = L.b[b];
+    int64_t r = t - L.start[b];
+    const int nk = B.k1 - B.k0 + 1, ni = B.i1 - B.i0 + 1;
+    k = B.k0 + (int)(r % nk);
+    r /= nk;
+    i = B.i0 + (int)(r % ni);
+    j = B.j0 + (int)(r / ni);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_v_slabs(DevGeom g, BoxList L, VFields F, Prof P, Psi9 psi)
+{
+    int j, i, k;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!slab_cell(L, t, j, i, k)) return;
+    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    float sxx_x = g.dtdx * D4F(b1, b2, F.sxx[c - si], F.sxx[c], F.sxx[c + si], F.sxx[c + 2 * si]);
+    float sxy_x = g.dtdx * D4B(b1, b2, F.sxy[c - 2 * si], F.sxy[c - si], F.sxy[c], F.sxy[c + si]);
+    float sxz_x = g.dtdx * D4B(b1, b2, F.sxz[c - 2 * si], F.sxz[c - si], F.sxz[c], F.sxz[c + si]);
+    float sxy_y = g.dtdy * D4B(b1, b2, F.sxy[c - 2 * sj], F.sxy[c - sj], F.sxy[c], F.sxy[c + sj]);
+    float syy_y = g.dtdy * D4F(b1, b2, F.syy[c - sj], F.syy[c], F.syy[c + sj], F.syy[c + 2 * sj]);
+    float syz_y = g.dtdy * D4B(b1, b2, F.syz[c - 2 * sj], F.syz[c - sj], F.syz[c], F.syz[c + sj]);
+    float sxz_z = g.dtdz * D4B(b1, b2, F.sxz[c - 2], F.sxz[c - 1], F.sxz[c], F.sxz[c + 1]);
+    float syz_z = g.dtdz * D4B(b1, b2, F.syz[c - 2], F.syz[c - 1], F.syz[c], F.syz[c + 1]);
+    float szz_z = g.dtdz * D4F(b1, b2, F.szz[c - 1], F.szz[c], F.szz[c + 1], F.szz[c + 2]);
+    const int hx = slab_h(i, g.nx, g.fw, 1);
+    const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+    const int hz = slab_h(k, g.nz, g.fw, 1);
+    v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z, szz_z);
+    F.vx[c] += (sxx_x + sxy_y + sxz_z) / F.rip[c];
+    F.vy[c] += (syy_y + sxy_x + syz_z) / F.rjp[c];
+    F.vz[c] += (szz_z + sxz_x + syz_y) / F.rkp[c];
+}
+
+template <bool VISCO>
+__global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F, Prof P, Psi9 psi, ViscoF V)
+{
+    int j, i, k;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!slab_cell(L, t, j, i, k)) return;
+    const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
+    const float b1 = g.b1, b2 = g.b2;
+    const float DX = g.dx, DY = g.dy, DZ = g.dz;
+    float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], F.vx[c], F.vx[c + si]) / DX;
+    float vxy = D4F(b1, b2, F.vx[c - sj], F.vx[c], F.vx[c + sj], F.vx[c + 2 * sj]) / DY;
+    float vxz = D4F(b1, b2, F.vx[c - 1], F.vx[c], F.vx[c + 1], F.vx[c + 2]) / DZ;
+    float vyx = D4F(b1, b2, F.vy[c - si], F.vy[c], F.vy[c + si], F.vy[c + 2 * si]) / DX;
+    float vyy = D4B(b1, b2, F.vy[c - 2 * sj], F.vy[c - sj], F.vy[c], F.vy[c + sj]) / DY;
+    float vyz = D4F(b1, b2, F.vy[c - 1], F.vy[c], F.vy[c + 1], F.vy[c + 2]) / DZ;
+    float vzx = D4F(b1, b2, F.vz[c - si], F.vz[c], F.vz[c + si], F.vz[c + 2 * si]) / DX;
+    float vzy = D4F(b1, b2, F.vz[c - sj], F.vz[c], F.vz[c + sj], F.vz[c + 2 * sj]) / DY;
+    float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], F.vz[c], F.vz[c + 1]) / DZ;
+    const int hx = slab_h(i, g.nx, g.fw, 1);
+    const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+    const int hz = slab_h(k, g.nz, g.fw, 1);
+    s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
+    if (!VISCO) {
+        s_apply(g, F, c, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
+        return;
+    }
+    float sn[6] = {F.sxx[c], F.syy[c], F.szz[c], F.sxy[c], F.syz[c], F.sxz[c]};
+    float rn[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) rn[n] = V.r[n][c];
+    stress_visco(g.dt, V.eta, V.L, F.pi[c], F.u[c], F.uipjp[c], F.ujpkp[c], F.uipkp[c], V.taus[c], V.taup[c],
+                 V.tausipjp[c], V.tausjpkp[c], V.tausipkp[c], vxy + vyx, vyz + vzy, vxz + vzx, vxx + vyy + vzz,
+                 vyy + vzz, vxx + vzz, vxx + vyy, sn, rn);
+    F.sxx[c] = sn[0];
+    F.syy[c] = sn[1];
+    F.szz[c] = sn[2];
+    F.sxy[c] = sn[3];
+    F.syz[c] = sn[4];
+    F.sxz[c] = sn[5];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) V.r[n][c] = rn[n];
+}
 
 // ---------------------------------------------------------------------------
 // host launchers
@@ -365,6 +519,22 @@ static int slab_boxes(const ifos_geom *g, Box *out)
         out[n++] = B;
     }
     return n;
+}
+
+// the slabs as a cell list for the one-thread-per-cell kernels
+static BoxList slab_list(const ifos_geom *g)
+{
+    BoxList L;
+    Box b[6];
+    L.n = slab_boxes(g, b);
+    L.start[0] = 0;
+    for (int q = 0; q < L.n; ++q) {
+        L.b[q] = b[q];
+        L.start[q + 1] = L.start[q] + (int64_t)(b[q].i1 - b[q].i0 + 1) * (b[q].j1 - b[q].j0 + 1) *
+                                          (b[q].k1 - b[q].k0 + 1);
+    }
+    for (int q = L.n + 1; q < 7; ++q) L.start[q] = L.start[L.n];
+    return L;
 }
 
 static Box region_box(const ifos_geom *g, int region)
@@ -440,17 +610,17 @@ static int launch_v(const MarchCfg &c, const MTiles &T, const DevGeom &G, const 
     return hip_check(hipGetLastError(), "k_v_march");
 }
 
-template <bool CPML>
+template <bool CPML, bool VISCO = false>
 static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
-                    const Psi9 &psi, const Shots &S, hipStream_t s)
+                    const Psi9 &psi, const Shots &S, hipStream_t s, const ViscoF &V = ViscoF{})
 {
     if (T.total == 0) return IFOS_OK;
     const dim3 grid((unsigned)(T.per * 8));
     switch (c.ti) {
-    case 1: hipLaunchKernelGGL((k_s_march<1, CPML>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S); break;
-    case 2: hipLaunchKernelGGL((k_s_march<2, CPML>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S); break;
-    case 4: hipLaunchKernelGGL((k_s_march<4, CPML>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S); break;
-    case 8: hipLaunchKernelGGL((k_s_march<8, CPML>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S); break;
+    case 1: hipLaunchKernelGGL((k_s_march<1, CPML, VISCO>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, V); break;
+    case 2: hipLaunchKernelGGL((k_s_march<2, CPML, VISCO>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
+    case 4: hipLaunchKernelGGL((k_s_march<4, CPML, VISCO>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
+    case 8: hipLaunchKernelGGL((k_s_march<8, CPML, VISCO>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
     default: set_error("march: TI=%d not in {1,2,4,8}", c.ti); return IFOS_EINVAL;
     }
     return hip_check(hipGetLastError(), "k_s_march");
@@ -467,6 +637,20 @@ static SFields sfields(const ifos_fields *f, const ifos_model *m)
                  m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
     return F;
 }
+static ViscoF viscof(const ifos_visco *v)
+{
+    ViscoF V = {};
+    for (int n = 0; n < 6; ++n) V.r[n] = v->r[n];
+    V.taus = v->taus;
+    V.taup = v->taup;
+    V.tausipjp = v->tausipjp;
+    V.tausjpkp = v->tausjpkp;
+    V.tausipkp = v->tausipkp;
+    V.eta = v->eta;
+    V.L = v->L;
+    return V;
+}
+
 static Psi9 psi9(const ifos_psi *p)
 {
     Psi9 q = {};
@@ -487,21 +671,26 @@ static int update_v_region(const ifos_geom *g, int region, const Box *box, const
     const Shots one = {1, 0, {0, 0, 0}};
     const Snap none = {nullptr, 0, 0, 0};
     const MarchCfg &c = g_default_cfg;
-    Box boxes[6];
-    int nb = 1;
-    if (box) boxes[0] = *box;
-    else if (region == IFOS_REGION_CPML) nb = slab_boxes(g, boxes);
-    else boxes[0] = region_box(g, region);
-    const MTiles T = make_tiles(boxes, nb, c.ti, c.jc, 1, 0);
-    const int cat = region == IFOS_REGION_ALL ? IFOS_K_V_FUSED
-                    : region == IFOS_REGION_CPML ? IFOS_K_V_CPML : IFOS_K_V_INTERIOR;
+    if (region == IFOS_REGION_CPML && !box) {
+        const BoxList L = slab_list(g);
+        const int64_t n = L.start[L.n];
+        if (n == 0) return IFOS_OK;
+        ProfScope ps_(IFOS_K_V_CPML, s);
+        hipLaunchKernelGGL(k_v_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, L, F, P, psi);
+        return hip_check(hipGetLastError(), "k_v_slabs");
+    }
+    Box boxes[1];
+    boxes[0] = box ? *box : region_box(g, region);
+    const MTiles T = make_tiles(boxes, 1, c.ti, c.jc, 1, 0);
+    const int cat = region == IFOS_REGION_ALL ? IFOS_K_V_FUSED : IFOS_K_V_INTERIOR;
     ProfScope ps_(cat, s);
     if (region == IFOS_REGION_INTERIOR) return launch_v<false, false>(c, T, G, F, P, psi, one, none, s);
     return launch_v<true, false>(c, T, G, F, P, psi, one, none, s);
 }
 
 static int update_s_region(const ifos_geom *g, int region, const Box *box, const ifos_fields *f,
-                           const ifos_model *m, const float *d_prof, const ifos_psi *psi_v, hipStream_t s)
+                           const ifos_model *m, const float *d_prof, const ifos_psi *psi_v, hipStream_t s,
+                           const ifos_visco *visco = nullptr)
 {
     DevGeom G = make_dev_geom(g);
     const SFields F = sfields(f, m);
@@ -509,15 +698,28 @@ static int update_s_region(const ifos_geom *g, int region, const Box *box, const
     const Psi9 psi = psi9(psi_v);
     const Shots one = {1, 0, {0, 0, 0}};
     const MarchCfg &c = g_default_cfg;
-    Box boxes[6];
-    int nb = 1;
-    if (box) boxes[0] = *box;
-    else if (region == IFOS_REGION_CPML) nb = slab_boxes(g, boxes);
-    else boxes[0] = region_box(g, region);
-    const MTiles T = make_tiles(boxes, nb, c.ti, c.jc, 1, 0);
-    const int cat = region == IFOS_REGION_ALL ? IFOS_K_S_FUSED
-                    : region == IFOS_REGION_CPML ? IFOS_K_S_CPML : IFOS_K_S_INTERIOR;
+    if (region == IFOS_REGION_CPML && !box) {
+        const BoxList L = slab_list(g);
+        const int64_t n = L.start[L.n];
+        if (n == 0) return IFOS_OK;
+        ProfScope ps_(IFOS_K_S_CPML, s);
+        const dim3 grid((unsigned)((n + 255) / 256));
+        if (visco)
+            hipLaunchKernelGGL(k_s_slabs<true>, grid, dim3(256), 0, s, G, L, F, P, psi, viscof(visco));
+        else
+            hipLaunchKernelGGL(k_s_slabs<false>, grid, dim3(256), 0, s, G, L, F, P, psi, ViscoF{});
+        return hip_check(hipGetLastError(), "k_s_slabs");
+    }
+    Box boxes[1];
+    boxes[0] = box ? *box : region_box(g, region);
+    const MTiles T = make_tiles(boxes, 1, c.ti, c.jc, 1, 0);
+    const int cat = region == IFOS_REGION_ALL ? IFOS_K_S_FUSED : IFOS_K_S_INTERIOR;
     ProfScope ps_(cat, s);
+    if (visco) {
+        const ViscoF V = viscof(visco);
+        if (region == IFOS_REGION_INTERIOR) return launch_s<false, true>(c, T, G, F, P, psi, one, s, V);
+        return launch_s<true, true>(c, T, G, F, P, psi, one, s, V);
+    }
     if (region == IFOS_REGION_INTERIOR) return launch_s<false>(c, T, G, F, P, psi, one, s);
     return launch_s<true>(c, T, G, F, P, psi, one, s);
 }
@@ -552,6 +754,26 @@ extern "C" int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *
         return IFOS_EINVAL;
     }
     return update_s_region(g, region, nullptr, f, m, d_prof, psi_v, (hipStream_t)stream);
+}
+
+extern "C" int ifos_update_s_visco(const ifos_geom *g, int region, const ifos_fields *f, const ifos_model *m,
+                                   const ifos_visco *visco, const float *d_prof, const ifos_psi *psi_v,
+                                   void *stream)
+{
+    if (!g || !f || !m || !visco) {
+        set_error("ifos_update_s_visco: null argument");
+        return IFOS_EINVAL;
+    }
+    if (visco->L != 1) {
+        set_error("ifos_update_s_visco: L=%d (one relaxation mechanism is supported, as in update_s_ssg.c)",
+                  visco->L);
+        return IFOS_EINVAL;
+    }
+    if (region != IFOS_REGION_INTERIOR && (!d_prof || !psi_v)) {
+        set_error("ifos_update_s_visco: CPML region needs profiles and psi arrays");
+        return IFOS_EINVAL;
+    }
+    return update_s_region(g, region, nullptr, f, m, d_prof, psi_v, (hipStream_t)stream, visco);
 }
 
 // box-restricted interior updates (the reference's update_v / update_s_elastic
@@ -617,7 +839,7 @@ int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
 }
 
 int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m, const float *d_prof,
-                   const ifos_psi *psi_v, const Shots &S, const MarchCfg &c, hipStream_t s)
+                   const ifos_psi *psi_v, const Shots &S, const MarchCfg &c, hipStream_t s, const ViscoF *V)
 {
     DevGeom G = make_dev_geom(g);
     const SFields F = sfields(f, m);
@@ -627,15 +849,16 @@ int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
     if (!c.split) {
         const Box all = region_box(g, IFOS_REGION_ALL);
         const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group);
-        return launch_s<true>(c, T, G, F, P, psi, S, s);
+        return V ? launch_s<true, true>(c, T, G, F, P, psi, S, s, *V) : launch_s<true>(c, T, G, F, P, psi, S, s);
     }
     const Box in = region_box(g, IFOS_REGION_INTERIOR);
     Box sl[6];
     const int nb = slab_boxes(g, sl);
     const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group);
     const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group);
-    int rc = launch_s<false>(c, Ti, G, F, P, psi, S, s);
-    if (!rc) rc = launch_s<true>(c, Ts, G, F, P, psi, S, s);
+    int rc = V ? launch_s<false, true>(c, Ti, G, F, P, psi, S, s, *V) : launch_s<false>(c, Ti, G, F, P, psi, S, s);
+    if (!rc)
+        rc = V ? launch_s<true, true>(c, Ts, G, F, P, psi, S, s, *V) : launch_s<true>(c, Ts, G, F, P, psi, S, s);
     return rc;
 }
 

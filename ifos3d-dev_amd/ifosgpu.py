@@ -44,6 +44,22 @@ class IfosPsi(C.Structure):
     _fields_ = [("p", C.c_void_p * 9)]
 
 
+class IfosVisco(C.Structure):
+    _fields_ = [("r", C.c_void_p * 6), ("taus", C.c_void_p), ("taup", C.c_void_p), ("tausipjp", C.c_void_p),
+                ("tausjpkp", C.c_void_p), ("tausipkp", C.c_void_p), ("eta", C.c_float), ("L", C.c_int)]
+
+
+def visco_struct(r, taus, taup, tausipjp, tausjpkp, tausipkp, eta, L=1):
+    v = IfosVisco()
+    for n in range(6):
+        v.r[n] = r[n].data_ptr()
+    v.taus, v.taup = taus.data_ptr(), taup.data_ptr()
+    v.tausipjp, v.tausjpkp, v.tausipkp = tausipjp.data_ptr(), tausjpkp.data_ptr(), tausipkp.data_ptr()
+    v.eta = float(eta)
+    v.L = L
+    return v
+
+
 class IfosShot(C.Structure):
     _fields_ = [("src_i", C.c_int), ("src_j", C.c_int), ("src_k", C.c_int), ("src_type", C.c_int),
                 ("tshift", C.c_float), ("fc", C.c_float), ("amp", C.c_float), ("source_shape", C.c_int),
@@ -267,6 +283,19 @@ def update_s(g, region, f, m, prof, psi, stream=None):
           "ifos_update_s")
 
 
+def update_s_visco(g, region, f, m, visco, prof, psi, stream=None):
+    """visco: an IfosVisco (visco_struct)"""
+    check(lib().ifos_update_s_visco(g.ref, region, C.byref(fields_struct(f)), C.byref(model_struct(m)),
+                                    C.byref(visco), _ptr(prof),
+                                    C.byref(psi_struct(psi)) if psi is not None else None, stream),
+          "ifos_update_s_visco")
+
+
+def av_tau(g, d_taus, d_out, stream=None):
+    check(lib().ifos_av_tau(g.ref, _ptr(d_taus), _ptr(d_out[0]), _ptr(d_out[1]), _ptr(d_out[2]), stream),
+          "ifos_av_tau")
+
+
 def add_forces(g, f, m, d_src, d_sig, nsrc, nt, alpha=0.0, beta=0.0, stream=None):
     check(lib().ifos_add_forces(g.ref, C.byref(fields_struct(f)), C.byref(model_struct(m)), _ptr(d_src),
                                 _ptr(d_sig), nsrc, nt, C.c_float(alpha), C.c_float(beta), stream),
@@ -366,6 +395,10 @@ class ShotContext:
         check(lib().ifos_ctx_set_freqs(self.h, fv.ctypes.data_as(C.c_void_p), len(fv), tast),
               "ifos_ctx_set_freqs")
 
+    def set_visco(self, d_taus, d_taup, eta, L=1, stream=None):
+        check(lib().ifos_ctx_set_visco(self.h, _ptr(d_taus), _ptr(d_taup), C.c_float(eta), L, stream),
+              "ifos_ctx_set_visco")
+
     def run_shot(self, src, rec_ijk, d_syn, grads, d_obs=None, shape=1, want_adjoint=True, stream=None):
         """src = (i, j, k, tshift, fc, amp, type); returns the L2 misfit (FP64 sum)."""
         rec = np.ascontiguousarray(rec_ijk, np.int32)
@@ -423,6 +456,10 @@ class BatchContext:
         fv = np.asarray(finv, np.float32)
         check(lib().ifos_batch_set_freqs(self.h, fv.ctypes.data_as(C.c_void_p), len(fv), tast),
               "ifos_batch_set_freqs")
+
+    def set_visco(self, d_taus, d_taup, eta, L=1):
+        check(lib().ifos_batch_set_visco(self.h, _ptr(d_taus), _ptr(d_taup), C.c_float(eta), L),
+              "ifos_batch_set_visco")
 
     def run(self, srcs, rec_ijk, d_syn=None, grads=None, d_obs=None, shape=1, want_adjoint=True, use_graph=False):
         """srcs: list of (i, j, k, tshift, fc, amp, type), one per shot; rec_ijk:

@@ -25,7 +25,8 @@
  * Other host wavefield/DFT arrays are stale while the device works.
  *
  * Supported configuration: one rank (NPROCX*NPROCY*NPROCZ == 1), FDORDER 4,
- * ABS_TYPE 1 (CPML), L 0 (elastic), SEISMO 1.  Anything else stops the run
+ * ABS_TYPE 1 (CPML), L 0 (elastic) or 1 (one relaxation mechanism: update_s,
+ * update_s_CPML), SEISMO 1.  Anything else stops the run
  * with a message, as the reference's err() does (util.c:30-40).
  */
 #ifndef IFOS3D_DROPIN_H
@@ -87,6 +88,36 @@ double update_s_CPML_elastic(int nx1, int nx2, int ny1, int ny2, int nz1,
                              float ***psi_vyy, float ***psi_vzy,
                              float ***psi_vxz, float ***psi_vyz,
                              float ***psi_vzz);
+
+/* fd.h:247-254 (update_s_ssg.c:28): viscoelastic, one relaxation
+ * mechanism (L = 1); the box must be the CPML_ini_elastic interior box */
+double update_s(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2,
+                float ***vx, float ***vy, float ***vz, float ***sxx,
+                float ***syy, float ***szz, float ***sxy, float ***syz,
+                float ***sxz, float ***rxx, float ***ryy, float ***rzz,
+                float ***rxy, float ***ryz, float ***rxz, float ***pi,
+                float ***u, float ***uipjp, float ***ujpkp, float ***uipkp,
+                float ***taus, float ***tausipjp, float ***tausjpkp,
+                float ***tausipkp, float ***taup, float *eta);
+
+/* fd.h:256-263 (update_s_ssg_CPML.c:30) */
+double update_s_CPML(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2,
+                     float ***vx, float ***vy, float ***vz, float ***sxx,
+                     float ***syy, float ***szz, float ***sxy, float ***syz,
+                     float ***sxz, float ***rxx, float ***ryy, float ***rzz,
+                     float ***rxy, float ***ryz, float ***rxz, float ***pi,
+                     float ***u, float ***uipjp, float ***ujpkp,
+                     float ***uipkp, float ***taus, float ***tausipjp,
+                     float ***tausjpkp, float ***tausipkp, float ***taup,
+                     float *eta, float *K_x, float *a_x, float *b_x,
+                     float *K_x_half, float *a_x_half, float *b_x_half,
+                     float *K_y, float *a_y, float *b_y, float *K_y_half,
+                     float *a_y_half, float *b_y_half, float *K_z, float *a_z,
+                     float *b_z, float *K_z_half, float *a_z_half,
+                     float *b_z_half, float ***psi_vxx, float ***psi_vyx,
+                     float ***psi_vzx, float ***psi_vxy, float ***psi_vyy,
+                     float ***psi_vzy, float ***psi_vxz, float ***psi_vyz,
+                     float ***psi_vzz);
 
 /* fd.h:130 (psource.c:26) */
 void psource(int nt, float ***sxx, float ***syy, float ***szz,

@@ -142,6 +142,25 @@ int ifos_update_v(const ifos_geom *g, int region, const ifos_fields *f,
 int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *f,
                   const ifos_model *m, const float *d_prof,
                   const ifos_psi *psi_v, void *stream);
+/* viscoelastic arrays (device, device layout): r = rxx ryy rzz rxy ryz rxz
+ * memory variables; taus, taup relaxation-time ratios (cell values, halos
+ * [0..N+1] valid); tausipjp, tausjpkp, tausipkp their averages
+ * (ifos_av_tau); eta = DT / relaxation time of the mechanism; L = 1 */
+typedef struct ifos_visco {
+    float *r[6];
+    const float *taus, *taup, *tausipjp, *tausjpkp, *tausipkp;
+    float eta;
+    int L;
+} ifos_visco;
+/* update_s_ssg.c:28-245 (INTERIOR) / update_s_ssg_CPML.c:30-772 (CPML): the
+ * viscoelastic stress update with one relaxation mechanism */
+int ifos_update_s_visco(const ifos_geom *g, int region, const ifos_fields *f,
+                        const ifos_model *m, const ifos_visco *visco,
+                        const float *d_prof, const ifos_psi *psi_v,
+                        void *stream);
+/* av_mat.c:56-60: arithmetic averages of taus (halos wrapped first) */
+int ifos_av_tau(const ifos_geom *g, const float *d_taus, float *d_tausipjp,
+                float *d_tausjpkp, float *d_tausipkp, void *stream);
 /* the same interior updates over an explicit box {i0,i1,j0,j1,k0,k1}
  * inside [1..N]^3 (update_v and update_s_elastic take the box as
  * arguments), without CPML */
@@ -226,6 +245,11 @@ int ifos_ctx_set_model(ifos_ctx *c, const float *d_rho, const float *d_pi,
 int ifos_ctx_set_cpml(ifos_ctx *c, float vppml, float fpml);
 /* DFT frequencies and TAST (ntast = ifos_ntast(finv[nf-1], tast, dt)) */
 int ifos_ctx_set_freqs(ifos_ctx *c, const float *finv, int nf, int tast);
+/* L = 1: viscoelastic stress update with one relaxation mechanism
+ * (update_s_ssg.c), taus/taup in device layout (copied, halos wrapped,
+ * averaged), eta = DT / relaxation time; L = 0: elastic (the default) */
+int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float *d_taup,
+                       float eta, int L, void *stream);
 /* runs one shot; d_syn [3][ntr][nt+1] (device, written); d_grad[3]
  * (device layout, accumulated); *l2 (host, accumulated) */
 int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn,
@@ -253,6 +277,9 @@ int ifos_batch_set_model(ifos_batch *b, const float *d_rho,
                          const float *d_pi, const float *d_u);
 int ifos_batch_set_cpml(ifos_batch *b, float vppml, float fpml);
 int ifos_batch_set_freqs(ifos_batch *b, const float *finv, int nf, int tast);
+/* as ifos_ctx_set_visco, for every shot of the batch */
+int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const float *d_taup,
+                         float eta, int L);
 /* runs ns (<= nshots) shots with equal ntr and want_adjoint, synchronously.
  * d_syn: device [ns][3][ntr][nt+1] or NULL; d_grad[3]: device layout,
  * accumulated over the shots in order (NULL: no gradient); l2: host [ns]

@@ -872,8 +872,18 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
     int ntast;
     const size_t sl = (size_t)(nt_all + 2);
 
+    const int visco = io->L > 0 && io->taus && io->taup;
+    oreal *taus = NULL, *taup = NULL, *tavg[3] = {NULL, NULL, NULL}, *rmem[6] = {NULL};
+    og_visco V;
+    memset(&V, 0, sizeof V);
     og_interior_box(g, box);
     ntast = og_ntast(io->finv[nf - 1], io->tast, g->dt);
+    if (visco) {
+        taus = (oreal *)calloc(fs, sizeof(oreal));
+        taup = (oreal *)calloc(fs, sizeof(oreal));
+        for (n = 0; n < 3; n++) tavg[n] = (oreal *)calloc(fs, sizeof(oreal));
+        for (n = 0; n < 6; n++) rmem[n] = (oreal *)calloc(fs, sizeof(oreal));
+    }
 
     rho = (oreal *)calloc(fs, sizeof(oreal));
     pi = (oreal *)calloc(fs, sizeof(oreal));
@@ -905,6 +915,24 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
     recsrc = (og_source *)calloc((size_t)(ntr > 0 ? ntr : 1), sizeof(og_source));
 
     og_av_mat(g, rho, u, NULL, avg[0], avg[1], avg[2], NULL, NULL, NULL, avg[3], avg[4], avg[5]);
+    if (visco) {
+        /* ifos3d.c:445-457, matcopy of taus/taup, av_mat.c:56-60 */
+        oreal *tsa[3];
+        memcpy(taus, io->taus, fs * sizeof(oreal));
+        memcpy(taup, io->taup, fs * sizeof(oreal));
+        og_matcopy_wrap(g, taus);
+        og_matcopy_wrap(g, taup);
+        for (n = 0; n < 3; n++) tsa[n] = tavg[n];
+        og_av_mat(g, rho, u, taus, avg[0], avg[1], avg[2], tsa[0], tsa[1], tsa[2], avg[3], avg[4], avg[5]);
+        V.taus = taus;
+        V.taup = taup;
+        V.tausipjp = tavg[0];
+        V.tausjpkp = tavg[1];
+        V.tausipkp = tavg[2];
+        V.eta = io->eta;
+        V.L = io->L;
+        for (n = 0; n < 6; n++) V.r[n] = rmem[n];
+    }
     og_cpml_coeff(g, io->vppml, io->fpml, prof, alpha);
     og_wavelet(g, &io->src, 1, io->source_shape, sig);
 
@@ -915,10 +943,17 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
                     avg[3], avg[4], avg[5], &io->src, sig, sig, sig, 1, 0);
         og_update_v_cpml(g, box, nt, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
                          avg[3], avg[4], avg[5], &io->src, sig, sig, sig, 1, 0, prof, psi_s);
-        og_update_s_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
-                            pi, u, avg[0], avg[1], avg[2]);
-        og_update_s_cpml_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
-                                 pi, u, avg[0], avg[1], avg[2], prof, psi_v);
+        if (visco) {
+            og_update_s_visco(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                              pi, u, avg[0], avg[1], avg[2], &V);
+            og_update_s_cpml_visco(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                   pi, u, avg[0], avg[1], avg[2], prof, psi_v, &V);
+        } else {
+            og_update_s_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                pi, u, avg[0], avg[1], avg[2]);
+            og_update_s_cpml_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                     pi, u, avg[0], avg[1], avg[2], prof, psi_v);
+        }
         og_psource(g, nt, f[3], f[4], f[5], &io->src, sig, 1);
         if (io->want_adjoint && nt % ntast == 0)
             og_discfourier(g, nt, f[0], f[1], f[2], F[0], F[1], F[2], F[3], F[4], F[5],
@@ -955,6 +990,8 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
 
     /* adjoint, ifos3d.c:905-960: fresh wavefield, receivers inject */
     for (n = 0; n < 9; n++) memset(f[n], 0, fs * sizeof(oreal));
+    if (visco)
+        for (n = 0; n < 6; n++) memset(rmem[n], 0, fs * sizeof(oreal));
     for (n = 0; n < 9; n++) {
         size_t sz = (n < 3) ? npx : (n < 6) ? npy : npz;
         memset(psi_s[n], 0, sz * sizeof(oreal));
@@ -966,10 +1003,17 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
                     avg[3], avg[4], avg[5], recsrc, dx, dy, dz, ntr, 1);
         og_update_v_cpml(g, box, nt, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
                          avg[3], avg[4], avg[5], recsrc, dx, dy, dz, ntr, 1, prof, psi_s);
-        og_update_s_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
-                            pi, u, avg[0], avg[1], avg[2]);
-        og_update_s_cpml_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
-                                 pi, u, avg[0], avg[1], avg[2], prof, psi_v);
+        if (visco) {
+            og_update_s_visco(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                              pi, u, avg[0], avg[1], avg[2], &V);
+            og_update_s_cpml_visco(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                   pi, u, avg[0], avg[1], avg[2], prof, psi_v, &V);
+        } else {
+            og_update_s_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                pi, u, avg[0], avg[1], avg[2]);
+            og_update_s_cpml_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                     pi, u, avg[0], avg[1], avg[2], prof, psi_v);
+        }
         if (nt % ntast == 0)
             og_discfourier(g, nt, f[0], f[1], f[2], F[6], F[7], F[8], F[9], F[10], F[11],
                            io->finv, nf, ntast, 1);
@@ -990,5 +1034,8 @@ done:
     for (n = 0; n < 18; n++) free(prof[n]);
     for (n = 0; n < 12; n++) free(F[n]);
     free(sig); free(dx); free(dy); free(dz); free(recsrc);
+    free(taus); free(taup);
+    for (n = 0; n < 3; n++) free(tavg[n]);
+    for (n = 0; n < 6; n++) free(rmem[n]);
     return rc;
 }

@@ -227,6 +227,11 @@ typedef struct og_shot_io {
     /* optional: the frequency fields after the shot (6*nf canonical each),
      * NULL if not wanted */
     oreal *ffwd, *fadj;
+    /* optional viscoelastic run (L = 1): taus, taup canonical (cell values),
+     * eta; L = 0 (or NULL taus) runs the elastic update of ifos3d.c */
+    const oreal *taus, *taup;
+    oreal eta;
+    int L;
 } og_shot_io;
 
 int og_run_shot(const og_grid *g, og_shot_io *io);

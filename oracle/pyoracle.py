@@ -235,7 +235,9 @@ class Oracle:
         return self.lib.og_ntast(C.c_float(fmax), tast, C.c_float(dt))
 
     def run_shot(self, g: Grid, rho, pi, u, src, rec_ijk, finv, tast=100, vppml=4000.0, fpml=50.0,
-                 shape=1, obs=None, want_adjoint=True, want_fields=False):
+                 shape=1, obs=None, want_adjoint=True, want_fields=False, visco=None):
+        """visco = (taus, taup, eta): the viscoelastic update with one
+        relaxation mechanism instead of the elastic one"""
         class Io(C.Structure):
             _fields_ = [("rho", C.c_void_p), ("pi", C.c_void_p), ("u", C.c_void_p), ("src", OgSource),
                         ("source_shape", C.c_int), ("ntr", C.c_int), ("rec_ijk", C.c_void_p),
@@ -244,7 +246,9 @@ class Oracle:
                         ("vppml", C.c_float), ("fpml", C.c_float), ("want_adjoint", C.c_int),
                         ("syn_x", C.c_void_p), ("syn_y", C.c_void_p), ("syn_z", C.c_void_p),
                         ("l2", C.c_void_p), ("grad1", C.c_void_p), ("grad2", C.c_void_p),
-                        ("grad3", C.c_void_p), ("ffwd", C.c_void_p), ("fadj", C.c_void_p)]
+                        ("grad3", C.c_void_p), ("ffwd", C.c_void_p), ("fadj", C.c_void_p),
+                        ("taus", C.c_void_p), ("taup", C.c_void_p),
+                        ("eta", C.c_float if self.precision == "f32" else C.c_double), ("L", C.c_int)]
         rec = np.ascontiguousarray(rec_ijk, np.int32)
         ntr = rec.shape[0]
         fv = np.asarray(finv, np.float32)
@@ -262,6 +266,10 @@ class Oracle:
                 syn[0].ctypes.data, syn[1].ctypes.data, syn[2].ctypes.data, l2.ctypes.data,
                 grads[0].ctypes.data, grads[1].ctypes.data, grads[2].ctypes.data,
                 ff.ctypes.data if ff is not None else None, fa.ctypes.data if fa is not None else None)
+        if visco is not None:
+            taus = np.ascontiguousarray(visco[0], self.dtype)
+            taup = np.ascontiguousarray(visco[1], self.dtype)
+            io.taus, io.taup, io.eta, io.L = taus.ctypes.data, taup.ctypes.data, visco[2], 1
         og = g.og()
         rc = self.lib.og_run_shot(C.byref(og), C.byref(io))
         if rc != 0:

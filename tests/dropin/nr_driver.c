@@ -143,6 +143,17 @@ static void residual1(float **obs, float **syn, float **diff, int ntr, int ns, f
  * finv[nf], tast, vppml, fpml.  Outputs: syn [3][ntr][nt+1], grads [3]
  * canonical (accumulated), *l2.  Returns 0, or -1 on the NaN probe.
  */
+/* optional viscoelastic run: relaxation ratios (canonical) and eta; NULL
+ * switches back to the elastic update */
+static const float *g_taus, *g_taup;
+static float g_eta;
+void dd_set_visco(const float *taus_c, const float *taup_c, float eta)
+{
+    g_taus = taus_c;
+    g_taup = taup_c;
+    g_eta = eta;
+}
+
 int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, float dt, int nt, int fdcoeff,
                 int free_surf, const float *rho_c, const float *pi_c, const float *u_c, const int *src,
                 const float *src_f, int shape, int ntr, const int *rec_ijk, const float *obs, const float *finv,
@@ -160,10 +171,11 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
     int **recpos;
     float *sig;
     ifos_geom geo;
+    float ***rm[6] = {NULL}, ***taus = NULL, ***taup = NULL, ***tsa[3] = {NULL}, *eta = NULL;
 
     NX = nx; NY = ny; NZ = nz; FW = fw; DX = dx; DY = dy; DZ = dz; DT = dt; NT = nt;
     TIME = (NT - 1) * DT;
-    FDORDER = 4; FDCOEFF = fdcoeff; ABS_TYPE = 1; FREE_SURF = free_surf; L = 0; SEISMO = 1;
+    FDORDER = 4; FDCOEFF = fdcoeff; ABS_TYPE = 1; FREE_SURF = free_surf; L = g_taus ? 1 : 0; SEISMO = 1;
     POS[1] = POS[2] = POS[3] = 0; NPROCX = NPROCY = NPROCZ = 1; ALPHA = BETA = 0.0f;
     VPPML = vppml; FPML = fpml;
 
@@ -201,7 +213,18 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
     from_canon(pi, pi_c, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
     from_canon(u, u_c, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
     matcopy1(rho); matcopy1(pi); matcopy1(u);
-    av_mat(rho, pi, u, NULL, NULL, uipjp, ujpkp, uipkp, NULL, NULL, NULL, rjp, rkp, rip);
+    if (L) {  /* ifos3d.c:445-457 */
+        for (n = 0; n < 6; n++) rm[n] = nr3(1, NY, 1, NX, 1, NZ);
+        taus = nr3(0, NY + 1, 0, NX + 1, 0, NZ + 1);
+        taup = nr3(0, NY + 1, 0, NX + 1, 0, NZ + 1);
+        for (n = 0; n < 3; n++) tsa[n] = nr3(1, NY, 1, NX, 1, NZ);
+        eta = (float *)calloc(2, sizeof(float));
+        eta[1] = g_eta;
+        from_canon(taus, g_taus, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+        from_canon(taup, g_taup, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+        matcopy1(taus); matcopy1(taup);
+    }
+    av_mat(rho, pi, u, taus, taup, uipjp, ujpkp, uipkp, tsa[0], tsa[1], tsa[2], rjp, rkp, rip);
     CPML_ini_elastic(xb, yb, zb);
     /* per axis: K alpha a b K_half alpha_half a_half b_half (fd.h:57-62 order) */
     CPML_coeff(prof[0], prof[1], prof[2], prof[3], prof[4], prof[5], prof[6], prof[7],
@@ -223,8 +246,26 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
 
 #define PROF_V prof[0], prof[2], prof[3], prof[4], prof[6], prof[7], prof[8], prof[10], prof[11], prof[12], \
                prof[14], prof[15], prof[16], prof[18], prof[19], prof[20], prof[22], prof[23]
+#define STRESS_UPDATE()                                                                                       \
+    do {                                                                                                      \
+        if (L) {                                                                                              \
+            update_s(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, rm[0], \
+                     rm[1], rm[2], rm[3], rm[4], rm[5], pi, u, uipjp, ujpkp, uipkp, taus, tsa[0], tsa[1],      \
+                     tsa[2], taup, eta);                                                                      \
+            update_s_CPML(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz, sxz,   \
+                          rm[0], rm[1], rm[2], rm[3], rm[4], rm[5], pi, u, uipjp, ujpkp, uipkp, taus, tsa[0],  \
+                          tsa[1], tsa[2], taup, eta, PROF_V, psi[9], psi[10], psi[11], psi[12], psi[13],      \
+                          psi[14], psi[15], psi[16], psi[17]);                                                \
+        } else {                                                                                              \
+            update_s_elastic(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz,   \
+                             sxz, pi, u, uipjp, ujpkp, uipkp);                                                \
+            update_s_CPML_elastic(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy,   \
+                                  syz, sxz, pi, u, uipjp, ujpkp, uipkp, PROF_V, psi[9], psi[10], psi[11],     \
+                                  psi[12], psi[13], psi[14], psi[15], psi[16], psi[17]);                      \
+        }                                                                                                     \
+    } while (0)
     /* forward, ifos3d.c:738-814 */
-    zero_wavefield(NX, NY, NZ, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, NULL, NULL, NULL, NULL, NULL, NULL,
+    zero_wavefield(NX, NY, NZ, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, rm[0], rm[1], rm[2], rm[3], rm[4], rm[5],
                    psi[0], psi[1], psi[2], psi[3], psi[4], psi[5], psi[6], psi[7], psi[8], psi[9], psi[10],
                    psi[11], psi[12], psi[13], psi[14], psi[15], psi[16], psi[17]);
     zero_invers(NX, NY, NZ, F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], F[11], nf, 0);
@@ -235,11 +276,7 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
         update_v_CPML(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], n, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, rho,
                       rjp, rkp, rip, srcpos, signals, signals, signals, 1, absorb, 0, PROF_V, psi[0], psi[1],
                       psi[2], psi[3], psi[4], psi[5], psi[6], psi[7], psi[8]);
-        update_s_elastic(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, pi, u,
-                         uipjp, ujpkp, uipkp);
-        update_s_CPML_elastic(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz, sxz,
-                              pi, u, uipjp, ujpkp, uipkp, PROF_V, psi[9], psi[10], psi[11], psi[12], psi[13],
-                              psi[14], psi[15], psi[16], psi[17]);
+        STRESS_UPDATE();
         psource(n, sxx, syy, szz, srcpos, signals, 1);
         if (obs && n % ntast == 0)
             discfourier(1, NX, 1, NY, 1, NZ, n, vx, vy, vz, F[0], F[1], F[2], F[3], F[4], F[5], (float *)finv, nf,
@@ -261,7 +298,7 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
         for (r = 1; r <= ntr; r++) recsrc[n][r] = (float)recpos[n][r];
 
     /* adjoint, ifos3d.c:905-975 */
-    zero_wavefield(NX, NY, NZ, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, NULL, NULL, NULL, NULL, NULL, NULL,
+    zero_wavefield(NX, NY, NZ, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, rm[0], rm[1], rm[2], rm[3], rm[4], rm[5],
                    psi[0], psi[1], psi[2], psi[3], psi[4], psi[5], psi[6], psi[7], psi[8], psi[9], psi[10],
                    psi[11], psi[12], psi[13], psi[14], psi[15], psi[16], psi[17]);
     for (n = 1; n <= NT; n++) {
@@ -271,11 +308,7 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
         update_v_CPML(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], n, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, rho,
                       rjp, rkp, rip, recsrc, diff[0], diff[1], diff[2], ntr, absorb, 1, PROF_V, psi[0], psi[1],
                       psi[2], psi[3], psi[4], psi[5], psi[6], psi[7], psi[8]);
-        update_s_elastic(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, pi, u,
-                         uipjp, ujpkp, uipkp);
-        update_s_CPML_elastic(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1], vx, vy, vz, sxx, syy, szz, sxy, syz, sxz,
-                              pi, u, uipjp, ujpkp, uipkp, PROF_V, psi[9], psi[10], psi[11], psi[12], psi[13],
-                              psi[14], psi[15], psi[16], psi[17]);
+        STRESS_UPDATE();
         if (n % ntast == 0)
             discfourier(1, NX, 1, NY, 1, NZ, n, vx, vy, vz, F[6], F[7], F[8], F[9], F[10], F[11], (float *)finv, nf,
                         ntast, 0, 1);
@@ -307,6 +340,12 @@ done:
     for (n = 1; n <= 3; n++) free(recpos[n]);
     free(recpos + 1);
     free(sig);
+    if (L) {
+        for (n = 0; n < 6; n++) nr3_free(rm[n], 1, 1, 1);
+        nr3_free(taus, 0, 0, 0); nr3_free(taup, 0, 0, 0);
+        for (n = 0; n < 3; n++) nr3_free(tsa[n], 1, 1, 1);
+        free(eta);
+    }
     (void)j;
     return rc;
 }

@@ -44,11 +44,13 @@ def _setup(ig, g, finv, tast, prec):
     return G
 
 
-def _plain(ig, G, g, true, start, srcs, rec, finv, tast, prec):
+def _plain(ig, G, g, true, start, srcs, rec, finv, tast, prec, visco=None):
     """observed data, seismograms, misfits and accumulated gradient, one shot at a time"""
     ctx = ig.ShotContext(G, len(finv), len(rec), prec)
     ctx.set_cpml(4000.0, 50.0)
     ctx.set_freqs(finv, tast)
+    if visco is not None:
+        ctx.set_visco(G.to_dev(visco[0]), G.to_dev(visco[1]), visco[2])
     ntr = len(rec)
     obs, syn, l2 = [], [], []
     d_syn = ig.DBuf((3, ntr, g.nt + 1))
@@ -70,11 +72,14 @@ def _plain(ig, G, g, true, start, srcs, rec, finv, tast, prec):
     return out
 
 
-def _batched(ig, G, g, true, start, srcs, rec, finv, tast, prec, chunk, use_graph, obs=None, tiles=None):
+def _batched(ig, G, g, true, start, srcs, rec, finv, tast, prec, chunk, use_graph, obs=None, tiles=None,
+             visco=None):
     ns, ntr = len(srcs), len(rec)
     b = ig.BatchContext(G, ns, len(finv), ntr, prec, chunk)
     if tiles is not None:
         b.set_tiles(*tiles)
+    if visco is not None:
+        b.set_visco(G.to_dev(visco[0]), G.to_dev(visco[1]), visco[2])
     b.set_cpml(4000.0, 50.0)
     b.set_freqs(finv, tast)
     d_syn = ig.DBuf((ns, 3, ntr, g.nt + 1))
@@ -204,3 +209,27 @@ def test_batch_autotune_keeps_results():
     for n in range(3):
         assert max_ulp(G.from_dev(grads[n]), ref["grads"][n]) == 0, f"gradient {n}"
     b.close()
+
+
+@pytest.mark.parametrize("tiles", [None, ((4, 0, 1, 1), (2, 0, 4, 1))])
+def test_batch_viscoelastic(tiles):
+    """the batched path with the viscoelastic update against the plain one"""
+    import ifosgpu as ig
+    c = CASES["slab_sources"]
+    g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=100)
+    rec = np.array([[9 + 2 * n, 10, 19] for n in range(8)] + [[20, 5, 21]], np.int32)
+    finv, tast = [40.0, 60.0], 100
+    rng = np.random.default_rng(21)
+    visco = ((0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
+             (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32), np.float32(0.019))
+    _note(f"batched viscoelastic, tiles {tiles}")
+    true, start = _models(g)
+    G = _setup(ig, g, finv, tast, ig.DFT_COMPAT)
+    ref = _plain(ig, G, g, true, start, c["srcs"], rec, finv, tast, ig.DFT_COMPAT, visco=visco)
+    out = _batched(ig, G, g, true, start, c["srcs"], rec, finv, tast, ig.DFT_COMPAT, 4, False, tiles=tiles,
+                   visco=visco)
+    for n in range(len(c["srcs"])):
+        assert max_ulp(out["syn"][n], ref["syn"][n]) == 0, f"synthetics, shot {n}"
+    for n in range(3):
+        assert max_ulp(out["grads"][n], ref["grads"][n]) == 0, f"gradient {n}"
+    out["ctx"].close()

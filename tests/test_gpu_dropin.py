@@ -73,3 +73,30 @@ def test_dropin_force_source_forward():
     syn, _, _ = run_dropin(g, start, src, 1, cs.SHOT_REC, None, [50.0])
     for c in range(3):
         assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0
+
+
+def test_dropin_viscoelastic_shot():
+    """L = 1: the driver calls update_s / update_s_CPML (and av_mat forms the
+    relaxation-ratio averages); the oracle's viscoelastic shot must match"""
+    g = cs.shot_grid(0)
+    print("drop-in viscoelastic gradient shot", flush=True)
+    o = Oracle("f32")
+    start, true = cs.shot_models(g)
+    rng = np.random.default_rng(12)
+    taus = (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32)
+    taup = (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32)
+    eta = np.float32(0.017)
+    vis = (taus, taup, eta)
+    obs = o.run_shot(g, *true, cs.SHOT_SRC, cs.SHOT_REC, cs.SHOT_FINV, want_adjoint=False, visco=vis)["syn"]
+    ref = o.run_shot(g, *start, cs.SHOT_SRC, cs.SHOT_REC, cs.SHOT_FINV, obs=obs, visco=vis)
+    lib = C.CDLL(DRIVER)
+    lib.dd_set_visco(_p(taus), _p(taup), C.c_float(eta))
+    try:
+        syn, grads, l2 = run_dropin(g, start, cs.SHOT_SRC, 1, cs.SHOT_REC, obs, cs.SHOT_FINV)
+    finally:
+        lib.dd_set_visco(None, None, C.c_float(0.0))
+    for c in range(3):
+        assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0, f"synthetic comp {c}"
+    assert np.float32(l2) == np.float32(ref["l2"])
+    for n in range(3):
+        assert max_ulp(g.interior(grads[n]), g.interior(ref["grads"][n])) == 0, f"grad{n + 1}"

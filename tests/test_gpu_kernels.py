@@ -120,6 +120,58 @@ def test_stress_update_regions(env, free_surf, fdcoeff):
         assert np.array_equal(dpsi2[n].numpy(), po[n]), f"fused psi_v[{n}]"
 
 
+@pytest.mark.parametrize("free_surf,fdcoeff", [(0, 2), (1, 1)])
+def test_visco_stress_update_regions(env, free_surf, fdcoeff):
+    """update_s / update_s_CPML (one relaxation mechanism), two steps so the
+    memory variables carry over; interior+CPML launches and the fused one"""
+    _, ig = env
+    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=29)
+    _note(f"viscoelastic stress update parity fdcoeff={fdcoeff} free_surf={free_surf}")
+    o, rng, rho, pi, u, avg, prof, f, psi = _setup(g, 31 + fdcoeff + 2 * free_surf)
+    taus = (0.01 + 0.05 * rng.random(g.shape)).astype(np.float32)
+    taup = (0.01 + 0.05 * rng.random(g.shape)).astype(np.float32)
+    for a in (taus, taup):
+        o.matcopy_wrap(g, a)
+    tsa = o.av_tau(g, rho, u, taus)
+    r6 = [(1e-2 * rng.standard_normal(g.shape)).astype(np.float32) for _ in range(6)]
+    eta = np.float32(0.0137)
+    G = _geom(ig, g)
+    # averages on the device
+    dtaus = G.to_dev(taus)
+    dtsa = [G.empty() for _ in range(3)]
+    ig.av_tau(G, dtaus, dtsa)
+    ig.sync()
+    for n in range(3):
+        assert max_ulp(g.interior(G.from_dev(dtsa[n])), g.interior(tsa[n])) == 0, f"tau average {n}"
+    box = o.box(g)
+    fo = [a.copy() for a in f]
+    po = [p.copy() for p in psi]
+    ro = [a.copy() for a in r6]
+    V = o.visco(ro, taus, taup, *tsa, eta)
+    for _ in range(2):
+        o.update_s_visco(g, box, fo, pi, u, avg[0], avg[1], avg[2], V)
+        o.update_s_cpml_visco(g, box, fo, pi, u, avg[0], avg[1], avg[2], prof, po, V)
+    dtaup = G.to_dev(taup)
+    for fused in (False, True):
+        dmodel, df, dpsi, dprof = _upload(ig, G, rho, pi, u, avg, f, psi, prof)
+        dr = [G.to_dev(a) for a in r6]
+        vs = ig.visco_struct(dr, dtaus, dtaup, *dtsa, eta)
+        for _ in range(2):
+            if fused:
+                ig.update_s_visco(G, ig.REGION_ALL, df, dmodel, vs, dprof, dpsi)
+            else:
+                ig.update_s_visco(G, ig.REGION_INTERIOR, df, dmodel, vs, dprof, dpsi)
+                ig.update_s_visco(G, ig.REGION_CPML, df, dmodel, vs, dprof, dpsi)
+        ig.sync()
+        tag = "fused " if fused else ""
+        for n in range(3, 9):
+            assert max_ulp(g.interior(G.from_dev(df[n])), g.interior(fo[n])) == 0, tag + NAMES[n]
+        for n in range(6):
+            assert max_ulp(g.interior(G.from_dev(dr[n])), g.interior(ro[n])) == 0, f"{tag}r[{n}]"
+        for n in range(9):
+            assert np.array_equal(dpsi[n].numpy(), po[n]), f"{tag}psi_v[{n}]"
+
+
 def test_sources_and_receivers(env):
     _, ig = env
     g = small_grid()

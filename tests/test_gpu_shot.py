@@ -112,3 +112,43 @@ def test_force_source_shot():
     ctx.close()
     for c in range(3):
         assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0
+
+
+@pytest.mark.parametrize("free_surf", [0, 1])
+def test_viscoelastic_gradient_shot(free_surf):
+    """the plain path with the viscoelastic update (one relaxation
+    mechanism) against the oracle's viscoelastic shot, bit for bit"""
+    import ifosgpu as ig
+    g = small_grid(nx=32, ny=30, nz=34, fw=7, nt=120, free_surf=free_surf)
+    src = (16, 11, 17, 0.0, 50.0, 1.0, 1)
+    rec = np.array([[8 + 2 * n, 9, 17] for n in range(8)], np.int32)
+    finv = [40.0, 70.0]
+    _note(f"viscoelastic gradient shot free_surf={free_surf}")
+    o = Oracle("f32")
+    start, true = _models(g, 3)
+    rng = np.random.default_rng(7)
+    taus = (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32)
+    taup = (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32)
+    eta = np.float32(0.021)
+    vis = (taus, taup, eta)
+    obs = o.run_shot(g, *true, src, rec, finv, want_adjoint=False, visco=vis)["syn"]
+    ref = o.run_shot(g, *start, src, rec, finv, obs=obs, visco=vis)
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt, g.fdcoeff, g.free_surf)
+    ctx = ig.ShotContext(G, len(finv), len(rec))
+    ctx.set_cpml(4000.0, 50.0)
+    ctx.set_freqs(finv)
+    ctx.set_visco(G.to_dev(taus), G.to_dev(taup), eta)
+    ctx.set_model(*[G.to_dev(a) for a in start])
+    d_obs = ig.DBuf.from_numpy(np.stack(obs))
+    d_syn = ig.DBuf((3, len(rec), g.nt + 1))
+    grads = [G.empty() for _ in range(3)]
+    l2 = ctx.run_shot(src, rec, d_syn, grads, d_obs=d_obs)
+    ig.sync()
+    syn = d_syn.numpy()
+    for c in range(3):
+        assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0, f"synthetic comp {c}"
+    for n in range(3):
+        got = g.interior(G.from_dev(grads[n]))
+        assert max_ulp(got, g.interior(ref["grads"][n])) == 0, f"gradient {n}"
+    assert abs(l2 - ref["l2"]) <= 1e-5 * abs(ref["l2"])
+    ctx.close()
