@@ -399,7 +399,7 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->nf = 0;
     b->tast = 100;
     b->ntr = 0;
-    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 1, 0};
+    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0};
     b->L = 0;
     b->eta = 0.0f;
     for (int n = 0; n < 6; ++n) b->r[n] = nullptr;
@@ -846,9 +846,10 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
 // the batch's own buffers (ns shots, `reps` launches each) and keeps the
 // fastest.  The wavefields are zeroed afterwards.
 static const MarchCfg k_candidates[] = {
-    {4, 0, 1, 0},  {4, 0, 0, 0},  {4, 0, 16, 0}, {4, 0, 64, 0}, {4, 0, 256, 0}, {8, 0, 0, 0},
-    {8, 0, 64, 0}, {2, 0, 0, 0},  {2, 0, 64, 0}, {4, 32, 0, 0}, {4, 64, 0, 0},  {4, 16, 0, 0},
-    {8, 32, 0, 0}, {1, 0, 0, 0},  {4, 0, 0, 1},  {8, 0, 0, 1},  {4, 32, 0, 1},
+    {4, 0, 1, 0, 0},  {4, 0, 0, 0, 0},  {4, 0, 64, 0, 0}, {8, 0, 0, 0, 0},  {2, 0, 0, 0, 0},
+    {4, 32, 0, 0, 0}, {4, 0, 1, 0, 1},  {4, 0, 0, 0, 1},  {4, 0, 64, 0, 1}, {8, 0, 0, 0, 1},
+    {2, 0, 0, 0, 1},  {4, 32, 0, 0, 1}, {8, 32, 0, 0, 1}, {1, 0, 0, 0, 1},  {4, 0, 0, 1, 1},
+    {8, 0, 0, 1, 1},  {2, 0, 16, 0, 1},
 };
 
 extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v, int *best_s, float *ms_v,
@@ -920,15 +921,16 @@ extern "C" int ifos_batch_tune_candidates(int *cfg, int max)
 {
     const int nc = (int)(sizeof(k_candidates) / sizeof(k_candidates[0]));
     for (int c = 0; c < nc && c < max; ++c) {
-        cfg[4 * c] = k_candidates[c].ti;
-        cfg[4 * c + 1] = k_candidates[c].jc;
-        cfg[4 * c + 2] = k_candidates[c].group;
-        cfg[4 * c + 3] = k_candidates[c].split;
+        cfg[5 * c] = k_candidates[c].ti;
+        cfg[5 * c + 1] = k_candidates[c].jc;
+        cfg[5 * c + 2] = k_candidates[c].group;
+        cfg[5 * c + 3] = k_candidates[c].split;
+        cfg[5 * c + 4] = k_candidates[c].pipe;
     }
     return nc;
 }
 
-extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[4], const int cfg_s[4])
+extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[5], const int cfg_s[5])
 {
     if (!b) return IFOS_EINVAL;
     for (const int *c : {cfg_v, cfg_s})
@@ -936,8 +938,8 @@ extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[4], const int
             set_error("ifos_batch_set_tiles: TI must be 1, 2, 4 or 8");
             return IFOS_EINVAL;
         }
-    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3]};
-    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3]};
+    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3], cfg_v[4]};
+    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3], cfg_s[4]};
     drop_graphs(b);
     return IFOS_OK;
 }

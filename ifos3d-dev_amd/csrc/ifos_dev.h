@@ -114,10 +114,12 @@ struct BoxList {
 // Tile shape of a march launch: TI i-rows per block (1, 2, 4, 8), jc
 // j-planes per tile (<= 0: automatic), `group` consecutive tiles per shot
 // before the next shot (1: shots fastest, <= 0: one shot after the other),
-// split: interior box and CPML slabs in separate launches.
+// split: interior box and CPML slabs in separate launches; pipe: the next
+// plane's operands are loaded one iteration ahead (two planes in flight).
 struct MarchCfg {
     int ti, jc, group;
     int split;
+    int pipe;
 };
 
 // Raw buffer access: one 32-bit byte offset per thread, the neighbour

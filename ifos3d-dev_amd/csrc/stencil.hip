@@ -221,7 +221,31 @@ __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
 #define LD(r, dj, di, dk) bload(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
 #define ST(v, r) bstore(v, r, vo + 8, 2 * SJ + 2 * SI)
 
-template <int TI, bool CPML, bool SNAP>
+// Operands of one plane j of the velocity march: the in-plane neighbours,
+// the model, the old velocities, and the j-queue values first needed at j
+// (sxy, syz at j+1, syy at j+2).  D = 0 loads plane j of the current offset,
+// D = 1 the next plane (prefetched one iteration ahead when PIPE).
+struct VPlane {
+    float sxx_a, sxx_b, sxx_c, sxx_d, sxy_a, sxy_b, sxy_d, sxz_xa, sxz_xb, sxz_0, sxz_xd, sxz_za, sxz_zb, sxz_zd,
+        syz_za, syz_zb, syz_zd, szz_a, szz_b, szz_c, szz_d, rip, rjp, rkp, vx0, vy0, vz0, sxy_n, syz_n, syy_n;
+};
+#define VLOAD(P, D)                                                                                            \
+    do {                                                                                                       \
+        P.sxx_a = LD(rsxx, D, -1, 0); P.sxx_b = LD(rsxx, D, 0, 0); P.sxx_c = LD(rsxx, D, 1, 0);                  \
+        P.sxx_d = LD(rsxx, D, 2, 0);                                                                           \
+        P.sxy_a = LD(rsxy, D, -2, 0); P.sxy_b = LD(rsxy, D, -1, 0); P.sxy_d = LD(rsxy, D, 1, 0);                 \
+        P.sxz_xa = LD(rsxz, D, -2, 0); P.sxz_xb = LD(rsxz, D, -1, 0); P.sxz_0 = LD(rsxz, D, 0, 0);               \
+        P.sxz_xd = LD(rsxz, D, 1, 0);                                                                          \
+        P.sxz_za = LD(rsxz, D, 0, -2); P.sxz_zb = LD(rsxz, D, 0, -1); P.sxz_zd = LD(rsxz, D, 0, 1);              \
+        P.syz_za = LD(rsyz, D, 0, -2); P.syz_zb = LD(rsyz, D, 0, -1); P.syz_zd = LD(rsyz, D, 0, 1);              \
+        P.szz_a = LD(rszz, D, 0, -1); P.szz_b = LD(rszz, D, 0, 0); P.szz_c = LD(rszz, D, 0, 1);                  \
+        P.szz_d = LD(rszz, D, 0, 2);                                                                           \
+        P.rip = LD(rrip, D, 0, 0); P.rjp = LD(rrjp, D, 0, 0); P.rkp = LD(rrkp, D, 0, 0);                         \
+        P.vx0 = LD(rvx, D, 0, 0); P.vy0 = LD(rvy, D, 0, 0); P.vz0 = LD(rvz, D, 0, 0);                            \
+        P.sxy_n = LD(rsxy, (D) + 1, 0, 0); P.syz_n = LD(rsyz, (D) + 1, 0, 0); P.syy_n = LD(rsyy, (D) + 2, 0, 0); \
+    } while (0)
+
+template <int TI, bool CPML, bool SNAP, bool PIPE>
 __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VFields F, Prof P, Psi9 psi, Shots S,
                                                     Snap N)
 {
@@ -245,48 +269,37 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
     const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
 
-    // j queues: sxy and syz at j-2..j+1, syy at j-1..j+2
-    float sxy_m2 = LD(rsxy, -2, 0, 0), sxy_m1 = LD(rsxy, -1, 0, 0), sxy_0 = LD(rsxy, 0, 0, 0),
-          sxy_p1 = LD(rsxy, 1, 0, 0);
-    float syz_m2 = LD(rsyz, -2, 0, 0), syz_m1 = LD(rsyz, -1, 0, 0), syz_0 = LD(rsyz, 0, 0, 0),
-          syz_p1 = LD(rsyz, 1, 0, 0);
-    float syy_m1 = LD(rsyy, -1, 0, 0), syy_0 = LD(rsyy, 0, 0, 0), syy_p1 = LD(rsyy, 1, 0, 0),
-          syy_p2 = LD(rsyy, 2, 0, 0);
+    // j queues: sxy and syz at j-2..j, syy at j-1..j+1 (the next element of
+    // each arrives with the plane's operands)
+    float sxy_m2 = LD(rsxy, -2, 0, 0), sxy_m1 = LD(rsxy, -1, 0, 0), sxy_0 = LD(rsxy, 0, 0, 0);
+    float syz_m2 = LD(rsyz, -2, 0, 0), syz_m1 = LD(rsyz, -1, 0, 0), syz_0 = LD(rsyz, 0, 0, 0);
+    float syy_m1 = LD(rsyy, -1, 0, 0), syy_0 = LD(rsyy, 0, 0, 0), syy_p1 = LD(rsyy, 1, 0, 0);
+    VPlane cur, nxt;
+    VLOAD(cur, 0);
 
     for (int j = jb; j <= je; ++j, vo += SJ) {
-        const float sxx_a = LD(rsxx, 0, -1, 0), sxx_b = LD(rsxx, 0, 0, 0), sxx_c = LD(rsxx, 0, 1, 0),
-                    sxx_d = LD(rsxx, 0, 2, 0);
-        const float sxy_a = LD(rsxy, 0, -2, 0), sxy_b = LD(rsxy, 0, -1, 0), sxy_d = LD(rsxy, 0, 1, 0);
-        const float sxz_xa = LD(rsxz, 0, -2, 0), sxz_xb = LD(rsxz, 0, -1, 0), sxz_0 = LD(rsxz, 0, 0, 0),
-                    sxz_xd = LD(rsxz, 0, 1, 0);
-        const float sxz_za = LD(rsxz, 0, 0, -2), sxz_zb = LD(rsxz, 0, 0, -1), sxz_zd = LD(rsxz, 0, 0, 1);
-        const float syz_za = LD(rsyz, 0, 0, -2), syz_zb = LD(rsyz, 0, 0, -1), syz_zd = LD(rsyz, 0, 0, 1);
-        const float szz_a = LD(rszz, 0, 0, -1), szz_b = LD(rszz, 0, 0, 0), szz_c = LD(rszz, 0, 0, 1),
-                    szz_d = LD(rszz, 0, 0, 2);
-        const float rip = LD(rrip, 0, 0, 0), rjp = LD(rrjp, 0, 0, 0), rkp = LD(rrkp, 0, 0, 0);
-        const float vx0 = LD(rvx, 0, 0, 0), vy0 = LD(rvy, 0, 0, 0), vz0 = LD(rvz, 0, 0, 0);
-        float sxx_x = g.dtdx * D4F(b1, b2, sxx_a, sxx_b, sxx_c, sxx_d);
-        float sxy_y = g.dtdy * D4B(b1, b2, sxy_m2, sxy_m1, sxy_0, sxy_p1);
-        float sxz_z = g.dtdz * D4B(b1, b2, sxz_za, sxz_zb, sxz_0, sxz_zd);
-        float syy_y = g.dtdy * D4F(b1, b2, syy_m1, syy_0, syy_p1, syy_p2);
-        float sxy_x = g.dtdx * D4B(b1, b2, sxy_a, sxy_b, sxy_0, sxy_d);
-        float syz_z = g.dtdz * D4B(b1, b2, syz_za, syz_zb, syz_0, syz_zd);
-        float szz_z = g.dtdz * D4F(b1, b2, szz_a, szz_b, szz_c, szz_d);
-        float sxz_x = g.dtdx * D4B(b1, b2, sxz_xa, sxz_xb, sxz_0, sxz_xd);
-        float syz_y = g.dtdy * D4B(b1, b2, syz_m2, syz_m1, syz_0, syz_p1);
+        if (PIPE && j < je) VLOAD(nxt, 1);
+        float sxx_x = g.dtdx * D4F(b1, b2, cur.sxx_a, cur.sxx_b, cur.sxx_c, cur.sxx_d);
+        float sxy_y = g.dtdy * D4B(b1, b2, sxy_m2, sxy_m1, sxy_0, cur.sxy_n);
+        float sxz_z = g.dtdz * D4B(b1, b2, cur.sxz_za, cur.sxz_zb, cur.sxz_0, cur.sxz_zd);
+        float syy_y = g.dtdy * D4F(b1, b2, syy_m1, syy_0, syy_p1, cur.syy_n);
+        float sxy_x = g.dtdx * D4B(b1, b2, cur.sxy_a, cur.sxy_b, sxy_0, cur.sxy_d);
+        float syz_z = g.dtdz * D4B(b1, b2, cur.syz_za, cur.syz_zb, syz_0, cur.syz_zd);
+        float szz_z = g.dtdz * D4F(b1, b2, cur.szz_a, cur.szz_b, cur.szz_c, cur.szz_d);
+        float sxz_x = g.dtdx * D4B(b1, b2, cur.sxz_xa, cur.sxz_xb, cur.sxz_0, cur.sxz_xd);
+        float syz_y = g.dtdy * D4B(b1, b2, syz_m2, syz_m1, syz_0, cur.syz_n);
         if (CPML) {
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z,
                    szz_z);
         }
-        const float nvx = vx0 + (sxx_x + sxy_y + sxz_z) / rip;
-        const float nvy = vy0 + (syy_y + sxy_x + syz_z) / rjp;
-        const float nvz = vz0 + (szz_z + sxz_x + syz_y) / rkp;
-        if (j < je) {  // next plane's queue operands, issued before this plane's stores
-            sxy_m2 = sxy_m1; sxy_m1 = sxy_0; sxy_0 = sxy_p1; sxy_p1 = LD(rsxy, 2, 0, 0);
-            syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1; syz_p1 = LD(rsyz, 2, 0, 0);
-            syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = syy_p2; syy_p2 = LD(rsyy, 3, 0, 0);
-        }
+        const float nvx = cur.vx0 + (sxx_x + sxy_y + sxz_z) / cur.rip;
+        const float nvy = cur.vy0 + (syy_y + sxy_x + syz_z) / cur.rjp;
+        const float nvz = cur.vz0 + (szz_z + sxz_x + syz_y) / cur.rkp;
+        sxy_m2 = sxy_m1; sxy_m1 = sxy_0; sxy_0 = cur.sxy_n;
+        syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = cur.syz_n;
+        syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = cur.syy_n;
+        if (!PIPE && j < je) VLOAD(nxt, 1);  // next plane's operands, issued before this plane's stores
         ST(nvx, rvx);
         ST(nvy, rvy);
         ST(nvz, rvz);
@@ -295,10 +308,40 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
             bstore(nvy, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + g.fbytes);
             bstore(nvz, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + 2 * g.fbytes);
         }
+        cur = nxt;
     }
 }
+#undef VLOAD
 
-template <int TI, bool CPML, bool VISCO>
+// Operands of one plane j of the stress march (plus the j-queue values first
+// needed at j: vx, vz at j+2, vy at j+1), and for the viscoelastic update
+// the memory variables and relaxation ratios of the cell.
+struct SPlane {
+    float vx_xa, vx_xb, vx_xd, vx_zb, vx_zc, vx_zd, vy_xb, vy_xc, vy_xd, vy_zb, vy_zc, vy_zd, vz_xb, vz_xc, vz_xd,
+        vz_za, vz_zb, vz_zd, uipjp, ujpkp, uipkp, pi, u, sxx0, syy0, szz0, sxy0, syz0, sxz0, vx_n, vz_n, vy_n;
+    float r0[6], ts, tp, tsipjp, tsjpkp, tsipkp;
+};
+#define SLOAD(P, D)                                                                                            \
+    do {                                                                                                       \
+        P.vx_xa = LD(rvx, D, -2, 0); P.vx_xb = LD(rvx, D, -1, 0); P.vx_xd = LD(rvx, D, 1, 0);                    \
+        P.vx_zb = LD(rvx, D, 0, -1); P.vx_zc = LD(rvx, D, 0, 1); P.vx_zd = LD(rvx, D, 0, 2);                     \
+        P.vy_xb = LD(rvy, D, -1, 0); P.vy_xc = LD(rvy, D, 1, 0); P.vy_xd = LD(rvy, D, 2, 0);                     \
+        P.vy_zb = LD(rvy, D, 0, -1); P.vy_zc = LD(rvy, D, 0, 1); P.vy_zd = LD(rvy, D, 0, 2);                     \
+        P.vz_xb = LD(rvz, D, -1, 0); P.vz_xc = LD(rvz, D, 1, 0); P.vz_xd = LD(rvz, D, 2, 0);                     \
+        P.vz_za = LD(rvz, D, 0, -2); P.vz_zb = LD(rvz, D, 0, -1); P.vz_zd = LD(rvz, D, 0, 1);                    \
+        P.uipjp = LD(ruipjp, D, 0, 0); P.ujpkp = LD(rujpkp, D, 0, 0); P.uipkp = LD(ruipkp, D, 0, 0);             \
+        P.pi = LD(rpi, D, 0, 0); P.u = LD(ru, D, 0, 0);                                                        \
+        P.sxx0 = LD(rsxx, D, 0, 0); P.syy0 = LD(rsyy, D, 0, 0); P.szz0 = LD(rszz, D, 0, 0);                      \
+        P.sxy0 = LD(rsxy, D, 0, 0); P.syz0 = LD(rsyz, D, 0, 0); P.sxz0 = LD(rsxz, D, 0, 0);                      \
+        P.vx_n = LD(rvx, (D) + 2, 0, 0); P.vz_n = LD(rvz, (D) + 2, 0, 0); P.vy_n = LD(rvy, (D) + 1, 0, 0);       \
+        if (VISCO) {                                                                                           \
+            for (int q_ = 0; q_ < 6; ++q_) P.r0[q_] = LD(rr[q_], D, 0, 0);                                     \
+            P.ts = LD(rtaus, D, 0, 0); P.tp = LD(rtaup, D, 0, 0);                                              \
+            P.tsipjp = LD(rtsipjp, D, 0, 0); P.tsjpkp = LD(rtsjpkp, D, 0, 0); P.tsipkp = LD(rtsipkp, D, 0, 0);  \
+        }                                                                                                      \
+    } while (0)
+
+template <int TI, bool CPML, bool VISCO, bool PIPE>
 __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S,
                                                     ViscoF V)
 {
@@ -332,31 +375,24 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
     const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
 
-    // j queues: vx, vz at j-1..j+2, vy at j-2..j+1
-    float vx_m1 = LD(rvx, -1, 0, 0), vx_0 = LD(rvx, 0, 0, 0), vx_p1 = LD(rvx, 1, 0, 0), vx_p2 = LD(rvx, 2, 0, 0);
-    float vz_m1 = LD(rvz, -1, 0, 0), vz_0 = LD(rvz, 0, 0, 0), vz_p1 = LD(rvz, 1, 0, 0), vz_p2 = LD(rvz, 2, 0, 0);
-    float vy_m2 = LD(rvy, -2, 0, 0), vy_m1 = LD(rvy, -1, 0, 0), vy_0 = LD(rvy, 0, 0, 0), vy_p1 = LD(rvy, 1, 0, 0);
+    // j queues: vx, vz at j-1..j+1, vy at j-2..j
+    float vx_m1 = LD(rvx, -1, 0, 0), vx_0 = LD(rvx, 0, 0, 0), vx_p1 = LD(rvx, 1, 0, 0);
+    float vz_m1 = LD(rvz, -1, 0, 0), vz_0 = LD(rvz, 0, 0, 0), vz_p1 = LD(rvz, 1, 0, 0);
+    float vy_m2 = LD(rvy, -2, 0, 0), vy_m1 = LD(rvy, -1, 0, 0), vy_0 = LD(rvy, 0, 0, 0);
+    SPlane cur, nxt;
+    SLOAD(cur, 0);
 
     for (int j = jb; j <= je; ++j, vo += SJ) {
-        const float vx_xa = LD(rvx, 0, -2, 0), vx_xb = LD(rvx, 0, -1, 0), vx_xd = LD(rvx, 0, 1, 0);
-        const float vx_zb = LD(rvx, 0, 0, -1), vx_zc = LD(rvx, 0, 0, 1), vx_zd = LD(rvx, 0, 0, 2);
-        const float vy_xb = LD(rvy, 0, -1, 0), vy_xc = LD(rvy, 0, 1, 0), vy_xd = LD(rvy, 0, 2, 0);
-        const float vy_zb = LD(rvy, 0, 0, -1), vy_zc = LD(rvy, 0, 0, 1), vy_zd = LD(rvy, 0, 0, 2);
-        const float vz_xb = LD(rvz, 0, -1, 0), vz_xc = LD(rvz, 0, 1, 0), vz_xd = LD(rvz, 0, 2, 0);
-        const float vz_za = LD(rvz, 0, 0, -2), vz_zb = LD(rvz, 0, 0, -1), vz_zd = LD(rvz, 0, 0, 1);
-        const float uipjp = LD(ruipjp, 0, 0, 0), ujpkp = LD(rujpkp, 0, 0, 0), uipkp = LD(ruipkp, 0, 0, 0);
-        const float pi = LD(rpi, 0, 0, 0), u = LD(ru, 0, 0, 0);
-        const float sxx0 = LD(rsxx, 0, 0, 0), syy0 = LD(rsyy, 0, 0, 0), szz0 = LD(rszz, 0, 0, 0);
-        const float sxy0 = LD(rsxy, 0, 0, 0), syz0 = LD(rsyz, 0, 0, 0), sxz0 = LD(rsxz, 0, 0, 0);
-        float vxx = D4B(b1, b2, vx_xa, vx_xb, vx_0, vx_xd) / DX;
-        float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, vx_p2) / DY;
-        float vxz = D4F(b1, b2, vx_zb, vx_0, vx_zc, vx_zd) / DZ;
-        float vyx = D4F(b1, b2, vy_xb, vy_0, vy_xc, vy_xd) / DX;
-        float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, vy_p1) / DY;
-        float vyz = D4F(b1, b2, vy_zb, vy_0, vy_zc, vy_zd) / DZ;
-        float vzx = D4F(b1, b2, vz_xb, vz_0, vz_xc, vz_xd) / DX;
-        float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, vz_p2) / DY;
-        float vzz = D4B(b1, b2, vz_za, vz_zb, vz_0, vz_zd) / DZ;
+        if (PIPE && j < je) SLOAD(nxt, 1);
+        float vxx = D4B(b1, b2, cur.vx_xa, cur.vx_xb, vx_0, cur.vx_xd) / DX;
+        float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, cur.vx_n) / DY;
+        float vxz = D4F(b1, b2, cur.vx_zb, vx_0, cur.vx_zc, cur.vx_zd) / DZ;
+        float vyx = D4F(b1, b2, cur.vy_xb, vy_0, cur.vy_xc, cur.vy_xd) / DX;
+        float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, cur.vy_n) / DY;
+        float vyz = D4F(b1, b2, cur.vy_zb, vy_0, cur.vy_zc, cur.vy_zd) / DZ;
+        float vzx = D4F(b1, b2, cur.vz_xb, vz_0, cur.vz_xc, cur.vz_xd) / DX;
+        float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, cur.vz_n) / DY;
+        float vzz = D4B(b1, b2, cur.vz_za, cur.vz_zb, vz_0, cur.vz_zd) / DZ;
         if (CPML) {
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
@@ -364,48 +400,44 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
         const float vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
         const float vxxyyzz = vxx + vyy + vzz;
         const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
-        float nsxy, nsyz, nsxz, nsxx, nsyy, nszz;
+        float sn[6], rn[6];
         if (!VISCO) {
             // update_s_ssg_elastic.c:117-138
-            const float fipjp = uipjp * DT, fjpkp = ujpkp * DT, fipkp = uipkp * DT;
-            const float f = 2.0f * u;
-            nsxy = sxy0 + (fipjp * vxyyx);
-            nsyz = syz0 + (fjpkp * vyzzy);
-            nsxz = sxz0 + (fipkp * vxzzx);
-            nsxx = sxx0 + DT * ((pi * vxxyyzz) - (f * vyyzz));
-            nsyy = syy0 + DT * ((pi * vxxyyzz) - (f * vxxzz));
-            nszz = szz0 + DT * ((pi * vxxyyzz) - (f * vxxyy));
+            const float fipjp = cur.uipjp * DT, fjpkp = cur.ujpkp * DT, fipkp = cur.uipkp * DT;
+            const float f = 2.0f * cur.u;
+            sn[3] = cur.sxy0 + (fipjp * vxyyx);
+            sn[4] = cur.syz0 + (fjpkp * vyzzy);
+            sn[5] = cur.sxz0 + (fipkp * vxzzx);
+            sn[0] = cur.sxx0 + DT * ((cur.pi * vxxyyzz) - (f * vyyzz));
+            sn[1] = cur.syy0 + DT * ((cur.pi * vxxyyzz) - (f * vxxzz));
+            sn[2] = cur.szz0 + DT * ((cur.pi * vxxyyzz) - (f * vxxyy));
         } else {
-            const float ts = LD(rtaus, 0, 0, 0), tp = LD(rtaup, 0, 0, 0);
-            const float tsipjp = LD(rtsipjp, 0, 0, 0), tsjpkp = LD(rtsjpkp, 0, 0, 0),
-                        tsipkp = LD(rtsipkp, 0, 0, 0);
-            float rn[6] = {LD(rr[0], 0, 0, 0), LD(rr[1], 0, 0, 0), LD(rr[2], 0, 0, 0),
-                           LD(rr[3], 0, 0, 0), LD(rr[4], 0, 0, 0), LD(rr[5], 0, 0, 0)};
-            float sn[6] = {sxx0, syy0, szz0, sxy0, syz0, sxz0};
-            stress_visco(DT, V.eta, V.L, pi, u, uipjp, ujpkp, uipkp, ts, tp, tsipjp, tsjpkp, tsipkp, vxyyx, vyzzy,
-                         vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn, rn);
-            nsxx = sn[0]; nsyy = sn[1]; nszz = sn[2]; nsxy = sn[3]; nsyz = sn[4]; nsxz = sn[5];
-            const float nrxx = rn[0], nryy = rn[1], nrzz = rn[2], nrxy = rn[3], nryz = rn[4], nrxz = rn[5];
-            ST(nrxx, rr[0]);
-            ST(nryy, rr[1]);
-            ST(nrzz, rr[2]);
-            ST(nrxy, rr[3]);
-            ST(nryz, rr[4]);
-            ST(nrxz, rr[5]);
+            sn[0] = cur.sxx0; sn[1] = cur.syy0; sn[2] = cur.szz0;
+            sn[3] = cur.sxy0; sn[4] = cur.syz0; sn[5] = cur.sxz0;
+#pragma unroll
+            for (int n = 0; n < 6; ++n) rn[n] = cur.r0[n];
+            stress_visco(DT, V.eta, V.L, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
+                         cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn,
+                         rn);
         }
-        if (j < je) {
-            vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2; vx_p2 = LD(rvx, 3, 0, 0);
-            vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_p2; vz_p2 = LD(rvz, 3, 0, 0);
-            vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = LD(rvy, 2, 0, 0);
+        vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = cur.vx_n;
+        vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = cur.vz_n;
+        vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = cur.vy_n;
+        if (!PIPE && j < je) SLOAD(nxt, 1);
+        ST(sn[3], rsxy);
+        ST(sn[4], rsyz);
+        ST(sn[5], rsxz);
+        ST(sn[0], rsxx);
+        ST(sn[1], rsyy);
+        ST(sn[2], rszz);
+        if (VISCO) {
+#pragma unroll
+            for (int n = 0; n < 6; ++n) ST(rn[n], rr[n]);
         }
-        ST(nsxy, rsxy);
-        ST(nsyz, rsyz);
-        ST(nsxz, rsxz);
-        ST(nsxx, rsxx);
-        ST(nsyy, rsyy);
-        ST(nszz, rszz);
+        cur = nxt;
     }
 }
+#undef SLOAD
 #undef LD
 #undef ST
 
@@ -592,22 +624,45 @@ static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int g
 
 // default tile shape of the march kernels (i-rows per block, j chunk,
 // shot grouping); the batched path's autotuner overrides it per kernel
-static MarchCfg g_default_cfg = {4, 0, 0, 0};
+static MarchCfg g_default_cfg = {4, 0, 0, 0, 0};
+
+template <bool CPML, bool SNAP, bool PIPE>
+static void launch_v_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
+                        const Psi9 &psi, const Shots &S, const Snap &N, hipStream_t s)
+{
+    switch (ti) {
+    case 1: hipLaunchKernelGGL((k_v_march<1, CPML, SNAP, PIPE>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, N); break;
+    case 2: hipLaunchKernelGGL((k_v_march<2, CPML, SNAP, PIPE>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, N); break;
+    case 4: hipLaunchKernelGGL((k_v_march<4, CPML, SNAP, PIPE>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, N); break;
+    default: hipLaunchKernelGGL((k_v_march<8, CPML, SNAP, PIPE>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, N); break;
+    }
+}
 
 template <bool CPML, bool SNAP>
 static int launch_v(const MarchCfg &c, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
                     const Psi9 &psi, const Shots &S, const Snap &N, hipStream_t s)
 {
     if (T.total == 0) return IFOS_OK;
-    const dim3 grid((unsigned)(T.per * 8));
-    switch (c.ti) {
-    case 1: hipLaunchKernelGGL((k_v_march<1, CPML, SNAP>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, N); break;
-    case 2: hipLaunchKernelGGL((k_v_march<2, CPML, SNAP>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, N); break;
-    case 4: hipLaunchKernelGGL((k_v_march<4, CPML, SNAP>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, N); break;
-    case 8: hipLaunchKernelGGL((k_v_march<8, CPML, SNAP>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, N); break;
-    default: set_error("march: TI=%d not in {1,2,4,8}", c.ti); return IFOS_EINVAL;
+    if (c.ti != 1 && c.ti != 2 && c.ti != 4 && c.ti != 8) {
+        set_error("march: TI=%d not in {1,2,4,8}", c.ti);
+        return IFOS_EINVAL;
     }
+    const dim3 grid((unsigned)(T.per * 8));
+    if (c.pipe) launch_v_ti<CPML, SNAP, true>(c.ti, grid, T, G, F, P, psi, S, N, s);
+    else launch_v_ti<CPML, SNAP, false>(c.ti, grid, T, G, F, P, psi, S, N, s);
     return hip_check(hipGetLastError(), "k_v_march");
+}
+
+template <bool CPML, bool VISCO, bool PIPE>
+static void launch_s_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
+                        const Psi9 &psi, const Shots &S, const ViscoF &V, hipStream_t s)
+{
+    switch (ti) {
+    case 1: hipLaunchKernelGGL((k_s_march<1, CPML, VISCO, PIPE>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, V); break;
+    case 2: hipLaunchKernelGGL((k_s_march<2, CPML, VISCO, PIPE>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
+    case 4: hipLaunchKernelGGL((k_s_march<4, CPML, VISCO, PIPE>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
+    default: hipLaunchKernelGGL((k_s_march<8, CPML, VISCO, PIPE>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
+    }
 }
 
 template <bool CPML, bool VISCO = false>
@@ -615,14 +670,13 @@ static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const 
                     const Psi9 &psi, const Shots &S, hipStream_t s, const ViscoF &V = ViscoF{})
 {
     if (T.total == 0) return IFOS_OK;
-    const dim3 grid((unsigned)(T.per * 8));
-    switch (c.ti) {
-    case 1: hipLaunchKernelGGL((k_s_march<1, CPML, VISCO>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, V); break;
-    case 2: hipLaunchKernelGGL((k_s_march<2, CPML, VISCO>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
-    case 4: hipLaunchKernelGGL((k_s_march<4, CPML, VISCO>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
-    case 8: hipLaunchKernelGGL((k_s_march<8, CPML, VISCO>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
-    default: set_error("march: TI=%d not in {1,2,4,8}", c.ti); return IFOS_EINVAL;
+    if (c.ti != 1 && c.ti != 2 && c.ti != 4 && c.ti != 8) {
+        set_error("march: TI=%d not in {1,2,4,8}", c.ti);
+        return IFOS_EINVAL;
     }
+    const dim3 grid((unsigned)(T.per * 8));
+    if (c.pipe) launch_s_ti<CPML, VISCO, true>(c.ti, grid, T, G, F, P, psi, S, V, s);
+    else launch_s_ti<CPML, VISCO, false>(c.ti, grid, T, G, F, P, psi, S, V, s);
     return hip_check(hipGetLastError(), "k_s_march");
 }
 

@@ -486,16 +486,16 @@ class BatchContext:
         return list(arr)
 
     def set_tiles(self, cfg_v=None, cfg_s=None):
-        """tile shapes {TI, jc, group, split} of the velocity / stress marches"""
-        cv = (C.c_int * 4)(*cfg_v) if cfg_v is not None else None
-        cs = (C.c_int * 4)(*cfg_s) if cfg_s is not None else None
+        """tile shapes {TI, jc, group, split, pipe} of the velocity / stress marches"""
+        cv = (C.c_int * 5)(*(tuple(cfg_v) + (0,) * (5 - len(cfg_v)))) if cfg_v is not None else None
+        cs = (C.c_int * 5)(*(tuple(cfg_s) + (0,) * (5 - len(cfg_s)))) if cfg_s is not None else None
         check(lib().ifos_batch_set_tiles(self.h, cv, cs), "ifos_batch_set_tiles")
 
     @staticmethod
     def tune_candidates():
-        buf = (C.c_int * 256)()
+        buf = (C.c_int * 320)()
         n = lib().ifos_batch_tune_candidates(buf, 64)
-        return [tuple(buf[4 * c:4 * c + 4]) for c in range(n)]
+        return [tuple(buf[5 * c:5 * c + 5]) for c in range(n)]
 
     def autotune(self, ns=None, reps=3):
         """times every candidate tile shape; returns (best_v, best_s, table)"""

@@ -166,8 +166,8 @@ def test_batch_partial_and_rejects():
     b.close()
 
 
-@pytest.mark.parametrize("tiles", [((4, 0, 1, 0), (2, 0, 4, 1)), ((8, 5, 0, 1), (1, 0, 16, 0)),
-                                   ((1, 3, 2, 1), (8, 0, 1, 1))])
+@pytest.mark.parametrize("tiles", [((4, 0, 1, 0, 0), (2, 0, 4, 1, 1)), ((8, 5, 0, 1, 1), (1, 0, 16, 0, 0)),
+                                   ((1, 3, 2, 1, 0), (8, 0, 1, 1, 1)), ((2, 4, 3, 0, 1), (4, 7, 0, 0, 1))])
 def test_batch_tile_shapes(tiles):
     """every tile shape the autotuner may pick gives the same bits"""
     import ifosgpu as ig
@@ -211,7 +211,7 @@ def test_batch_autotune_keeps_results():
     b.close()
 
 
-@pytest.mark.parametrize("tiles", [None, ((4, 0, 1, 1), (2, 0, 4, 1))])
+@pytest.mark.parametrize("tiles", [None, ((4, 0, 1, 1, 1), (2, 0, 4, 1, 1))])
 def test_batch_viscoelastic(tiles):
     """the batched path with the viscoelastic update against the plain one"""
     import ifosgpu as ig
