@@ -295,105 +295,11 @@ __global__ __launch_bounds__(256) void k_copy(v4f *__restrict__ dst, const v4f *
 }
 
 // ---------------------------------------------------------------------------
-// gradient_F.c:66-139.  One thread per cell, frequencies in order.
-struct G9 {
-    float xx, xy, xz, yx, yy, yz, zx, zy, zz;
-};
-
-__device__ inline G9 grad9(const DevGeom &g, const float *fx, const float *fy, const float *fz, int64_t c)
-{
-    const int64_t sj = g.sj, si = g.si;
-    const float b1 = g.b1, b2 = g.b2;
-    G9 o;
-    o.xx = (b1 * (fx[c] - fx[c - si]) + b2 * (fx[c + si] - fx[c - 2 * si])) / g.dx;
-    o.xy = (b1 * (fx[c + sj] - fx[c]) + b2 * (fx[c + 2 * sj] - fx[c - sj])) / g.dy;
-    o.xz = (b1 * (fx[c + 1] - fx[c]) + b2 * (fx[c + 2] - fx[c - 1])) / g.dz;
-    o.yx = (b1 * (fy[c + si] - fy[c]) + b2 * (fy[c + 2 * si] - fy[c - si])) / g.dx;
-    o.yy = (b1 * (fy[c] - fy[c - sj]) + b2 * (fy[c + sj] - fy[c - 2 * sj])) / g.dy;
-    o.yz = (b1 * (fy[c + 1] - fy[c]) + b2 * (fy[c + 2] - fy[c - 1])) / g.dz;
-    o.zx = (b1 * (fz[c + si] - fz[c]) + b2 * (fz[c + 2 * si] - fz[c - si])) / g.dx;
-    o.zy = (b1 * (fz[c + sj] - fz[c]) + b2 * (fz[c + 2 * sj] - fz[c - sj])) / g.dy;
-    o.zz = (b1 * (fz[c] - fz[c - 1]) + b2 * (fz[c + 1] - fz[c - 2])) / g.dz;
-    return o;
-}
-
+// gradient_F.c:70-72: 1 / (4 pi^2 f^2) of each frequency, the double
+// product rounded to float before the reciprocal
 struct GradFreq {
     float fdummy[IFOS_MAX_NF];
 };
-
-// one shot's contribution to the three gradients of cell c, frequencies in
-// order (gradient_F.c:79-139)
-__device__ inline void grad_accum(const DevGeom &g, const FPtrs &Ff, const FPtrs &Fb, int nf, int64_t fstride,
-                                  const GradFreq &GF, float r, float p, float m, int64_t c, float &a1, float &a2,
-                                  float &a3)
-{
-    for (int l = 0; l < nf; ++l) {
-        const int64_t co = c + (int64_t)l * fstride;
-        const float fd = GF.fdummy[l];
-        G9 f = grad9(g, Ff.p[0], Ff.p[1], Ff.p[2], co);
-        G9 b = grad9(g, Fb.p[0], Fb.p[1], Fb.p[2], co);
-        G9 fi = grad9(g, Ff.p[3], Ff.p[4], Ff.p[5], co);
-        G9 bi = grad9(g, Fb.p[3], Fb.p[4], Fb.p[5], co);
-        float gl = (f.xx + f.yy + f.zz) * (b.xx + b.yy + b.zz) + (fi.xx + fi.yy + fi.zz) * (bi.xx + bi.yy + bi.zz);
-        gl = -gl * fd;
-        float gm = 2.0f * f.xx * b.xx + 2.0f * f.yy * b.yy + 2.0f * f.zz * b.zz + 2.0f * fi.xx * bi.xx +
-                   2.0f * fi.yy * bi.yy + 2.0f * fi.zz * bi.zz + (f.xy + f.yx) * (b.xy + b.yx) +
-                   (fi.xy + fi.yx) * (bi.xy + bi.yx) + (f.xz + f.zx) * (b.xz + b.zx) +
-                   (fi.xz + fi.zx) * (bi.xz + bi.zx) + (f.yz + f.zy) * (b.yz + b.zy) +
-                   (fi.yz + fi.zy) * (bi.yz + bi.zy);
-        gm = -gm * fd;
-        float gr = (Ff.p[0][co] * Fb.p[0][co] + Ff.p[1][co] * Fb.p[1][co] + Ff.p[2][co] * Fb.p[2][co]) +
-                   (Ff.p[3][co] * Fb.p[3][co] + Ff.p[4][co] * Fb.p[4][co] + Ff.p[5][co] * Fb.p[5][co]);
-        const double srp = sqrt((double)(r * p)), sru = sqrt((double)(r * m));
-        a1 = (float)(a1 + srp * 2 * gl);
-        a2 = (float)(a2 + (-4 * sru * gl + 2 * sru * gm));
-        a3 += gr + m / r * gm + (p - 2.0f * m) / r * gl;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_gradient(DevGeom g, FPtrs Ff, FPtrs Fb, int nf, int64_t fstride,
-                                                 GradFreq GF, const float *rho, const float *pi, const float *u,
-                                                 float *g1, float *g2, float *g3)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int j = blockIdx.z + 1;
-    if (k > g.nz || i > g.nx) return;
-    const int64_t c = at(g, j, i, k);
-    float a1 = g1[c], a2 = g2[c], a3 = g3[c];
-    grad_accum(g, Ff, Fb, nf, fstride, GF, rho[c], pi[c], u[c], c, a1, a2, a3);
-    g1[c] = a1;
-    g2[c] = a2;
-    g3[c] = a3;
-}
-
-// several shots accumulated in shot order, as consecutive gradient_F calls
-// would (shot s's DFT fields at +s*fshot elements)
-__global__ __launch_bounds__(256) void k_gradient_shots(DevGeom g, FPtrs Ff, FPtrs Fb, int nf, int64_t fstride,
-                                                       int64_t fshot, int ns, GradFreq GF, const float *rho,
-                                                       const float *pi, const float *u, float *g1, float *g2,
-                                                       float *g3)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int j = blockIdx.z + 1;
-    if (k > g.nz || i > g.nx) return;
-    const int64_t c = at(g, j, i, k);
-    const float r = rho[c], p = pi[c], m = u[c];
-    float a1 = g1[c], a2 = g2[c], a3 = g3[c];
-    for (int s = 0; s < ns; ++s) {
-        FPtrs fs = Ff, bs = Fb;
-#pragma unroll
-        for (int n = 0; n < 6; ++n) {
-            fs.p[n] += (int64_t)s * fshot;
-            bs.p[n] += (int64_t)s * fshot;
-        }
-        grad_accum(g, fs, bs, nf, fstride, GF, r, p, m, c, a1, a2, a3);
-    }
-    g1[c] = a1;
-    g2[c] = a2;
-    g3[c] = a3;
-}
 
 // ---------------------------------------------------------------------------
 // matcopy.c:31-307 (single rank): three passes, y faces then x then z
@@ -629,17 +535,20 @@ extern "C" int ifos_gradient_F(const ifos_geom *g, float *const d_Ff[6], float *
         float fd = (float)(2.0 * 2.0 * finv[l] * finv[l] * M_PI * M_PI);
         GF.fdummy[l] = 1 / fd;
     }
-    FPtrs Ff, Fb;
-    for (int n = 0; n < 6; ++n) {
-        Ff.p[n] = d_Ff[n];
-        Fb.p[n] = d_Fb[n];
+    (void)G;
+    // one march launch per frequency, in order (gradient_F.c:66-139)
+    const size_t fe = ifos_field_elems(g);
+    ProfScope ps_(IFOS_K_GRADIENT, (hipStream_t)stream);
+    for (int l = 0; l < nf; ++l) {
+        const float *f[12];
+        for (int n = 0; n < 6; ++n) {
+            f[n] = d_Ff[n] + (size_t)l * fe;
+            f[6 + n] = d_Fb[n] + (size_t)l * fe;
+        }
+        int rc = launch_grad_march(g, f, GF.fdummy[l], m, d_grad, (hipStream_t)stream);
+        if (rc) return rc;
     }
-    dim3 block, grid = cell_grid(g, block);
-    { ProfScope ps_(IFOS_K_GRADIENT, (hipStream_t)stream);
-    hipLaunchKernelGGL(k_gradient, grid, block, 0, (hipStream_t)stream, G, Ff, Fb, nf,
-                       (int64_t)ifos_field_elems(g), GF, m->rho, m->pi, m->u, d_grad[0], d_grad[1], d_grad[2]);
-    }
-    return hip_check(hipGetLastError(), "k_gradient");
+    return IFOS_OK;
 }
 
 namespace ifos {
@@ -652,16 +561,21 @@ int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const
         float fd = (float)(2.0 * 2.0 * finv[l] * finv[l] * M_PI * M_PI);
         GF.fdummy[l] = 1 / fd;
     }
-    FPtrs Ff, Fb;
-    for (int n = 0; n < 6; ++n) {
-        Ff.p[n] = d_Ff[n];
-        Fb.p[n] = d_Fb[n];
-    }
-    dim3 block, grid = cell_grid(g, block);
+    (void)G;
+    // shots in order, frequencies in order: one march launch each
+    const size_t fe = ifos_field_elems(g);
     ProfScope ps_(IFOS_K_GRADIENT, st);
-    hipLaunchKernelGGL(k_gradient_shots, grid, block, 0, st, G, Ff, Fb, nf, (int64_t)ifos_field_elems(g), fshot, ns,
-                       GF, m->rho, m->pi, m->u, d_grad[0], d_grad[1], d_grad[2]);
-    return hip_check(hipGetLastError(), "k_gradient_shots");
+    for (int s = 0; s < ns; ++s)
+        for (int l = 0; l < nf; ++l) {
+            const float *f[12];
+            for (int n = 0; n < 6; ++n) {
+                f[n] = d_Ff[n] + (size_t)s * fshot + (size_t)l * fe;
+                f[6 + n] = d_Fb[n] + (size_t)s * fshot + (size_t)l * fe;
+            }
+            int rc = launch_grad_march(g, f, GF.fdummy[l], m, d_grad, st);
+            if (rc) return rc;
+        }
+    return IFOS_OK;
 }
 }  // namespace ifos
 

@@ -187,17 +187,19 @@ __global__ void k_bsurface(DevGeom g, BFields F, float *sxy, float *syz, const f
     szz[c] += h;
 }
 
-// chunked DFT: nsnap snapshots of one shot's v, every accumulator of the
-// cell updated nsnap times in registers.  trig[(c*NF + l)*2 + {0,1}] are the
-// cos/sin factors of snapshot c (disc_fourier.c:42-44).  One thread per cell
-// of [1..N]^3 and shot (blockIdx.z = shot * ny + j - 1).
+// chunked DFT: nsnap (<= CH) snapshots of one shot's v, every accumulator
+// of the cell updated nsnap times in registers.  trig[(c*NF + l)*2 + {0,1}]
+// are the cos/sin factors of snapshot c (disc_fourier.c:42-44).  One thread
+// per cell of [1..N]^3 and shot (blockIdx.z = shot * ny + j - 1); the
+// frequencies are walked in order with the next frequency's accumulators
+// loaded while the current one is updated.
 struct FBase {
     float *p[6];
 };
 
-template <int NF, bool KAHAN>
-__global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring, int64_t rstride, uint32_t rbytes,
-                                                  int nsnap, FBase F, FBase Cc, int64_t fshot, uint32_t tot,
+template <int NF, int CH, bool KAHAN>
+__global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring, int64_t rstride, int nsnap,
+                                                  FBase F, FBase Cc, int64_t fshot, uint32_t tot,
                                                   const double *__restrict__ trig)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -205,35 +207,42 @@ __global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring,
     const int j = (int)(blockIdx.z % (unsigned)g.ny) + 1;
     const int s = (int)(blockIdx.z / (unsigned)g.ny);
     if (k > g.nz || i > g.nx) return;
-    const uint32_t vo = (uint32_t)at(g, j, i, k) * 4u;
-    const rsrc_t rr = make_rsrc(ring + (int64_t)s * rstride, rbytes);
-    float v[IFOS_CHUNK_MAX][3];
+    const int64_t cell = at(g, j, i, k);
+    const uint32_t vo = (uint32_t)cell * 4u;
+    const int64_t fe = g.fbytes / 4;
+    const float *rp = ring + (int64_t)s * rstride + cell;
+    float v[CH][3];
 #pragma unroll
-    for (int c = 0; c < IFOS_CHUNK_MAX; ++c)
+    for (int c = 0; c < CH; ++c)
         if (c < nsnap)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) v[c][n] = bload(rr, vo, (uint32_t)(3 * c + n) * g.fbytes);
+            for (int n = 0; n < 3; ++n) v[c][n] = __builtin_nontemporal_load(rp + (3 * c + n) * fe);
     rsrc_t r[6], rc[6];
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
         r[n] = make_rsrc(F.p[n] + (int64_t)s * fshot, tot);
         if (KAHAN) rc[n] = make_rsrc(Cc.p[n] + (int64_t)s * fshot, tot);
     }
-    float a[6][NF], cc[6][NF];
+    float a[6], cm[6], an[6], cn[6];
 #pragma unroll
-    for (int l = 0; l < NF; ++l)
+    for (int n = 0; n < 6; ++n) {
+        a[n] = bload(r[n], vo, 0);
+        cm[n] = KAHAN ? bload(rc[n], vo, 0) : 0.0f;
+    }
 #pragma unroll
-        for (int n = 0; n < 6; ++n) {
-            a[n][l] = bload(r[n], vo, (uint32_t)l * g.fbytes);
-            if (KAHAN) cc[n][l] = bload(rc[n], vo, (uint32_t)l * g.fbytes);
+    for (int l = 0; l < NF; ++l) {
+        if (l + 1 < NF) {
+#pragma unroll
+            for (int n = 0; n < 6; ++n) {
+                an[n] = bload(r[n], vo, (uint32_t)(l + 1) * g.fbytes);
+                cn[n] = KAHAN ? bload(rc[n], vo, (uint32_t)(l + 1) * g.fbytes) : 0.0f;
+            }
         }
 #pragma unroll
-    for (int l = 0; l < NF; ++l)
-#pragma unroll
         for (int n = 0; n < 6; ++n) {
-            float acc = a[n][l], cmp = KAHAN ? cc[n][l] : 0.0f;
+            float acc = a[n], cmp = cm[n];
 #pragma unroll
-            for (int c = 0; c < IFOS_CHUNK_MAX; ++c)
+            for (int c = 0; c < CH; ++c)
                 if (c < nsnap) {
                     const double term = (double)v[c][n % 3] * trig[(c * NF + l) * 2 + (n < 3 ? 0 : 1)];
                     if (KAHAN) {
@@ -248,6 +257,14 @@ __global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring,
             bstore(acc, r[n], vo, (uint32_t)l * g.fbytes);
             if (KAHAN) bstore(cmp, rc[n], vo, (uint32_t)l * g.fbytes);
         }
+        if (l + 1 < NF) {
+#pragma unroll
+            for (int n = 0; n < 6; ++n) {
+                a[n] = an[n];
+                cm[n] = cn[n];
+            }
+        }
+    }
 }
 
 static __global__ void k_bfill(float *d, int64_t n, float v)
@@ -347,8 +364,8 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
         return IFOS_EINVAL;
     }
     const size_t fe = ifos_field_elems(g);
-    if ((double)chunk * 3.0 * fe * 4.0 >= 4294967295.0 || (double)nf_max * fe * 4.0 >= 4294967295.0) {
-        set_error("ifos_batch_create: snapshot ring or DFT array of one shot exceeds 4 GiB (lower chunk)");
+    if (3.0 * fe * 4.0 >= 4294967295.0 || (double)nf_max * fe * 4.0 >= 4294967295.0) {
+        set_error("ifos_batch_create: one snapshot or the DFT array of one shot exceeds 4 GiB");
         return IFOS_EINVAL;
     }
     if ((size_t)nshots * g->ny > 65535) {
@@ -535,21 +552,26 @@ static int dft_chunk(ifos_batch *b, int back, int ns, int q0, int nsnap, hipStre
         if (b->C[0]) Cc.p[n] = b->C[(back ? 6 : 0) + n];
     }
     const int64_t rstride = (int64_t)b->chunk * 3 * b->fe;
-    const uint32_t rbytes = (uint32_t)(rstride * 4);
     const int64_t fshot = (int64_t)b->nf_max * b->fe;
     const uint32_t tot = (uint32_t)((size_t)b->nf * b->fe * 4);
     const double *tr = b->trig[back] + (size_t)q0 * b->nf * 2;
     dim3 block(64, 4), grid((unsigned)((g->nz + 63) / 64), (unsigned)((g->nx + 3) / 4), (unsigned)(ns * g->ny));
     const bool kahan = b->prec == IFOS_DFT_KAHAN;
     ProfScope ps_(IFOS_K_DFT, st);
-#define IFOS_CHUNK_CASE(NFV)                                                                                      \
-    case NFV:                                                                                                     \
-        if (kahan)                                                                                                \
-            hipLaunchKernelGGL((k_dft_chunk<NFV, true>), grid, block, 0, st, G, b->ring, rstride, rbytes, nsnap, F, \
-                               Cc, fshot, tot, tr);                                                               \
-        else                                                                                                      \
-            hipLaunchKernelGGL((k_dft_chunk<NFV, false>), grid, block, 0, st, G, b->ring, rstride, rbytes, nsnap,  \
-                               F, Cc, fshot, tot, tr);                                                            \
+#define IFOS_CHUNK_LAUNCH(NFV, CHV)                                                                             \
+    do {                                                                                                        \
+        if (kahan)                                                                                              \
+            hipLaunchKernelGGL((k_dft_chunk<NFV, CHV, true>), grid, block, 0, st, G, b->ring, rstride, nsnap, F, \
+                               Cc, fshot, tot, tr);                                                             \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_dft_chunk<NFV, CHV, false>), grid, block, 0, st, G, b->ring, rstride, nsnap,   \
+                               F, Cc, fshot, tot, tr);                                                          \
+    } while (0)
+#define IFOS_CHUNK_CASE(NFV)                                                                                    \
+    case NFV:                                                                                                   \
+        if (b->chunk <= 4) IFOS_CHUNK_LAUNCH(NFV, 4);                                                           \
+        else if (b->chunk <= 8) IFOS_CHUNK_LAUNCH(NFV, 8);                                                      \
+        else IFOS_CHUNK_LAUNCH(NFV, 16);                                                                        \
         break;
     switch (b->nf) {
         IFOS_CHUNK_CASE(1)
@@ -562,6 +584,7 @@ static int dft_chunk(ifos_batch *b, int back, int ns, int q0, int nsnap, hipStre
         IFOS_CHUNK_CASE(8)
     default: set_error("dft_chunk: nf=%d", b->nf); return IFOS_EINVAL;
     }
+#undef IFOS_CHUNK_LAUNCH
 #undef IFOS_CHUNK_CASE
     return hip_check(hipGetLastError(), "k_dft_chunk");
 }
@@ -581,7 +604,6 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
     const Shots S = {ns, (int64_t)b->fe, {b->pe[0], b->pe[1], b->pe[2]}};
     const BFields BF = {f.vx, f.vy, f.vz, f.sxx, f.syy, f.szz, (int64_t)b->fe};
     const int64_t rstride = (int64_t)b->chunk * 3 * b->fe;
-    const uint32_t rbytes = (uint32_t)(rstride * 4);
     const int ntr = b->ntr, nt2 = g->nt + 2;
     const dim3 rgrid((unsigned)((ntr + 255) / 256), (unsigned)ns);
     const Snap nosnap = {nullptr, 0, 0, 0};
@@ -600,7 +622,7 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
     for (int nt = 1; nt <= g->nt; ++nt) {
         const bool samp = dft && nt % b->ntast == 0 && (nt / b->ntast) <= b->nsamp;
         Snap N = nosnap;
-        if (samp) N = Snap{b->ring, rstride, rbytes, (uint32_t)((size_t)nsnap * 3 * b->fe * 4)};
+        if (samp) N = Snap{b->ring + (size_t)nsnap * 3 * b->fe, rstride, (uint32_t)(3 * (size_t)b->fe * 4), 0};
         int rc = 0;
         // sources in CPML cells: before the fused update
         if (!back && b->any_slab_src) {

@@ -83,8 +83,9 @@ struct Shots {
     int64_t fstride;
     int64_t pstride[3];
 };
-// velocity snapshot ring of the batched path: shot s, component n of the
-// current slot at byte (s*rstride)*4 + soff + n*fbytes + cell offset
+// velocity snapshot of the batched path: shot s, component n of the slot at
+// ring + s*rstride + soff/4 + n*(field elements) + cell (ring points at the
+// slot; rbytes covers the slot's three components)
 struct Snap {
     float *ring;
     int64_t rstride;
@@ -154,6 +155,9 @@ int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
 // fields at d_F*[n] + s*fshot
 int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,
                           const ifos_model *m, const float *finv, int nf, float *const d_grad[3], hipStream_t st);
+// gradient_F for one frequency of one shot, march kernel (stencil.hip)
+int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, const ifos_model *m,
+                      float *const d_grad[3], hipStream_t s);
 void set_error(const char *fmt, ...);
 int hip_check(hipError_t e, const char *what);
 

@@ -438,6 +438,95 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
     }
 }
 #undef SLOAD
+// ---------------------------------------------------------------------------
+// gradient_F.c:79-139 for one frequency of one shot as a march over j: the
+// y-derivatives of the 12 DFT fields come from 4-deep register queues and the
+// three gradients are read, updated and written once per call.  Calls in
+// frequency order (and shot order) repeat the reference's per-cell
+// accumulation order exactly.  f[0..2] forward Re x y z, f[3..5] forward Im,
+// f[6..8] adjoint Re, f[9..11] adjoint Im.
+struct GradArgs {
+    const float *f[12];
+    float fd;  // 1 / (4 pi^2 f^2) as gradient_F.c:70-72 rounds it
+    const float *rho, *pi, *u;
+    float *g1, *g2, *g3;
+};
+
+template <int TI>
+__global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, GradArgs A)
+{
+    int k, i, jb, je, shot;
+    Box R;
+    if (!decode_tile(T, TI, k, i, jb, je, shot, R)) return;
+    const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
+    const float b1 = g.b1, b2 = g.b2;
+    rsrc_t rf[12];
+#pragma unroll
+    for (int n = 0; n < 12; ++n) rf[n] = make_rsrc(A.f[n], g.fbytes);
+    const rsrc_t rrho = make_rsrc(A.rho, g.fbytes), rpi = make_rsrc(A.pi, g.fbytes), ru = make_rsrc(A.u, g.fbytes),
+                 rg1 = make_rsrc(A.g1, g.fbytes), rg2 = make_rsrc(A.g2, g.fbytes), rg3 = make_rsrc(A.g3, g.fbytes);
+    uint32_t vo = (uint32_t)((int64_t)jb * g.sj + (int64_t)i * g.si + k + IFOS_KOFF - 2) * 4u;
+    // queues per triple t: x component at j-1..j+2, y at j-2..j+1, z at j-1..j+2
+    float qx[4][4], qy[4][4], qz[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        qx[t][0] = LD(rf[3 * t], -1, 0, 0); qx[t][1] = LD(rf[3 * t], 0, 0, 0); qx[t][2] = LD(rf[3 * t], 1, 0, 0);
+        qy[t][0] = LD(rf[3 * t + 1], -2, 0, 0); qy[t][1] = LD(rf[3 * t + 1], -1, 0, 0);
+        qy[t][2] = LD(rf[3 * t + 1], 0, 0, 0);
+        qz[t][0] = LD(rf[3 * t + 2], -1, 0, 0); qz[t][1] = LD(rf[3 * t + 2], 0, 0, 0);
+        qz[t][2] = LD(rf[3 * t + 2], 1, 0, 0);
+    }
+    for (int j = jb; j <= je; ++j, vo += SJ) {
+        float d[4][9];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const rsrc_t fx = rf[3 * t], fy = rf[3 * t + 1], fz = rf[3 * t + 2];
+            qx[t][3] = LD(fx, 2, 0, 0);
+            qy[t][3] = LD(fy, 1, 0, 0);
+            qz[t][3] = LD(fz, 2, 0, 0);
+            const float x_c = qx[t][1], y_c = qy[t][2], z_c = qz[t][1];
+            d[t][0] = (b1 * (x_c - LD(fx, 0, -1, 0)) + b2 * (LD(fx, 0, 1, 0) - LD(fx, 0, -2, 0))) / g.dx;
+            d[t][1] = (b1 * (qx[t][2] - x_c) + b2 * (qx[t][3] - qx[t][0])) / g.dy;
+            d[t][2] = (b1 * (LD(fx, 0, 0, 1) - x_c) + b2 * (LD(fx, 0, 0, 2) - LD(fx, 0, 0, -1))) / g.dz;
+            d[t][3] = (b1 * (LD(fy, 0, 1, 0) - y_c) + b2 * (LD(fy, 0, 2, 0) - LD(fy, 0, -1, 0))) / g.dx;
+            d[t][4] = (b1 * (y_c - qy[t][1]) + b2 * (qy[t][3] - qy[t][0])) / g.dy;
+            d[t][5] = (b1 * (LD(fy, 0, 0, 1) - y_c) + b2 * (LD(fy, 0, 0, 2) - LD(fy, 0, 0, -1))) / g.dz;
+            d[t][6] = (b1 * (LD(fz, 0, 1, 0) - z_c) + b2 * (LD(fz, 0, 2, 0) - LD(fz, 0, -1, 0))) / g.dx;
+            d[t][7] = (b1 * (qz[t][2] - z_c) + b2 * (qz[t][3] - qz[t][0])) / g.dy;
+            d[t][8] = (b1 * (z_c - LD(fz, 0, 0, -1)) + b2 * (LD(fz, 0, 0, 1) - LD(fz, 0, 0, -2))) / g.dz;
+        }
+        // d[t] = xx xy xz yx yy yz zx zy zz of f (t 0), fi (1), b (2), bi (3)
+        const float *f = d[0], *fi = d[1], *bb = d[2], *bi = d[3];
+        float gl = (f[0] + f[4] + f[8]) * (bb[0] + bb[4] + bb[8]) + (fi[0] + fi[4] + fi[8]) * (bi[0] + bi[4] + bi[8]);
+        gl = -gl * A.fd;
+        float gm = 2.0f * f[0] * bb[0] + 2.0f * f[4] * bb[4] + 2.0f * f[8] * bb[8] + 2.0f * fi[0] * bi[0] +
+                   2.0f * fi[4] * bi[4] + 2.0f * fi[8] * bi[8] + (f[1] + f[3]) * (bb[1] + bb[3]) +
+                   (fi[1] + fi[3]) * (bi[1] + bi[3]) + (f[2] + f[6]) * (bb[2] + bb[6]) +
+                   (fi[2] + fi[6]) * (bi[2] + bi[6]) + (f[5] + f[7]) * (bb[5] + bb[7]) +
+                   (fi[5] + fi[7]) * (bi[5] + bi[7]);
+        gm = -gm * A.fd;
+        const float gr = (qx[0][1] * qx[2][1] + qy[0][2] * qy[2][2] + qz[0][1] * qz[2][1]) +
+                         (qx[1][1] * qx[3][1] + qy[1][2] * qy[3][2] + qz[1][1] * qz[3][1]);
+        const float r = LD(rrho, 0, 0, 0), p = LD(rpi, 0, 0, 0), m = LD(ru, 0, 0, 0);
+        float a1 = LD(rg1, 0, 0, 0), a2 = LD(rg2, 0, 0, 0), a3 = LD(rg3, 0, 0, 0);
+        const double srp = sqrt((double)(r * p)), sru = sqrt((double)(r * m));
+        a1 = (float)(a1 + srp * 2 * gl);
+        a2 = (float)(a2 + (-4 * sru * gl + 2 * sru * gm));
+        a3 += gr + m / r * gm + (p - 2.0f * m) / r * gl;
+        ST(a1, rg1);
+        ST(a2, rg2);
+        ST(a3, rg3);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                qx[t][q] = qx[t][q + 1];
+                qy[t][q] = qy[t][q + 1];
+                qz[t][q] = qz[t][q + 1];
+            }
+    }
+}
+
 #undef LD
 #undef ST
 
@@ -916,4 +1005,27 @@ int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
     return rc;
 }
 
+}  // namespace ifos
+
+namespace ifos {
+// gradient_F over [1..N]^3 for one frequency: f[12] point at that
+// frequency's fields (forward Re xyz, Im xyz, adjoint Re xyz, Im xyz)
+int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, const ifos_model *m,
+                      float *const d_grad[3], hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    GradArgs A;
+    for (int n = 0; n < 12; ++n) A.f[n] = f[n];
+    A.fd = fd;
+    A.rho = m->rho;
+    A.pi = m->pi;
+    A.u = m->u;
+    A.g1 = d_grad[0];
+    A.g2 = d_grad[1];
+    A.g3 = d_grad[2];
+    const Box all = region_box(g, IFOS_REGION_ALL);
+    const MTiles T = make_tiles(&all, 1, 4, 0, 1, 0);
+    hipLaunchKernelGGL(k_grad_march<4>, dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
+    return hip_check(hipGetLastError(), "k_grad_march");
+}
 }  // namespace ifos
