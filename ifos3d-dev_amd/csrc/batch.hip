@@ -25,6 +25,7 @@
 #include <math.h>
 #include <stdio.h>
 
+#include <algorithm>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -287,6 +288,10 @@ struct ifos_batch {
     int64_t pe[3];             // elements of one psi_x / psi_y / psi_z array
     float *fields[9];          // [S][fe] each
     float *psi_s[9], *psi_v[9];
+    // second wavefield set and velocity-derivative memory of the fused step
+    // (double buffering); cur_set says which set holds the current state
+    float *fields_alt[9], *psi_v_alt[9];
+    int fused, fz_ti, fz_jc, cur_set, last_fused;
     float *model[9];
     float *prof;
     float *F[12], *C[12];      // [S][nf_max][fe]
@@ -349,7 +354,7 @@ extern "C" size_t ifos_batch_bytes(const ifos_geom *g, int nshots, int nf_max, i
     const size_t fe = ifos_field_elems(g);
     size_t psi = 0;  // (a viscoelastic batch adds 6 * nshots + 5 fields)
     for (int n = 0; n < 9; ++n) psi += ifos_psi_elems(g, n);
-    size_t per_shot = 9 * fe + 2 * psi + 12 * (size_t)nf_max * fe * (dft_precision == IFOS_DFT_KAHAN ? 2 : 1) +
+    size_t per_shot = 18 * fe + 3 * psi + 12 * (size_t)nf_max * fe * (dft_precision == IFOS_DFT_KAHAN ? 2 : 1) +
                       (size_t)chunk * 3 * fe + (size_t)(g->nt + 2) * (1 + 6 * (size_t)ntr_max);
     return 4 * ((size_t)nshots * per_shot + 9 * fe);
 }
@@ -392,6 +397,8 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     for (int n = 0; n < 9 && !rc; ++n) rc = balloc(b, (void **)&b->fields[n], S * fe * 4);
     for (int n = 0; n < 9 && !rc; ++n) rc = balloc(b, (void **)&b->psi_s[n], S * b->pe[n / 3] * 4);
     for (int n = 0; n < 9 && !rc; ++n) rc = balloc(b, (void **)&b->psi_v[n], S * b->pe[n / 3] * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = balloc(b, (void **)&b->fields_alt[n], S * fe * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = balloc(b, (void **)&b->psi_v_alt[n], S * b->pe[n / 3] * 4);
     for (int n = 0; n < 9 && !rc; ++n) rc = balloc(b, (void **)&b->model[n], fe * 4);
     if (!rc) rc = balloc(b, (void **)&b->prof, 18 * (size_t)(2 * g->fw + 1) * 4);
     for (int n = 0; n < 12; ++n) b->C[n] = nullptr;
@@ -417,6 +424,11 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->tast = 100;
     b->ntr = 0;
     b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0};
+    b->fused = 1;
+    b->fz_ti = 16;
+    b->fz_jc = 0;
+    b->cur_set = 0;
+    b->last_fused = 0;
     b->L = 0;
     b->eta = 0.0f;
     for (int n = 0; n < 6; ++n) b->r[n] = nullptr;
@@ -523,6 +535,14 @@ static ifos_fields bfields(const ifos_batch *b)
     ifos_fields f;
     float **p = (float **)&f;
     for (int n = 0; n < 9; ++n) p[n] = b->fields[n];
+    return f;
+}
+
+static ifos_fields fields_of(float *const *set)
+{
+    ifos_fields f;
+    float **p = (float **)&f;
+    for (int n = 0; n < 9; ++n) p[n] = set[n];
     return f;
 }
 
@@ -680,12 +700,112 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
     return IFOS_OK;
 }
 
+// the fused form of bpass: v(1) alone, then one fused launch per step doing
+// sigma(n) + psource(n) + v(n+1) from one buffer set into the other.  The
+// stress of the last step feeds nothing the pass returns and is not formed.
+// Needs no free surface, no point in a CPML slab and the elastic update.
+static bool fused_ok(const ifos_batch *b)
+{
+    return b->fused && !b->g.free_surf && !b->any_slab_src && !b->any_slab_rec && !b->L;
+}
+
+static int bpass_fused(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t st)
+{
+    const ifos_geom *g = &b->g;
+    DevGeom G = make_dev_geom(g);
+    ifos_model m = bmodel(b);
+    float *cur[9], *nxt[9], *pvc[9], *pvn[9];
+    for (int n = 0; n < 9; ++n) {
+        cur[n] = b->fields[n];
+        nxt[n] = b->fields_alt[n];
+        pvc[n] = b->psi_v[n];
+        pvn[n] = b->psi_v_alt[n];
+    }
+    ifos_psi ps;
+    for (int n = 0; n < 9; ++n) ps.p[n] = b->psi_s[n];
+    const Shots S = {ns, (int64_t)b->fe, {b->pe[0], b->pe[1], b->pe[2]}};
+    const int64_t rstride = (int64_t)b->chunk * 3 * b->fe;
+    const int ntr = b->ntr, nt2 = g->nt + 2;
+    const dim3 rgrid((unsigned)((ntr + 255) / 256), (unsigned)ns);
+    int nsnap = 0, q0 = 0, set = 0;
+    for (int nt = 1; nt <= g->nt; ++nt) {
+        const bool samp = dft && nt % b->ntast == 0 && (nt / b->ntast) <= b->nsamp;
+        Snap N = {nullptr, 0, 0, 0};
+        if (samp) N = Snap{b->ring + (size_t)nsnap * 3 * b->fe, rstride, (uint32_t)(3 * (size_t)b->fe * 4), 0};
+        int rc = 0;
+        if (nt == 1) {
+            ifos_fields f = fields_of(cur);
+            rc = launch_v_shots(g, &f, &m, b->prof, &ps, S, samp ? &N : nullptr, b->cfg_v, st);
+        } else {
+            // sigma(nt-1) + psource(nt-1) + v(nt): cur -> nxt
+            FusedArgs A;
+            A.vx = cur[0]; A.vy = cur[1]; A.vz = cur[2];
+            A.sxx = cur[3]; A.syy = cur[4]; A.szz = cur[5]; A.sxy = cur[6]; A.syz = cur[7]; A.sxz = cur[8];
+            A.nvx = nxt[0]; A.nvy = nxt[1]; A.nvz = nxt[2];
+            A.nsxx = nxt[3]; A.nsyy = nxt[4]; A.nszz = nxt[5]; A.nsxy = nxt[6]; A.nsyz = nxt[7]; A.nsxz = nxt[8];
+            A.pi = m.pi; A.u = m.u; A.uipjp = m.uipjp; A.ujpkp = m.ujpkp; A.uipkp = m.uipkp;
+            A.rip = m.rip; A.rjp = m.rjp; A.rkp = m.rkp;
+            for (int n = 0; n < 9; ++n) {
+                A.psv_old.p[n] = pvc[n];
+                A.psv_new.p[n] = pvn[n];
+                A.pss.p[n] = b->psi_s[n];
+            }
+            A.src = back ? nullptr : b->src;
+            A.sig = b->sig;
+            A.ldsig = nt2;
+            A.nt = nt - 1;
+            A.ntall = g->nt;
+            rc = launch_vs_fused(g, A, b->prof, S, samp ? &N : nullptr, b->fz_ti, b->fz_jc, st);
+            for (int n = 0; n < 9; ++n) {
+                std::swap(cur[n], nxt[n]);
+                std::swap(pvc[n], pvn[n]);
+            }
+            set ^= 1;
+        }
+        const BFields BF = {cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], (int64_t)b->fe};
+        if (!rc && !back) {
+            ProfScope ps_(IFOS_K_SOURCES, st);
+            hipLaunchKernelGGL(k_bforces, dim3(ns), dim3(64), 0, st, G, BF, m.rip, m.rjp, m.rkp, b->src, b->srccls,
+                               b->sig, 1, nt, nt2, 0, N);
+            rc = hip_check(hipGetLastError(), "k_bforces");
+        }
+        if (!rc && back && ntr > 0) {
+            ProfScope ps_(IFOS_K_SOURCES, st);
+            hipLaunchKernelGGL(k_badjoint, rgrid, dim3(256), 0, st, G, BF, b->rec, b->chain, b->rcls, b->adj, ntr,
+                               nt, nt2, 0, N);
+            hipLaunchKernelGGL(k_badjoint, rgrid, dim3(256), 0, st, G, BF, b->rec, b->chain, b->rcls, b->adj, ntr,
+                               nt, nt2, 2, N);
+            rc = hip_check(hipGetLastError(), "k_badjoint");
+        }
+        if (!rc && samp) {
+            ++nsnap;
+            if (nsnap == b->chunk) {
+                rc = dft_chunk(b, back, ns, q0, nsnap, st);
+                q0 += nsnap;
+                nsnap = 0;
+            }
+        }
+        if (!rc && !back && seis && ntr > 0) {
+            ProfScope ps_(IFOS_K_SOURCES, st);
+            hipLaunchKernelGGL(k_bseismo, rgrid, dim3(256), 0, st, G, BF, b->rec, ntr, nt, g->nt + 1, b->syn);
+            rc = hip_check(hipGetLastError(), "k_bseismo");
+        }
+        if (rc) return rc;
+    }
+    b->cur_set = set;
+    if (nsnap > 0) return dft_chunk(b, back, ns, q0, nsnap, st);
+    return IFOS_OK;
+}
+
 static int zero_state(ifos_batch *b, int ns, hipStream_t st)
 {
     int rc = 0;
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->fields[n], (size_t)ns * b->fe, st);
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->psi_s[n], (size_t)ns * b->pe[n / 3], st);
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->psi_v[n], (size_t)ns * b->pe[n / 3], st);
+    for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->fields_alt[n], (size_t)ns * b->fe, st);
+    for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->psi_v_alt[n], (size_t)ns * b->pe[n / 3], st);
+    b->cur_set = 0;
     if (b->L)  // zero_wavefield.c:80-95
         for (int n = 0; n < 6 && !rc; ++n) rc = fill(b->r[n], (size_t)ns * b->fe, st);
     return rc;
@@ -695,14 +815,17 @@ static int zero_state(ifos_batch *b, int ns, hipStream_t st)
 static int run_pass(ifos_batch *b, int back, int ns, int dft, int seis, int use_graph)
 {
     hipStream_t st = b->st;
-    if (!use_graph) return bpass(b, back, ns, dft, seis, st);
+    const bool fz = fused_ok(b);
+    b->last_fused = fz;
+    auto pass = [&](hipStream_t s) { return fz ? bpass_fused(b, back, ns, dft, seis, s) : bpass(b, back, ns, dft, seis, s); };
+    if (!use_graph) return pass(st);
     auto key = std::make_tuple(back, ns, dft, seis, b->ntr, (int)b->any_slab_src, (int)b->any_slab_rec);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         hipGraph_t graph;
         int rc = hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         if (rc) return rc;
-        int rc2 = bpass(b, back, ns, dft, seis, st);
+        int rc2 = pass(st);
         rc = hip_check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
         if (rc2) return rc2;
         if (rc) return rc;
@@ -721,7 +844,8 @@ static int check_probe(ifos_batch *b, int ns, const char *what)
     const size_t off = ifos_offset(g, g->ny / 2, g->nx / 2, g->nz / 2);
     std::vector<float> h(ns);
     for (int s = 0; s < ns; ++s) {
-        int rc = hip_check(hipMemcpyAsync(&h[s], b->fields[1] + (size_t)s * b->fe + off, 4, hipMemcpyDeviceToHost,
+        const float *vy = b->cur_set ? b->fields_alt[1] : b->fields[1];
+        int rc = hip_check(hipMemcpyAsync(&h[s], vy + (size_t)s * b->fe + off, 4, hipMemcpyDeviceToHost,
                                           b->st),
                            "probe");
         if (rc) return rc;
@@ -965,6 +1089,24 @@ extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[5], const int
     drop_graphs(b);
     return IFOS_OK;
 }
+
+// the fused step on (when the configuration allows it) or off, its
+// workgroup rows (8 or 16) and j chunk (<= 0: automatic)
+extern "C" int ifos_batch_set_fused(ifos_batch *b, int on, int ti, int jc)
+{
+    if (!b || (ti != 8 && ti != 16)) {
+        set_error("ifos_batch_set_fused: ti must be 8 or 16");
+        return IFOS_EINVAL;
+    }
+    b->fused = on != 0;
+    b->fz_ti = ti;
+    b->fz_jc = jc;
+    drop_graphs(b);
+    return IFOS_OK;
+}
+
+// 1 if the last run used the fused step
+extern "C" int ifos_batch_last_fused(const ifos_batch *b) { return b ? b->last_fused : 0; }
 
 // DFT accumulators of one shot of the last batch (device pointers into the
 // batch's arrays; frequency l of component n at d_F[n] + l * field elements)

@@ -155,6 +155,24 @@ int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
 // fields at d_F*[n] + s*fshot
 int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,
                           const ifos_model *m, const float *finv, int nf, float *const d_grad[3], hipStream_t st);
+// arguments of the fused stress(n) + velocity(n+1) step (stencil.hip)
+struct FusedArgs {
+    const float *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *syz, *sxz;  // v^n, sigma^(n-1)
+    float *nvx, *nvy, *nvz, *nsxx, *nsyy, *nszz, *nsxy, *nsyz, *nsxz;  // v^(n+1), sigma^n
+    const float *pi, *u, *uipjp, *ujpkp, *uipkp, *rip, *rjp, *rkp;
+    Psi9 psv_old, psv_new;  // velocity-derivative memory: read, written by the owner
+    Psi9 pss;               // stress-derivative memory, updated in place by the owner
+    const int *src;         // [S][4] i j k type (null: no explosive source)
+    const float *sig;       // [S][ldsig]
+    int ldsig, nt, ntall;   // psource of step nt (psource.c:26-54)
+};
+struct FTiles {
+    int ntk, nti, ntj, jc, ns, total, per;
+};
+
+int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof, const Shots &S, const Snap *snap,
+                    int ti, int jc, hipStream_t s);
+
 // gradient_F for one frequency of one shot, march kernel (stencil.hip)
 int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, const ifos_model *m,
                       float *const d_grad[3], hipStream_t s);

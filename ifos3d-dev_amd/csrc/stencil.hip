@@ -1029,3 +1029,300 @@ int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, co
     return hip_check(hipGetLastError(), "k_grad_march");
 }
 }  // namespace ifos
+
+// ---------------------------------------------------------------------------
+// Fused time step of the batched path: the stress update of step n (with the
+// explosive sources of step n) and the velocity update of step n+1 in one
+// march over j, so each wavefield array crosses HBM once per step.
+//
+// The fields are double-buffered: v^n and sigma^(n-1) are read from one set,
+// sigma^n and v^(n+1) written to the other, so no block ever reads a value
+// another block has already overwritten.  A workgroup of 64 x TI lanes covers
+// k-columns [k0-2, k0+62) x i-rows [i0-2, i0+TI-2); every lane computes
+// sigma^n at its own (i, k), the 60 x (TI-4) inner lanes own their cells
+// (they store sigma^n and the CPML memory of the stress update, and compute
+// v^(n+1)).  The five stress components the velocity update needs in the
+// (i, k) plane pass through LDS (a ring of four planes); the j-neighbours stay
+// in each lane's register queues.  Plane j of sigma^n is computed two planes
+// ahead of plane j-2 of v^(n+1).  Halo planes, rows and columns outside
+// [1..N]^3 hold zero stress (no free surface).
+// CPML of the velocity derivatives reading the previous memory from `po`
+// and, for the owner, writing the new memory to `pn` (else pn.p[0] == null)
+__device__ inline void s_cpml2(const DevGeom &g, const Prof &P, const Psi9 &po, const Psi9 &pn, bool own, int j,
+                               int i, int k, int hx, int hy, int hz, float &vxx, float &vyx, float &vzx,
+                               float &vxy, float &vyy, float &vzy, float &vxz, float &vyz, float &vzz)
+{
+#define C2(n, a_, d, K, A_, B_)                       \
+    do {                                              \
+        const float p_ = (B_) * po.p[n][a_] + (A_) * (d); \
+        if (own) pn.p[n][a_] = p_;                    \
+        d = d / (K) + p_;                             \
+    } while (0)
+    if (hx) {
+        const int64_t a = psi_x_at(g, j, hx, k);
+        C2(0, a, vxx, P.K(0, hx), P.A(0, hx), P.B(0, hx));
+        C2(1, a, vyx, P.KH(0, hx), P.AH(0, hx), P.BH(0, hx));
+        C2(2, a, vzx, P.KH(0, hx), P.AH(0, hx), P.BH(0, hx));
+    }
+    if (hy) {
+        const int64_t a = psi_y_at(g, hy, i, k);
+        C2(3, a, vxy, P.KH(1, hy), P.AH(1, hy), P.BH(1, hy));
+        C2(4, a, vyy, P.K(1, hy), P.A(1, hy), P.B(1, hy));
+        C2(5, a, vzy, P.KH(1, hy), P.AH(1, hy), P.BH(1, hy));
+    }
+    if (hz) {
+        const int64_t a = psi_z_at(g, j, i, hz);
+        const int kax = (k <= g.fw) ? 1 : 2;
+        C2(6, a, vxz, P.KH(kax, hz), P.AH(2, hz), P.BH(2, hz));
+        C2(7, a, vyz, P.KH(kax, hz), P.AH(2, hz), P.BH(2, hz));
+        C2(8, a, vzz, P.K(kax, hz), P.A(2, hz), P.B(2, hz));
+    }
+#undef C2
+}
+
+// absolute-plane buffer access: vcol is the byte offset of (-2, i-2, k-2)
+#define LDJ(r, jj, di, dk) bload(r, vcol + ((dk) + 2) * 4, (uint32_t)((jj) + 2) * SJ + ((di) + 2) * SI)
+#define STJ(v, r, jj) bstore(v, r, vcol + 8, (uint32_t)((jj) + 2) * SJ + 2 * SI)
+
+template <int TI, bool SNAP>
+__global__ __launch_bounds__(64 * TI) void k_vs_fused(DevGeom g, FTiles T, FusedArgs A, Prof P, Shots S, Snap N)
+{
+    __shared__ float lds[4][5][TI][64];  // sxx sxy sxz syz szz of sigma^n, plane slot j & 3
+    const int b = blockIdx.x;
+    const int L = (b & 7) * T.per + (b >> 3);
+    if (L >= T.total) return;  // whole workgroup
+    const int tiles = T.ntk * T.nti * T.ntj;
+    const int shot = L / tiles;
+    int t = L % tiles;
+    const int tk = t % T.ntk;
+    t /= T.ntk;
+    const int ti = t % T.nti, tj = t / T.nti;
+    const int lx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const int k = tk * 60 - 1 + lx;        // k0 = 1 + 60 tk, lanes from k0 - 2
+    const int i = ti * (TI - 4) - 1 + ty;  // i0 = 1 + (TI-4) ti, rows from i0 - 2
+    const int jb = 1 + tj * T.jc, je = min(jb + T.jc - 1, g.ny);
+    const bool col = k >= 1 && k <= g.nz && i >= 1 && i <= g.nx;  // (i, k) inside [1..N]^2
+    const bool inner = col && lx >= 2 && lx < 62 && ty >= 2 && ty < TI - 2;
+    const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
+    const uint32_t vcol = col ? (uint32_t)((int64_t)i * g.si + k - 2 + IFOS_KOFF) * 4u : 0u;
+    const float b1 = g.b1, b2 = g.b2;
+    const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
+    const int64_t so = (int64_t)shot * S.fstride;
+    const rsrc_t rvx = make_rsrc(A.vx + so, g.fbytes), rvy = make_rsrc(A.vy + so, g.fbytes),
+                 rvz = make_rsrc(A.vz + so, g.fbytes), rsxx = make_rsrc(A.sxx + so, g.fbytes),
+                 rsyy = make_rsrc(A.syy + so, g.fbytes), rszz = make_rsrc(A.szz + so, g.fbytes),
+                 rsxy = make_rsrc(A.sxy + so, g.fbytes), rsyz = make_rsrc(A.syz + so, g.fbytes),
+                 rsxz = make_rsrc(A.sxz + so, g.fbytes);
+    const rsrc_t rpi = make_rsrc(A.pi, g.fbytes), ru = make_rsrc(A.u, g.fbytes),
+                 ruipjp = make_rsrc(A.uipjp, g.fbytes), rujpkp = make_rsrc(A.ujpkp, g.fbytes),
+                 ruipkp = make_rsrc(A.uipkp, g.fbytes), rrip = make_rsrc(A.rip, g.fbytes),
+                 rrjp = make_rsrc(A.rjp, g.fbytes), rrkp = make_rsrc(A.rkp, g.fbytes);
+    const rsrc_t wvx = make_rsrc(A.nvx + so, g.fbytes), wvy = make_rsrc(A.nvy + so, g.fbytes),
+                 wvz = make_rsrc(A.nvz + so, g.fbytes), wsxx = make_rsrc(A.nsxx + so, g.fbytes),
+                 wsyy = make_rsrc(A.nsyy + so, g.fbytes), wszz = make_rsrc(A.nszz + so, g.fbytes),
+                 wsxy = make_rsrc(A.nsxy + so, g.fbytes), wsyz = make_rsrc(A.nsyz + so, g.fbytes),
+                 wsxz = make_rsrc(A.nsxz + so, g.fbytes);
+    rsrc_t rring;
+    if (SNAP) rring = make_rsrc(N.ring + (int64_t)shot * N.rstride, N.rbytes);
+    Psi9 pvo = A.psv_old, pvn = A.psv_new, pss = A.pss;
+    shift_psi(pvo, S, shot);
+    shift_psi(pvn, S, shot);
+    shift_psi(pss, S, shot);
+    const int hx = col ? slab_h(i, g.nx, g.fw, 1) : 0;
+    const int hz = col ? slab_h(k, g.nz, g.fw, 1) : 0;
+    // explosive source of this shot, psource.c:40-50
+    int si_ = -9, sj_ = -9, sk_ = -9;
+    float amp = 0.0f;
+    if (A.src) {
+        const int *p = A.src + 4 * shot;
+        if (p[3] == 1) {
+            si_ = p[0];
+            sj_ = p[1];
+            sk_ = p[2];
+            const float *sg = A.sig + (int64_t)shot * A.ldsig;
+            const double den = 2.0 * g.dx * g.dy * g.dz;
+            const int nt = A.nt;
+            if (nt == 1) amp = (float)(sg[nt + 1] / den);
+            if (nt > 1 && nt < A.ntall) amp = (float)((sg[nt + 1] - sg[nt - 1]) / den);
+            if (nt == A.ntall) amp = (float)(-sg[nt - 1] / den);
+        }
+    }
+
+    // v^n queues of the own column, planes j-2..j+2 at iteration j
+    float qvx[5], qvy[5], qvz[5];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int p = jb - 4 + q;
+        const bool ok = col && p >= -2;
+        qvx[q] = ok ? LDJ(rvx, p, 0, 0) : 0.0f;
+        qvy[q] = ok ? LDJ(rvy, p, 0, 0) : 0.0f;
+        qvz[q] = ok ? LDJ(rvz, p, 0, 0) : 0.0f;
+    }
+    qvx[4] = qvy[4] = qvz[4] = 0.0f;
+    // sigma^n queues of the own column: sxy, syz planes j-4..j, syy j-3..j
+    float qxy[5] = {0, 0, 0, 0, 0}, qyz[5] = {0, 0, 0, 0, 0}, qyy[4] = {0, 0, 0, 0};
+
+    for (int j = jb - 2; j <= je + 2; ++j) {
+        // ---- phase A: sigma^n at plane j
+        {
+            const int p = j + 2;
+            const bool ok = col && p <= g.ny + 2;
+            qvx[4] = ok ? LDJ(rvx, p, 0, 0) : 0.0f;
+            qvy[4] = ok ? LDJ(rvy, p, 0, 0) : 0.0f;
+            qvz[4] = ok ? LDJ(rvz, p, 0, 0) : 0.0f;
+        }
+        float sn[6] = {0, 0, 0, 0, 0, 0};  // sxx syy szz sxy syz sxz
+        if (col && j >= 1 && j <= g.ny) {
+            const float vx_0 = qvx[2], vy_0 = qvy[2], vz_0 = qvz[2];
+            const float vx_xa = LDJ(rvx, j, -2, 0), vx_xb = LDJ(rvx, j, -1, 0), vx_xd = LDJ(rvx, j, 1, 0);
+            const float vx_zb = LDJ(rvx, j, 0, -1), vx_zc = LDJ(rvx, j, 0, 1), vx_zd = LDJ(rvx, j, 0, 2);
+            const float vy_xb = LDJ(rvy, j, -1, 0), vy_xc = LDJ(rvy, j, 1, 0), vy_xd = LDJ(rvy, j, 2, 0);
+            const float vy_zb = LDJ(rvy, j, 0, -1), vy_zc = LDJ(rvy, j, 0, 1), vy_zd = LDJ(rvy, j, 0, 2);
+            const float vz_xb = LDJ(rvz, j, -1, 0), vz_xc = LDJ(rvz, j, 1, 0), vz_xd = LDJ(rvz, j, 2, 0);
+            const float vz_za = LDJ(rvz, j, 0, -2), vz_zb = LDJ(rvz, j, 0, -1), vz_zd = LDJ(rvz, j, 0, 1);
+            const float uipjp = LDJ(ruipjp, j, 0, 0), ujpkp = LDJ(rujpkp, j, 0, 0), uipkp = LDJ(ruipkp, j, 0, 0);
+            const float pi = LDJ(rpi, j, 0, 0), u = LDJ(ru, j, 0, 0);
+            const float sxx0 = LDJ(rsxx, j, 0, 0), syy0 = LDJ(rsyy, j, 0, 0), szz0 = LDJ(rszz, j, 0, 0);
+            const float sxy0 = LDJ(rsxy, j, 0, 0), syz0 = LDJ(rsyz, j, 0, 0), sxz0 = LDJ(rsxz, j, 0, 0);
+            float vxx = D4B(b1, b2, vx_xa, vx_xb, vx_0, vx_xd) / DX;
+            float vxy = D4F(b1, b2, qvx[1], vx_0, qvx[3], qvx[4]) / DY;
+            float vxz = D4F(b1, b2, vx_zb, vx_0, vx_zc, vx_zd) / DZ;
+            float vyx = D4F(b1, b2, vy_xb, vy_0, vy_xc, vy_xd) / DX;
+            float vyy = D4B(b1, b2, qvy[0], qvy[1], vy_0, qvy[3]) / DY;
+            float vyz = D4F(b1, b2, vy_zb, vy_0, vy_zc, vy_zd) / DZ;
+            float vzx = D4F(b1, b2, vz_xb, vz_0, vz_xc, vz_xd) / DX;
+            float vzy = D4F(b1, b2, qvz[1], vz_0, qvz[3], qvz[4]) / DY;
+            float vzz = D4B(b1, b2, vz_za, vz_zb, vz_0, vz_zd) / DZ;
+            const bool own = inner && j >= jb && j <= je;
+            const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+            if (hx | hy | hz)
+                s_cpml2(g, P, pvo, pvn, own, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
+            // update_s_ssg_elastic.c:117-138
+            const float fipjp = uipjp * DT, fjpkp = ujpkp * DT, fipkp = uipkp * DT;
+            const float f = 2.0f * u;
+            const float vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+            const float vxxyyzz = vxx + vyy + vzz;
+            const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
+            sn[3] = sxy0 + (fipjp * vxyyx);
+            sn[4] = syz0 + (fjpkp * vyzzy);
+            sn[5] = sxz0 + (fipkp * vxzzx);
+            sn[0] = sxx0 + DT * ((pi * vxxyyzz) - (f * vyyzz));
+            sn[1] = syy0 + DT * ((pi * vxxyyzz) - (f * vxxzz));
+            sn[2] = szz0 + DT * ((pi * vxxyyzz) - (f * vxxyy));
+            if (i == si_ && j == sj_ && k == sk_) {  // psource after the stress update
+                sn[0] += amp;
+                sn[1] += amp;
+                sn[2] += amp;
+            }
+            if (own) {
+                STJ(sn[3], wsxy, j);
+                STJ(sn[4], wsyz, j);
+                STJ(sn[5], wsxz, j);
+                STJ(sn[0], wsxx, j);
+                STJ(sn[1], wsyy, j);
+                STJ(sn[2], wszz, j);
+            }
+        }
+        const int slot = j & 3;
+        lds[slot][0][ty][lx] = sn[0];
+        lds[slot][1][ty][lx] = sn[3];
+        lds[slot][2][ty][lx] = sn[5];
+        lds[slot][3][ty][lx] = sn[4];
+        lds[slot][4][ty][lx] = sn[2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            qxy[q] = qxy[q + 1];
+            qyz[q] = qyz[q + 1];
+        }
+        qxy[4] = sn[3];
+        qyz[4] = sn[4];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) qyy[q] = qyy[q + 1];
+        qyy[3] = sn[1];
+        __syncthreads();
+
+        // ---- phase B: v^(n+1) at plane jv = j - 2
+        const int jv = j - 2;
+        if (inner && jv >= jb && jv <= je) {
+            const int sl = jv & 3;
+            const float(*X)[64] = lds[sl][0];
+            const float(*XY)[64] = lds[sl][1];
+            const float(*XZ)[64] = lds[sl][2];
+            const float(*YZ)[64] = lds[sl][3];
+            const float(*ZZ)[64] = lds[sl][4];
+            const float sxy_0 = qxy[2], syz_0 = qyz[2], sxz_0 = XZ[ty][lx];
+            float sxx_x = g.dtdx * D4F(b1, b2, X[ty - 1][lx], X[ty][lx], X[ty + 1][lx], X[ty + 2][lx]);
+            float sxy_y = g.dtdy * D4B(b1, b2, qxy[0], qxy[1], sxy_0, qxy[3]);
+            float sxz_z = g.dtdz * D4B(b1, b2, XZ[ty][lx - 2], XZ[ty][lx - 1], sxz_0, XZ[ty][lx + 1]);
+            float syy_y = g.dtdy * D4F(b1, b2, qyy[0], qyy[1], qyy[2], qyy[3]);
+            float sxy_x = g.dtdx * D4B(b1, b2, XY[ty - 2][lx], XY[ty - 1][lx], sxy_0, XY[ty + 1][lx]);
+            float syz_z = g.dtdz * D4B(b1, b2, YZ[ty][lx - 2], YZ[ty][lx - 1], syz_0, YZ[ty][lx + 1]);
+            float szz_z = g.dtdz * D4F(b1, b2, ZZ[ty][lx - 1], ZZ[ty][lx], ZZ[ty][lx + 1], ZZ[ty][lx + 2]);
+            float sxz_x = g.dtdx * D4B(b1, b2, XZ[ty - 2][lx], XZ[ty - 1][lx], sxz_0, XZ[ty + 1][lx]);
+            float syz_y = g.dtdy * D4B(b1, b2, qyz[0], qyz[1], syz_0, qyz[3]);
+            const int hy = slab_h(jv, g.ny, g.fw, !g.free_surf);
+            if (hx | hy | hz)
+                v_cpml(g, P, pss, jv, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z,
+                       szz_z);
+            const float rip = LDJ(rrip, jv, 0, 0), rjp = LDJ(rrjp, jv, 0, 0), rkp = LDJ(rrkp, jv, 0, 0);
+            const float nvx = qvx[0] + (sxx_x + sxy_y + sxz_z) / rip;
+            const float nvy = qvy[0] + (syy_y + sxy_x + syz_z) / rjp;
+            const float nvz = qvz[0] + (szz_z + sxz_x + syz_y) / rkp;
+            STJ(nvx, wvx, jv);
+            STJ(nvy, wvy, jv);
+            STJ(nvz, wvz, jv);
+            if (SNAP) {
+                const uint32_t o = (uint32_t)(jv + 2) * SJ + 2 * SI + N.soff;
+                bstore(nvx, rring, vcol + 8, o);
+                bstore(nvy, rring, vcol + 8, o + g.fbytes);
+                bstore(nvz, rring, vcol + 8, o + 2 * g.fbytes);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            qvx[q] = qvx[q + 1];
+            qvy[q] = qvy[q + 1];
+            qvz[q] = qvz[q + 1];
+        }
+    }
+}
+#undef LDJ
+#undef STJ
+
+namespace ifos {
+// one fused step for S.n shots (TI 8 or 16 i-rows per workgroup, jc j-planes
+// per tile, <= 0 automatic)
+int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof, const Shots &S, const Snap *snap,
+                    int ti, int jc, hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    const Prof P = {d_prof, 2 * g->fw + 1};
+    if (ti != 8 && ti != 16) {
+        set_error("fused step: TI=%d not in {8,16}", ti);
+        return IFOS_EINVAL;
+    }
+    FTiles T;
+    T.ntk = (g->nz + 59) / 60;
+    T.nti = (g->nx + ti - 5) / (ti - 4);
+    if (jc <= 0) {
+        jc = 64;
+        while (jc > 8 && (int64_t)T.ntk * T.nti * ((g->ny + jc - 1) / jc) * S.n < 2048) jc /= 2;
+    }
+    T.jc = jc;
+    T.ntj = (g->ny + jc - 1) / jc;
+    T.ns = S.n;
+    T.total = T.ntk * T.nti * T.ntj * S.n;
+    T.per = (T.total + 7) / 8;
+    const Snap N = snap ? *snap : Snap{nullptr, 0, 0, 0};
+    const dim3 grid((unsigned)(T.per * 8));
+    ProfScope ps_(IFOS_K_V_FUSED, s);
+    if (ti == 8) {
+        if (snap) hipLaunchKernelGGL((k_vs_fused<8, true>), grid, dim3(64, 8), 0, s, G, T, A, P, S, N);
+        else hipLaunchKernelGGL((k_vs_fused<8, false>), grid, dim3(64, 8), 0, s, G, T, A, P, S, N);
+    } else {
+        if (snap) hipLaunchKernelGGL((k_vs_fused<16, true>), grid, dim3(64, 16), 0, s, G, T, A, P, S, N);
+        else hipLaunchKernelGGL((k_vs_fused<16, false>), grid, dim3(64, 16), 0, s, G, T, A, P, S, N);
+    }
+    return hip_check(hipGetLastError(), "k_vs_fused");
+}
+}  // namespace ifos

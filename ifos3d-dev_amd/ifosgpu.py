@@ -485,6 +485,12 @@ class BatchContext:
         check(lib().ifos_batch_dft(self.h, shot, back, arr), "ifos_batch_dft")
         return list(arr)
 
+    def set_fused(self, on=True, ti=16, jc=0):
+        check(lib().ifos_batch_set_fused(self.h, int(on), ti, jc), "ifos_batch_set_fused")
+
+    def last_fused(self):
+        return bool(lib().ifos_batch_last_fused(self.h))
+
     def set_tiles(self, cfg_v=None, cfg_s=None):
         """tile shapes {TI, jc, group, split, pipe} of the velocity / stress marches"""
         cv = (C.c_int * 5)(*(tuple(cfg_v) + (0,) * (5 - len(cfg_v)))) if cfg_v is not None else None

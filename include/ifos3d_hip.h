@@ -305,6 +305,13 @@ int ifos_batch_tune_candidates(int *cfg, int max);
  * clears the wavefields */
 int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
                         int *best_s, float *ms_v, float *ms_s);
+/* the fused step (stress of step n and velocity of step n+1 in one launch,
+ * double-buffered fields): on by default, used when there is no free
+ * surface, no source or receiver inside a CPML slab and L = 0; ti = 8 or 16
+ * workgroup rows, jc j-planes per tile (<= 0: automatic) */
+int ifos_batch_set_fused(ifos_batch *b, int on, int ti, int jc);
+/* 1 if the last ifos_batch_run used the fused step */
+int ifos_batch_last_fused(const ifos_batch *b);
 /* DFT accumulators of one shot of the last batch: frequency l of
  * component n (Re x y z, Im x y z) at d_F[n] + l * ifos_field_elems(g) */
 int ifos_batch_dft(ifos_batch *b, int shot, int back, float *d_F[6]);

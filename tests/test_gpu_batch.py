@@ -233,3 +233,50 @@ def test_batch_viscoelastic(tiles):
     for n in range(3):
         assert max_ulp(out["grads"][n], ref["grads"][n]) == 0, f"gradient {n}"
     out["ctx"].close()
+
+
+@pytest.mark.parametrize("fz,prec_name,chunk,use_graph", [((16, 0), "compat", 4, False), ((8, 5), "compat", 3, True),
+                                                          ((16, 7), "kahan", 5, False), ((8, 0), "compat", 8, False)])
+def test_batch_fused_step(fz, prec_name, chunk, use_graph):
+    """the fused stress(n) + velocity(n+1) step (all points inside the
+    interior box, no free surface) against the plain path, bit for bit"""
+    import ifosgpu as ig
+    g = small_grid(nx=40, ny=37, nz=70, fw=8, nt=130)
+    srcs = [(18, 12, 19, 0.0, 50.0, 1.0, 1), (25, 14, 40, 0.0, 45.0, 1.0, 1), (12, 20, 55, 0.001, 55.0, 1e6, 4)]
+    rec = np.array([[10 + 2 * n, 10, 19 + 3 * n] for n in range(9)] + [[14, 10, 22], [14, 10, 22]], np.int32)
+    finv, tast = [40.0, 60.0, 80.0], 100
+    prec = ig.DFT_KAHAN if prec_name == "kahan" else ig.DFT_COMPAT
+    _note(f"fused step ti={fz[0]} jc={fz[1]} prec={prec_name} chunk={chunk} graph={use_graph}")
+    true, start = _models(g)
+    G = _setup(ig, g, finv, tast, prec)
+    ref = _plain(ig, G, g, true, start, srcs, rec, finv, tast, prec)
+    ns, ntr = len(srcs), len(rec)
+    b = ig.BatchContext(G, ns, len(finv), ntr, prec, chunk)
+    b.set_fused(True, *fz)
+    b.set_cpml(4000.0, 50.0)
+    b.set_freqs(finv, tast)
+    d_syn = ig.DBuf((ns, 3, ntr, g.nt + 1))
+    b.set_model(*[G.to_dev(a) for a in true])
+    b.run(srcs, rec, d_syn=d_syn, want_adjoint=False, use_graph=use_graph)
+    assert b.last_fused()
+    obs_b = d_syn.numpy().copy()
+    for n in range(ns):
+        assert max_ulp(obs_b[n], ref["obs"][n].numpy()) == 0, f"observed data, shot {n}"
+    b.set_model(*[G.to_dev(a) for a in start])
+    for rep in range(2 if use_graph else 1):
+        grads = [G.empty() for _ in range(3)]
+        obs = [ig.DBuf.from_numpy(obs_b[n]) for n in range(ns)]
+        l2 = b.run(srcs, rec, d_syn=d_syn, grads=grads, d_obs=obs, use_graph=use_graph)
+        assert b.last_fused()
+        syn = d_syn.numpy()
+        for n in range(ns):
+            assert max_ulp(syn[n], ref["syn"][n]) == 0, f"rep {rep}: synthetics, shot {n}"
+            assert l2[n] == ref["l2"][n], f"rep {rep}: misfit, shot {n}"
+        fe = G.field_elems()
+        dft_last = [ig.DBuf.wrap(p, (len(finv) * fe,)).numpy() for p in b.dft_ptrs(ns - 1, 1)]
+        for n in range(6):
+            assert max_ulp(dft_last[n], ref["dft_last"][n]) == 0, f"rep {rep}: adjoint DFT component {n}"
+        for n in range(3):
+            assert max_ulp(G.from_dev(grads[n]), ref["grads"][n]) == 0, f"rep {rep}: gradient {n}"
+        ig.DBuf((1 << 20,)).zero_()
+    b.close()
