@@ -32,10 +32,12 @@ HBM_PEAK_GBS = 8000.0
 CAT_KERNEL = {"v_interior": "ifos::k_v_march<64, 4, false>", "s_interior": "ifos::k_s_march<64, 4, false>",
               "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs", "dft": "ifos::k_dft_nf<{nf}>",
               "dft_kahan": "ifos::k_dft_kahan_nf<{nf}>", "gradient": "ifos::k_gradient"}
-BATCH_KERNEL = {"v_fused": "ifos::k_v_march<64, 4, true, true>", "s_fused": "ifos::k_s_march<64, 4, true>",
+BATCH_KERNEL = {"v_fused": "ifos::k_v_march<4, true, true, false>", "s_fused": "ifos::k_s_march<4, true, false, false>",
+                "vs_fused": "ifos::k_vs_fused<16, false>",
                 "dft": "ifos::k_dft_chunk<{nf}, false>", "dft_kahan": "ifos::k_dft_chunk<{nf}, true>",
                 "gradient": "ifos::k_gradient_shots"}
-CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other"]
+CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other",
+        "vs_fused"]
 
 
 def log(msg):
@@ -83,7 +85,13 @@ def algorithmic_bytes_batched(w, nf, S, chunk, ntast, kahan=False):
     v_avg = v_base + 12 * cells * S * nsamp / w.nt
     s_avg = (60 * cells + 24 * axes) * S + 20 * cells
     dft_avg = (12 * nsamp + acc * nf * nchunks) * cells * S / nchunks
-    return {"v_fused": v_avg, "s_fused": s_avg, "dft": dft_avg, "gradient": (48 * nf * S + 36) * cells}
+    # fused step: v and sigma read and written once (72 B), the whole model
+    # once per tile (32 B shared), 48 B per active CPML axis (the stress
+    # memory read and rewritten, the velocity memory updated in place), the
+    # snapshot at sampling steps
+    vs_avg = (72 * cells + 48 * axes) * S + 32 * cells + 12 * cells * S * nsamp / w.nt
+    return {"v_fused": v_avg, "s_fused": s_avg, "dft": dft_avg, "gradient": (48 * nf * S + 36) * cells,
+            "vs_fused": vs_avg}
 
 
 def _prof_start(ig):
@@ -134,7 +142,7 @@ def observed_data(w, G, d_true, S, chunk):
     return obs
 
 
-def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_graph, autotune=True):
+def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_graph, autotune=True, fused=True):
     import ifosgpu as ig
     ntr = len(w.receivers)
     log(f"batched path: {S} shots per batch, DFT chunk {chunk}, graph {use_graph}, "
@@ -143,11 +151,13 @@ def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_grap
     b.set_cpml(w.vppml, w.fpml)
     b.set_freqs(w.finv, w.tast)
     b.set_model(*d_start)
+    if not fused:
+        b.set_fused(False)
     tune = None
     if autotune:
         bv, bs, table = b.autotune(reps=2)
-        tune = {"v": bv, "s": bs, "table": table}
-        log(f"autotune: velocity march {bv}, stress march {bs}")
+        tune = {"v": bv, "s": bs, "table": table, "fused": b.fused_table}
+        log(f"autotune: velocity march {bv}, stress march {bs}, fused step {b.fused_table}")
     grads = [G.empty() for _ in range(3)]
     batches = [list(range(b0, min(b0 + S, len(w.shots)))) for b0 in range(0, len(w.shots), S)]
 
@@ -173,8 +183,9 @@ def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_grap
     elapsed = time.perf_counter() - t0
     prof = _prof_stop(ig) if not use_graph else None
     gnorm = [float(np.linalg.norm(G.from_dev(t).astype(np.float64))) for t in grads]
+    fused_used = b.last_fused()
     b.close()
-    return dict(elapsed=elapsed, shots=nshots, prof=prof, l2=l2, gnorm=gnorm, tune=tune)
+    return dict(elapsed=elapsed, shots=nshots, prof=prof, l2=l2, gnorm=gnorm, tune=tune, fused=fused_used)
 
 
 def run_plain(w, G, d_start, obs, shots, precision):
@@ -306,6 +317,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=8, help="DFT samples per accumulator pass")
     ap.add_argument("--graph", action="store_true", help="replay the time loops from hipGraphs")
     ap.add_argument("--no-autotune", action="store_true", help="default tile shapes instead of the autotuner's")
+    ap.add_argument("--no-fused", action="store_true", help="separate velocity and stress launches per step")
     ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
     ap.add_argument("--precision", default="compat", choices=["compat", "kahan"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -343,7 +355,7 @@ def main():
     batched = None
     if args.path == "batched":
         batched = run_batched(w, G, d_start, obs, args.steps, args.warmup, prec, S, args.chunk, args.graph,
-                              not args.no_autotune)
+                              not args.no_autotune, not args.no_fused)
         batched["value"] = batched["shots"] / batched["elapsed"]
         log(f"batched path: {batched['value']:.4f} shots/s")
 
@@ -371,7 +383,8 @@ def main():
                 "traffic_source": os.path.relpath(tpath, ROOT) if traffic else None,
                 "algorithmic_bytes_per_launch": int(ab[dom]), "avg_launch_ms": round(avg_ms, 5),
                 "rocprof_kernel": kname or None}
-        st_cats = [k for k in ("v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused") if k in ab]
+        st_cats = [k for k in ("v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "vs_fused")
+                   if k in ab]
         st_bytes = sum(ab[k] * res["prof"][k][1] for k in st_cats)
         st_ms = sum(res["prof"][k][0] for k in st_cats)
         hbm_frac_stencils = (st_bytes / (st_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if st_ms > 0 else None
@@ -385,8 +398,9 @@ def main():
             cpu = cpu_baseline(w, host_start)
         except Exception as e:  # the baseline is reported, never the product
             log(f"CPU baseline failed: {e}")
-    path_desc = (f"batched: {S} shots per launch, fused interior+CPML marches, DFT in chunks of {args.chunk} "
-                 f"samples, {'hipGraph replay' if args.graph else 'direct launches'}"
+    path_desc = (f"batched: {S} shots per launch, "
+                 f"{'fused stress+velocity step (one launch per time step)' if batched and batched['fused'] else 'separate velocity and stress marches'}, "
+                 f"DFT in chunks of {args.chunk} samples, {'hipGraph replay' if args.graph else 'direct launches'}"
                  if args.path == "batched" else
                  "plain (one kernel per reference routine per step, one shot per launch)")
     out = {

@@ -292,6 +292,8 @@ struct ifos_batch {
     // (double buffering); cur_set says which set holds the current state
     float *fields_alt[9], *psi_v_alt[9];
     int fused, fz_ti, fz_jc, cur_set, last_fused;
+    int tune_fused[16][2], tune_fused_n;  // the autotuner's fused-step table
+    float tune_fused_ms[16];
     float *model[9];
     float *prof;
     float *F[12], *C[12];      // [S][nf_max][fe]
@@ -429,6 +431,7 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->fz_jc = 0;
     b->cur_set = 0;
     b->last_fused = 0;
+    b->tune_fused_n = 0;
     b->L = 0;
     b->eta = 0.0f;
     for (int n = 0; n < 6; ++n) b->r[n] = nullptr;
@@ -1051,11 +1054,59 @@ extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
             if (kind == 1 && ms < bs) { bs = ms; is = c; }
         }
     }
+    // the fused step: (ti, jc) candidates, timed like the others
+    static const int fz[][2] = {{16, 0}, {16, 16}, {16, 32}, {16, 64}, {8, 0}, {8, 16}, {8, 32}};
+    float bf = 1e30f;
+    int ifz = 0;
+    b->tune_fused_n = 0;
+    if (!rc && !b->g.free_surf && !b->L) {
+        FusedArgs A;
+        A.vx = b->fields[0]; A.vy = b->fields[1]; A.vz = b->fields[2];
+        A.sxx = b->fields[3]; A.syy = b->fields[4]; A.szz = b->fields[5];
+        A.sxy = b->fields[6]; A.syz = b->fields[7]; A.sxz = b->fields[8];
+        A.nvx = b->fields_alt[0]; A.nvy = b->fields_alt[1]; A.nvz = b->fields_alt[2];
+        A.nsxx = b->fields_alt[3]; A.nsyy = b->fields_alt[4]; A.nszz = b->fields_alt[5];
+        A.nsxy = b->fields_alt[6]; A.nsyz = b->fields_alt[7]; A.nsxz = b->fields_alt[8];
+        A.pi = m.pi; A.u = m.u; A.uipjp = m.uipjp; A.ujpkp = m.ujpkp; A.uipkp = m.uipkp;
+        A.rip = m.rip; A.rjp = m.rjp; A.rkp = m.rkp;
+        for (int n = 0; n < 9; ++n) {
+            A.psv_old.p[n] = b->psi_v[n];
+            A.psv_new.p[n] = b->psi_v_alt[n];
+            A.pss.p[n] = b->psi_s[n];
+        }
+        A.src = nullptr;
+        A.sig = b->sig;
+        A.ldsig = g->nt + 2;
+        A.nt = 1;
+        A.ntall = g->nt;
+        const int nfz = (int)(sizeof(fz) / sizeof(fz[0]));
+        for (int c = 0; c < nfz && !rc; ++c) {
+            for (int r = 0; r <= reps && !rc; ++r) {
+                if (r == 1) rc = hip_check(hipEventRecord(e0, st), "record");
+                if (!rc) rc = launch_vs_fused(g, A, b->prof, S, nullptr, fz[c][0], fz[c][1], st);
+            }
+            if (!rc) rc = hip_check(hipEventRecord(e1, st), "record");
+            if (!rc) rc = hip_check(hipEventSynchronize(e1), "sync");
+            float ms = 0.0f;
+            if (!rc) rc = hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            ms /= reps;
+            b->tune_fused[c][0] = fz[c][0];
+            b->tune_fused[c][1] = fz[c][1];
+            b->tune_fused_ms[c] = ms;
+            b->tune_fused_n = c + 1;
+            if (ms < bf) { bf = ms; ifz = c; }
+        }
+        if (!rc) {
+            b->fz_ti = fz[ifz][0];
+            b->fz_jc = fz[ifz][1];
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     b->cfg_v = k_candidates[iv];
     b->cfg_s = k_candidates[is];
+    drop_graphs(b);
     if (best_v) *best_v = iv;
     if (best_s) *best_s = is;
     rc = zero_state(b, b->S, st);
@@ -1103,6 +1154,19 @@ extern "C" int ifos_batch_set_fused(ifos_batch *b, int on, int ti, int jc)
     b->fz_jc = jc;
     drop_graphs(b);
     return IFOS_OK;
+}
+
+// the autotuner's fused-step timings: out[3*c] = ti, jc, ms*1000; returns count
+extern "C" int ifos_batch_tune_fused(const ifos_batch *b, float *out, int max)
+{
+    if (!b) return 0;
+    int n = 0;
+    for (; n < b->tune_fused_n && n < max; ++n) {
+        out[3 * n] = (float)b->tune_fused[n][0];
+        out[3 * n + 1] = (float)b->tune_fused[n][1];
+        out[3 * n + 2] = b->tune_fused_ms[n];
+    }
+    return n;
 }
 
 // 1 if the last run used the fused step

@@ -1080,6 +1080,7 @@ __device__ inline void s_cpml2(const DevGeom &g, const Prof &P, const Psi9 &po, 
 #undef C2
 }
 
+namespace ifos {
 // absolute-plane buffer access: vcol is the byte offset of (-2, i-2, k-2)
 #define LDJ(r, jj, di, dk) bload(r, vcol + ((dk) + 2) * 4, (uint32_t)((jj) + 2) * SJ + ((di) + 2) * SI)
 #define STJ(v, r, jj) bstore(v, r, vcol + 8, (uint32_t)((jj) + 2) * SJ + 2 * SI)
@@ -1289,7 +1290,6 @@ __global__ __launch_bounds__(64 * TI) void k_vs_fused(DevGeom g, FTiles T, Fused
 #undef LDJ
 #undef STJ
 
-namespace ifos {
 // one fused step for S.n shots (TI 8 or 16 i-rows per workgroup, jc j-planes
 // per tile, <= 0 automatic)
 int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof, const Shots &S, const Snap *snap,
@@ -1315,7 +1315,7 @@ int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof,
     T.per = (T.total + 7) / 8;
     const Snap N = snap ? *snap : Snap{nullptr, 0, 0, 0};
     const dim3 grid((unsigned)(T.per * 8));
-    ProfScope ps_(IFOS_K_V_FUSED, s);
+    ProfScope ps_(IFOS_K_VS_FUSED, s);
     if (ti == 8) {
         if (snap) hipLaunchKernelGGL((k_vs_fused<8, true>), grid, dim3(64, 8), 0, s, G, T, A, P, S, N);
         else hipLaunchKernelGGL((k_vs_fused<8, false>), grid, dim3(64, 8), 0, s, G, T, A, P, S, N);

@@ -512,4 +512,8 @@ class BatchContext:
         check(lib().ifos_batch_autotune(self.h, ns or self.nshots, reps, C.byref(bv), C.byref(bs), mv, ms),
               "ifos_batch_autotune")
         table = [dict(cfg=c, v_ms=round(mv[n], 4), s_ms=round(ms[n], 4)) for n, c in enumerate(cands)]
+        buf = (C.c_float * 48)()
+        nfz = lib().ifos_batch_tune_fused(self.h, buf, 16)
+        self.fused_table = [dict(ti=int(buf[3 * c]), jc=int(buf[3 * c + 1]), ms=round(buf[3 * c + 2], 4))
+                            for c in range(nfz)]
         return cands[bv.value], cands[bs.value], table

@@ -95,6 +95,7 @@ enum {
     IFOS_K_V_INTERIOR = 0, IFOS_K_V_CPML, IFOS_K_V_FUSED,
     IFOS_K_S_INTERIOR, IFOS_K_S_CPML, IFOS_K_S_FUSED,
     IFOS_K_DFT, IFOS_K_GRADIENT, IFOS_K_SOURCES, IFOS_K_OTHER,
+    IFOS_K_VS_FUSED,  /* the batched path's fused stress + velocity step */
     IFOS_K_NCAT
 };
 
@@ -310,6 +311,10 @@ int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
  * surface, no source or receiver inside a CPML slab and L = 0; ti = 8 or 16
  * workgroup rows, jc j-planes per tile (<= 0: automatic) */
 int ifos_batch_set_fused(ifos_batch *b, int on, int ti, int jc);
+/* the fused-step timings of the last ifos_batch_autotune (which also picks
+ * the fused step's ti and jc): out[3c] = ti, jc, average ms; returns the
+ * count (at most max) */
+int ifos_batch_tune_fused(const ifos_batch *b, float *out, int max);
 /* 1 if the last ifos_batch_run used the fused step */
 int ifos_batch_last_fused(const ifos_batch *b);
 /* DFT accumulators of one shot of the last batch: frequency l of
