@@ -1099,6 +1099,8 @@ extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
         if (!rc) {
             b->fz_ti = fz[ifz][0];
             b->fz_jc = fz[ifz][1];
+            // the fused step replaces one velocity and one stress launch
+            b->fused = bf < bv + bs;
         }
     }
     (void)hipEventDestroy(e0);

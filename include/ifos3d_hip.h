@@ -312,8 +312,9 @@ int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
  * workgroup rows, jc j-planes per tile (<= 0: automatic) */
 int ifos_batch_set_fused(ifos_batch *b, int on, int ti, int jc);
 /* the fused-step timings of the last ifos_batch_autotune (which also picks
- * the fused step's ti and jc): out[3c] = ti, jc, average ms; returns the
- * count (at most max) */
+ * the fused step's ti and jc, and turns the fused step on only if it beats
+ * the best velocity + stress launch pair): out[3c] = ti, jc, average ms;
+ * returns the count (at most max) */
 int ifos_batch_tune_fused(const ifos_batch *b, float *out, int max);
 /* 1 if the last ifos_batch_run used the fused step */
 int ifos_batch_last_fused(const ifos_batch *b);
