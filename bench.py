@@ -36,6 +36,9 @@ BATCH_KERNEL = {"v_fused": "ifos::k_v_march<4, true, true, false>", "s_fused": "
                 "vs_fused": "ifos::k_vs_fused<16, false>",
                 "dft": "ifos::k_dft_chunk<{nf}, false>", "dft_kahan": "ifos::k_dft_chunk<{nf}, true>",
                 "gradient": "ifos::k_gradient_shots"}
+# tile shapes {TI, jc, group, split, pipe} used without the autotuner (its
+# usual choice on grad276)
+DEFAULT_TILES = ((4, 32, 0, 0, 0), (4, 32, 0, 0, 1))
 CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other",
         "vs_fused"]
 
@@ -120,7 +123,7 @@ def setup(w):
     return G, [G.to_dev(a) for a in true], [G.to_dev(a) for a in start], start
 
 
-def observed_data(w, G, d_true, S, chunk):
+def observed_data(w, G, d_true, S, chunk, fused=True):
     """forward runs of every shot position on the true model (untimed setup),
     through the batched path; one device buffer [3][ntr][nt+1] per shot"""
     import ifosgpu as ig
@@ -129,6 +132,8 @@ def observed_data(w, G, d_true, S, chunk):
     b.set_cpml(w.vppml, w.fpml)
     b.set_freqs(w.finv, w.tast)
     b.set_model(*d_true)
+    b.set_fused(fused)
+    b.set_tiles(DEFAULT_TILES[0], DEFAULT_TILES[1])
     obs = []
     for b0 in range(0, len(w.shots), S):
         srcs = w.shots[b0:b0 + S]
@@ -151,8 +156,8 @@ def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_grap
     b.set_cpml(w.vppml, w.fpml)
     b.set_freqs(w.finv, w.tast)
     b.set_model(*d_start)
-    if not fused:
-        b.set_fused(False)
+    b.set_fused(fused)
+    b.set_tiles(DEFAULT_TILES[0], DEFAULT_TILES[1])
     tune = None
     if autotune:
         bv, bs, table = b.autotune(reps=2)
@@ -318,6 +323,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay the time loops from hipGraphs")
     ap.add_argument("--no-autotune", action="store_true", help="default tile shapes instead of the autotuner's")
     ap.add_argument("--no-fused", action="store_true", help="separate velocity and stress launches per step")
+    ap.add_argument("--nt", type=int, default=0,
+                    help="shorter time loop (profiling runs only: the line then reports a smaller workload)")
     ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
     ap.add_argument("--precision", default="compat", choices=["compat", "kahan"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -337,13 +344,16 @@ def main():
     copy = copy_probe()
     log(f"copy probe: {copy}")
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
+    if args.nt > 0:
+        w.nt = args.nt
+        w.name += f"_nt{args.nt}"
     prec = ig.DFT_KAHAN if args.precision == "kahan" else ig.DFT_COMPAT
     kahan = prec == ig.DFT_KAHAN
     nf = len(w.finv)
     S = min(args.shots_per_batch, len(w.shots))
     ntast = ig.ntast(w.finv[-1], w.tast, w.dt)
     G, d_true, d_start, host_start = setup(w)
-    obs = observed_data(w, G, d_true, S, args.chunk)
+    obs = observed_data(w, G, d_true, S, args.chunk, not args.no_fused)
     del d_true
 
     plain = None
