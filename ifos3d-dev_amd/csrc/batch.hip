@@ -425,7 +425,7 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->nf = 0;
     b->tast = 100;
     b->ntr = 0;
-    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0};
+    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0, 1, 1};
     b->fused = 1;
     b->fz_ti = 16;
     b->fz_jc = 0;
@@ -995,10 +995,11 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
 // the batch's own buffers (ns shots, `reps` launches each) and keeps the
 // fastest.  The wavefields are zeroed afterwards.
 static const MarchCfg k_candidates[] = {
-    {4, 0, 1, 0, 0},  {4, 0, 0, 0, 0},  {4, 0, 64, 0, 0}, {8, 0, 0, 0, 0},  {2, 0, 0, 0, 0},
-    {4, 32, 0, 0, 0}, {4, 0, 1, 0, 1},  {4, 0, 0, 0, 1},  {4, 0, 64, 0, 1}, {8, 0, 0, 0, 1},
-    {2, 0, 0, 0, 1},  {4, 32, 0, 0, 1}, {8, 32, 0, 0, 1}, {1, 0, 0, 0, 1},  {4, 0, 0, 1, 1},
-    {8, 0, 0, 1, 1},  {2, 0, 16, 0, 1},
+    {4, 32, 0, 0, 0, 1, 1}, {4, 32, 0, 0, 1, 1, 1}, {4, 0, 0, 0, 0, 1, 1},  {4, 0, 0, 0, 1, 1, 1},
+    {8, 32, 0, 0, 0, 1, 1}, {8, 32, 0, 0, 1, 1, 1}, {2, 0, 0, 0, 1, 1, 1},  {1, 0, 0, 0, 1, 1, 1},
+    {4, 0, 0, 1, 1, 1, 1},  {8, 32, 0, 0, 0, 2, 1}, {4, 32, 0, 0, 1, 2, 1}, {4, 32, 0, 0, 0, 1, 2},
+    {4, 32, 0, 0, 1, 1, 2}, {8, 32, 0, 0, 0, 1, 2}, {8, 32, 0, 0, 1, 1, 2}, {8, 32, 0, 0, 0, 1, 4},
+    {8, 32, 0, 0, 1, 1, 4}, {4, 32, 0, 0, 0, 1, 4}, {4, 32, 0, 0, 1, 1, 4}, {2, 32, 0, 0, 1, 1, 2},
 };
 
 extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v, int *best_s, float *ms_v,
@@ -1120,25 +1121,34 @@ extern "C" int ifos_batch_tune_candidates(int *cfg, int max)
 {
     const int nc = (int)(sizeof(k_candidates) / sizeof(k_candidates[0]));
     for (int c = 0; c < nc && c < max; ++c) {
-        cfg[5 * c] = k_candidates[c].ti;
-        cfg[5 * c + 1] = k_candidates[c].jc;
-        cfg[5 * c + 2] = k_candidates[c].group;
-        cfg[5 * c + 3] = k_candidates[c].split;
-        cfg[5 * c + 4] = k_candidates[c].pipe;
+        int *o = cfg + IFOS_TILE_FIELDS * c;
+        o[0] = k_candidates[c].ti;
+        o[1] = k_candidates[c].jc;
+        o[2] = k_candidates[c].group;
+        o[3] = k_candidates[c].split;
+        o[4] = k_candidates[c].pipe;
+        o[5] = k_candidates[c].kb;
+        o[6] = k_candidates[c].sb;
     }
     return nc;
 }
 
-extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[5], const int cfg_s[5])
+extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[IFOS_TILE_FIELDS],
+                                    const int cfg_s[IFOS_TILE_FIELDS])
 {
     if (!b) return IFOS_EINVAL;
-    for (const int *c : {cfg_v, cfg_s})
+    for (const int *c : {cfg_v, cfg_s}) {
         if (c && c[0] != 1 && c[0] != 2 && c[0] != 4 && c[0] != 8) {
             set_error("ifos_batch_set_tiles: TI must be 1, 2, 4 or 8");
             return IFOS_EINVAL;
         }
-    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3], cfg_v[4]};
-    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3], cfg_s[4]};
+        if (c && (c[5] < 1 || c[6] < 1 || c[5] * c[6] > c[0] || c[0] % (c[5] * c[6]))) {
+            set_error("ifos_batch_set_tiles: kb=%d times sb=%d must divide TI=%d", c[5], c[6], c[0]);
+            return IFOS_EINVAL;
+        }
+    }
+    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3], cfg_v[4], cfg_v[5], cfg_v[6]};
+    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3], cfg_s[4], cfg_s[5], cfg_s[6]};
     drop_graphs(b);
     return IFOS_OK;
 }

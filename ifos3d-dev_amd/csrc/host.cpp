@@ -1,6 +1,7 @@
 // host.cpp -- host side of libifos3d_hip.so: geometry, error reporting, the
 // CPML profiles and source wavelet (computed once per run on the CPU, with
 // the reference's float/double promotions), and the plain one-shot driver.
+#include <cmath>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
@@ -50,6 +51,14 @@ DevGeom make_dev_geom(const ifos_geom *g)
     d.dz = g->dz;
     d.dt = g->dt;
     d.fbytes = (uint32_t)(ifos_field_elems(g) * 4);
+    // exact reciprocal spacings: each spacing 2^e with 2^-e a normal float
+    auto pow2 = [](float h, float *r) {
+        int e = 0;
+        const double m = std::frexp((double)h, &e);
+        *r = 1.0f / h;
+        return m == 0.5 && std::isnormal(h) && std::isnormal(*r) && (double)*r * (double)h == 1.0;
+    };
+    d.p2 = pow2(g->dx, &d.rdx) && pow2(g->dy, &d.rdy) && pow2(g->dz, &d.rdz);
     d.dtdx = g->dt / g->dx;
     d.dtdy = g->dt / g->dy;
     d.dtdz = g->dt / g->dz;

@@ -24,7 +24,29 @@ struct DevGeom {
     float dtdx, dtdy, dtdz;  // DT/DX ... in float, update_v_ssg.c:95-97
     float b1, b2;            // 4th-order coefficients, update_v_ssg.c:98-100
     uint32_t fbytes;         // bytes of one device-layout field
+    float rdx, rdy, rdz;     // 1/DX ... when p2
+    int p2;                  // every spacing a power of two with a normal reciprocal
 };
+
+// The nine velocity derivatives xx xy xz yx yy yz zx zy zz (their 4th-order
+// numerators) divided by the spacing of their second axis, as
+// update_s_ssg_elastic.c:99-115 divides them.  When every spacing is 2^e with
+// 2^-e a normal float, x * 2^-e is the same real number as x / 2^e and rounds
+// to the same float (subnormal results included), so the multiply replaces
+// the IEEE division bit for bit; otherwise the division is kept.
+__device__ __forceinline__ void div_spacing(const DevGeom &g, float &xx, float &xy, float &xz, float &yx, float &yy,
+                                            float &yz, float &zx, float &zy, float &zz)
+{
+    if (g.p2) {
+        xx *= g.rdx; xy *= g.rdy; xz *= g.rdz;
+        yx *= g.rdx; yy *= g.rdy; yz *= g.rdz;
+        zx *= g.rdx; zy *= g.rdy; zz *= g.rdz;
+    } else {
+        xx = xx / g.dx; xy = xy / g.dy; xz = xz / g.dz;
+        yx = yx / g.dx; yy = yy / g.dy; yz = yz / g.dz;
+        zx = zx / g.dx; zy = zy / g.dy; zz = zz / g.dz;
+    }
+}
 
 __host__ __device__ inline int64_t at(const DevGeom &g, int j, int i, int k)
 {
@@ -121,6 +143,8 @@ struct MarchCfg {
     int ti, jc, group;
     int split;
     int pipe;
+    int kb;  // wavefronts of a workgroup side by side along k (divides ti; <= 1: one)
+    int sb;  // shots of a workgroup, one wavefront row each (kb * sb divides ti)
 };
 
 // Raw buffer access: one 32-bit byte offset per thread, the neighbour

@@ -174,9 +174,10 @@ __device__ inline void stress_visco(float DT, float eta, int L, float pi, float 
 // L = (b % 8) * per + b / 8 gives each XCD a contiguous range.
 struct MTiles {
     Box b[6];
-    int kw[6], rpw[6], ntk[6], nti[6];
+    int kw[6], rpw[6], ntk[6], nti[6], kb[6];
     int start[7];  // cumulative tile counts per box
     int nb, jc, ns, group, total, per;
+    int sb, nsg;  // shots per workgroup (one wavefront row each), shot groups
 };
 
 __device__ inline bool decode_tile(const MTiles &T, int TI, int &k, int &i, int &jb, int &je, int &shot,
@@ -185,23 +186,29 @@ __device__ inline bool decode_tile(const MTiles &T, int TI, int &k, int &i, int 
     const int b = blockIdx.x;
     const int L = (b & 7) * T.per + (b >> 3);
     if (L >= T.total) return false;
-    const int gs = T.group * T.ns;
+    const int gs = T.group * T.nsg;
     const int rem = L % gs;
-    shot = rem / T.group;
+    const int sgrp = rem / T.group;
     int t = (L / gs) * T.group + rem % T.group;
     if (t >= T.start[T.nb]) return false;
     int q = 0;
     while (q + 1 < T.nb && t >= T.start[q + 1]) ++q;
     t -= T.start[q];
     R = T.b[q];
-    const int kw = T.kw[q], rpw = T.rpw[q];
+    const int kw = T.kw[q], rpw = T.rpw[q], kb = T.kb[q];
     const int dk = (int)threadIdx.x % kw, di = (int)threadIdx.x / kw;
     if (di >= rpw) return false;
     const int tk = t % T.ntk[q];
     t /= T.ntk[q];
     const int ti = t % T.nti[q], tj = t / T.nti[q];
-    k = R.k0 + tk * kw + dk;
-    i = R.i0 + (ti * TI + (int)threadIdx.y) * rpw + di;
+    // wavefront y of the workgroup: k column y % kb, shot (y / kb) % sb of
+    // the shot group, i row y / (kb sb)
+    const int y = (int)threadIdx.y, sb = T.sb;
+    const int wk = y % kb, ws = (y / kb) % sb, wi = y / (kb * sb);
+    shot = sgrp * sb + ws;
+    if (shot >= T.ns) return false;
+    k = R.k0 + (tk * kb + wk) * kw + dk;
+    i = R.i0 + (ti * (TI / (kb * sb)) + wi) * rpw + di;
     if (k > R.k1 || i > R.i1) return false;
     jb = R.j0 + tj * T.jc;
     je = min(jb + T.jc - 1, R.j1);
@@ -384,15 +391,16 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 
     for (int j = jb; j <= je; ++j, vo += SJ) {
         if (PIPE && j < je) SLOAD(nxt, 1);
-        float vxx = D4B(b1, b2, cur.vx_xa, cur.vx_xb, vx_0, cur.vx_xd) / DX;
-        float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, cur.vx_n) / DY;
-        float vxz = D4F(b1, b2, cur.vx_zb, vx_0, cur.vx_zc, cur.vx_zd) / DZ;
-        float vyx = D4F(b1, b2, cur.vy_xb, vy_0, cur.vy_xc, cur.vy_xd) / DX;
-        float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, cur.vy_n) / DY;
-        float vyz = D4F(b1, b2, cur.vy_zb, vy_0, cur.vy_zc, cur.vy_zd) / DZ;
-        float vzx = D4F(b1, b2, cur.vz_xb, vz_0, cur.vz_xc, cur.vz_xd) / DX;
-        float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, cur.vz_n) / DY;
-        float vzz = D4B(b1, b2, cur.vz_za, cur.vz_zb, vz_0, cur.vz_zd) / DZ;
+        float vxx = D4B(b1, b2, cur.vx_xa, cur.vx_xb, vx_0, cur.vx_xd);
+        float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, cur.vx_n);
+        float vxz = D4F(b1, b2, cur.vx_zb, vx_0, cur.vx_zc, cur.vx_zd);
+        float vyx = D4F(b1, b2, cur.vy_xb, vy_0, cur.vy_xc, cur.vy_xd);
+        float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, cur.vy_n);
+        float vyz = D4F(b1, b2, cur.vy_zb, vy_0, cur.vy_zc, cur.vy_zd);
+        float vzx = D4F(b1, b2, cur.vz_xb, vz_0, cur.vz_xc, cur.vz_xd);
+        float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, cur.vz_n);
+        float vzz = D4B(b1, b2, cur.vz_za, cur.vz_zb, vz_0, cur.vz_zd);
+        div_spacing(g, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
         if (CPML) {
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
@@ -485,15 +493,16 @@ __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, Gra
             qy[t][3] = LD(fy, 1, 0, 0);
             qz[t][3] = LD(fz, 2, 0, 0);
             const float x_c = qx[t][1], y_c = qy[t][2], z_c = qz[t][1];
-            d[t][0] = (b1 * (x_c - LD(fx, 0, -1, 0)) + b2 * (LD(fx, 0, 1, 0) - LD(fx, 0, -2, 0))) / g.dx;
-            d[t][1] = (b1 * (qx[t][2] - x_c) + b2 * (qx[t][3] - qx[t][0])) / g.dy;
-            d[t][2] = (b1 * (LD(fx, 0, 0, 1) - x_c) + b2 * (LD(fx, 0, 0, 2) - LD(fx, 0, 0, -1))) / g.dz;
-            d[t][3] = (b1 * (LD(fy, 0, 1, 0) - y_c) + b2 * (LD(fy, 0, 2, 0) - LD(fy, 0, -1, 0))) / g.dx;
-            d[t][4] = (b1 * (y_c - qy[t][1]) + b2 * (qy[t][3] - qy[t][0])) / g.dy;
-            d[t][5] = (b1 * (LD(fy, 0, 0, 1) - y_c) + b2 * (LD(fy, 0, 0, 2) - LD(fy, 0, 0, -1))) / g.dz;
-            d[t][6] = (b1 * (LD(fz, 0, 1, 0) - z_c) + b2 * (LD(fz, 0, 2, 0) - LD(fz, 0, -1, 0))) / g.dx;
-            d[t][7] = (b1 * (qz[t][2] - z_c) + b2 * (qz[t][3] - qz[t][0])) / g.dy;
-            d[t][8] = (b1 * (z_c - LD(fz, 0, 0, -1)) + b2 * (LD(fz, 0, 0, 1) - LD(fz, 0, 0, -2))) / g.dz;
+            d[t][0] = b1 * (x_c - LD(fx, 0, -1, 0)) + b2 * (LD(fx, 0, 1, 0) - LD(fx, 0, -2, 0));
+            d[t][1] = b1 * (qx[t][2] - x_c) + b2 * (qx[t][3] - qx[t][0]);
+            d[t][2] = b1 * (LD(fx, 0, 0, 1) - x_c) + b2 * (LD(fx, 0, 0, 2) - LD(fx, 0, 0, -1));
+            d[t][3] = b1 * (LD(fy, 0, 1, 0) - y_c) + b2 * (LD(fy, 0, 2, 0) - LD(fy, 0, -1, 0));
+            d[t][4] = b1 * (y_c - qy[t][1]) + b2 * (qy[t][3] - qy[t][0]);
+            d[t][5] = b1 * (LD(fy, 0, 0, 1) - y_c) + b2 * (LD(fy, 0, 0, 2) - LD(fy, 0, 0, -1));
+            d[t][6] = b1 * (LD(fz, 0, 1, 0) - z_c) + b2 * (LD(fz, 0, 2, 0) - LD(fz, 0, -1, 0));
+            d[t][7] = b1 * (qz[t][2] - z_c) + b2 * (qz[t][3] - qz[t][0]);
+            d[t][8] = b1 * (z_c - LD(fz, 0, 0, -1)) + b2 * (LD(fz, 0, 0, 1) - LD(fz, 0, 0, -2));
+            div_spacing(g, d[t][0], d[t][1], d[t][2], d[t][3], d[t][4], d[t][5], d[t][6], d[t][7], d[t][8]);
         }
         // d[t] = xx xy xz yx yy yz zx zy zz of f (t 0), fi (1), b (2), bi (3)
         const float *f = d[0], *fi = d[1], *bb = d[2], *bi = d[3];
@@ -580,16 +589,17 @@ __global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F
     if (!slab_cell(L, t, j, i, k)) return;
     const int64_t c = at(g, j, i, k), sj = g.sj, si = g.si;
     const float b1 = g.b1, b2 = g.b2;
-    const float DX = g.dx, DY = g.dy, DZ = g.dz;
-    float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], F.vx[c], F.vx[c + si]) / DX;
-    float vxy = D4F(b1, b2, F.vx[c - sj], F.vx[c], F.vx[c + sj], F.vx[c + 2 * sj]) / DY;
-    float vxz = D4F(b1, b2, F.vx[c - 1], F.vx[c], F.vx[c + 1], F.vx[c + 2]) / DZ;
-    float vyx = D4F(b1, b2, F.vy[c - si], F.vy[c], F.vy[c + si], F.vy[c + 2 * si]) / DX;
-    float vyy = D4B(b1, b2, F.vy[c - 2 * sj], F.vy[c - sj], F.vy[c], F.vy[c + sj]) / DY;
-    float vyz = D4F(b1, b2, F.vy[c - 1], F.vy[c], F.vy[c + 1], F.vy[c + 2]) / DZ;
-    float vzx = D4F(b1, b2, F.vz[c - si], F.vz[c], F.vz[c + si], F.vz[c + 2 * si]) / DX;
-    float vzy = D4F(b1, b2, F.vz[c - sj], F.vz[c], F.vz[c + sj], F.vz[c + 2 * sj]) / DY;
-    float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], F.vz[c], F.vz[c + 1]) / DZ;
+
+    float vxx = D4B(b1, b2, F.vx[c - 2 * si], F.vx[c - si], F.vx[c], F.vx[c + si]);
+    float vxy = D4F(b1, b2, F.vx[c - sj], F.vx[c], F.vx[c + sj], F.vx[c + 2 * sj]);
+    float vxz = D4F(b1, b2, F.vx[c - 1], F.vx[c], F.vx[c + 1], F.vx[c + 2]);
+    float vyx = D4F(b1, b2, F.vy[c - si], F.vy[c], F.vy[c + si], F.vy[c + 2 * si]);
+    float vyy = D4B(b1, b2, F.vy[c - 2 * sj], F.vy[c - sj], F.vy[c], F.vy[c + sj]);
+    float vyz = D4F(b1, b2, F.vy[c - 1], F.vy[c], F.vy[c + 1], F.vy[c + 2]);
+    float vzx = D4F(b1, b2, F.vz[c - si], F.vz[c], F.vz[c + si], F.vz[c + 2 * si]);
+    float vzy = D4F(b1, b2, F.vz[c - sj], F.vz[c], F.vz[c + sj], F.vz[c + 2 * sj]);
+    float vzz = D4B(b1, b2, F.vz[c - 2], F.vz[c - 1], F.vz[c], F.vz[c + 1]);
+    div_spacing(g, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
     const int hx = slab_h(i, g.nx, g.fw, 1);
     const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
     const int hz = slab_h(k, g.nz, g.fw, 1);
@@ -671,22 +681,32 @@ static Box region_box(const ifos_geom *g, int region)
 
 // tiles of boxes for TI i-rows per block; jc <= 0 picks the j chunk so a
 // launch has >= 4096 workgroups (16 per CU) when the boxes allow it
-static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int group)
+static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int group, int kb = 1, int sb = 1)
 {
     MTiles T = {};
     T.nb = 0;
     T.ns = ns;
+    // shots side by side in a workgroup share its model loads
+    int sbq = sb < 1 ? 1 : sb;
+    while (sbq > 1 && (sbq > ns || ti % sbq)) sbq /= 2;
+    T.sb = sbq;
+    T.nsg = (ns + sbq - 1) / sbq;
     T.start[0] = 0;
     for (int q = 0; q < nb; ++q) {
         const Box &B = boxes[q];
         if (B.i1 < B.i0 || B.j1 < B.j0 || B.k1 < B.k0) continue;
         const int nk = B.k1 - B.k0 + 1, ni = B.i1 - B.i0 + 1;
         const int kw = nk >= 64 ? 64 : nk, rpw = 64 / kw;
+        const int nwk = (nk + kw - 1) / kw;  // wavefront columns along k
+        int kbq = kb < 1 ? 1 : kb;
+        while (kbq > 1 && (kbq > nwk || (ti / sbq) % kbq)) kbq /= 2;
+        const int rows = (ti / (kbq * sbq)) * rpw;  // i-rows of a workgroup
         T.b[T.nb] = B;
         T.kw[T.nb] = kw;
         T.rpw[T.nb] = rpw;
-        T.ntk[T.nb] = (nk + kw - 1) / kw;
-        T.nti[T.nb] = (ni + ti * rpw - 1) / (ti * rpw);
+        T.kb[T.nb] = kbq;
+        T.ntk[T.nb] = (nwk + kbq - 1) / kbq;
+        T.nti[T.nb] = (ni + rows - 1) / rows;
         ++T.nb;
     }
     auto count = [&](int c) {
@@ -697,7 +717,7 @@ static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int g
     };
     if (jc <= 0) {
         jc = 64;
-        while (jc > 8 && count(jc) * ns < 4096) jc /= 2;
+        while (jc > 8 && count(jc) * T.nsg < 4096) jc /= 2;
     }
     T.jc = jc;
     for (int q = 0; q < T.nb; ++q)
@@ -706,14 +726,14 @@ static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int g
     const int tiles = T.start[T.nb];
     T.group = group <= 0 ? tiles : std::min(group, tiles);
     if (T.group < 1) T.group = 1;
-    T.total = ((tiles + T.group - 1) / T.group) * T.group * ns;
+    T.total = ((tiles + T.group - 1) / T.group) * T.group * T.nsg;
     T.per = (T.total + 7) / 8;
     return T;
 }
 
 // default tile shape of the march kernels (i-rows per block, j chunk,
 // shot grouping); the batched path's autotuner overrides it per kernel
-static MarchCfg g_default_cfg = {4, 0, 0, 0, 0};
+static MarchCfg g_default_cfg = {4, 0, 0, 0, 0, 1, 1};
 
 template <bool CPML, bool SNAP, bool PIPE>
 static void launch_v_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
@@ -964,15 +984,15 @@ int launch_v_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
     ProfScope ps_(IFOS_K_V_FUSED, s);
     if (!c.split) {
         const Box all = region_box(g, IFOS_REGION_ALL);
-        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group);
+        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
         return snap ? launch_v<true, true>(c, T, G, F, P, psi, S, N, s)
                     : launch_v<true, false>(c, T, G, F, P, psi, S, N, s);
     }
     const Box in = region_box(g, IFOS_REGION_INTERIOR);
     Box sl[6];
     const int nb = slab_boxes(g, sl);
-    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group);
-    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group);
+    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
+    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
     int rc = snap ? launch_v<false, true>(c, Ti, G, F, P, psi, S, N, s)
                   : launch_v<false, false>(c, Ti, G, F, P, psi, S, N, s);
     if (!rc)
@@ -991,14 +1011,14 @@ int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
     ProfScope ps_(IFOS_K_S_FUSED, s);
     if (!c.split) {
         const Box all = region_box(g, IFOS_REGION_ALL);
-        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group);
+        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
         return V ? launch_s<true, true>(c, T, G, F, P, psi, S, s, *V) : launch_s<true>(c, T, G, F, P, psi, S, s);
     }
     const Box in = region_box(g, IFOS_REGION_INTERIOR);
     Box sl[6];
     const int nb = slab_boxes(g, sl);
-    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group);
-    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group);
+    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
+    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
     int rc = V ? launch_s<false, true>(c, Ti, G, F, P, psi, S, s, *V) : launch_s<false>(c, Ti, G, F, P, psi, S, s);
     if (!rc)
         rc = V ? launch_s<true, true>(c, Ts, G, F, P, psi, S, s, *V) : launch_s<true>(c, Ts, G, F, P, psi, S, s);
@@ -1185,15 +1205,16 @@ __global__ __launch_bounds__(64 * TI) void k_vs_fused(DevGeom g, FTiles T, Fused
             const float pi = LDJ(rpi, j, 0, 0), u = LDJ(ru, j, 0, 0);
             const float sxx0 = LDJ(rsxx, j, 0, 0), syy0 = LDJ(rsyy, j, 0, 0), szz0 = LDJ(rszz, j, 0, 0);
             const float sxy0 = LDJ(rsxy, j, 0, 0), syz0 = LDJ(rsyz, j, 0, 0), sxz0 = LDJ(rsxz, j, 0, 0);
-            float vxx = D4B(b1, b2, vx_xa, vx_xb, vx_0, vx_xd) / DX;
-            float vxy = D4F(b1, b2, qvx[1], vx_0, qvx[3], qvx[4]) / DY;
-            float vxz = D4F(b1, b2, vx_zb, vx_0, vx_zc, vx_zd) / DZ;
-            float vyx = D4F(b1, b2, vy_xb, vy_0, vy_xc, vy_xd) / DX;
-            float vyy = D4B(b1, b2, qvy[0], qvy[1], vy_0, qvy[3]) / DY;
-            float vyz = D4F(b1, b2, vy_zb, vy_0, vy_zc, vy_zd) / DZ;
-            float vzx = D4F(b1, b2, vz_xb, vz_0, vz_xc, vz_xd) / DX;
-            float vzy = D4F(b1, b2, qvz[1], vz_0, qvz[3], qvz[4]) / DY;
-            float vzz = D4B(b1, b2, vz_za, vz_zb, vz_0, vz_zd) / DZ;
+            float vxx = D4B(b1, b2, vx_xa, vx_xb, vx_0, vx_xd);
+            float vxy = D4F(b1, b2, qvx[1], vx_0, qvx[3], qvx[4]);
+            float vxz = D4F(b1, b2, vx_zb, vx_0, vx_zc, vx_zd);
+            float vyx = D4F(b1, b2, vy_xb, vy_0, vy_xc, vy_xd);
+            float vyy = D4B(b1, b2, qvy[0], qvy[1], vy_0, qvy[3]);
+            float vyz = D4F(b1, b2, vy_zb, vy_0, vy_zc, vy_zd);
+            float vzx = D4F(b1, b2, vz_xb, vz_0, vz_xc, vz_xd);
+            float vzy = D4F(b1, b2, qvz[1], vz_0, qvz[3], qvz[4]);
+            float vzz = D4B(b1, b2, vz_za, vz_zb, vz_0, vz_zd);
+            div_spacing(g, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
             const bool own = inner && j >= jb && j <= je;
             const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
             if (hx | hy | hz)

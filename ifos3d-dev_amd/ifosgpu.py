@@ -491,17 +491,26 @@ class BatchContext:
     def last_fused(self):
         return bool(lib().ifos_batch_last_fused(self.h))
 
+    TILE_DEFAULTS = (4, 0, 0, 0, 0, 1, 1)
+
+    @classmethod
+    def _tile(cls, cfg):
+        cfg = tuple(cfg)
+        return (C.c_int * len(cls.TILE_DEFAULTS))(*(cfg + cls.TILE_DEFAULTS[len(cfg):]))
+
     def set_tiles(self, cfg_v=None, cfg_s=None):
-        """tile shapes {TI, jc, group, split, pipe} of the velocity / stress marches"""
-        cv = (C.c_int * 5)(*(tuple(cfg_v) + (0,) * (5 - len(cfg_v)))) if cfg_v is not None else None
-        cs = (C.c_int * 5)(*(tuple(cfg_s) + (0,) * (5 - len(cfg_s)))) if cfg_s is not None else None
+        """tile shapes {TI, jc, group, split, pipe, kb, sb} of the velocity / stress
+        marches (missing trailing fields take TILE_DEFAULTS)"""
+        cv = self._tile(cfg_v) if cfg_v is not None else None
+        cs = self._tile(cfg_s) if cfg_s is not None else None
         check(lib().ifos_batch_set_tiles(self.h, cv, cs), "ifos_batch_set_tiles")
 
-    @staticmethod
-    def tune_candidates():
-        buf = (C.c_int * 320)()
+    @classmethod
+    def tune_candidates(cls):
+        nf = len(cls.TILE_DEFAULTS)
+        buf = (C.c_int * (64 * nf))()
         n = lib().ifos_batch_tune_candidates(buf, 64)
-        return [tuple(buf[5 * c:5 * c + 5]) for c in range(n)]
+        return [tuple(buf[nf * c:nf * c + nf]) for c in range(n)]
 
     def autotune(self, ns=None, reps=3):
         """times every candidate tile shape; returns (best_v, best_s, table)"""

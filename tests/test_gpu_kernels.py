@@ -90,12 +90,13 @@ def test_velocity_update_regions(env, free_surf, fdcoeff):
         assert np.array_equal(dpsi2[n].numpy(), po[n]), f"fused psi_s[{n}]"
 
 
+@pytest.mark.parametrize("dh", [2.0, 2.5])  # power-of-two spacing: multiply by 1/dh; else IEEE division
 @pytest.mark.parametrize("free_surf", [0, 1])
 @pytest.mark.parametrize("fdcoeff", [1, 2])
-def test_stress_update_regions(env, free_surf, fdcoeff):
+def test_stress_update_regions(env, free_surf, fdcoeff, dh):
     _, ig = env
-    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=29)
-    _note(f"stress update parity fdcoeff={fdcoeff} free_surf={free_surf}")
+    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=29, dh=dh)
+    _note(f"stress update parity fdcoeff={fdcoeff} free_surf={free_surf} dh={dh}")
     o, rng, rho, pi, u, avg, prof, f, psi = _setup(g, 21 + fdcoeff + 2 * free_surf)
     G = _geom(ig, g)
     dmodel, df, dpsi, dprof = _upload(ig, G, rho, pi, u, avg, f, psi, prof)
@@ -290,10 +291,11 @@ def test_dft_compat_and_kahan(env, nf):
     assert max(ek) <= max(ec)
 
 
-def test_gradient_F(env):
+@pytest.mark.parametrize("dh", [2.0, 3.0])
+def test_gradient_F(env, dh):
     _, ig = env
-    g = small_grid(nx=26, ny=24, nz=28)
-    _note("gradient_F parity")
+    g = small_grid(nx=26, ny=24, nz=28, dh=dh)
+    _note(f"gradient_F parity dh={dh}")
     o = Oracle("f32")
     rng = np.random.default_rng(9)
     rho, pi, u = random_model(g, rng)
