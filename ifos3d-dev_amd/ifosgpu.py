@@ -421,6 +421,65 @@ class ShotContext:
         return list(arr)
 
 
+class Lbfgs:
+    """L-BFGS history on the device (lbfgs.c, lbfgs_save.c) with FP64 dots.
+    Vectors are compact: numpar blocks of NX*NY*NZ cells, cell
+    ((j-1)*NX + i-1)*NZ + k-1."""
+
+    def __init__(self, g, numpar=2, bfgsnum=5):
+        self.g, self.numpar, self.bfgsnum = g, numpar, bfgsnum
+        self.n = numpar * g.nx * g.ny * g.nz
+        h = C.c_void_p()
+        check(lib().ifos_lbfgs_create(g.ref, numpar, bfgsnum, C.byref(h)), "ifos_lbfgs_create")
+        self.h = h
+
+    def savegrad(self, grad1, grad2=None):
+        check(lib().ifos_lbfgs_savegrad(self.h, _ptr(grad1), _ptr(grad2)), "ifos_lbfgs_savegrad")
+
+    def update(self, grad1, grad2, iteration) -> float:
+        """replaces grad1 (and grad2) by the L-BFGS direction; returns h0"""
+        h0 = C.c_double()
+        check(lib().ifos_lbfgs_update(self.h, _ptr(grad1), _ptr(grad2), iteration, C.byref(h0)),
+              "ifos_lbfgs_update")
+        return h0.value
+
+    def _slot(self, fn, w):
+        f = getattr(lib(), fn)
+        f.restype = C.c_void_p
+        p = f(self.h, w)
+        if not p:
+            raise IfosError(f"{fn}: slot {w} out of range")
+        return DBuf.wrap(p, (self.n,))
+
+    def mod(self, w) -> DBuf:
+        """slot w (1..bfgsnum) of bfgsmod, the model step the caller writes"""
+        return self._slot("ifos_lbfgs_mod", w)
+
+    def grad(self, w) -> DBuf:
+        return self._slot("ifos_lbfgs_grad", w)
+
+    def scale(self):
+        out = (C.c_double * self.bfgsnum)()
+        check(lib().ifos_lbfgs_scale(self.h, out), "ifos_lbfgs_scale")
+        return np.array(out[:], np.float64)
+
+    def close(self):
+        if self.h:
+            lib().ifos_lbfgs_destroy(self.h)
+            self.h = None
+
+
+def conjugate(g, grads, priors, cdf):
+    """conjugategrad.c on device fields: grads[3], priors[6]; returns
+    (beta[3], the six FP64 sums or None for cdf 1)"""
+    Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads])
+    Pp = (C.c_void_p * 6)(*[t.data_ptr() for t in priors])
+    beta = (C.c_float * 3)()
+    sums = (C.c_double * 6)()
+    check(lib().ifos_conjugate(g.ref, Gp, Pp, beta, cdf, sums), "ifos_conjugate")
+    return np.array(beta[:], np.float32), (None if cdf == 1 else np.array(sums[:], np.float64))
+
+
 class BatchContext:
     """The batched path: several shots of one model per launch (ifos_batch_*)."""
 

@@ -326,6 +326,35 @@ int ifos_batch_last_fused(const ifos_batch *b);
  * component n (Re x y z, Im x y z) at d_F[n] + l * ifos_field_elems(g) */
 int ifos_batch_dft(ifos_batch *b, int shot, int back, float *d_F[6]);
 
+/* ---- optimiser steps on the device, dot products reduced in FP64 (the
+ * reference accumulates them in FP32 and all-reduces MPI_FLOAT) ---- */
+typedef struct ifos_lbfgs ifos_lbfgs;
+/* the L-BFGS history of ifos3d.c:373-381: bfgsgrad and bfgsmod, bfgsnum
+ * slots of numpar * NX*NY*NZ floats (numpar 1: grad1 only, 2: grad1 and
+ * grad2, lbfgs.c:49-71), and bfgsscale; all zero */
+int ifos_lbfgs_create(const ifos_geom *g, int numpar, int bfgsnum, ifos_lbfgs **out);
+void ifos_lbfgs_destroy(ifos_lbfgs *b);
+/* device pointers of slot w (1..bfgsnum) of bfgsmod (the model step that
+ * modelupdate.c:146-149 writes, cell ((j-1)*NX + i-1)*NZ + k-1, grad2's
+ * block after grad1's) and of bfgsgrad */
+float *ifos_lbfgs_mod(ifos_lbfgs *b, int w);
+float *ifos_lbfgs_grad(ifos_lbfgs *b, int w);
+/* bfgsscale[0..bfgsnum-1] (1 / y.s of each slot) */
+int ifos_lbfgs_scale(const ifos_lbfgs *b, double *scale);
+/* lbfgs_save.c:25-54: slot 1 of bfgsgrad = the gradient (first iteration) */
+int ifos_lbfgs_savegrad(ifos_lbfgs *b, const float *d_grad1, const float *d_grad2);
+/* lbfgs.c:26-181 for iteration >= 2: y = previous - current gradient,
+ * rho = 1 / y.s, h0 = y.s / y.y, the two-loop recursion over the last
+ * bfgsnum pairs, the current gradient saved, d_grad1 (and d_grad2) replaced
+ * by the update direction.  *h0_out (may be NULL) receives h0. */
+int ifos_lbfgs_update(ifos_lbfgs *b, float *d_grad1, float *d_grad2, int iteration, double *h0_out);
+/* conjugategrad.c:26-110 on the device-layout fields: cdf 1 copies the
+ * gradients to all six priors; else beta from the six sums (written to
+ * sums[6] when not NULL), the reference's clamps, the new direction in
+ * d_grad and d_prior[3..5]; beta[3] receives the factors */
+int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float *const d_prior[6], float beta[3], int cdf,
+                   double *sums);
+
 #ifdef __cplusplus
 }
 #endif
