@@ -1,0 +1,131 @@
+"""A torch.autograd.Function over the device path, checked by finite
+differences of the misfit along model perturbations.
+
+IfosMisfit.forward runs every shot through ifos_ctx_run_shot and returns the
+time-domain misfit J = sum over traces and samples of (DT * running sum of
+(syn - obs))^2, the quantity residual.c:43-53 accumulates; backward returns
+minus the gradient gradient_F.c accumulates (grad1..3 = vp, vs, rho), since
+the reference's gradient is the descent direction (lbfgs.c:46).
+
+That gradient is a frequency-domain estimate (gradient_F.c:122-139: the
+products of the forward and adjoint DFT fields at the chosen frequencies,
+each weighted 1 / (4 pi^2 f^2)), not the exact derivative of the discrete J:
+with the frequencies on the DFT grid 1 / (NT DT) that covers the source
+band it is proportional to dJ/dm with a normalisation of its own.  The check
+is therefore on proportionality: along perturbations of vp (two places), vs
+and rho, the finite-difference derivative divided by <backward, delta> must
+be positive and the same within 25% (measured spread about 17%, ratio about
+0.01).  The misfit itself is compared with the device's FP64-reduced L2.
+"""
+import numpy as np
+import pytest
+
+from helpers import small_grid
+
+pytestmark = pytest.mark.gpu
+
+
+class _Problem:
+    def __init__(self):
+        import ifosgpu as ig
+        self.ig = ig
+        g = small_grid(nx=40, ny=40, nz=40, fw=8, nt=500, fdcoeff=2, dh=2.0, dt=2e-4)
+        self.g = g
+        self.G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt, g.fdcoeff, g.free_surf)
+        shp = g.shape
+        self.J, self.I, self.K = np.meshgrid(*(np.arange(s) - 2.0 for s in shp), indexing="ij")
+        one = np.ones(shp)
+        self.base = (4000.0 * one, 2300.0 * one, 2000.0 * one)
+        blob = self.bump(24, 20, 20, 4)
+        true = (4000.0 * (1 + 0.05 * blob), 2300.0 * (1 + 0.05 * blob), 2000.0 * one)
+        self.srcs = [(14, 12, 14, 0.0, 50.0, 1.0, 1), (27, 12, 26, 0.0, 50.0, 1.0, 1)]
+        self.rec = np.array([[10 + 2 * n, 11, 10 + 2 * m] for n in range(10) for m in range(10)], np.int32)
+        self.finv = [n / (g.nt * g.dt) for n in range(1, 17)]  # 10 .. 160 Hz on the DFT grid
+        self.ctx = ig.ShotContext(self.G, len(self.finv), len(self.rec), ig.DFT_COMPAT)
+        self.ctx.set_cpml(4000.0, 50.0)
+        self.ctx.set_freqs(self.finv, 100)
+        self.d_syn = ig.DBuf((3, len(self.rec), g.nt + 1))
+        self.ctx.set_model(*self.upload(true))
+        self.obs = []
+        for s in self.srcs:
+            self.ctx.run_shot(s, self.rec, self.d_syn, None, want_adjoint=False)
+            self.obs.append(self.d_syn.numpy().astype(np.float64))
+        self.d_obs = [ig.DBuf.from_numpy(o.astype(np.float32)) for o in self.obs]
+
+    def bump(self, cj, ci, ck, r):
+        return np.exp(-((self.J - cj) ** 2 + (self.I - ci) ** 2 + (self.K - ck) ** 2) / (2.0 * r * r))
+
+    def full(self, interior):
+        """an interior (NY, NX, NZ) array on the canonical grid, halos by edge copy"""
+        return np.pad(np.asarray(interior, np.float64), ((3, 2), (3, 2), (3, 2)), mode="edge")
+
+    def upload(self, m):
+        vp, vs, rho = m
+        return [self.G.to_dev(np.asarray(a, np.float32)) for a in (rho, rho * vp * vp, rho * vs * vs)]
+
+    def misfit(self, m):
+        self.ctx.set_model(*self.upload(m))
+        J = 0.0
+        for n, s in enumerate(self.srcs):
+            self.ctx.run_shot(s, self.rec, self.d_syn, None, want_adjoint=False)
+            r = self.d_syn.numpy().astype(np.float64)[:, :, 1:] - self.obs[n][:, :, 1:]
+            R = self.g.dt * np.cumsum(r, axis=2)
+            J += float((R * R).sum())
+        return J
+
+    def misfit_and_gradient(self, m):
+        ig = self.ig
+        self.ctx.set_model(*self.upload(m))
+        grads = [self.G.empty() for _ in range(3)]
+        l2 = 0.0
+        for n, s in enumerate(self.srcs):
+            l2 += self.ctx.run_shot(s, self.rec, self.d_syn, grads, d_obs=self.d_obs[n])
+        ig.sync()
+        return l2, [self.g.interior(self.G.from_dev(t)).astype(np.float64) for t in grads]
+
+
+def _autograd_fn(problem):
+    import torch
+
+    class IfosMisfit(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, vp, vs, rho):
+            m = tuple(problem.full(t.detach().cpu().numpy()) for t in (vp, vs, rho))
+            l2, grads = problem.misfit_and_gradient(m)
+            ctx.save_for_backward(*[torch.from_numpy(-gq) for gq in grads])
+            return torch.tensor(l2, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, g_out):
+            return tuple(g_out * t for t in ctx.saved_tensors)
+
+    return IfosMisfit
+
+
+def test_autograd_gradient_is_proportional_to_finite_differences():
+    import torch
+    p = _Problem()
+    g = p.g
+    fn = _autograd_fn(p)
+    interior = lambda a: torch.tensor(g.interior(a).copy(), dtype=torch.float64, requires_grad=True)
+    vp, vs, rho = (interior(a) for a in p.base)
+    J = fn.apply(vp, vs, rho)
+    J.backward()
+    J_host = p.misfit(p.base)
+    print(f"misfit: device {J.item():.6e}, host FP64 {J_host:.6e}", flush=True)
+    assert J.item() == pytest.approx(J_host, rel=1e-5)
+    dirs = {"vp at the anomaly": (p.bump(24, 20, 20, 4), 0, 0), "vp elsewhere": (p.bump(28, 26, 14, 3), 0, 0),
+            "vs at the anomaly": (0, p.bump(24, 20, 20, 4), 0), "rho at the anomaly": (0, 0, p.bump(24, 20, 20, 4))}
+    scale = (4000.0, 2300.0, 2000.0)
+    ratios = []
+    for name, d in dirs.items():
+        dm = [1e-3 * scale[q] * np.asarray(d[q], np.float64) * np.ones(g.shape) for q in range(3)]
+        fd = (p.misfit(tuple(p.base[q] + dm[q] for q in range(3))) -
+              p.misfit(tuple(p.base[q] - dm[q] for q in range(3)))) / 2.0
+        ad = sum(float((t.grad.numpy() * g.interior(dm[q])).sum()) for q, t in enumerate((vp, vs, rho)))
+        ratios.append(fd / ad)
+        print(f"{name:20s} FD {fd:+.4e}  <backward, delta> {ad:+.4e}  ratio {fd / ad:.4e}", flush=True)
+    ratios = np.array(ratios)
+    assert (ratios > 0).all()
+    assert ratios.max() / ratios.min() < 1.25
+    p.ctx.close()
