@@ -28,14 +28,26 @@ sys.path.insert(0, os.path.join(ROOT, "ifos3d-dev_amd"))
 
 METRIC = "gradient shots/sec at 256³ (batched vs plain); Gcell-updates/s; % HBM peak"
 HBM_PEAK_GBS = 8000.0
-# kernel (as rocprofv3 names it) behind each launch-timer category
-CAT_KERNEL = {"v_interior": "ifos::k_v_march<64, 4, false>", "s_interior": "ifos::k_s_march<64, 4, false>",
-              "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs", "dft": "ifos::k_dft_nf<{nf}>",
-              "dft_kahan": "ifos::k_dft_kahan_nf<{nf}>", "gradient": "ifos::k_gradient"}
-BATCH_KERNEL = {"v_fused": "ifos::k_v_march<4, true, true, false>", "s_fused": "ifos::k_s_march<4, true, false, false>",
-                "vs_fused": "ifos::k_vs_fused<16, false>",
-                "dft": "ifos::k_dft_chunk<{nf}, false>", "dft_kahan": "ifos::k_dft_chunk<{nf}, true>",
-                "gradient": "ifos::k_gradient_shots"}
+# kernel (as rocprofv3 names it) behind each launch-timer category; the
+# march kernels' template arguments follow the tile shape in use
+
+
+def kernel_names(path, nf, chunk, tiles=None):
+    b = lambda x: "true" if x else "false"
+    if path == "plain":
+        return {"v_interior": "ifos::k_v_march<4, false, false, false>",
+                "s_interior": "ifos::k_s_march<4, false, false, false>",
+                "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs<false>",
+                "dft": f"ifos::k_dft_nf<{nf}>", "dft_kahan": f"ifos::k_dft_kahan_nf<{nf}>",
+                "gradient": "ifos::k_grad_march<4>"}
+    tv, ts = tiles
+    return {"v_fused": f"ifos::k_v_march<{tv[0]}, true, false, {b(tv[4])}>",
+            "s_fused": f"ifos::k_s_march<{ts[0]}, true, false, {b(ts[4])}>",
+            "vs_fused": "ifos::k_vs_fused<16, false>",
+            "dft": f"ifos::k_dft_chunk<{nf}, {chunk}, false>", "dft_kahan": f"ifos::k_dft_chunk<{nf}, {chunk}, true>",
+            "gradient": "ifos::k_grad_march<4>"}
+
+
 # tile shapes {TI, jc, group, split, pipe} used without the autotuner (its
 # usual choice on grad276)
 DEFAULT_TILES = ((4, 32, 0, 0, 0), (4, 32, 0, 0, 1))
@@ -391,8 +403,11 @@ def main():
         path_tag = "batched" if args.path == "batched" else "plain"
         traffic = None
         tpath = os.path.join(ROOT, "profiles", f"{w.name}_{path_tag}_traffic.txt")
-        kname = (BATCH_KERNEL if args.path == "batched" else CAT_KERNEL).get(
-            "dft_kahan" if dom == "dft" and kahan else dom, "").format(nf=nf)
+        tiles = None
+        if args.path == "batched":
+            tune = batched.get("tune")
+            tiles = (tuple(tune["v"]), tuple(tune["s"])) if tune else DEFAULT_TILES
+        kname = kernel_names(path_tag, nf, args.chunk, tiles).get("dft_kahan" if dom == "dft" and kahan else dom, "")
         if os.path.exists(tpath) and kname:
             import profsum
             traffic = profsum.read_traffic(tpath).get(kname)
