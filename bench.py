@@ -50,7 +50,7 @@ def kernel_names(path, nf, chunk, tiles=None):
 
 # tile shapes {TI, jc, group, split, pipe} used without the autotuner (its
 # usual choice on grad276)
-DEFAULT_TILES = ((4, 32, 0, 0, 0), (4, 32, 0, 0, 1))
+DEFAULT_TILES = ((4, 32, 0, 0, 0, 1, 2), (4, 32, 0, 0, 0, 1, 2))
 CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other",
         "vs_fused"]
 
@@ -135,7 +135,7 @@ def setup(w):
     return G, [G.to_dev(a) for a in true], [G.to_dev(a) for a in start], start
 
 
-def observed_data(w, G, d_true, S, chunk, fused=True):
+def observed_data(w, G, d_true, S, chunk, fused=False):
     """forward runs of every shot position on the true model (untimed setup),
     through the batched path; one device buffer [3][ntr][nt+1] per shot"""
     import ifosgpu as ig
@@ -159,7 +159,7 @@ def observed_data(w, G, d_true, S, chunk, fused=True):
     return obs
 
 
-def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_graph, autotune=True, fused=True):
+def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_graph, autotune=True, fused=False):
     import ifosgpu as ig
     ntr = len(w.receivers)
     log(f"batched path: {S} shots per batch, DFT chunk {chunk}, graph {use_graph}, "
@@ -334,7 +334,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=8, help="DFT samples per accumulator pass")
     ap.add_argument("--graph", action="store_true", help="replay the time loops from hipGraphs")
     ap.add_argument("--no-autotune", action="store_true", help="default tile shapes instead of the autotuner's")
-    ap.add_argument("--no-fused", action="store_true", help="separate velocity and stress launches per step")
+    ap.add_argument("--fused", action="store_true",
+                    help="fused stress+velocity step without the autotuner's verdict (the autotuner "
+                         "turns it on by itself only where it is faster)")
     ap.add_argument("--tiles", default="",
                     help="tile shapes used without the autotuner, velocity/stress march, each "
                          "TI,jc,group,split,pipe[,kb[,sb]] (e.g. 4,32,0,0,0/4,32,0,0,1)")
@@ -373,7 +375,7 @@ def main():
     S = min(args.shots_per_batch, len(w.shots))
     ntast = ig.ntast(w.finv[-1], w.tast, w.dt)
     G, d_true, d_start, host_start = setup(w)
-    obs = observed_data(w, G, d_true, S, args.chunk, not args.no_fused)
+    obs = observed_data(w, G, d_true, S, args.chunk, args.fused)
     del d_true
 
     plain = None
@@ -385,7 +387,7 @@ def main():
     batched = None
     if args.path == "batched":
         batched = run_batched(w, G, d_start, obs, args.steps, args.warmup, prec, S, args.chunk, args.graph,
-                              not args.no_autotune, not args.no_fused)
+                              not args.no_autotune, args.fused)
         batched["value"] = batched["shots"] / batched["elapsed"]
         log(f"batched path: {batched['value']:.4f} shots/s")
 

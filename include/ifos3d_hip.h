@@ -311,7 +311,7 @@ int ifos_batch_tune_candidates(int *cfg, int max);
 int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
                         int *best_s, float *ms_v, float *ms_s);
 /* the fused step (stress of step n and velocity of step n+1 in one launch,
- * double-buffered fields): on by default, used when there is no free
+ * double-buffered fields): off by default, used when on and there is no free
  * surface, no source or receiver inside a CPML slab and L = 0; ti = 8 or 16
  * workgroup rows, jc j-planes per tile (<= 0: automatic) */
 int ifos_batch_set_fused(ifos_batch *b, int on, int ti, int jc);
