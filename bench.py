@@ -339,7 +339,7 @@ def main():
                          "turns it on by itself only where it is faster)")
     ap.add_argument("--tiles", default="",
                     help="tile shapes used without the autotuner, velocity/stress march, each "
-                         "TI,jc,group,split,pipe[,kb[,sb]] (e.g. 4,32,0,0,0/4,32,0,0,1)")
+                         "TI,jc,group,split,pipe[,kb[,sb[,lds]]] (e.g. 4,32,0,0,0/4,32,0,0,1)")
     ap.add_argument("--nt", type=int, default=0,
                     help="shorter time loop (profiling runs only: the line then reports a smaller workload)")
     ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
@@ -349,8 +349,8 @@ def main():
     if args.tiles:
         global DEFAULT_TILES
         DEFAULT_TILES = tuple(tuple(int(x) for x in part.split(",")) for part in args.tiles.split("/"))
-        if len(DEFAULT_TILES) != 2 or any(len(t) not in (5, 6, 7) for t in DEFAULT_TILES):
-            ap.error("--tiles wants two groups of five to seven integers separated by '/'")
+        if len(DEFAULT_TILES) != 2 or any(len(t) not in (5, 6, 7, 8) for t in DEFAULT_TILES):
+            ap.error("--tiles wants two groups of five to eight integers separated by '/'")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(shots would be sharded one rank per GPU; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)

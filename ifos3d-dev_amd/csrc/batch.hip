@@ -425,7 +425,7 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->nf = 0;
     b->tast = 100;
     b->ntr = 0;
-    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0, 1, 1};
+    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0, 1, 1, 0};
     b->fused = 0;  // opt-in; the autotuner turns it on where it beats the separate marches
     b->fz_ti = 16;
     b->fz_jc = 0;
@@ -995,11 +995,11 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
 // the batch's own buffers (ns shots, `reps` launches each) and keeps the
 // fastest.  The wavefields are zeroed afterwards.
 static const MarchCfg k_candidates[] = {
-    {4, 32, 0, 0, 0, 1, 1}, {4, 32, 0, 0, 1, 1, 1}, {4, 0, 0, 0, 0, 1, 1},  {4, 0, 0, 0, 1, 1, 1},
-    {8, 32, 0, 0, 0, 1, 1}, {8, 32, 0, 0, 1, 1, 1}, {2, 0, 0, 0, 1, 1, 1},  {1, 0, 0, 0, 1, 1, 1},
-    {4, 0, 0, 1, 1, 1, 1},  {8, 32, 0, 0, 0, 2, 1}, {4, 32, 0, 0, 1, 2, 1}, {4, 32, 0, 0, 0, 1, 2},
-    {4, 32, 0, 0, 1, 1, 2}, {8, 32, 0, 0, 0, 1, 2}, {8, 32, 0, 0, 1, 1, 2}, {8, 32, 0, 0, 0, 1, 4},
-    {8, 32, 0, 0, 1, 1, 4}, {4, 32, 0, 0, 0, 1, 4}, {4, 32, 0, 0, 1, 1, 4}, {2, 32, 0, 0, 1, 1, 2},
+    {4, 32, 0, 0, 0, 1, 1, 0}, {4, 32, 0, 0, 1, 1, 1, 0}, {4, 0, 0, 0, 0, 1, 1, 0}, {8, 32, 0, 0, 0, 1, 1, 0},
+    {2, 0, 0, 0, 1, 1, 1, 0},  {1, 0, 0, 0, 1, 1, 1, 0},  {4, 0, 0, 1, 1, 1, 1, 0}, {4, 32, 0, 0, 0, 1, 2, 0},
+    {4, 32, 0, 0, 1, 1, 2, 0}, {8, 32, 0, 0, 0, 1, 2, 0}, {4, 32, 0, 0, 0, 1, 4, 0}, {2, 32, 0, 0, 1, 1, 2, 0},
+    {4, 32, 0, 0, 0, 1, 1, 1}, {8, 32, 0, 0, 0, 1, 1, 1}, {2, 32, 0, 0, 0, 1, 1, 1}, {4, 0, 0, 0, 0, 1, 1, 1},
+    {8, 0, 0, 0, 0, 1, 1, 1},  {4, 64, 0, 0, 0, 1, 1, 1}, {8, 16, 0, 0, 0, 1, 1, 1}, {2, 0, 0, 0, 0, 1, 1, 1},
 };
 
 extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v, int *best_s, float *ms_v,
@@ -1129,6 +1129,7 @@ extern "C" int ifos_batch_tune_candidates(int *cfg, int max)
         o[4] = k_candidates[c].pipe;
         o[5] = k_candidates[c].kb;
         o[6] = k_candidates[c].sb;
+        o[7] = k_candidates[c].lds;
     }
     return nc;
 }
@@ -1147,8 +1148,8 @@ extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[IFOS_TILE_FIE
             return IFOS_EINVAL;
         }
     }
-    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3], cfg_v[4], cfg_v[5], cfg_v[6]};
-    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3], cfg_s[4], cfg_s[5], cfg_s[6]};
+    if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3], cfg_v[4], cfg_v[5], cfg_v[6], cfg_v[7]};
+    if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3], cfg_s[4], cfg_s[5], cfg_s[6], cfg_s[7]};
     drop_graphs(b);
     return IFOS_OK;
 }

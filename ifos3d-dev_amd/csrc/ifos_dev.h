@@ -145,6 +145,7 @@ struct MarchCfg {
     int pipe;
     int kb;  // wavefronts of a workgroup side by side along k (divides ti; <= 1: one)
     int sb;  // shots of a workgroup, one wavefront row each (kb * sb divides ti)
+    int lds; // in-plane neighbours staged in LDS (64-lane rows, kb = sb = 1, ti >= 2)
 };
 
 // Raw buffer access: one 32-bit byte offset per thread, the neighbour

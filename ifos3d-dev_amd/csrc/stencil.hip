@@ -205,7 +205,9 @@ __device__ inline bool decode_tile(const MTiles &T, int TI, int &k, int &i, int 
     // the shot group, i row y / (kb sb)
     const int y = (int)threadIdx.y, sb = T.sb;
     const int wk = y % kb, ws = (y / kb) % sb, wi = y / (kb * sb);
-    shot = sgrp * sb + ws;
+    // one value per wavefront (blockDim.x is the wavefront): keep the shot,
+    // and with it the field descriptors, in scalar registers
+    shot = __builtin_amdgcn_readfirstlane(sgrp * sb + ws);
     if (shot >= T.ns) return false;
     k = R.k0 + (tk * kb + wk) * kw + dk;
     i = R.i0 + (ti * (TI / (kb * sb)) + wi) * rpw + di;
@@ -446,6 +448,192 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
     }
 }
 #undef SLOAD
+
+// ---------------------------------------------------------------------------
+// Stress march with the in-plane neighbours staged in LDS.  Tiles of 64
+// k-lanes (one row per wavefront, one shot per workgroup): at each plane j
+// every wavefront writes its row of vx, vy, vz (the centre of its j-queues)
+// to LDS, lanes 0-3 add the two k-neighbours on either side of the row, the
+// workgroup adds the rows just outside it (vx at i0-2, i0-1, i0+TI; vy and
+// vz at i0-1, i0+TI, i0+TI+1), and every i- and k-neighbour of the update is
+// then read from LDS instead of through L1: the only global loads left are
+// one stream per array.  Two LDS buffers alternate by plane, so one barrier
+// per plane suffices.  Every lane of the workgroup takes part in the loads
+// it is responsible for and in the barrier; only lanes inside the box update
+// and store.  Same arithmetic, in the same order, as k_s_march.
+template <int TI, bool CPML, bool VISCO>
+__global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S,
+                                                  ViscoF V)
+{
+    constexpr int RW = TI + 4;  // LDS rows: i0-2 .. i0+TI+1
+    __shared__ float sh[2][3][RW][68];
+    const int b = blockIdx.x;
+    const int L = (b & 7) * T.per + (b >> 3);
+    if (L >= T.total) return;  // whole workgroup
+    const int gs = T.group * T.nsg;
+    const int rem = L % gs;
+    const int shot = rem / T.group;
+    int t = (L / gs) * T.group + rem % T.group;
+    if (t >= T.start[T.nb] || shot >= T.ns) return;  // whole workgroup
+    int q = 0;
+    while (q + 1 < T.nb && t >= T.start[q + 1]) ++q;
+    t -= T.start[q];
+    const Box R = T.b[q];
+    const int tk = t % T.ntk[q];
+    t /= T.ntk[q];
+    const int ti = t % T.nti[q], tj = t / T.nti[q];
+    const int lane = (int)threadIdx.x, y = (int)threadIdx.y;
+    const int k0 = R.k0 + tk * 64, i0 = R.i0 + ti * TI;
+    const int k = k0 + lane, i = i0 + y;
+    const int jb = R.j0 + tj * T.jc, je = min(jb + T.jc - 1, R.j1);
+    const bool act = k <= R.k1 && i <= R.i1;                 // updates and stores
+    const bool live = k <= g.nz + 2 && i <= g.nx + 2;        // its row values exist in memory
+    const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
+    const float b1 = g.b1, b2 = g.b2, DT = g.dt;
+    const int64_t so = (int64_t)shot * S.fstride;
+    const rsrc_t rvx = make_rsrc(F.vx + so, g.fbytes), rvy = make_rsrc(F.vy + so, g.fbytes),
+                 rvz = make_rsrc(F.vz + so, g.fbytes), rsxx = make_rsrc(F.sxx + so, g.fbytes),
+                 rsyy = make_rsrc(F.syy + so, g.fbytes), rszz = make_rsrc(F.szz + so, g.fbytes),
+                 rsxy = make_rsrc(F.sxy + so, g.fbytes), rsyz = make_rsrc(F.syz + so, g.fbytes),
+                 rsxz = make_rsrc(F.sxz + so, g.fbytes), rpi = make_rsrc(F.pi, g.fbytes), ru = make_rsrc(F.u, g.fbytes),
+                 ruipjp = make_rsrc(F.uipjp, g.fbytes), rujpkp = make_rsrc(F.ujpkp, g.fbytes),
+                 ruipkp = make_rsrc(F.uipkp, g.fbytes);
+    if (CPML) shift_psi(psi, S, shot);
+    rsrc_t rr[6], rtaus, rtaup, rtsipjp, rtsjpkp, rtsipkp;
+    if (VISCO) {
+#pragma unroll
+        for (int n = 0; n < 6; ++n) rr[n] = make_rsrc(V.r[n] + so, g.fbytes);
+        rtaus = make_rsrc(V.taus, g.fbytes);
+        rtaup = make_rsrc(V.taup, g.fbytes);
+        rtsipjp = make_rsrc(V.tausipjp, g.fbytes);
+        rtsjpkp = make_rsrc(V.tausjpkp, g.fbytes);
+        rtsipkp = make_rsrc(V.tausipkp, g.fbytes);
+    }
+    // byte offset of (j-2, i0-2, k-2) at plane j: operand (j+dj, i0+r, k+dk)
+    // at vb + (dk+2)*4 + (dj+2)*SJ + (r+2)*SI, r in [-2, TI+1]
+    uint32_t vb = (uint32_t)((int64_t)jb * g.sj + (int64_t)i0 * g.si + k + IFOS_KOFF - 2) * 4u;
+    const uint32_t own = (uint32_t)(y + 2) * SI;
+#define LB(r, dj) bload(r, vb + 8u, (uint32_t)((dj) + 2) * SJ + own)
+    // lanes 0-3: k0-2, k0-1, k0+64, k0+65 of the own row (LDS columns 0, 1, 66, 67)
+    const int hk = lane < 2 ? k0 - 2 + lane : k0 + 62 + lane;
+    const bool hl = lane < 4 && hk <= g.nz + 2 && i <= g.nx + 2;
+    const int hcol = lane < 2 ? lane : lane + 64;
+    const uint32_t hoff = (uint32_t)((hk - k + 2) * 4);  // byte offset of hk from vb (>= 0)
+    int hx = 0;
+    if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
+    const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
+
+    // j queues of the own column: vx, vz at j-1..j+1, vy at j-2..j
+    float vx_m1 = 0.0f, vx_0 = 0.0f, vx_p1 = 0.0f, vz_m1 = 0.0f, vz_0 = 0.0f, vz_p1 = 0.0f;
+    float vy_m2 = 0.0f, vy_m1 = 0.0f, vy_0 = 0.0f;
+    if (live) {
+        vx_m1 = LB(rvx, -1); vx_0 = LB(rvx, 0); vx_p1 = LB(rvx, 1);
+        vz_m1 = LB(rvz, -1); vz_0 = LB(rvz, 0); vz_p1 = LB(rvz, 1);
+        vy_m2 = LB(rvy, -2); vy_m1 = LB(rvy, -1); vy_0 = LB(rvy, 0);
+    }
+
+    for (int j = jb; j <= je; ++j, vb += SJ) {
+        const int p = (j - jb) & 1;
+        // operands of the own cell (global, one stream per array)
+        float sxx0 = 0.0f, syy0 = 0.0f, szz0 = 0.0f, sxy0 = 0.0f, syz0 = 0.0f, sxz0 = 0.0f;
+        float c_pi = 0.0f, c_u = 0.0f, c_uipjp = 0.0f, c_ujpkp = 0.0f, c_uipkp = 0.0f;
+        float r0[6] = {0, 0, 0, 0, 0, 0}, ts = 0.0f, tp = 0.0f, tsipjp = 0.0f, tsjpkp = 0.0f, tsipkp = 0.0f;
+        if (act) {
+            sxx0 = LB(rsxx, 0); syy0 = LB(rsyy, 0); szz0 = LB(rszz, 0);
+            sxy0 = LB(rsxy, 0); syz0 = LB(rsyz, 0); sxz0 = LB(rsxz, 0);
+            c_uipjp = LB(ruipjp, 0); c_ujpkp = LB(rujpkp, 0); c_uipkp = LB(ruipkp, 0);
+            c_pi = LB(rpi, 0); c_u = LB(ru, 0);
+            if (VISCO) {
+#pragma unroll
+                for (int n = 0; n < 6; ++n) r0[n] = LB(rr[n], 0);
+                ts = LB(rtaus, 0); tp = LB(rtaup, 0);
+                tsipjp = LB(rtsipjp, 0); tsjpkp = LB(rtsjpkp, 0); tsipkp = LB(rtsipkp, 0);
+            }
+        }
+        // next queue entries: vx, vz at j+2, vy at j+1
+        float vx_n = 0.0f, vz_n = 0.0f, vy_n = 0.0f;
+        if (live) {
+            vx_n = LB(rvx, 2); vz_n = LB(rvz, 2); vy_n = LB(rvy, 1);
+        }
+        // stage plane j: own row, k-halo lanes, rows outside the workgroup
+        sh[p][0][y + 2][lane + 2] = vx_0;
+        sh[p][1][y + 2][lane + 2] = vy_0;
+        sh[p][2][y + 2][lane + 2] = vz_0;
+        if (lane < 4) {
+            const uint32_t ho = 2u * SJ + own;
+            sh[p][0][y + 2][hcol] = hl ? bload(rvx, vb + hoff, ho) : 0.0f;
+            sh[p][1][y + 2][hcol] = hl ? bload(rvy, vb + hoff, ho) : 0.0f;
+            sh[p][2][y + 2][hcol] = hl ? bload(rvz, vb + hoff, ho) : 0.0f;
+        }
+        for (int e = y; e < 9; e += TI) {
+            // (component, row relative to i0): vx -2 -1 TI; vy, vz -1 TI TI+1
+            const int c = e < 3 ? 0 : (e < 6 ? 1 : 2);
+            const int m = e - 3 * c;
+            const int r = c == 0 ? (m == 0 ? -2 : (m == 1 ? -1 : TI)) : (m == 0 ? -1 : (m == 1 ? TI : TI + 1));
+            const rsrc_t rc = c == 0 ? rvx : (c == 1 ? rvy : rvz);
+            const bool ok = i0 + r <= g.nx + 2 && k <= g.nz + 2;
+            sh[p][c][r + 2][lane + 2] = ok ? bload(rc, vb + 8u, 2u * SJ + (uint32_t)(r + 2) * SI) : 0.0f;
+        }
+        __syncthreads();
+        if (act) {
+            const float(*X)[68] = sh[p][0];
+            const float(*Y)[68] = sh[p][1];
+            const float(*Z)[68] = sh[p][2];
+            const int c0 = lane + 2, r0_ = y + 2;
+            float vxx = D4B(b1, b2, X[r0_ - 2][c0], X[r0_ - 1][c0], vx_0, X[r0_ + 1][c0]);
+            float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, vx_n);
+            float vxz = D4F(b1, b2, X[r0_][c0 - 1], vx_0, X[r0_][c0 + 1], X[r0_][c0 + 2]);
+            float vyx = D4F(b1, b2, Y[r0_ - 1][c0], vy_0, Y[r0_ + 1][c0], Y[r0_ + 2][c0]);
+            float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, vy_n);
+            float vyz = D4F(b1, b2, Y[r0_][c0 - 1], vy_0, Y[r0_][c0 + 1], Y[r0_][c0 + 2]);
+            float vzx = D4F(b1, b2, Z[r0_ - 1][c0], vz_0, Z[r0_ + 1][c0], Z[r0_ + 2][c0]);
+            float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, vz_n);
+            float vzz = D4B(b1, b2, Z[r0_][c0 - 2], Z[r0_][c0 - 1], vz_0, Z[r0_][c0 + 1]);
+            div_spacing(g, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
+            if (CPML) {
+                const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+                s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
+            }
+            const float vxyyx = vxy + vyx, vyzzy = vyz + vzy, vxzzx = vxz + vzx;
+            const float vxxyyzz = vxx + vyy + vzz;
+            const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
+            float sn[6], rn[6];
+            if (!VISCO) {
+                // update_s_ssg_elastic.c:117-138
+                const float fipjp = c_uipjp * DT, fjpkp = c_ujpkp * DT, fipkp = c_uipkp * DT;
+                const float f = 2.0f * c_u;
+                sn[3] = sxy0 + (fipjp * vxyyx);
+                sn[4] = syz0 + (fjpkp * vyzzy);
+                sn[5] = sxz0 + (fipkp * vxzzx);
+                sn[0] = sxx0 + DT * ((c_pi * vxxyyzz) - (f * vyyzz));
+                sn[1] = syy0 + DT * ((c_pi * vxxyyzz) - (f * vxxzz));
+                sn[2] = szz0 + DT * ((c_pi * vxxyyzz) - (f * vxxyy));
+            } else {
+                sn[0] = sxx0; sn[1] = syy0; sn[2] = szz0;
+                sn[3] = sxy0; sn[4] = syz0; sn[5] = sxz0;
+#pragma unroll
+                for (int n = 0; n < 6; ++n) rn[n] = r0[n];
+                stress_visco(DT, V.eta, V.L, c_pi, c_u, c_uipjp, c_ujpkp, c_uipkp, ts, tp, tsipjp, tsjpkp, tsipkp,
+                             vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn, rn);
+            }
+            const uint32_t st = 2u * SJ + own;
+            bstore(sn[3], rsxy, vb + 8u, st);
+            bstore(sn[4], rsyz, vb + 8u, st);
+            bstore(sn[5], rsxz, vb + 8u, st);
+            bstore(sn[0], rsxx, vb + 8u, st);
+            bstore(sn[1], rsyy, vb + 8u, st);
+            bstore(sn[2], rszz, vb + 8u, st);
+            if (VISCO) {
+#pragma unroll
+                for (int n = 0; n < 6; ++n) bstore(rn[n], rr[n], vb + 8u, st);
+            }
+        }
+        vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_n;
+        vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_n;
+        vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_n;
+    }
+#undef LB
+}
 // ---------------------------------------------------------------------------
 // gradient_F.c:79-139 for one frequency of one shot as a march over j: the
 // y-derivatives of the 12 DFT fields come from 4-deep register queues and the
@@ -733,7 +921,7 @@ static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int g
 
 // default tile shape of the march kernels (i-rows per block, j chunk,
 // shot grouping); the batched path's autotuner overrides it per kernel
-static MarchCfg g_default_cfg = {4, 0, 0, 0, 0, 1, 1};
+static MarchCfg g_default_cfg = {4, 0, 0, 0, 0, 1, 1, 0};
 
 template <bool CPML, bool SNAP, bool PIPE>
 static void launch_v_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
@@ -774,6 +962,27 @@ static void launch_s_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, co
     }
 }
 
+// the LDS-staged marches need 64-lane rows (every box at least 64 cells deep
+// in k), one k column and one shot per workgroup, and TI >= 2
+static bool lds_ok(const MarchCfg &c, const MTiles &T)
+{
+    if (!c.lds || c.ti < 2 || T.sb != 1) return false;
+    for (int q = 0; q < T.nb; ++q)
+        if (T.kw[q] != 64 || T.kb[q] != 1) return false;
+    return true;
+}
+
+template <bool CPML, bool VISCO>
+static void launch_s_lds_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
+                            const Psi9 &psi, const Shots &S, const ViscoF &V, hipStream_t s)
+{
+    switch (ti) {
+    case 2: hipLaunchKernelGGL((k_s_lds<2, CPML, VISCO>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
+    case 4: hipLaunchKernelGGL((k_s_lds<4, CPML, VISCO>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
+    default: hipLaunchKernelGGL((k_s_lds<8, CPML, VISCO>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
+    }
+}
+
 template <bool CPML, bool VISCO = false>
 static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
                     const Psi9 &psi, const Shots &S, hipStream_t s, const ViscoF &V = ViscoF{})
@@ -784,6 +993,10 @@ static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const 
         return IFOS_EINVAL;
     }
     const dim3 grid((unsigned)(T.per * 8));
+    if (lds_ok(c, T)) {
+        launch_s_lds_ti<CPML, VISCO>(c.ti, grid, T, G, F, P, psi, S, V, s);
+        return hip_check(hipGetLastError(), "k_s_lds");
+    }
     if (c.pipe) launch_s_ti<CPML, VISCO, true>(c.ti, grid, T, G, F, P, psi, S, V, s);
     else launch_s_ti<CPML, VISCO, false>(c.ti, grid, T, G, F, P, psi, S, V, s);
     return hip_check(hipGetLastError(), "k_s_march");

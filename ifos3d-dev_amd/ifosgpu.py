@@ -550,7 +550,7 @@ class BatchContext:
     def last_fused(self):
         return bool(lib().ifos_batch_last_fused(self.h))
 
-    TILE_DEFAULTS = (4, 0, 0, 0, 0, 1, 1)
+    TILE_DEFAULTS = (4, 0, 0, 0, 0, 1, 1, 0)
 
     @classmethod
     def _tile(cls, cfg):
@@ -558,7 +558,7 @@ class BatchContext:
         return (C.c_int * len(cls.TILE_DEFAULTS))(*(cfg + cls.TILE_DEFAULTS[len(cfg):]))
 
     def set_tiles(self, cfg_v=None, cfg_s=None):
-        """tile shapes {TI, jc, group, split, pipe, kb, sb} of the velocity / stress
+        """tile shapes {TI, jc, group, split, pipe, kb, sb, lds} of the velocity / stress
         marches (missing trailing fields take TILE_DEFAULTS)"""
         cv = self._tile(cfg_v) if cfg_v is not None else None
         cs = self._tile(cfg_s) if cfg_s is not None else None

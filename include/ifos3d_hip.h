@@ -290,16 +290,18 @@ int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns,
                    float *d_syn, float *const d_grad[3], double *l2,
                    int use_graph);
 /* tile shapes of the velocity and stress march launches, each
- * {TI, jc, group, split, pipe, kb, sb}: TI wavefronts per workgroup (1, 2, 4, 8);
+ * {TI, jc, group, split, pipe, kb, sb, lds}: TI wavefronts per workgroup (1, 2, 4, 8);
  * jc j-planes per tile (<= 0: automatic); group: tiles of one shot before
  * the next shot (1: shots fastest, <= 0: one shot after the other); split:
  * 1 = interior box and CPML slabs in separate launches, 0 = one CPML-aware
  * launch; pipe: 1 = the next plane's operands are loaded one plane ahead;
  * kb: the workgroup's wavefronts stand kb side by side along k (64 k each);
  * sb: sb shots of one tile share a workgroup (and its model loads); the
- * remaining TI / (kb sb) wavefronts go along i (kb sb divides TI).  NULL
- * leaves a shape unchanged. */
-#define IFOS_TILE_FIELDS 7
+ * remaining TI / (kb sb) wavefronts go along i (kb sb divides TI); lds: 1 =
+ * the in-plane neighbours are staged in LDS (used where every row is 64 k
+ * wide and kb = sb = 1, TI >= 2; elsewhere ignored).  NULL leaves a shape
+ * unchanged. */
+#define IFOS_TILE_FIELDS 8
 int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[IFOS_TILE_FIELDS], const int cfg_s[IFOS_TILE_FIELDS]);
 /* the autotuner's candidate shapes (IFOS_TILE_FIELDS ints each, at most
  * max); returns their number */

@@ -261,6 +261,38 @@ def test_batch_viscoelastic(tiles):
     out["ctx"].close()
 
 
+@pytest.mark.parametrize("nz,free_surf,visco,tiles", [
+    (70, 0, False, ((4, 32, 0, 0, 0, 1, 1, 1), (4, 32, 0, 0, 0, 1, 1, 1))),
+    (70, 1, False, ((2, 5, 0, 0, 0, 1, 1, 1), (8, 0, 1, 0, 0, 1, 1, 1))),
+    (70, 0, True, ((4, 0, 0, 0, 1, 1, 1, 1), (2, 7, 0, 0, 0, 1, 1, 1))),
+    (140, 0, False, ((4, 0, 0, 1, 0, 1, 1, 1), (4, 0, 0, 1, 0, 1, 1, 1))),
+    (140, 1, True, ((8, 9, 3, 0, 0, 1, 1, 1), (8, 9, 3, 0, 0, 1, 1, 1)))])
+def test_batch_lds_march(nz, free_surf, visco, tiles):
+    """the LDS-staged march (64-lane rows: nz >= 64; partial k and i tiles,
+    CPML slabs in the same launch or split off, free surface, viscoelastic)
+    gives the same bits as the plain path"""
+    import ifosgpu as ig
+    g = small_grid(nx=30, ny=28, nz=nz, fw=6, nt=80, free_surf=free_surf)
+    srcs = [(15, 10, 35, 0.0, 50.0, 1.0, 1), (4, 12, nz - 4, 0.0, 50.0, 1.0, 1), (20, 14, 64, 0.0, 45.0, 1.0, 1)]
+    rec = np.array([[3 + 3 * n, 9, 60 + n] for n in range(8)] + [[29, 27, nz], [12, 2, 1]], np.int32)
+    finv, tast = [40.0, 60.0], 100
+    vis = None
+    if visco:
+        rng = np.random.default_rng(5)
+        vis = ((0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
+               (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32), np.float32(0.019))
+    _note(f"lds march nz={nz} free_surf={free_surf} visco={visco} tiles {tiles}")
+    true, start = _models(g)
+    G = _setup(ig, g, finv, tast, ig.DFT_COMPAT)
+    ref = _plain(ig, G, g, true, start, srcs, rec, finv, tast, ig.DFT_COMPAT, visco=vis)
+    out = _batched(ig, G, g, true, start, srcs, rec, finv, tast, ig.DFT_COMPAT, 4, False, tiles=tiles, visco=vis)
+    for n in range(len(srcs)):
+        assert max_ulp(out["syn"][n], ref["syn"][n]) == 0, f"synthetics, shot {n}"
+    for n in range(3):
+        assert max_ulp(out["grads"][n], ref["grads"][n]) == 0, f"gradient {n}"
+    out["ctx"].close()
+
+
 @pytest.mark.parametrize("fz,prec_name,chunk,use_graph", [((16, 0), "compat", 4, False), ((8, 5), "compat", 3, True),
                                                           ((16, 7), "kahan", 5, False), ((8, 0), "compat", 8, False)])
 def test_batch_fused_step(fz, prec_name, chunk, use_graph):
