@@ -4,6 +4,7 @@
 // reference's expression order; every dot product is reduced in FP64 (the
 // reference sums in FP32 and all-reduces MPI_FLOAT) over a fixed grid of
 // partial sums, so a call's result does not depend on scheduling.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 #include "ifos_dev.h"
@@ -269,11 +270,11 @@ extern "C" int ifos_lbfgs_create(const ifos_geom *g, int numpar, int bfgsnum, if
 extern "C" void ifos_lbfgs_destroy(ifos_lbfgs *b)
 {
     if (!b) return;
-    hipFree(b->grad);
-    hipFree(b->mod);
-    hipFree(b->q);
-    hipFree(b->part);
-    hipFree(b->out);
+    (void)hipFree(b->grad);
+    (void)hipFree(b->mod);
+    (void)hipFree(b->q);
+    (void)hipFree(b->part);
+    (void)hipFree(b->out);
     delete b;
 }
 
@@ -393,8 +394,8 @@ extern "C" int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float 
         rc = hip_check(hipGetLastError(), "k_cg_dots");
     }
     if (!rc) rc = hip_check(hipMemcpy(s, out, sizeof(s), hipMemcpyDeviceToHost), "conjugate sums copy");
-    hipFree(part);
-    hipFree(out);
+    (void)hipFree(part);
+    (void)hipFree(out);
     if (rc) return rc;
     if (sums)
         for (int m = 0; m < 6; ++m) sums[m] = s[m];
@@ -415,4 +416,285 @@ extern "C" int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float 
     rc = hip_check(hipGetLastError(), "k_cg_update");
     if (!rc) rc = hip_check(hipDeviceSynchronize(), "ifos_conjugate");
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// precongrad.c:28-336, modelupdate.c:26-160 and steplength.c:26-91.
+
+namespace ifos {
+
+struct PreconArgs {
+    float *g[3];
+    const int *src;  // [nsrc][3] grid x, y, z of the sources (iround(srcpos / D))
+    const int *rec;  // [ntr][3] grid x, y, z of the receivers
+    int nsrc, ntr, damptype, free_surf, sh, rh;
+    int nxg, nyg, nzg, posx, posy, posz;
+};
+
+// 1 / (1 + c * exp(e)) in double, stored to float as the reference does
+__device__ inline float taper(double c, double e)
+{
+    return (float)(1.0 / (1.0 + c * exp(e)));
+}
+
+__global__ __launch_bounds__(256) void k_precon(DevGeom g, PreconArgs A)
+{
+    for (int64_t r = blockIdx.x; r < (int64_t)g.nx * g.ny; r += gridDim.x) {
+        const int j = (int)(r / g.nx) + 1, i = (int)(r % g.nx) + 1;
+        for (int k = 1 + (int)threadIdx.x; k <= g.nz; k += blockDim.x) {
+            const int64_t c = at(g, j, i, k);
+            float a = A.g[0][c], b = A.g[1][c], d = A.g[2][c];
+            // global indices: set only by the DAMPTYPE 1 and 2 loops
+            // (precongrad.c:45-47, :131-133); 0 elsewhere, as written there
+            int ii = 0, jj = 0, kk = 0;
+            if (A.damptype == 1 || A.damptype == 2) {
+                ii = i + A.posx * g.nx;
+                jj = j + A.posy * g.ny;
+                kk = k + A.posz * g.nz;
+            }
+            if (A.damptype == 2) {  // precongrad.c:135-150
+                for (int l = 0; l < A.nsrc; ++l) {
+                    const int sx = A.src[3 * l], sy = A.src[3 * l + 1], sz = A.src[3 * l + 2];
+                    const float rr = (float)sqrt((double)((jj - sy) * (jj - sy) + (ii - sx) * (ii - sx) +
+                                                          (kk - sz) * (kk - sz)));
+                    const float G = taper(1000.0, -0.7 * (double)rr);
+                    a = a * G; b = b * G; d = d * G;
+                }
+            }
+            if (A.damptype == 1 || A.damptype == 2) {  // precongrad.c:49-64, :152-166
+                for (int n = 0; n < A.ntr; ++n) {
+                    const int rx = A.rec[3 * n], ry = A.rec[3 * n + 1], rz = A.rec[3 * n + 2];
+                    const float rr = (float)sqrt((double)((jj - ry) * (jj - ry) + (ii - rx) * (ii - rx) +
+                                                          (kk - rz) * (kk - rz)));
+                    const float G = taper(1000.0, -0.7 * (double)rr * (double)rr);
+                    a = a * G; b = b * G; d = d * G;
+                }
+            }
+            float G0, G1;
+            if (A.damptype != 3) {  // precongrad.c:65-113, :167-214, :283-330
+                G0 = A.free_surf ? 0.0f : taper(2000.0, -0.6 * (double)(float)(jj * 1.0));
+                G1 = taper(2000.0, -0.6 * (double)(float)((A.nyg - jj) * 1.0));
+            } else {  // precongrad.c:222-236
+                float rr = (float)((jj - A.sh) * 1.0);
+                G0 = rr < 0 ? 0.0f : taper(1000.0, -0.7 * (double)rr * (double)rr);
+                rr = (float)((A.nyg - jj - A.rh) * 1.0);
+                G1 = rr < 0 ? 0.0f : taper(1000.0, -0.7 * (double)rr);
+            }
+            a = a * G0 * G1; b = b * G0 * G1; d = d * G0 * G1;
+            // x and z (the DAMPTYPE 3 tests r < 0 never hold: ii, kk >= 0)
+            G0 = taper(2000.0, -0.6 * (double)(float)(ii * 1.0));
+            G1 = taper(2000.0, -0.6 * (double)(float)((A.nxg - ii) * 1.0));
+            a = a * G0 * G1; b = b * G0 * G1; d = d * G0 * G1;
+            G0 = taper(2000.0, -0.6 * (double)(float)(kk * 1.0));
+            G1 = taper(2000.0, -0.6 * (double)(float)((A.nzg - kk) * 1.0));
+            a = a * G0 * G1; b = b * G0 * G1; d = d * G0 * G1;
+            A.g[0][c] = a;
+            A.g[1][c] = b;
+            A.g[2][c] = d;
+        }
+    }
+}
+
+// block maxima of vp, vs, rho (modelupdate.c:59-74) and of |grad| (:86-95)
+__global__ __launch_bounds__(kRedThreads) void k_mu_max(DevGeom g, const float *rho, const float *pi, const float *u,
+                                                       const float *g1, const float *g2, const float *g3,
+                                                       float *part)
+{
+    __shared__ float sh[6][kRedThreads];
+    float m[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t r = blockIdx.x; r < (int64_t)g.nx * g.ny; r += gridDim.x) {
+        const int j = (int)(r / g.nx) + 1, i = (int)(r % g.nx) + 1;
+        for (int k = 1 + (int)threadIdx.x; k <= g.nz; k += blockDim.x) {
+            const int64_t c = at(g, j, i, k);
+            const float vp = (float)sqrt((double)(pi[c] / rho[c]));
+            const float vs = (float)sqrt((double)(u[c] / rho[c]));
+            if (vp > m[0]) m[0] = vp;
+            if (vs > m[1]) m[1] = vs;
+            if (rho[c] > m[2]) m[2] = rho[c];
+            if (fabsf(g1[c]) > m[3]) m[3] = fabsf(g1[c]);
+            if (fabsf(g2[c]) > m[4]) m[4] = fabsf(g2[c]);
+            if (fabsf(g3[c]) > m[5]) m[5] = fabsf(g3[c]);
+        }
+    }
+    for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = m[q];
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = fmaxf(sh[q][threadIdx.x], sh[q][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+struct MuArgs {
+    float *g[3], *rho, *pi, *u, *mod;  // mod: bfgsmod slot (compact) or null
+    float step, s1, s2, s3, vp0, vs0, rho0, n1, n2, n3;  // n*: gradient normalisers (1 when none)
+    int numpar;
+    int64_t ncell;
+};
+
+// modelupdate.c:97-157 (the gradient normalisation folded in as a division)
+__global__ __launch_bounds__(256) void k_modelupdate(DevGeom g, MuArgs A, int normalise)
+{
+    for (int64_t r = blockIdx.x; r < (int64_t)g.nx * g.ny; r += gridDim.x) {
+        const int j = (int)(r / g.nx) + 1, i = (int)(r % g.nx) + 1;
+        for (int k = 1 + (int)threadIdx.x; k <= g.nz; k += blockDim.x) {
+            const int64_t c = at(g, j, i, k);
+            float gvp = A.g[0][c], gvs = A.g[1][c], grho = A.g[2][c];
+            if (normalise) {
+                gvp = gvp / A.n1;
+                gvs = gvs / A.n2;
+                grho = grho / A.n3;
+                A.g[0][c] = gvp;
+                A.g[1][c] = gvs;
+                A.g[2][c] = grho;
+            }
+            const float rho = A.rho[c], pi = A.pi[c], u = A.u[c];
+            const double vp = sqrt((double)(pi / rho)), vs = sqrt((double)(u / rho));
+            const float vpnew = (float)(vp + (double)(((A.step * A.s1) * gvp) * A.vp0));
+            const float vsnew = (float)(vs + (double)(((A.step * A.s2) * gvs) * A.vs0));
+            const float rhonew = rho + (((A.s3 * A.step) * grho) * A.rho0);
+            if (A.mod) {
+                const int64_t l = ((int64_t)(j - 1) * g.nx + (i - 1)) * g.nz + (k - 1);
+                A.mod[l] = (float)(((double)vpnew - vp) / (double)A.vp0);
+                if (A.numpar > 1) A.mod[A.ncell + l] = (float)(((double)vsnew - vs) / (double)A.vs0);
+            }
+            A.u[c] = (rhonew * vsnew) * vsnew;
+            A.pi[c] = (rhonew * vpnew) * vpnew;
+            A.rho[c] = rhonew;
+        }
+    }
+}
+
+}  // namespace ifos
+
+extern "C" int ifos_precon_grad(const ifos_geom *g, float *const d_grad[3], int damptype, const int *src_xyz, int nsrc,
+                                const int *rec_xyz, int ntr, const int nglob[3], const int pos[3], void *stream)
+{
+    if (!g || !d_grad || damptype < 0 || damptype > 3 || nsrc < 0 || ntr < 0 || (nsrc && !src_xyz) ||
+        (ntr && !rec_xyz)) {
+        set_error("ifos_precon_grad: bad arguments (DAMPTYPE must be 0..3)");
+        return IFOS_EINVAL;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    const DevGeom G = make_dev_geom(g);
+    PreconArgs A;
+    for (int n = 0; n < 3; ++n) A.g[n] = d_grad[n];
+    A.nsrc = damptype == 2 ? nsrc : 0;
+    A.ntr = (damptype == 1 || damptype == 2) ? ntr : 0;
+    A.damptype = damptype;
+    A.free_surf = g->free_surf;
+    A.sh = A.rh = g->fw + 40;  // precongrad.c:38-39
+    A.nxg = nglob ? nglob[0] : g->nx;
+    A.nyg = nglob ? nglob[1] : g->ny;
+    A.nzg = nglob ? nglob[2] : g->nz;
+    A.posx = pos ? pos[0] : 0;
+    A.posy = pos ? pos[1] : 0;
+    A.posz = pos ? pos[2] : 0;
+    int *d = nullptr;
+    const size_t nb = (size_t)(A.nsrc + A.ntr) * 3 * sizeof(int);
+    int rc = IFOS_OK;
+    if (nb) {
+        rc = hip_check(hipMalloc(&d, nb), "precon positions");
+        if (!rc && A.nsrc)
+            rc = hip_check(hipMemcpy(d, src_xyz, (size_t)A.nsrc * 3 * sizeof(int), hipMemcpyHostToDevice), "src");
+        if (!rc && A.ntr)
+            rc = hip_check(hipMemcpy(d + 3 * A.nsrc, rec_xyz, (size_t)A.ntr * 3 * sizeof(int), hipMemcpyHostToDevice),
+                           "rec");
+    }
+    A.src = d;
+    A.rec = d ? d + 3 * A.nsrc : nullptr;
+    if (!rc) {
+        const int64_t rows = (int64_t)g->nx * g->ny;
+        hipLaunchKernelGGL(k_precon, dim3((unsigned)(rows < 8192 ? rows : 8192)), dim3(256), 0, st, G, A);
+        rc = hip_check(hipGetLastError(), "k_precon");
+    }
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "ifos_precon_grad");
+    (void)hipFree(d);
+    return rc;
+}
+
+extern "C" int ifos_modelupdate(const ifos_geom *g, float *const d_grad[3], float *d_rho, float *d_pi, float *d_u,
+                                float *d_bfgsmod, int numpar, int lbfgs, float step, const float weight[3], float vp0,
+                                float vs0, float rho0, float maxima[3], void *stream)
+{
+    if (!g || !d_grad || !d_rho || !d_pi || !d_u || !weight || (lbfgs && !d_bfgsmod)) {
+        set_error("ifos_modelupdate: bad arguments");
+        return IFOS_EINVAL;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    const DevGeom G = make_dev_geom(g);
+    float *part = nullptr;
+    float m[6] = {0, 0, 0, 0, 0, 0};
+    int rc = hip_check(hipMalloc(&part, kRedBlocks * 6 * sizeof(float)), "modelupdate partials");
+    if (!rc) {
+        hipLaunchKernelGGL(k_mu_max, dim3(kRedBlocks), dim3(kRedThreads), 0, st, G, d_rho, d_pi, d_u, d_grad[0],
+                           d_grad[1], d_grad[2], part);
+        rc = hip_check(hipGetLastError(), "k_mu_max");
+    }
+    std::vector<float> h(kRedBlocks * 6);
+    if (!rc) rc = hip_check(hipMemcpyAsync(h.data(), part, h.size() * sizeof(float), hipMemcpyDeviceToHost, st), "max");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "modelupdate max");
+    (void)hipFree(part);
+    if (rc) return rc;
+    for (int b = 0; b < kRedBlocks; ++b)
+        for (int q = 0; q < 6; ++q) m[q] = std::max(m[q], h[6 * b + q]);
+    if (maxima)
+        for (int q = 0; q < 3; ++q) maxima[q] = m[q];
+    MuArgs A;
+    for (int n = 0; n < 3; ++n) A.g[n] = d_grad[n];
+    A.rho = d_rho;
+    A.pi = d_pi;
+    A.u = d_u;
+    A.mod = lbfgs ? d_bfgsmod : nullptr;
+    A.step = step;
+    A.s1 = weight[0];
+    A.s2 = weight[1];
+    A.s3 = weight[2];
+    A.vp0 = vp0;
+    A.vs0 = vs0;
+    A.rho0 = rho0;
+    A.numpar = numpar;
+    A.ncell = (int64_t)g->nx * g->ny * g->nz;
+    // modelupdate.c:84-115: without L-BFGS the gradients are divided by their
+    // maxima, when the vp and vs maxima are both nonzero
+    int normalise = 0;
+    A.n1 = m[3];
+    A.n2 = m[4];
+    A.n3 = m[5] == 0.0f ? 1.0f : m[5];
+    if (!lbfgs && A.n1 > 0.0f && A.n2 > 0.0f) normalise = 1;
+    const int64_t rows = (int64_t)g->nx * g->ny;
+    hipLaunchKernelGGL(k_modelupdate, dim3((unsigned)(rows < 8192 ? rows : 8192)), dim3(256), 0, st, G, A, normalise);
+    rc = hip_check(hipGetLastError(), "k_modelupdate");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "ifos_modelupdate");
+    return rc;
+}
+
+// steplength.c:26-91 on the host, in the reference's float arithmetic
+extern "C" int ifos_steplength(float L2[4], float step[5], int lbfgs, int it_group, float teststep)
+{
+    if (!L2 || !step) return IFOS_EINVAL;
+    float a = ((step[0] - step[1]) * (L2[0] - L2[2]) - (step[0] - step[2]) * (L2[0] - L2[1])) /
+              (step[0] * step[2] * (step[0] - step[2]) + step[0] * step[1] * (step[1] - step[0]) +
+               step[1] * step[2] * (step[2] - step[1]));
+    float b = (L2[0] - L2[1]) / (step[0] - step[1]);
+    b = b - a * (step[0] + step[1]);
+    const float c = L2[0] - a * step[0] * step[0] - b * step[0];
+    step[3] = 0.0f;
+    if (a > 0) step[3] = -b / (2 * a);
+    if (a < 0) {
+        if (L2[0] < L2[1] && L2[0] < L2[2]) step[3] = 0.1 * step[1];
+        if (L2[1] < L2[0] && L2[1] < L2[2]) step[3] = step[1];
+        if (L2[2] < L2[0] && L2[2] < L2[1]) step[3] = step[2];
+    }
+    if (step[3] > 2.5 * step[4]) step[3] = 2.5 * step[4];
+    if (lbfgs && it_group > 1 && step[3] > 1.0) step[3] = 1.0;
+    if (step[3] <= 0) step[3] = 0.1 * step[1];
+    if (step[3] < 0.1 * step[1]) step[3] = 0.1 * step[1];
+    L2[3] = a * step[3] * step[3] + b * step[3] + c;
+    step[4] = step[3] / 2;
+    if (step[4] < 0.2 * teststep) step[4] = 0.2 * teststep;
+    if (step[4] > teststep) step[4] = teststep;
+    if (lbfgs && it_group > 1) step[4] = 0.5;
+    return IFOS_OK;
 }

@@ -480,6 +480,36 @@ def conjugate(g, grads, priors, cdf):
     return np.array(beta[:], np.float32), (None if cdf == 1 else np.array(sums[:], np.float64))
 
 
+def precon_grad(g, grads, damptype, src_xyz=None, rec_xyz=None, nglob=None, pos=None, stream=None):
+    """precongrad.c on device gradients (three device-layout fields)"""
+    src = np.ascontiguousarray(src_xyz if src_xyz is not None else np.zeros((0, 3)), np.int32)
+    rec = np.ascontiguousarray(rec_xyz if rec_xyz is not None else np.zeros((0, 3)), np.int32)
+    Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads])
+    ng = (C.c_int * 3)(*nglob) if nglob is not None else None
+    ps = (C.c_int * 3)(*pos) if pos is not None else None
+    check(lib().ifos_precon_grad(g.ref, Gp, damptype, src.ctypes.data_as(C.c_void_p), len(src),
+                                 rec.ctypes.data_as(C.c_void_p), len(rec), ng, ps, stream), "ifos_precon_grad")
+
+
+def modelupdate(g, grads, rho, pi, u, step, weight, vp0, vs0, rho0, bfgsmod=None, numpar=2, stream=None):
+    """modelupdate.c on device fields; returns the model maxima (vp, vs, rho)"""
+    Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads])
+    w = (C.c_float * 3)(*weight)
+    mx = (C.c_float * 3)()
+    check(lib().ifos_modelupdate(g.ref, Gp, _ptr(rho), _ptr(pi), _ptr(u), _ptr(bfgsmod), numpar,
+                                 int(bfgsmod is not None), C.c_float(step), w, C.c_float(vp0), C.c_float(vs0),
+                                 C.c_float(rho0), mx, stream), "ifos_modelupdate")
+    return np.array(mx[:], np.float32)
+
+
+def steplength(L2, step, lbfgs, it_group, teststep):
+    """steplength.c: returns (L2[4], step[5]) after the call"""
+    l2 = (C.c_float * 4)(*[float(x) for x in L2[:4]] + [0.0] * (4 - len(L2[:4])))
+    st = (C.c_float * 5)(*[float(x) for x in step[:5]] + [0.0] * (5 - len(step[:5])))
+    check(lib().ifos_steplength(l2, st, int(lbfgs), int(it_group), C.c_float(teststep)), "ifos_steplength")
+    return np.array(l2[:], np.float32), np.array(st[:], np.float32)
+
+
 class BatchContext:
     """The batched path: several shots of one model per launch (ifos_batch_*)."""
 

@@ -356,6 +356,27 @@ int ifos_lbfgs_update(ifos_lbfgs *b, float *d_grad1, float *d_grad2, int iterati
  * d_grad and d_prior[3..5]; beta[3] receives the factors */
 int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float *const d_prior[6], float beta[3], int cdf,
                    double *sums);
+/* precongrad.c:28-336: tapers the three gradients (device layout, [1..N]^3)
+ * for DAMPTYPE 0..3: around the sources (src_xyz[nsrc][3], grid x y z, i.e.
+ * iround(srcpos / DX ...)), around the receivers (rec_xyz[ntr][3], recpos
+ * order x y z) and towards the model edges; nglob = NXG NYG NZG (NULL: this
+ * grid), pos = POS[1..3] (NULL: 0).  Keeps the reference's quirks: the
+ * global indices are 0 outside DAMPTYPE 1 and 2, and FREE_SURF zeroes the
+ * y factor. */
+int ifos_precon_grad(const ifos_geom *g, float *const d_grad[3], int damptype, const int *src_xyz, int nsrc,
+                     const int *rec_xyz, int ntr, const int nglob[3], const int pos[3], void *stream);
+/* modelupdate.c:26-160: vp, vs, rho += step * WEIGHT * grad * (VP0, VS0,
+ * RHO0), rho, pi, u rewritten; without L-BFGS the gradients are first divided
+ * by their maxima; with it, slot d_bfgsmod (compact, ifos_lbfgs_mod) receives
+ * the normalised vp (and, numpar 2, vs) steps.  maxima[3] (may be NULL):
+ * vp, vs, rho maxima of the model before the update. */
+int ifos_modelupdate(const ifos_geom *g, float *const d_grad[3], float *d_rho, float *d_pi, float *d_u,
+                     float *d_bfgsmod, int numpar, int lbfgs, float step, const float weight[3], float vp0,
+                     float vs0, float rho0, float maxima[3], void *stream);
+/* steplength.c:26-91 (host): the parabola through (step[0..2], L2[0..2]);
+ * step[3] and L2[3] receive its clamped minimum, step[4] the next test
+ * step (TESTSTEP = teststep) */
+int ifos_steplength(float L2[4], float step[5], int lbfgs, int it_group, float teststep);
 
 #ifdef __cplusplus
 }

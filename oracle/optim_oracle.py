@@ -133,3 +133,128 @@ def conjugate(grads, priors, cdf, dot="f64"):
         grads[n][...] = v
         priors[n + 3][...] = v
     return beta, sums
+
+
+# ---------------------------------------------------------------------------
+# precongrad.c:28-336, modelupdate.c:26-160, steplength.c:26-91
+
+def _taper(c, e):
+    """(float)(1 / (1 + c * exp(e))) with e in double, as the reference forms it"""
+    return np.float32(1.0 / (1.0 + c * np.exp(e)))
+
+
+def precon_grad(grads, damptype, src_xyz, rec_xyz, nglob, fw, free_surf, pos=(0, 0, 0)):
+    """grads: three interior arrays [NY][NX][NZ] (float32), updated in place.
+    src_xyz / rec_xyz: integer grid positions (x, y, z) per source / receiver."""
+    ny, nx, nz = grads[0].shape
+    nxg, nyg, nzg = nglob
+    sh = rh = fw + 40
+    J, I, K = np.meshgrid(np.arange(1, ny + 1), np.arange(1, nx + 1), np.arange(1, nz + 1), indexing="ij")
+    if damptype in (1, 2):
+        ii, jj, kk = I + pos[0] * nx, J + pos[1] * ny, K + pos[2] * nz
+    else:  # the global indices stay 0 outside DAMPTYPE 1 and 2 (precongrad.c:35)
+        ii = jj = kk = np.zeros_like(I)
+    g = [a.astype(np.float32) for a in grads]
+
+    def mul(G):
+        for n in range(3):
+            g[n] = (g[n] * G).astype(np.float32)
+
+    if damptype == 2:
+        for sx, sy, sz in src_xyz:
+            r = np.sqrt(((jj - sy) ** 2 + (ii - sx) ** 2 + (kk - sz) ** 2).astype(np.float64)).astype(np.float32)
+            mul(_taper(1000.0, -0.7 * r.astype(np.float64)))
+    if damptype in (1, 2):
+        for rx, ry, rz in rec_xyz:
+            r = np.sqrt(((jj - ry) ** 2 + (ii - rx) ** 2 + (kk - rz) ** 2).astype(np.float64)).astype(np.float32)
+            rd = r.astype(np.float64)
+            mul(_taper(1000.0, -0.7 * rd * rd))
+    f = lambda v: np.asarray(v, np.float64).astype(np.float32).astype(np.float64)
+    if damptype != 3:
+        G0 = np.zeros_like(g[0]) if free_surf else _taper(2000.0, -0.6 * f(jj * 1.0))
+        G1 = _taper(2000.0, -0.6 * f((nyg - jj) * 1.0))
+    else:
+        r = f((jj - sh) * 1.0)
+        G0 = np.where(r < 0, np.float32(0), _taper(1000.0, -0.7 * r * r)).astype(np.float32)
+        r = f((nyg - jj - rh) * 1.0)
+        G1 = np.where(r < 0, np.float32(0), _taper(1000.0, -0.7 * r)).astype(np.float32)
+    for n in range(3):
+        g[n] = ((g[n] * G0).astype(np.float32) * G1).astype(np.float32)
+    for lo, hi in ((ii, nxg - ii), (kk, nzg - kk)):
+        G0 = _taper(2000.0, -0.6 * f(lo * 1.0))
+        G1 = _taper(2000.0, -0.6 * f(hi * 1.0))
+        for n in range(3):
+            g[n] = ((g[n] * G0).astype(np.float32) * G1).astype(np.float32)
+    for n in range(3):
+        grads[n][...] = g[n]
+
+
+def modelupdate(grads, rho, pi, u, step, weight, vp0, vs0, rho0, lbfgs, numpar=2):
+    """interior arrays updated in place; returns (maxima[3], bfgsmod or None)"""
+    f32 = np.float32
+    vp = np.sqrt((pi / rho).astype(f32).astype(np.float64))
+    vs = np.sqrt((u / rho).astype(f32).astype(np.float64))
+    maxima = np.array([max(0.0, vp.astype(f32).max()), max(0.0, vs.astype(f32).max()), max(0.0, rho.max())], f32)
+    g = [a.astype(f32) for a in grads]
+    if not lbfgs:
+        m = [np.abs(a).max() for a in g]
+        if m[2] == 0.0:
+            m[2] = f32(1.0)
+        if m[0] > 0.0 and m[1] > 0.0:
+            g = [(g[n] / f32(m[n])).astype(f32) for n in range(3)]
+            for n in range(3):
+                grads[n][...] = g[n]
+    s1, s2, s3 = (f32(w) for w in weight)
+    step, vp0, vs0, rho0 = f32(step), f32(vp0), f32(vs0), f32(rho0)
+    vpnew = (vp + (((step * s1) * g[0]) * vp0).astype(f32).astype(np.float64)).astype(f32)
+    vsnew = (vs + (((step * s2) * g[1]) * vs0).astype(f32).astype(np.float64)).astype(f32)
+    rhonew = (rho + (((s3 * step) * g[2]) * rho0).astype(f32)).astype(f32)
+    mod = None
+    if lbfgs:
+        parts = [((vpnew.astype(np.float64) - vp) / np.float64(vp0)).astype(f32).ravel()]
+        if numpar > 1:
+            parts.append(((vsnew.astype(np.float64) - vs) / np.float64(vs0)).astype(f32).ravel())
+        mod = np.concatenate(parts)
+    u[...] = ((rhonew * vsnew).astype(f32) * vsnew).astype(f32)
+    pi[...] = ((rhonew * vpnew).astype(f32) * vpnew).astype(f32)
+    rho[...] = rhonew
+    return maxima, mod
+
+
+def steplength(L2, step, lbfgs, it_group, teststep):
+    """float32 scalars as steplength.c computes them; returns (L2[4], step[5])"""
+    f = np.float32
+    L2 = [f(x) for x in L2[:3]] + [f(0)]
+    s = [f(x) for x in step[:5]]
+    a = ((s[0] - s[1]) * (L2[0] - L2[2]) - (s[0] - s[2]) * (L2[0] - L2[1])) / \
+        (s[0] * s[2] * (s[0] - s[2]) + s[0] * s[1] * (s[1] - s[0]) + s[1] * s[2] * (s[2] - s[1]))
+    b = (L2[0] - L2[1]) / (s[0] - s[1])
+    b = b - a * (s[0] + s[1])
+    c = L2[0] - a * s[0] * s[0] - b * s[0]
+    s[3] = f(0)
+    if a > 0:
+        s[3] = -b / (f(2) * a)
+    if a < 0:
+        if L2[0] < L2[1] and L2[0] < L2[2]:
+            s[3] = f(0.1 * float(s[1]))
+        if L2[1] < L2[0] and L2[1] < L2[2]:
+            s[3] = s[1]
+        if L2[2] < L2[0] and L2[2] < L2[1]:
+            s[3] = s[2]
+    if float(s[3]) > 2.5 * float(s[4]):
+        s[3] = f(2.5 * float(s[4]))
+    if lbfgs and it_group > 1 and float(s[3]) > 1.0:
+        s[3] = f(1.0)
+    if s[3] <= 0:
+        s[3] = f(0.1 * float(s[1]))
+    if float(s[3]) < 0.1 * float(s[1]):
+        s[3] = f(0.1 * float(s[1]))
+    L2[3] = a * s[3] * s[3] + b * s[3] + c
+    s[4] = s[3] / f(2)
+    if float(s[4]) < 0.2 * float(teststep):
+        s[4] = f(0.2 * float(teststep))
+    if float(s[4]) > float(teststep):
+        s[4] = f(teststep)
+    if lbfgs and it_group > 1:
+        s[4] = f(0.5)
+    return np.array(L2, f), np.array(s, f)

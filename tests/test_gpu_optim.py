@@ -108,3 +108,67 @@ def test_conjugate_matches_oracle():
             g.interior(priors[n])[...] = op[n]
         dp = [G.to_dev(a) for a in priors]
         grads = [(0.6 * _field(g, rng) + 0.4 * a).astype(np.float32) for a in grads]
+
+
+# ---- precongrad.c and modelupdate.c on the device -------------------------
+from optim_oracle import modelupdate as mu_oracle, precon_grad as pg_oracle  # noqa: E402
+
+
+def _max_ulp(a, b):
+    a = np.ascontiguousarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.ascontiguousarray(b, np.float32).view(np.int32).astype(np.int64)
+    return int(np.abs(a - b).max()) if a.size else 0
+
+
+@pytest.mark.parametrize("damptype,free_surf", [(0, 0), (1, 0), (2, 0), (3, 0), (2, 1)])
+def test_precon_grad_matches_oracle(damptype, free_surf):
+    """double exp on the device against numpy's: the tapers are stored to
+    float, so a last-bit difference of exp rarely moves them; a cell's factor
+    chain is a product of (sources + receivers + 6) of them, so agreement is
+    to a few ulp"""
+    import ifosgpu as ig
+    g = small_grid(nx=22, ny=18, nz=26, fw=4, free_surf=free_surf)
+    _note(f"precon_grad DAMPTYPE={damptype} FREE_SURF={free_surf}")
+    G = _geom(ig, g)
+    rng = np.random.default_rng(60 + damptype)
+    grads = [_field(g, rng) for _ in range(3)]
+    src = np.array([[11, 5, 13], [3, 9, 20]], np.int32)
+    rec = np.array([[2 + 2 * n, 4, 10 + n] for n in range(9)], np.int32)
+    d = [G.to_dev(a) for a in grads]
+    ig.precon_grad(G, d, damptype, src, rec)
+    og = [g.interior(a).copy() for a in grads]
+    pg_oracle(og, damptype, src, rec, (g.nx, g.ny, g.nz), g.fw, free_surf)
+    for n in range(3):
+        got = g.interior(G.from_dev(d[n]))
+        assert _max_ulp(got, og[n]) <= 4, f"grad{n + 1}"
+
+
+@pytest.mark.parametrize("lbfgs,numpar", [(False, 2), (True, 2), (True, 1)])
+def test_modelupdate_matches_oracle(lbfgs, numpar):
+    import ifosgpu as ig
+    g = small_grid(nx=20, ny=16, nz=24, fw=4)
+    _note(f"modelupdate lbfgs={lbfgs} numpar={numpar}")
+    G = _geom(ig, g)
+    rng = np.random.default_rng(70 + numpar)
+    shp = g.shape
+    rho = (2000.0 * (1 + 0.1 * rng.random(shp))).astype(np.float32)
+    vp = (4000.0 * (1 + 0.1 * rng.random(shp))).astype(np.float32)
+    vs = (2300.0 * (1 + 0.1 * rng.random(shp))).astype(np.float32)
+    pi = (rho * vp * vp).astype(np.float32)
+    u = (rho * vs * vs).astype(np.float32)
+    grads = [_field(g, rng) for _ in range(3)]
+    d = [G.to_dev(a) for a in grads]
+    dm = [G.to_dev(a) for a in (rho, pi, u)]
+    n1 = g.nx * g.ny * g.nz
+    mod = ig.DBuf((numpar * n1,)) if lbfgs else None
+    mx = ig.modelupdate(G, d, *dm, 0.02, (1.0, 0.8, 0.5), 4000.0, 2300.0, 2000.0, bfgsmod=mod, numpar=numpar)
+    o = [g.interior(a).copy() for a in (rho, pi, u)]
+    og = [g.interior(a).copy() for a in grads]
+    mx_o, mod_o = mu_oracle(og, o[0], o[1], o[2], 0.02, (1.0, 0.8, 0.5), 4000.0, 2300.0, 2000.0, lbfgs, numpar)
+    np.testing.assert_array_equal(mx, mx_o)
+    for n, name in enumerate(("rho", "pi", "u")):
+        assert _max_ulp(g.interior(G.from_dev(dm[n])), o[n]) == 0, name
+    for n in range(3):
+        assert _max_ulp(g.interior(G.from_dev(d[n])), og[n]) == 0, f"grad{n + 1}"
+    if lbfgs:
+        assert _max_ulp(mod.numpy(), mod_o) == 0, "bfgsmod"

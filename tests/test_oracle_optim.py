@@ -96,3 +96,108 @@ def test_conjugate_first_call_copies():
     for n in range(3):
         np.testing.assert_array_equal(p[n], g[n])
         np.testing.assert_array_equal(p[n + 3], g[n])
+
+
+# ---- steplength.c, precongrad.c, modelupdate.c ----------------------------
+from optim_oracle import modelupdate, precon_grad, steplength  # noqa: E402
+
+
+@pytest.mark.parametrize("alpha,smin", [(2.0, 0.3), (0.5, 0.9), (7.0, 0.05)])
+def test_steplength_parabola_minimum(alpha, smin):
+    """three points of L2 = alpha (s - smin)^2 + 3: the fitted minimum is smin
+    (before the reference's clamps) and L2[3] is 3"""
+    st = [0.0, 0.5, 1.0, 0.0, 1.0]
+    L2 = [alpha * (s - smin) ** 2 + 3.0 for s in st[:3]]
+    l2, s = steplength(L2, st, 0, 1, 1.0)
+    expect = max(min(smin, 2.5 * st[4]), 0.1 * st[1])
+    assert s[3] == pytest.approx(expect, rel=1e-5)
+    if expect == smin:
+        assert l2[3] == pytest.approx(3.0, rel=1e-5)
+    assert s[4] == pytest.approx(min(max(s[3] / 2, 0.2), 1.0), rel=1e-6)
+
+
+def test_steplength_branches_match_the_library():
+    """the a < 0 branches and the clamps, oracle against the C ABI bit for bit"""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "ifos3d-dev_amd"))
+    import ifosgpu as ig
+    cases = [([3.0, 1.0, 2.0], [0, 0.5, 1.0, 0, 1.0], 0, 1),    # a > 0
+             ([1.0, 3.0, 2.0], [0, 0.5, 1.0, 0, 1.0], 0, 1),    # a < 0, L2[0] smallest
+             ([2.0, 1.0, 3.5], [0, 0.5, 1.0, 0, 0.1], 0, 1),    # clamp 2.5 step[4]
+             ([5.0, 2.0, 1.0], [0, 0.5, 1.0, 0, 4.0], 1, 3),    # L-BFGS: step <= 1, step[4] = 0.5
+             ([2.0, 3.0, 1.0], [0, 0.5, 1.0, 0, 1.0], 0, 1)]    # a < 0, L2[2] smallest
+    for L2, st, lb, it in cases:
+        a = steplength(L2, st, lb, it, 0.8)
+        b = ig.steplength(L2, st, lb, it, 0.8)
+        np.testing.assert_array_equal(a[0][3:], b[0][3:])
+        np.testing.assert_array_equal(a[1], b[1])
+
+
+def _grads(shape, seed=3):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal(shape).astype(np.float32) for _ in range(3)]
+
+
+def test_precon_damptype0_is_a_constant_factor():
+    """with the global indices left at 0 (DAMPTYPE 0), every cell is scaled by
+    the same six edge tapers 1/(1 + 2000 exp(-0.6 r)) at r = 0 and r = N*G"""
+    shape = (6, 5, 7)
+    g = _grads(shape)
+    g0 = [a.copy() for a in g]
+    precon_grad(g, 0, [], [], (5, 6, 7), 4, 0)
+    t = lambda r: np.float32(1.0 / (1.0 + 2000.0 * np.exp(-0.6 * r)))
+    for n in range(3):
+        expect = g0[n]
+        for G0, G1 in ((t(0.0), t(6.0)), (t(0.0), t(5.0)), (t(0.0), t(7.0))):
+            expect = (expect * G0).astype(np.float32) * G1
+        np.testing.assert_array_equal(g[n], expect.astype(np.float32))
+
+
+def test_precon_receiver_taper_and_quirks():
+    shape = (12, 10, 14)
+    g = _grads(shape)
+    g1 = [a.copy() for a in g]
+    precon_grad(g1, 1, [], [(4, 6, 7)], (10, 12, 14), 4, 0)
+    # at the receiver cell r = 0: factor 1 / 1001 times the edge tapers
+    t = lambda r: np.float32(1.0 / (1.0 + 2000.0 * np.exp(-0.6 * r)))
+    j, i, k = 6, 4, 7
+    expect = np.float32(g[0][j - 1, i - 1, k - 1] * np.float32(1.0 / 1001.0))
+    for lo, hi in ((j, 12 - j), (i, 10 - i), (k, 14 - k)):
+        expect = np.float32(np.float32(expect * t(lo)) * t(hi))
+    assert g1[0][j - 1, i - 1, k - 1] == expect
+    # three cells from the receiver the receiver factor is 1 / (1 + 1000 exp(-0.7 * 9))
+    g_norec = [a.copy() for a in g]
+    precon_grad(g_norec, 1, [], [], (10, 12, 14), 4, 0)
+    ratio = g1[0][j - 1, i - 1, k + 2] / g_norec[0][j - 1, i - 1, k + 2]
+    assert ratio == pytest.approx(1.0 / (1.0 + 1000.0 * np.exp(-0.7 * 9.0)), rel=1e-6)
+    # FREE_SURF zeroes the y factor of DAMPTYPE 0-2; DAMPTYPE 3 with j = 0 zeroes G[0]
+    for dt, fs in ((1, 1), (0, 1), (3, 0)):
+        gz = [a.copy() for a in g]
+        precon_grad(gz, dt, [], [(4, 6, 7)], (10, 12, 14), 4, fs)
+        assert all(not a.any() for a in gz)
+
+
+def test_modelupdate_by_hand():
+    """one cell: vp 4000 -> 4000 + 0.1*1*gvp*VP0 etc., pi/u/rho rebuilt;
+    without L-BFGS the gradients are first divided by their maxima"""
+    sh = (1, 1, 2)
+    rho = np.full(sh, 2000.0, np.float32)
+    pi = (rho * np.float32(4000.0) ** 2).astype(np.float32)
+    u = (rho * np.float32(2300.0) ** 2).astype(np.float32)
+    g = [np.array([[[0.5, -1.0]]], np.float32), np.array([[[2.0, 1.0]]], np.float32),
+         np.array([[[0.0, 0.0]]], np.float32)]
+    mx, mod = modelupdate(g, rho, pi, u, 0.1, (1.0, 1.0, 1.0), 4000.0, 2300.0, 2000.0, lbfgs=False)
+    assert list(mx) == [4000.0, 2300.0, 2000.0]
+    np.testing.assert_array_equal(g[0], [[[0.5, -1.0]]])           # divided by max |g| = 1
+    np.testing.assert_array_equal(g[1], [[[1.0, 0.5]]])            # divided by 2
+    vp = np.sqrt(pi / rho)
+    assert vp[0, 0, 0] == pytest.approx(4000.0 + 0.1 * 0.5 * 4000.0, rel=1e-6)
+    assert np.sqrt(u / rho)[0, 0, 1] == pytest.approx(2300.0 + 0.1 * 0.5 * 2300.0, rel=1e-6)
+    assert mod is None
+    rho2 = np.full(sh, 2000.0, np.float32)
+    pi2 = (rho2 * np.float32(4000.0) ** 2).astype(np.float32)
+    u2 = (rho2 * np.float32(2300.0) ** 2).astype(np.float32)
+    g2 = [np.array([[[0.5, -1.0]]], np.float32), np.array([[[2.0, 1.0]]], np.float32),
+          np.array([[[1.0, 0.0]]], np.float32)]
+    _, mod = modelupdate(g2, rho2, pi2, u2, 0.1, (1.0, 1.0, 1.0), 4000.0, 2300.0, 2000.0, lbfgs=True, numpar=2)
+    np.testing.assert_allclose(mod, [0.05, -0.1, 0.2, 0.1], rtol=1e-5)   # (vpnew - vp) / VP0, then vs
+    assert rho2[0, 0, 0] == np.float32(2000.0 + 0.1 * 1.0 * 2000.0)
