@@ -50,7 +50,7 @@ def kernel_names(path, nf, chunk, tiles=None):
 
 # tile shapes {TI, jc, group, split, pipe} used without the autotuner (its
 # usual choice on grad276)
-DEFAULT_TILES = ((4, 32, 0, 0, 0, 1, 2), (4, 32, 0, 0, 0, 1, 2))
+DEFAULT_TILES = ((4, 32, 0, 0, 0, 1, 1, 0), (4, 32, 0, 0, 1, 1, 1, 0))
 CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other",
         "vs_fused"]
 
