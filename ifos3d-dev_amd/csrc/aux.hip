@@ -551,6 +551,32 @@ extern "C" int ifos_gradient_F(const ifos_geom *g, float *const d_Ff[6], float *
     return IFOS_OK;
 }
 
+extern "C" int ifos_hess_F(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int ntr_hess,
+                           const ifos_model *m, const float *finv, int nf, float *const d_hess[3], void *stream)
+{
+    if (nf <= 0 || nf > IFOS_MAX_NF || ntr_hess < 0) {
+        set_error("ifos_hess_F: nf=%d outside [1,%d] or ntr_hess=%d < 0", nf, IFOS_MAX_NF, ntr_hess);
+        return IFOS_EINVAL;
+    }
+    const size_t fe = ifos_field_elems(g);
+    ProfScope ps_(IFOS_K_GRADIENT, (hipStream_t)stream);
+    // hess_F.c:72-77: receivers outer, frequencies inner, one march per pair
+    for (int n = 1; n <= ntr_hess; ++n)
+        for (int l = 1; l <= nf; ++l) {
+            const float fdummy = (float)(finv[l - 1] * M_PI * 2);
+            const float fd = 1 / fdummy;
+            const size_t mb = (size_t)(n * nf + l - 1);
+            const float *f[12];
+            for (int c = 0; c < 6; ++c) {
+                f[c] = d_Ff[c] + (size_t)(l - 1) * fe;
+                f[6 + c] = d_Fb[c] + mb * fe;
+            }
+            int rc = launch_grad_march(g, f, fd, m, d_hess, (hipStream_t)stream, true);
+            if (rc) return rc;
+        }
+    return IFOS_OK;
+}
+
 namespace ifos {
 int launch_gradient_shots(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int64_t fshot, int ns,
                           const ifos_model *m, const float *finv, int nf, float *const d_grad[3], hipStream_t st)

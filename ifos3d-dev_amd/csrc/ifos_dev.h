@@ -198,9 +198,10 @@ struct FTiles {
 int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof, const Shots &S, const Snap *snap,
                     int ti, int jc, hipStream_t s);
 
-// gradient_F for one frequency of one shot, march kernel (stencil.hip)
+// gradient_F for one frequency of one shot, march kernel (stencil.hip); with
+// hess, hess_F for one frequency of one receiver-side set
 int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, const ifos_model *m,
-                      float *const d_grad[3], hipStream_t s);
+                      float *const d_grad[3], hipStream_t s, bool hess = false);
 void set_error(const char *fmt, ...);
 int hip_check(hipError_t e, const char *what);
 

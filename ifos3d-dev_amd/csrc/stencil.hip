@@ -648,7 +648,10 @@ struct GradArgs {
     float *g1, *g2, *g3;
 };
 
-template <int TI>
+// HESS: hess_F.c:28-192 instead, the same fields read as the forward set
+// (f, fi) and one receiver-side set (b, bi), fd = 1 / (2 pi f) as
+// hess_F.c:75-77 rounds it, the three Hessian diagonals in place of g1..g3.
+template <int TI, bool HESS>
 __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, GradArgs A)
 {
     int k, i, jb, je, shot;
@@ -694,6 +697,42 @@ __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, Gra
         }
         // d[t] = xx xy xz yx yy yz zx zy zz of f (t 0), fi (1), b (2), bi (3)
         const float *f = d[0], *fi = d[1], *bb = d[2], *bi = d[3];
+        if (HESS) {
+            // hess_F.c:135-182, values of the cell: qx[t][1], qy[t][2], qz[t][1]
+            const float fdiv = f[0] + f[4] + f[8], fidiv = fi[0] + fi[4] + fi[8];
+            const float bdiv = bb[0] + bb[4] + bb[8], bidiv = bi[0] + bi[4] + bi[8];
+            float relam = fidiv * bdiv + fdiv * bidiv;
+            relam = A.fd * relam;
+            float imlam = fidiv * bidiv - fdiv * bdiv;
+            imlam = A.fd * imlam;
+            float remu = 2.0f * fi[0] * bb[0] + 2.0f * fi[4] * bb[4] + 2.0f * fi[8] * bb[8] + 2.0f * f[0] * bi[0] +
+                         2.0f * f[4] * bi[4] + 2.0f * f[8] * bi[8] + (fi[1] + fi[3]) * (bb[1] + bb[3]) +
+                         (f[1] + f[3]) * (bi[1] + bi[3]) + (fi[2] + fi[6]) * (bb[2] + bb[6]) +
+                         (f[2] + f[6]) * (bi[2] + bi[6]) + (fi[5] + fi[7]) * (bb[5] + bb[7]) +
+                         (f[5] + f[7]) * (bi[5] + bi[7]);
+            remu = A.fd * remu;
+            float immu = 2.0f * fi[0] * bi[0] + 2.0f * fi[4] * bi[4] + 2.0f * fi[8] * bi[8] - 2.0f * f[0] * bb[0] -
+                         2.0f * f[4] * bb[4] - 2.0f * f[8] * bb[8] + (fi[1] + fi[3]) * (bi[1] + bi[3]) -
+                         (f[1] + f[3]) * (bb[1] + bb[3]) + (fi[2] + fi[6]) * (bi[2] + bi[6]) -
+                         (f[2] + f[6]) * (bb[2] + bb[6]) + (fi[5] + fi[7]) * (bi[5] + bi[7]) -
+                         (f[5] + f[7]) * (bb[5] + bb[7]);
+            immu = A.fd * immu;
+            const float fx = qx[0][1], fy = qy[0][2], fz = qz[0][1], fix = qx[1][1], fiy = qy[1][2], fiz = qz[1][1];
+            const float bx = qx[2][1], by = qy[2][2], bz = qz[2][1], bix = qx[3][1], biy = qy[3][2], biz = qz[3][1];
+            const float rerho = (fx * bx + fy * by + fz * bz) - (fix * bix + fiy * biy + fiz * biz);
+            const float imrho = (fx * bix + fy * biy + fz * biz) + (fix * bx + fiy * by + fiz * bz);
+            const float revs = -2.0f * relam + remu, imvs = -2.0f * imlam + immu;
+            const float r = LD(rrho, 0, 0, 0), p = LD(rpi, 0, 0, 0), m = LD(ru, 0, 0, 0);
+            const float rerho1 = rerho + relam * (p - 2.0f * m) / r + remu * m / r;
+            const float imrho1 = imrho + imlam * (p - 2.0f * m) / r + immu * m / r;
+            float h1 = LD(rg1, 0, 0, 0), h2 = LD(rg2, 0, 0, 0), h3 = LD(rg3, 0, 0, 0);
+            h1 += (relam * relam + imlam * imlam) * r * p * 4.0f;
+            h2 += (revs * revs + imvs * imvs) * 4.0f * r * m;
+            h3 += rerho1 * rerho1 + imrho1 * imrho1;
+            ST(h1, rg1);
+            ST(h2, rg2);
+            ST(h3, rg3);
+        } else {
         float gl = (f[0] + f[4] + f[8]) * (bb[0] + bb[4] + bb[8]) + (fi[0] + fi[4] + fi[8]) * (bi[0] + bi[4] + bi[8]);
         gl = -gl * A.fd;
         float gm = 2.0f * f[0] * bb[0] + 2.0f * f[4] * bb[4] + 2.0f * f[8] * bb[8] + 2.0f * fi[0] * bi[0] +
@@ -713,6 +752,7 @@ __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, Gra
         ST(a1, rg1);
         ST(a2, rg2);
         ST(a3, rg3);
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -1244,7 +1284,7 @@ namespace ifos {
 // gradient_F over [1..N]^3 for one frequency: f[12] point at that
 // frequency's fields (forward Re xyz, Im xyz, adjoint Re xyz, Im xyz)
 int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, const ifos_model *m,
-                      float *const d_grad[3], hipStream_t s)
+                      float *const d_grad[3], hipStream_t s, bool hess)
 {
     DevGeom G = make_dev_geom(g);
     GradArgs A;
@@ -1258,7 +1298,8 @@ int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, co
     A.g3 = d_grad[2];
     const Box all = region_box(g, IFOS_REGION_ALL);
     const MTiles T = make_tiles(&all, 1, 4, 0, 1, 0);
-    hipLaunchKernelGGL(k_grad_march<4>, dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
+    if (hess) hipLaunchKernelGGL((k_grad_march<4, true>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
+    else hipLaunchKernelGGL((k_grad_march<4, false>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
     return hip_check(hipGetLastError(), "k_grad_march");
 }
 }  // namespace ifos

@@ -350,6 +350,16 @@ def gradient_F(g, Ff6, Fb6, m, finv, grads, stream=None):
                                 len(fv), Gp, stream), "ifos_gradient_F")
 
 
+def hess_F(g, Ff6, Fb6, ntr_hess, m, finv, hess, stream=None):
+    """hess_F.c: Ff6 [nf] and Fb6 [nf*(ntr_hess+1)] stacks of device fields"""
+    fv = np.asarray(finv, np.float32)
+    Fp = (C.c_void_p * 6)(*[t.data_ptr() for t in Ff6])
+    Bp = (C.c_void_p * 6)(*[t.data_ptr() for t in Fb6])
+    Hp = (C.c_void_p * 3)(*[t.data_ptr() for t in hess])
+    check(lib().ifos_hess_F(g.ref, Fp, Bp, ntr_hess, C.byref(model_struct(m)), fv.ctypes.data_as(C.c_void_p),
+                            len(fv), Hp, stream), "ifos_hess_F")
+
+
 def matcopy_wrap(g, d_m, stream=None):
     check(lib().ifos_matcopy_wrap(g.ref, _ptr(d_m), stream), "ifos_matcopy_wrap")
 

@@ -207,6 +207,14 @@ int ifos_gradient_F(const ifos_geom *g, float *const d_Ff[6],
                     void *stream);
 /* matcopy.c:31-307 single rank: periodic halos of a model array */
 int ifos_matcopy_wrap(const ifos_geom *g, float *d_m, void *stream);
+/* hess_F.c:28-192: accumulates the diagonal (pseudo-)Hessian d_hess[3]
+ * (vp, vs, rho) on [1..N]^3 from the forward DFT fields d_Ff (frequency l at
+ * + (l-1) * ifos_field_elems) and the receiver-side DFT fields d_Fb of
+ * receivers n = 1..ntr_hess (receiver n, frequency l at
+ * + (n*nf + l - 1) * ifos_field_elems, as the reference's Fbv* index them),
+ * receivers outer, frequencies inner */
+int ifos_hess_F(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int ntr_hess,
+                const ifos_model *m, const float *finv, int nf, float *const d_hess[3], void *stream);
 /* av_mat.c:28-80: writes the six averaged arrays of *out */
 int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *d_u,
                 float *d_uipjp, float *d_ujpkp, float *d_uipkp, float *d_rjp,

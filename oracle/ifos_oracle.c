@@ -845,6 +845,74 @@ void og_gradient_F(const og_grid *g, const oreal *fvx, const oreal *fvy,
     }
 }
 
+/* hess_F.c:28-192: the diagonal Hessian (approximate, pseudo-Hessian of the
+ * receiver-side Green's functions).  Forward fields f* of frequency l at
+ * (l-1)*fs; receiver-side fields b* of receiver n (1..ntr_hess) and
+ * frequency l at (n*nf + l - 1)*fs (hess_F.c:74); accumulated over n, then
+ * l, cell by cell, in the reference's float expression order. */
+void og_hess_F(const og_grid *g, const oreal *fvx, const oreal *fvy,
+               const oreal *fvz, const oreal *fivx, const oreal *fivy,
+               const oreal *fivz, const oreal *bvx, const oreal *bvy,
+               const oreal *bvz, const oreal *bivx, const oreal *bivy,
+               const oreal *bivz, oreal *hess1, oreal *hess2, oreal *hess3,
+               const oreal *rho, const oreal *pi, const oreal *u,
+               const float *finv, int nf, int ntr_hess)
+{
+    const size_t fs = og_field_size(g);
+    oreal b1, b2;
+    int n, l, i, j, k;
+    fd4_coeffs(g->fdcoeff, &b1, &b2);
+    for (n = 1; n <= ntr_hess; n++)
+        for (l = 1; l <= nf; l++) {
+            const int m = n * nf + l;
+            const size_t off = (size_t)(l - 1) * fs, offb = (size_t)(m - 1) * fs;
+            oreal fdummy = (oreal)(finv[l - 1] * M_PI * 2); /* hess_F.c:75-77 */
+            fdummy = 1 / fdummy;
+            for (j = 1; j <= g->ny; j++)
+                for (i = 1; i <= g->nx; i++)
+                    for (k = 1; k <= g->nz; k++) {
+                        const size_t c = og_idx(g, j, i, k), co = off + c, cb = offb + c;
+                        grad9 f, b, fi, bi;
+                        oreal relam, imlam, remu, immu, rerho, imrho, revs, imvs, rerho1, imrho1;
+                        fgrad9(g, co, fvx, fvy, fvz, b1, b2, &f);
+                        fgrad9(g, cb, bvx, bvy, bvz, b1, b2, &b);
+                        fgrad9(g, co, fivx, fivy, fivz, b1, b2, &fi);
+                        fgrad9(g, cb, bivx, bivy, bivz, b1, b2, &bi);
+                        /* hess_F.c:135-174 */
+                        relam = (fi.xx + fi.yy + fi.zz) * (b.xx + b.yy + b.zz) +
+                                (f.xx + f.yy + f.zz) * (bi.xx + bi.yy + bi.zz);
+                        relam = fdummy * relam;
+                        imlam = (fi.xx + fi.yy + fi.zz) * (bi.xx + bi.yy + bi.zz) -
+                                (f.xx + f.yy + f.zz) * (b.xx + b.yy + b.zz);
+                        imlam = fdummy * imlam;
+                        remu = 2 * fi.xx * b.xx + 2 * fi.yy * b.yy + 2 * fi.zz * b.zz +
+                               2 * f.xx * bi.xx + 2 * f.yy * bi.yy + 2 * f.zz * bi.zz +
+                               (fi.xy + fi.yx) * (b.xy + b.yx) + (f.xy + f.yx) * (bi.xy + bi.yx) +
+                               (fi.xz + fi.zx) * (b.xz + b.zx) + (f.xz + f.zx) * (bi.xz + bi.zx) +
+                               (fi.yz + fi.zy) * (b.yz + b.zy) + (f.yz + f.zy) * (bi.yz + bi.zy);
+                        remu = fdummy * remu;
+                        immu = 2 * fi.xx * bi.xx + 2 * fi.yy * bi.yy + 2 * fi.zz * bi.zz -
+                               2 * f.xx * b.xx - 2 * f.yy * b.yy - 2 * f.zz * b.zz +
+                               (fi.xy + fi.yx) * (bi.xy + bi.yx) - (f.xy + f.yx) * (b.xy + b.yx) +
+                               (fi.xz + fi.zx) * (bi.xz + bi.zx) - (f.xz + f.zx) * (b.xz + b.zx) +
+                               (fi.yz + fi.zy) * (bi.yz + bi.zy) - (f.yz + f.zy) * (b.yz + b.zy);
+                        immu = fdummy * immu;
+                        rerho = (fvx[co] * bvx[cb] + fvy[co] * bvy[cb] + fvz[co] * bvz[cb]) -
+                                (fivx[co] * bivx[cb] + fivy[co] * bivy[cb] + fivz[co] * bivz[cb]);
+                        imrho = (fvx[co] * bivx[cb] + fvy[co] * bivy[cb] + fvz[co] * bivz[cb]) +
+                                (fivx[co] * bvx[cb] + fivy[co] * bvy[cb] + fivz[co] * bvz[cb]);
+                        revs = -2 * relam + remu;
+                        imvs = -2 * imlam + immu;
+                        rerho1 = rerho + relam * (pi[c] - 2 * u[c]) / rho[c] + remu * u[c] / rho[c];
+                        imrho1 = imrho + imlam * (pi[c] - 2 * u[c]) / rho[c] + immu * u[c] / rho[c];
+                        /* hess_F.c:180-182 */
+                        hess1[c] += (relam * relam + imlam * imlam) * rho[c] * pi[c] * 4;
+                        hess2[c] += (revs * revs + imvs * imvs) * 4 * rho[c] * u[c];
+                        hess3[c] += rerho1 * rerho1 + imrho1 * imrho1;
+                    }
+        }
+}
+
 /* ifos3d.c:745-750 */
 int og_ntast(float fmax, int tast, float dt)
 {

@@ -196,6 +196,16 @@ void og_gradient_F(const og_grid *g, const oreal *fvx, const oreal *fvy,
                    oreal *grad3, const oreal *rho, const oreal *pi,
                    const oreal *u, const float *finv, int nf);
 
+/* hess_F.c:28-192; receiver-side fields of receiver n, frequency l at
+ * (n*nf + l - 1) * field size */
+void og_hess_F(const og_grid *g, const oreal *fvx, const oreal *fvy,
+               const oreal *fvz, const oreal *fivx, const oreal *fivy,
+               const oreal *fivz, const oreal *bvx, const oreal *bvy,
+               const oreal *bvz, const oreal *bivx, const oreal *bivy,
+               const oreal *bivz, oreal *hess1, oreal *hess2, oreal *hess3,
+               const oreal *rho, const oreal *pi, const oreal *u,
+               const float *finv, int nf, int ntr_hess);
+
 /* ntast, ifos3d.c:745-750 */
 int og_ntast(float fmax, int tast, float dt);
 

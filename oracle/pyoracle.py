@@ -222,6 +222,14 @@ class Oracle:
         self.lib.og_discfourier(C.byref(og), nt, _p(vx), _p(vy), _p(vz), _p(Fr[0]), _p(Fr[1]), _p(Fr[2]),
                                 _p(Fi[0]), _p(Fi[1]), _p(Fi[2]), _p(fv), len(fv), ntast, back)
 
+    def hess_F(self, g, Ff, Fb, hess, rho, pi, u, finv, ntr_hess):
+        """hess_F.c; Ff[6] [nf][...], Fb[6] [nf*(ntr_hess+1)][...] (receiver n,
+        frequency l at n*nf + l - 1), hess[3] accumulated in place"""
+        og = g.og()
+        fv = np.asarray(finv, np.float32)
+        self.lib.og_hess_F(C.byref(og), *[_p(a) for a in Ff], *[_p(a) for a in Fb], *[_p(a) for a in hess],
+                           _p(rho), _p(pi), _p(u), fv.ctypes.data_as(C.c_void_p), len(fv), ntr_hess)
+
     def gradient_F(self, g, Ff, Fb, grads, rho, pi, u, finv):
         """Ff, Fb: 6 arrays each (Re x y z, Im x y z) of shape (nf,)+g.shape"""
         og = g.og()

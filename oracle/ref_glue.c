@@ -413,6 +413,15 @@ void rg_gradient_F(void)
                S.finv, S.nf, 1);
 }
 
+/* hess_F over the F arrays (receiver n, frequency l at index n*nf + l),
+ * accumulated into grad1..3 */
+void rg_hess_F(int ntr_hess)
+{
+    hess_F(NX, NY, NZ, S.F[0], S.F[1], S.F[2], S.F[3], S.F[4], S.F[5], S.F[6], S.F[7], S.F[8], S.F[9],
+           S.F[10], S.F[11], W[RG_GRAD1], W[RG_GRAD2], W[RG_GRAD3], NT + 1, W[RG_RHO], W[RG_PI], W[RG_U],
+           S.finv, S.nf, ntr_hess);
+}
+
 void rg_matcopy(void)
 {
     matcopy(W[RG_RHO], W[RG_PI], W[RG_U], W[RG_TAUS], W[RG_TAUP]);

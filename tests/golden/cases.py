@@ -77,6 +77,28 @@ def grad_inputs(g, seed):
     return F, grads, (rho, pi, u)
 
 
+HESS_NTR = 2  # receiver-side Green's-function sets of the hess_F case
+
+
+def hess_inputs(g, seed, ntr=HESS_NTR):
+    """forward DFT fields [nf], receiver-side fields [nf * (ntr + 1)] (entries
+    of receiver n, frequency l at n*nf + l - 1; receiver 0 unused by hess_F),
+    three Hessian accumulators and a model"""
+    nf = len(DFT_FINV)
+    Ff = [det_uniform((nf,) + g.shape, seed + 110 + n, -1.0, 1.0) for n in range(6)]
+    Fb = [det_uniform((nf * (ntr + 1),) + g.shape, seed + 120 + n, -1.0, 1.0) for n in range(6)]
+    for a in Ff + Fb:  # zero halos (zero_invers.c:43-79)
+        a[:, :3] = 0
+        a[:, 3 + g.ny:] = 0
+        a[:, :, :3] = 0
+        a[:, :, 3 + g.nx:] = 0
+        a[:, :, :, :3] = 0
+        a[:, :, :, 3 + g.nz:] = 0
+    hess = [det_uniform(g.shape, seed + 130 + n, 0.0, 1.0) for n in range(3)]
+    rho, pi, u = det_model(g, seed + 135)
+    return Ff, Fb, hess, (rho, pi, u)
+
+
 def residual_inputs(ntr, ns, seed):
     obs = [det_uniform((ntr, ns + 1), seed + c, -1.0, 1.0) for c in range(3)]
     syn = [det_uniform((ntr, ns + 1), seed + 10 + c, -1.0, 1.0) for c in range(3)]

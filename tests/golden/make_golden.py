@@ -118,6 +118,23 @@ def main():
     r.call("gradient_F")
     C["gradient"] = {f"grad{n + 1}": rec(g.interior(r.get(f"grad{n + 1}"))) for n in range(3)}
 
+    # g9: hess_F (hess_F.c:28-192), two receiver-side sets
+    Ff, Fb, hess, (rho, pi, u) = cs.hess_inputs(g, 9)
+    nf = len(cs.DFT_FINV)
+    r = Ref(g, nfmax=nf * (cs.HESS_NTR + 1))
+    for n in range(6):
+        for m in range(nf):
+            r.F_set(n, m + 1, Ff[n][m])
+        for m in range(nf * (cs.HESS_NTR + 1)):
+            r.F_set(6 + n, m + 1, Fb[n][m])
+    for n in range(3):
+        r.set(f"grad{n + 1}", hess[n])
+    for name, a in zip(("rho", "pi", "u"), (rho, pi, u)):
+        r.set(name, a)
+    r.set_freqs(cs.DFT_FINV)
+    r.call("hess_F", cs.HESS_NTR)
+    C["hess"] = {f"hess{n + 1}": rec(g.interior(r.get(f"grad{n + 1}"))) for n in range(3)}
+
     # g5: residual
     g = cs.dft_grid()
     ntr = 5

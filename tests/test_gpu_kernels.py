@@ -291,6 +291,37 @@ def test_dft_compat_and_kahan(env, nf):
     assert max(ek) <= max(ec)
 
 
+@pytest.mark.parametrize("dh,ntr", [(2.0, 2), (3.0, 3)])
+def test_hess_F(env, dh, ntr):
+    """hess_F.c: forward fields [nf], receiver-side fields [nf*(ntr+1)]"""
+    _, ig = env
+    g = small_grid(nx=22, ny=20, nz=24, dh=dh)
+    _note(f"hess_F parity dh={dh} ntr_hess={ntr}")
+    o = Oracle("f32")
+    rng = np.random.default_rng(19)
+    rho, pi, u = random_model(g, rng)
+    finv = [40.0, 60.0]
+    nf = len(finv)
+    Ff = [rng.standard_normal((nf,) + g.shape).astype(np.float32) for _ in range(6)]
+    Fb = [rng.standard_normal((nf * (ntr + 1),) + g.shape).astype(np.float32) for _ in range(6)]
+    for a in Ff + Fb:  # zero halos (zero_invers.c:43-79)
+        a[:, :3] = 0; a[:, 3 + g.ny:] = 0; a[:, :, :3] = 0; a[:, :, 3 + g.nx:] = 0
+        a[:, :, :, :3] = 0; a[:, :, :, 3 + g.nz:] = 0
+    hess = [rng.random(g.shape).astype(np.float32) for _ in range(3)]
+    ho = [a.copy() for a in hess]
+    o.hess_F(g, Ff, Fb, ho, rho, pi, u, finv, ntr)
+    G = _geom(ig, g)
+    dFf = [G.to_dev(a) for a in Ff]
+    dFb = [G.to_dev(a) for a in Fb]
+    dm = [G.to_dev(a) for a in (rho, pi, u)] + [G.empty() for _ in range(6)]
+    dh_ = [G.to_dev(a) for a in hess]
+    ig.hess_F(G, dFf, dFb, ntr, dm, finv, dh_)
+    ig.sync()
+    for n in range(3):
+        got = g.interior(G.from_dev(dh_[n]))
+        assert max_ulp(got, g.interior(ho[n])) == 0, f"hess{n + 1}"
+
+
 @pytest.mark.parametrize("dh", [2.0, 3.0])
 def test_gradient_F(env, dh):
     _, ig = env

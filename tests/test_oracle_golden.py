@@ -113,6 +113,16 @@ def test_gradient_F(gold):
         check(gold["gradient"][f"grad{n + 1}"], g.interior(grads[n]), f"grad{n + 1}")
 
 
+def test_hess_F(gold):
+    """hess_F.c on seeded DFT fields, two receiver-side sets"""
+    g = cs.dft_grid()
+    o = Oracle("f32")
+    Ff, Fb, hess, (rho, pi, u) = cs.hess_inputs(g, 9)
+    o.hess_F(g, Ff, Fb, hess, rho, pi, u, cs.DFT_FINV, cs.HESS_NTR)
+    for n in range(3):
+        check(gold["hess"][f"hess{n + 1}"], g.interior(hess[n]), f"hess{n + 1}")
+
+
 def test_residual(gold):
     g = cs.dft_grid()
     o = Oracle("f32")
