@@ -360,6 +360,23 @@ def hess_F(g, Ff6, Fb6, ntr_hess, m, finv, hess, stream=None):
                             len(fv), Hp, stream), "ifos_hess_F")
 
 
+def su_read(path, ntr, ns, d_section):
+    """readseis.c: ntr SU traces of ns samples into d_section [ntr][ns+1]"""
+    check(lib().ifos_su_read(os.fsencode(path), ntr, ns, _ptr(d_section)), "ifos_su_read")
+
+
+def su_write(path, d_section, ntr, ns, rec_xyz, dx, dy, dz, dt, ndt=1, rec_id=None, src_xyz=None):
+    """outseis.c (SU): d_section [ntr][ns+1] to path; rec_xyz [ntr][3] grid
+    indices, rec_id global receiver numbers, src_xyz the shot position (m)"""
+    rx = np.ascontiguousarray(rec_xyz, np.int32).reshape(-1)
+    rid = None if rec_id is None else np.ascontiguousarray(rec_id, np.int32)
+    sp = None if src_xyz is None else np.ascontiguousarray(src_xyz, np.float32)
+    check(lib().ifos_su_write(os.fsencode(path), _ptr(d_section), ntr, ns, rx.ctypes.data_as(C.c_void_p),
+                              None if rid is None else rid.ctypes.data_as(C.c_void_p),
+                              None if sp is None else sp.ctypes.data_as(C.c_void_p),
+                              C.c_float(dx), C.c_float(dy), C.c_float(dz), C.c_float(dt), ndt), "ifos_su_write")
+
+
 def matcopy_wrap(g, d_m, stream=None):
     check(lib().ifos_matcopy_wrap(g.ref, _ptr(d_m), stream), "ifos_matcopy_wrap")
 

@@ -215,6 +215,22 @@ int ifos_matcopy_wrap(const ifos_geom *g, float *d_m, void *stream);
  * receivers outer, frequencies inner */
 int ifos_hess_F(const ifos_geom *g, float *const d_Ff[6], float *const d_Fb[6], int ntr_hess,
                 const ifos_model *m, const float *finv, int nf, float *const d_hess[3], void *stream);
+/* readseis.c:27-66 (SU files): reads ntr traces of ns samples from path
+ * into the device section d_section [ntr][ns+1] (sample j at index j,
+ * index 0 set to 0), skipping each 240-byte trace header; samples past the
+ * end of a short file are 0, as in readseis.c.  An unreadable file is an
+ * IFOS_EINVAL error. */
+int ifos_su_read(const char *path, int ntr, int ns, float *d_section);
+/* outseis.c:30-275 with SEIS_FORMAT 1 (SU) as saveseis.c:38-80 calls it:
+ * writes the device section d_section [ntr][ns+1] as ntr SU traces.
+ * rec_xyz [ntr][3] holds each trace's receiver grid indices (x, y, z) and
+ * rec_id [ntr] its global receiver number (tracl; NULL = 1..ntr); src_xyz
+ * (x, y, z in metres, as srcpos[1..3]) is the shot's position, or NULL for
+ * the reference's nsrc != 1 case (zero source coordinates).  Header words
+ * the reference leaves unassigned (cdp, selev, swdep, unass) are written as
+ * 0. */
+int ifos_su_write(const char *path, const float *d_section, int ntr, int ns, const int *rec_xyz,
+                  const int *rec_id, const float *src_xyz, float dx, float dy, float dz, float dt, int ndt);
 /* av_mat.c:28-80: writes the six averaged arrays of *out */
 int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *d_u,
                 float *d_uipjp, float *d_ujpkp, float *d_uipkp, float *d_rjp,

@@ -301,6 +301,26 @@ class Ref:
     def available() -> bool:
         return os.path.exists(REFLIB)
 
+    @staticmethod
+    def outseis(path, comp, sec, rec_xyz, rec_id, src_xyz, dx, dy, dz, dt, ndt):
+        """outseis.c (SU) of section [ntr][ns+1] into path"""
+        lib = C.CDLL(REFLIB)
+        s = np.ascontiguousarray(sec, np.float32)
+        rx = np.ascontiguousarray(rec_xyz, np.int32)
+        rid = np.ascontiguousarray(rec_id, np.int32)
+        sp = None if src_xyz is None else np.ascontiguousarray(src_xyz, np.float32)
+        lib.rg_outseis(os.fsencode(path), comp, _p(s), s.shape[0], s.shape[1] - 1, _p(rx), _p(rid),
+                       None if sp is None else _p(sp), C.c_float(dx), C.c_float(dy), C.c_float(dz),
+                       C.c_float(dt), ndt)
+
+    @staticmethod
+    def readseis(prefix, ishot, comp, ntr, ns):
+        """readseis.c of "<prefix>_v?.su.shot<ishot>.0" -> [ntr][ns+1]"""
+        lib = C.CDLL(REFLIB)
+        out = np.zeros((ntr, ns + 1), np.float32)
+        lib.rg_readseis(os.fsencode(prefix), ishot, comp, ntr, ns, _p(out))
+        return out
+
     def __init__(self, g: Grid, vppml=4000.0, fpml=50.0, nfmax=1, nsrc=1, ntr=1):
         self.lib = C.CDLL(REFLIB)
         self.lib.rg_residual.restype = C.c_float

@@ -542,3 +542,53 @@ float rg_run_shot(int shape, int tast, const float *obs_x, const float *obs_y, c
     rg_gradient_F();
     return l2;
 }
+
+/* outseis.c with SEIS_FORMAT 1 into path: sec [ntr][ns+1] by sample,
+ * rec_xyz [ntr][3] grid indices of trace t's receiver, rec_id [ntr] its
+ * global number (recpos_loc[4][t]); src_xyz the one source (nsrc = 1, as
+ * saveseis.c:38-39 passes it) or NULL for nsrc = 0 */
+void rg_outseis(const char *path, int comp, const float *sec, int ntr, int ns, const int *rec_xyz,
+                const int *rec_id, const float *src_xyz, float dx, float dy, float dz, float dt, int ndt)
+{
+    int t, j, nrec = 1;
+    float **section, **srcpos;
+    int **recpos, **recpos_loc;
+    for (t = 0; t < ntr; t++)
+        if (rec_id[t] > nrec) nrec = rec_id[t];
+    section = fmatrix(1, ntr, 1, ns);
+    srcpos = fmatrix(1, 7, 1, 1);
+    recpos = imatrix(1, 3, 1, nrec);
+    recpos_loc = imatrix(1, 4, 1, ntr);
+    for (t = 1; t <= ntr; t++) {
+        for (j = 1; j <= ns; j++) section[t][j] = sec[(size_t)(t - 1) * (ns + 1) + j];
+        recpos_loc[4][t] = rec_id[t - 1];
+        for (j = 0; j < 3; j++) recpos[j + 1][rec_id[t - 1]] = rec_xyz[3 * (t - 1) + j];
+    }
+    for (j = 1; j <= 7; j++) srcpos[j][1] = 0.0f;
+    if (src_xyz)
+        for (j = 1; j <= 3; j++) srcpos[j][1] = src_xyz[j - 1];
+    DX = dx; DY = dy; DZ = dz; DT = dt; NDT = ndt;
+    outseis(stderr, fopen(path, "w"), comp, section, recpos, recpos_loc, ntr, srcpos, src_xyz ? 1 : 0, ns, 1);
+    free_matrix(section, 1, ntr, 1, ns);
+    free_matrix(srcpos, 1, 7, 1, 1);
+    free_imatrix(recpos, 1, 3, 1, nrec);
+    free_imatrix(recpos_loc, 1, 4, 1, ntr);
+}
+
+/* readseis.c for component comp of shot ishot from the file
+ * "<prefix>_v?.su.shot<ishot>.0" (RUN_MULTIPLE_SHOTS = 0, MYID = 0) into
+ * out [ntr][ns+1] by sample */
+void rg_readseis(const char *prefix, int ishot, int comp, int ntr, int ns, float *out)
+{
+    int t, j;
+    float **section = fmatrix(1, ntr, 1, ns), **sectionf = fmatrix(1, ntr, 1, ns);
+    snprintf(SEIS_OBS_FILE, STRING_SIZE, "%s", prefix);
+    RUN_MULTIPLE_SHOTS = 0;
+    MYID = 0;
+    if (!FP) FP = stderr;
+    readseis(ishot, section, sectionf, ntr, ns, comp);
+    for (t = 1; t <= ntr; t++)
+        for (j = 1; j <= ns; j++) out[(size_t)(t - 1) * (ns + 1) + j] = section[t][j];
+    free_matrix(section, 1, ntr, 1, ns);
+    free_matrix(sectionf, 1, ntr, 1, ns);
+}

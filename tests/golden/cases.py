@@ -139,3 +139,21 @@ def visco_inputs(g, seed):
     taus = det_uniform(g.shape, seed + 410, 0.01, 0.06)
     taup = det_uniform(g.shape, seed + 411, 0.01, 0.06)
     return f, model, psi, r, taus, taup
+
+
+# SU seismogram files (outseis.c / readseis.c): (ntr, ns, receiver ids,
+# source position or None for nsrc != 1, dx, dy, dz, dt, ndt)
+SU_CASES = (
+    (4, 9, (3, 7, 8, 12), (12.3, 4.56, 30.1), 0.8, 1.25, 0.7, 2.7e-4, 2),
+    (3, 5, (1, 2, 3), None, 0.0005, 2.5, 0.125, 1.3e-3, 1),
+)
+
+
+def su_inputs(case, seed):
+    """a section [ntr][ns+1] (index 0 zero) and receiver grid indices [ntr][3]"""
+    ntr, ns = case[0], case[1]
+    rng = np.random.default_rng(seed)
+    sec = np.zeros((ntr, ns + 1), np.float32)
+    sec[:, 1:] = rng.standard_normal((ntr, ns)).astype(np.float32)
+    rec_xyz = rng.integers(1, 60, size=(ntr, 3)).astype(np.int32)
+    return sec, rec_xyz

@@ -135,6 +135,26 @@ def main():
     r.call("hess_F", cs.HESS_NTR)
     C["hess"] = {f"hess{n + 1}": rec(g.interior(r.get(f"grad{n + 1}"))) for n in range(3)}
 
+    # g10: SU seismogram files (outseis.c SEIS_FORMAT 1, readseis.c); the
+    # header words outseis.c leaves unassigned are zeroed before storing
+    import tempfile
+    from suio_oracle import mask_unassigned
+    for n, case in enumerate(cs.SU_CASES):
+        ntr, ns, rid, src, dx, dy, dz, dt, ndt = case
+        sec, rxyz = cs.su_inputs(case, 100 + n)
+        with tempfile.TemporaryDirectory() as td:
+            prefix = os.path.join(td, "obs")
+            path = f"{prefix}_vy.su.shot1.0"
+            Ref.outseis(path, 2, sec, rxyz, rid, src, dx, dy, dz, dt, ndt)
+            data = open(path, "rb").read()
+            full = Ref.readseis(prefix, 1, 2, ntr, ns)
+            cut = 240 + 4 * ns + 240 + 4 * (ns // 2)  # second trace ends mid-way
+            with open(path, "r+b") as fh:
+                fh.truncate(cut)
+            short = Ref.readseis(prefix, 1, 2, ntr, ns)
+        C[f"su_{n}"] = {"bytes": mask_unassigned(data, ns).hex(), "read": rec(full), "read_short": rec(short),
+                        "short_len": cut}
+
     # g5: residual
     g = cs.dft_grid()
     ntr = 5

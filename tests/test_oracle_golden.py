@@ -181,3 +181,18 @@ def test_visco_stress(gold, fdc, fs):
         check(want[name], g.interior(r6[n]), name)
     for n in range(9):
         check(want[f"psi{n}"], psv[n], f"psi{n}")
+
+
+@pytest.mark.parametrize("n", range(len(cs.SU_CASES)))
+def test_su_io(gold, n):
+    """outseis.c (SU) bytes and readseis.c samples, full and short file"""
+    from suio_oracle import su_read, su_write
+    ntr, ns, rid, src, dx, dy, dz, dt, ndt = cs.SU_CASES[n]
+    sec, rxyz = cs.su_inputs(cs.SU_CASES[n], 100 + n)
+    want = bytes.fromhex(gold[f"su_{n}"]["bytes"])
+    got = su_write(sec, rxyz, dx, dy, dz, dt, ndt, rec_id=rid, src_xyz=src)
+    assert len(got) == len(want) == ntr * (240 + 4 * ns)
+    bad = [i for i in range(len(want)) if got[i] != want[i]]
+    assert not bad, f"SU bytes differ at {bad[:8]}"
+    check(gold[f"su_{n}"]["read"], su_read(want, ntr, ns), "readseis")
+    check(gold[f"su_{n}"]["read_short"], su_read(want[:gold[f"su_{n}"]["short_len"]], ntr, ns), "short readseis")
