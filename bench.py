@@ -205,6 +205,116 @@ def run_batched(w, G, d_start, obs, steps, warmup, precision, S, chunk, use_grap
     return dict(elapsed=elapsed, shots=nshots, prof=prof, l2=l2, gnorm=gnorm, tune=tune, fused=fused_used)
 
 
+def run_levels(args):
+    """the uneven grid-level workload: forward solves of three levels (32
+    shots each) in one ifos_levels_run call, against the same solves level by
+    level through ifos_batch_run and against plain one-shot launches (a
+    bounded sample of shots per level, extrapolated to all of them)"""
+    import ifosgpu as ig
+    import workloads as wl
+    levels = wl.levels3(args.level_shots)
+    models = wl.level_models(levels)
+    ctxs, geoms = [], []
+    for w, m in zip(levels, models):
+        G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+        b = ig.BatchContext(G, len(w.shots), 1, len(w.receivers), ig.DFT_COMPAT, 1)
+        b.set_cpml(w.vppml, w.fpml)
+        b.set_model(*[G.to_dev(a) for a in m])
+        tune = None
+        if not args.no_autotune:
+            bv, bs, _ = b.autotune(reps=2)
+            tune = {"v": bv, "s": bs}
+        log(f"{w.name}: {w.nx}^3, dh {w.dh}, dt {w.dt:.4e}, NT {w.nt}, {len(w.shots)} shots, tiles {tune}")
+        ctxs.append(b)
+        geoms.append((G, tune))
+    srcs = [w.shots for w in levels]
+    recs = [w.receivers for w in levels]
+    d_syn = [ig.DBuf((len(w.shots), 3, len(w.receivers), w.nt + 1)) for w in levels]
+    solves = sum(len(w.shots) for w in levels)
+    cell_steps = sum(w.cells * w.nt * len(w.shots) for w in levels)
+    for n in range(args.warmup):
+        ig.levels_run(ctxs, srcs, recs, d_syn=d_syn, use_graph=True)
+        log(f"warmup {n + 1}/{args.warmup} done")
+    ig.sync()
+    t0 = time.perf_counter()
+    for n in range(args.steps):
+        ig.levels_run(ctxs, srcs, recs, d_syn=d_syn, use_graph=True)
+        log(f"timed levels run {n + 1}/{args.steps}")
+    ig.sync()
+    el = time.perf_counter() - t0
+    together = solves * args.steps / el
+    log(f"levels together (one graph): {together:.3f} solves/s")
+    # the same solves level after level, each level's batch alone (graph replay)
+    for b, w, d in zip(ctxs, levels, d_syn):
+        b.run(w.shots, w.receivers, d_syn=d, want_adjoint=False, use_graph=True)
+    ig.sync()
+    t0 = time.perf_counter()
+    for b, w, d in zip(ctxs, levels, d_syn):
+        b.run(w.shots, w.receivers, d_syn=d, want_adjoint=False, use_graph=True)
+    ig.sync()
+    seq = solves / (time.perf_counter() - t0)
+    log(f"levels one after another: {seq:.3f} solves/s")
+    for b in ctxs:
+        b.close()
+    # plain one-shot launches: the first shots of every level
+    plain_t, plain_n = 0.0, 0
+    for w, (G, _), m in zip(levels, geoms, models):
+        ctx = ig.ShotContext(G, 1, len(w.receivers), ig.DFT_COMPAT)
+        ctx.set_cpml(w.vppml, w.fpml)
+        ctx.set_freqs(w.finv, w.tast)
+        ctx.set_model(*[G.to_dev(a) for a in m])
+        d = ig.DBuf((3, len(w.receivers), w.nt + 1))
+        ctx.run_shot(w.shots[0], w.receivers, d, None, want_adjoint=False)
+        ig.sync()
+        t0 = time.perf_counter()
+        for src in w.shots[:args.plain_shots]:
+            ctx.run_shot(src, w.receivers, d, None, want_adjoint=False)
+        ig.sync()
+        dt_ = time.perf_counter() - t0
+        plain_t += dt_ * len(w.shots) / args.plain_shots
+        plain_n += len(w.shots)
+        log(f"plain {w.name}: {args.plain_shots / dt_:.3f} solves/s")
+        ctx.close()
+    plain = plain_n / plain_t
+    # stencil bytes of the whole run: 140 B per cell and shot-step (v 60, sigma 80)
+    # less the model share (32 B) the shots of a batch read together, + CPML
+    abytes = 0.0
+    for w in levels:
+        _, _, axes = slab_stats(w)
+        S = len(w.shots)
+        abytes += ((108 * w.cells + 48 * axes) * S + 32 * w.cells) * w.nt
+    achieved = abytes / (el / args.steps) / 1e9
+    desc = "; ".join(f"{w.extra['interior']}^3+{w.fw} CPML at dh {w.dh} m, NT {w.nt}" for w in levels)
+    out = {
+        "metric": "forward solves/s, 3 uneven grid levels x 32 shots in one graph launch",
+        "value": round(together, 4),
+        "unit": "forward solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * el / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"levels3: {desc}; {args.level_shots} explosive Ricker shots per level "
+                               f"(fc 5-20 Hz), 64 receivers; elastic FP32 4th order, forward only",
+                   "path": "ifos_levels_run: one hipGraph, one branch per level, autotuned tiles per level"},
+        "gcell_updates_per_s": round(cell_steps * args.steps / el / 1e9, 3),
+        "levels_sequential_solves_per_s": round(seq, 4),
+        "plain_solves_per_s": round(plain, 4),
+        "plain_sample": f"{args.plain_shots} shots per level, extrapolated to {plain_n}",
+        "together_over_plain": round(together / plain, 3),
+        "roofline": {"kernel": "all (concurrent levels)", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "autotune": [t for _, t in geoms],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_plain(w, G, d_start, obs, shots, precision):
     """the one-shot-per-launch path on the first `shots` shot positions"""
     import ifosgpu as ig
@@ -328,7 +438,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed batches (batched) or shots (plain)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84"])
+    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84", "levels3"])
+    ap.add_argument("--level-shots", type=int, default=32, help="shots per grid level (levels3)")
     ap.add_argument("--path", default="batched", choices=["batched", "plain"])
     ap.add_argument("--shots-per-batch", type=int, default=16)
     ap.add_argument("--chunk", type=int, default=8, help="DFT samples per accumulator pass")
@@ -365,6 +476,9 @@ def main():
     log(f"device: {ig.device_name()}")
     copy = copy_probe()
     log(f"copy probe: {copy}")
+    if args.config == "levels3":
+        run_levels(args)
+        return
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
     if args.nt > 0:
         w.nt = args.nt

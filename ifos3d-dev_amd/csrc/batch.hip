@@ -26,6 +26,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -316,8 +317,19 @@ struct ifos_batch {
     float *r[6], *taus, *taup, *tavg[3];
     hipStream_t st;
     std::vector<void *> allocs;
-    std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
+    // captured time loops with the buffer set (cur_set) each leaves current
+    std::map<std::tuple<int, int, int, int, int, int, int>, std::pair<hipGraphExec_t, int>> graphs;
+    // identity of this context (never reused) and of its launch state
+    // (bumped whenever cached graphs are dropped): the key of multi-level
+    // graphs that include this context
+    long long gen, ver;
+    // the multi-level forward graph last captured with this context first
+    hipGraphExec_t lv_exec;
+    std::vector<long long> lv_key;
+    std::vector<int> lv_sets;
 };
+
+static std::atomic<long long> g_batch_gen{0};
 
 static int balloc(ifos_batch *b, void **p, size_t bytes)
 {
@@ -333,8 +345,12 @@ static int balloc(ifos_batch *b, void **p, size_t bytes)
 
 static void drop_graphs(ifos_batch *b)
 {
-    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second.first);
     b->graphs.clear();
+    if (b->lv_exec) (void)hipGraphExecDestroy(b->lv_exec);
+    b->lv_exec = nullptr;
+    b->lv_key.clear();
+    ++b->ver;
 }
 
 extern "C" int ifos_batch_destroy(ifos_batch *b)
@@ -432,6 +448,9 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->cur_set = 0;
     b->last_fused = 0;
     b->tune_fused_n = 0;
+    b->gen = ++g_batch_gen;
+    b->ver = 0;
+    b->lv_exec = nullptr;
     b->L = 0;
     b->eta = 0.0f;
     for (int n = 0; n < 6; ++n) b->r[n] = nullptr;
@@ -836,9 +855,10 @@ static int run_pass(ifos_batch *b, int back, int ns, int dft, int seis, int use_
         rc = hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         (void)hipGraphDestroy(graph);
         if (rc) return rc;
-        it = b->graphs.emplace(key, exec).first;
+        it = b->graphs.emplace(key, std::make_pair(exec, b->cur_set)).first;
     }
-    return hip_check(hipGraphLaunch(it->second, st), "hipGraphLaunch");
+    b->cur_set = it->second.second;  // the replayed loop ends in the set the captured one did
+    return hip_check(hipGraphLaunch(it->second.first, st), "hipGraphLaunch");
 }
 
 static int check_probe(ifos_batch *b, int ns, const char *what)
@@ -863,11 +883,12 @@ static int check_probe(ifos_batch *b, int ns, const char *what)
     return IFOS_OK;
 }
 
-extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, float *d_syn,
-                              float *const d_grad[3], double *l2, int use_graph)
+// checks the shots of one batch, uploads their sources and receivers and
+// zeroes the state they start from (ifos3d.c:727-739)
+static int batch_prepare(ifos_batch *b, const ifos_shot *shots, int ns, const char *who)
 {
     if (!b || !shots || ns < 1 || ns > b->S) {
-        set_error("ifos_batch_run: need 1..%d shots", b ? b->S : 0);
+        set_error("%s: need 1..%d shots", who, b ? b->S : 0);
         return IFOS_EINVAL;
     }
     const ifos_geom *g = &b->g;
@@ -876,31 +897,31 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
     for (int s = 0; s < ns; ++s) {
         const ifos_shot &sh = shots[s];
         if (sh.want_adjoint != want_adj || sh.ntr != ntr) {
-            set_error("ifos_batch_run: shots of one batch need the same ntr and want_adjoint");
+            set_error("%s: shots of one batch need the same ntr and want_adjoint", who);
             return IFOS_EINVAL;
         }
         if (sh.ntr > b->ntr_max) {
-            set_error("ifos_batch_run: ntr=%d > ntr_max=%d", sh.ntr, b->ntr_max);
+            set_error("%s: ntr=%d > ntr_max=%d", who, sh.ntr, b->ntr_max);
             return IFOS_EINVAL;
         }
         if (sh.src_i < 1 || sh.src_i > g->nx || sh.src_j < 1 || sh.src_j > g->ny || sh.src_k < 1 || sh.src_k > g->nz) {
-            set_error("ifos_batch_run: source of shot %d outside the grid", s);
+            set_error("%s: source of shot %d outside the grid", who, s);
             return IFOS_EINVAL;
         }
         for (int r = 0; r < sh.ntr; ++r) {
             const int *p = sh.rec_ijk + 3 * r;
             if (p[0] < 1 || p[0] > g->nx || p[1] < 1 || p[1] > g->ny || p[2] < 1 || p[2] > g->nz) {
-                set_error("ifos_batch_run: receiver %d of shot %d outside the grid", r, s);
+                set_error("%s: receiver %d of shot %d outside the grid", who, r, s);
                 return IFOS_EINVAL;
             }
         }
         if (want_adj && !sh.d_obs) {
-            set_error("ifos_batch_run: shot %d has no observed data", s);
+            set_error("%s: shot %d has no observed data", who, s);
             return IFOS_EINVAL;
         }
     }
     if (want_adj && b->nf < 1) {
-        set_error("ifos_batch_run: frequencies not set");
+        set_error("%s: frequencies not set", who);
         return IFOS_EINVAL;
     }
     hipStream_t st = b->st;
@@ -956,7 +977,21 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
     const size_t synshot = 3 * (size_t)ntr * (g->nt + 1);
     if (!rc) rc = fill(b->syn, (size_t)ns * synshot, st);
     if (!rc) rc = hip_check(hipStreamSynchronize(st), "batch uploads");
+    return rc;
+}
+
+extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, float *d_syn,
+                              float *const d_grad[3], double *l2, int use_graph)
+{
+    int rc = batch_prepare(b, shots, ns, "ifos_batch_run");
     if (rc) return rc;
+    const ifos_geom *g = &b->g;
+    hipStream_t st = b->st;
+    const int want_adj = shots[0].want_adjoint;
+    const int ntr = shots[0].ntr;
+    const size_t nt2 = g->nt + 2;
+    const size_t fshot = (size_t)b->nf_max * b->fe;
+    const size_t synshot = 3 * (size_t)ntr * (g->nt + 1);
 
     rc = run_pass(b, 0, ns, want_adj, 1, use_graph);
     if (!rc) rc = check_probe(b, ns, "forward");
@@ -987,6 +1022,132 @@ extern "C" int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns, flo
     }
     if (d_grad) rc = launch_gradient_shots(g, Ff, Fb, (int64_t)fshot, ns, &m, b->finv, b->nf, d_grad, st);
     if (!rc) rc = hip_check(hipStreamSynchronize(st), "batch gradient");
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// grid levels of uneven size run together: each level is a batch context of
+// its own grid, and the forward time loops of all levels are enqueued on the
+// levels' own streams, or captured once into one graph whose branches are
+// those loops (fork from the first level's stream, join back into it) and
+// replayed.  A level's shots give the same seismograms as ifos_batch_run on
+// that level alone.
+static int level_pass(ifos_batch *b, int ns, hipStream_t s)
+{
+    const bool fz = fused_ok(b);
+    b->last_fused = fz;
+    return fz ? bpass_fused(b, 0, ns, 0, 1, s) : bpass(b, 0, ns, 0, 1, s);
+}
+
+// captures the forward loops of all levels into one graph
+static int capture_levels(ifos_batch *const *lv, int nlev, const int *ns, hipGraphExec_t *exec)
+{
+    hipStream_t s0 = lv[0]->st;
+    std::vector<hipEvent_t> ev(nlev, nullptr);
+    int rc = IFOS_OK;
+    for (int l = 0; l < nlev && !rc; ++l)
+        rc = hip_check(hipEventCreateWithFlags(&ev[l], hipEventDisableTiming), "hipEventCreate");
+    if (rc) {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        return rc;
+    }
+    rc = hip_check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc) {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        return rc;
+    }
+    // fork: every other level's stream joins the capture behind s0
+    int rc2 = hip_check(hipEventRecord(ev[0], s0), "fork");
+    std::vector<bool> joined(nlev, false);
+    for (int l = 1; l < nlev && !rc2; ++l) {
+        rc2 = hip_check(hipStreamWaitEvent(lv[l]->st, ev[0], 0), "fork wait");
+        joined[l] = !rc2;
+    }
+    for (int l = 0; l < nlev && !rc2; ++l) rc2 = level_pass(lv[l], ns[l], lv[l]->st);
+    // join every stream that entered the capture, also after an error, so
+    // that the capture can end
+    for (int l = 1; l < nlev; ++l)
+        if (joined[l]) {
+            int r = hip_check(hipEventRecord(ev[l], lv[l]->st), "join");
+            if (!r) r = hip_check(hipStreamWaitEvent(s0, ev[l], 0), "join wait");
+            if (!rc2) rc2 = r;
+        }
+    hipGraph_t graph = nullptr;
+    rc = hip_check(hipStreamEndCapture(s0, &graph), "hipStreamEndCapture");
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (!rc && !rc2) rc = hip_check(hipGraphInstantiate(exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc2 ? rc2 : rc;
+}
+
+extern "C" int ifos_levels_run(ifos_batch *const *lv, int nlev, const ifos_shot *const *shots, const int *ns,
+                               float *const *d_syn, int use_graph)
+{
+    if (!lv || !shots || !ns || nlev < 1 || nlev > IFOS_MAX_LEVELS) {
+        set_error("ifos_levels_run: need 1..%d levels", IFOS_MAX_LEVELS);
+        return IFOS_EINVAL;
+    }
+    for (int l = 0; l < nlev; ++l) {
+        if (!lv[l] || !shots[l]) {
+            set_error("ifos_levels_run: level %d has no context or no shots", l);
+            return IFOS_EINVAL;
+        }
+        for (int m = 0; m < l; ++m)
+            if (lv[m] == lv[l]) {
+                set_error("ifos_levels_run: levels %d and %d share one context", m, l);
+                return IFOS_EINVAL;
+            }
+        for (int s = 0; s < ns[l] && s < lv[l]->S; ++s)
+            if (shots[l][s].want_adjoint) {
+                set_error("ifos_levels_run: forward solves only (shot %d of level %d asks for the adjoint)", s, l);
+                return IFOS_EINVAL;
+            }
+    }
+    int rc = IFOS_OK;
+    for (int l = 0; l < nlev && !rc; ++l) rc = batch_prepare(lv[l], shots[l], ns[l], "ifos_levels_run");
+    if (rc) return rc;
+    if (!use_graph) {
+        for (int l = 0; l < nlev && !rc; ++l) rc = level_pass(lv[l], ns[l], lv[l]->st);
+        for (int l = 0; l < nlev; ++l) {
+            int r = hip_check(hipStreamSynchronize(lv[l]->st), "levels forward");
+            if (!rc) rc = r;
+        }
+    } else {
+        std::vector<long long> key;
+        for (int l = 0; l < nlev; ++l) {
+            const ifos_batch *b = lv[l];
+            key.insert(key.end(), {b->gen, b->ver, ns[l], b->ntr, (long long)b->any_slab_src,
+                                   (long long)b->any_slab_rec, (long long)fused_ok(b)});
+        }
+        ifos_batch *b0 = lv[0];
+        if (!b0->lv_exec || b0->lv_key != key) {
+            if (b0->lv_exec) (void)hipGraphExecDestroy(b0->lv_exec);
+            b0->lv_exec = nullptr;
+            b0->lv_key.clear();
+            hipGraphExec_t exec = nullptr;
+            rc = capture_levels(lv, nlev, ns, &exec);
+            if (rc) return rc;
+            b0->lv_exec = exec;
+            b0->lv_key = key;
+            b0->lv_sets.resize(nlev);
+            for (int l = 0; l < nlev; ++l) b0->lv_sets[l] = lv[l]->cur_set;
+        }
+        for (int l = 0; l < nlev; ++l) lv[l]->cur_set = b0->lv_sets[l];
+        rc = hip_check(hipGraphLaunch(b0->lv_exec, b0->st), "hipGraphLaunch");
+        if (!rc) rc = hip_check(hipStreamSynchronize(b0->st), "levels forward");
+    }
+    for (int l = 0; l < nlev && !rc; ++l) {
+        ifos_batch *b = lv[l];
+        rc = check_probe(b, ns[l], "forward");
+        if (!rc && d_syn && d_syn[l]) {
+            const size_t synshot = 3 * (size_t)b->ntr * (b->g.nt + 1);
+            rc = hip_check(hipMemcpyAsync(d_syn[l], b->syn, (size_t)ns[l] * synshot * 4, hipMemcpyDeviceToDevice,
+                                          b->st),
+                           "copy seismograms");
+        }
+        if (!rc) rc = hip_check(hipStreamSynchronize(b->st), "levels seismograms");
+    }
     return rc;
 }
 

@@ -584,6 +584,14 @@ class BatchContext:
         d_syn: device buffer [ns][3][ntr][nt+1] or None.
         Returns the per-shot L2 misfits (FP64)."""
         ns = len(srcs)
+        shots, recs = self._shots(srcs, rec_ijk, d_obs, shape, want_adjoint)
+        l2 = (C.c_double * ns)()
+        Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads]) if grads is not None else None
+        check(lib().ifos_batch_run(self.h, shots, ns, _ptr(d_syn), Gp, l2, int(use_graph)), "ifos_batch_run")
+        return list(l2)
+
+    def _shots(self, srcs, rec_ijk, d_obs=None, shape=1, want_adjoint=True):
+        ns = len(srcs)
         recs = rec_ijk if isinstance(rec_ijk, (list, tuple)) else [rec_ijk] * ns
         recs = [np.ascontiguousarray(r, np.int32) for r in recs]
         shots = (IfosShot * ns)()
@@ -591,10 +599,7 @@ class BatchContext:
             shots[n] = IfosShot(src[0], src[1], src[2], src[6], src[3], src[4], src[5], shape, recs[n].shape[0],
                                 recs[n].ctypes.data, d_obs[n].data_ptr() if d_obs is not None else None,
                                 int(want_adjoint))
-        l2 = (C.c_double * ns)()
-        Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads]) if grads is not None else None
-        check(lib().ifos_batch_run(self.h, shots, ns, _ptr(d_syn), Gp, l2, int(use_graph)), "ifos_batch_run")
-        return list(l2)
+        return shots, recs
 
     def dft_ptrs(self, shot, back):
         arr = (C.c_void_p * 6)()
@@ -642,3 +647,24 @@ class BatchContext:
         self.fused_table = [dict(ti=int(buf[3 * c]), jc=int(buf[3 * c + 1]), ms=round(buf[3 * c + 2], 4))
                             for c in range(nfz)]
         return cands[bv.value], cands[bs.value], table
+
+
+def levels_run(levels, srcs, rec_ijk, d_syn=None, shape=1, use_graph=True):
+    """ifos_levels_run: forward solves of several grid levels together.
+    levels: BatchContexts (one grid each); srcs[l], rec_ijk[l]: the shots and
+    receivers of level l (as BatchContext.run takes them); d_syn[l]: device
+    buffer [ns][3][ntr][nt+1] or None."""
+    nlev = len(levels)
+    keep = []
+    shot_arrs = (C.POINTER(IfosShot) * nlev)()
+    ns = (C.c_int * nlev)()
+    for l, b in enumerate(levels):
+        shots, recs = b._shots(srcs[l], rec_ijk[l], None, shape, False)
+        keep.append((shots, recs))
+        shot_arrs[l] = C.cast(shots, C.POINTER(IfosShot))
+        ns[l] = len(srcs[l])
+    lv = (C.c_void_p * nlev)(*[b.h.value for b in levels])
+    sy = None
+    if d_syn is not None:
+        sy = (C.c_void_p * nlev)(*[t.data_ptr() if t is not None else None for t in d_syn])
+    check(lib().ifos_levels_run(lv, nlev, shot_arrs, ns, sy, int(use_graph)), "ifos_levels_run")

@@ -12,6 +12,14 @@ receivers, time step), in the canonical/device layouts of ifosgpu.
            16 explosive Ricker shots (10 Hz) on a 4x4 grid, 1024 receivers
            (32x32) near the top, 6 DFT frequencies 4-14 Hz.
 
+* levels3:  three coarse-to-fine grid levels of one 768 m cube, interiors
+           96^3 at dh 8 m, 128^3 at 6 m and 192^3 at 4 m, each + 10-cell
+           CPML, dt = 0.9 CFL per level and NT = ceil(0.6 s / dt); the
+           seeded random field (as grad276) built on the finest level and
+           sampled at each level's cell positions; 32 explosive Ricker shots
+           (seed 4, fc uniform in 5-20 Hz) at the same physical positions on
+           every level; 64 receivers (8x8) near the top.  Forward solves.
+
 The CFL limit is the reference's (checkfd_ssg.c:473-510):
 dt <= dh / (gamma * sqrt(3) * vmax), gamma = 1.184614 at 4th order.
 """
@@ -70,6 +78,51 @@ def grad276() -> Workload:
     finv = [4.0, 6.0, 8.0, 10.0, 12.0, 14.0]
     return Workload("grad276", n, n, n, fw, dh, float(np.float32(dt)), 1000, finv, 100, 5500.0, 10.0, shots, rec,
                     seed=5)
+
+
+LEVELS3 = ((96, 8.0), (128, 6.0), (192, 4.0))  # (interior cells per axis, dh), one 768 m cube
+
+
+def levels3(nshots: int = 32) -> list:
+    """the three grid levels of one survey (see the module docstring)"""
+    fw, side, tmax = 10, 768.0, 0.6
+    rng = np.random.default_rng(4)
+    # physical shot positions (x, depth, z) and frequencies, shared by the levels
+    pos = np.column_stack([rng.uniform(0.1 * side, 0.9 * side, nshots), rng.uniform(40.0, 120.0, nshots),
+                           rng.uniform(0.1 * side, 0.9 * side, nshots)])
+    fcs = rng.uniform(5.0, 20.0, nshots)
+    out = []
+    for n, dh in LEVELS3:
+        dt = float(np.float32(cfl_dt(dh, 5500.0)))
+        nt = int(math.ceil(tmax / dt))
+        idx = lambda x: fw + 1 + int(round(x / dh))
+        shots = [(idx(p[0]), idx(p[1]), idx(p[2]), 0.0, float(f), 1.0, 1) for p, f in zip(pos, fcs)]
+        rec = np.array([[idx(side * (a + 0.5) / 8), fw + 1, idx(side * (b + 0.5) / 8)]
+                        for a in range(8) for b in range(8)], np.int32)
+        out.append(Workload(f"levels3_{n}", n + 2 * fw, n + 2 * fw, n + 2 * fw, fw, dh, dt, nt, [10.0], 100,
+                            5500.0, 10.0, shots, rec, seed=4, extra={"interior": n}))
+    return out
+
+
+def level_models(levels: list):
+    """true models (rho, pi, u) of each level: the random field of the finest
+    level sampled at every level's cell positions"""
+    fine = levels[-1]
+    shape = (fine.ny + 5, fine.nx + 5, fine.nz + 5)
+    vp_f = _minmax(_grf(shape, 160.0 / fine.dh, fine.seed), 2500.0, 5500.0)
+    ratio_f = _minmax(_grf(shape, 160.0 / fine.dh, fine.seed + 1000), 1.7, 2.0)
+    out = []
+    for w in levels:
+        # canonical index c (grid index c - 2) of this level -> nearest canonical fine index
+        def fidx(m, w=w):
+            x = (np.arange(m) - 2 - w.fw - 1) * w.dh
+            return np.clip(np.rint(x / fine.dh).astype(int) + fine.fw + 1 + 2, 0, shape[0] - 1)
+        j, i, k = fidx(w.ny + 5), fidx(w.nx + 5), fidx(w.nz + 5)
+        vp = vp_f[np.ix_(j, i, k)]
+        vs = vp / ratio_f[np.ix_(j, i, k)]
+        rho = 310.0 * vp ** 0.25
+        out.append([np.ascontiguousarray(t, np.float32) for t in (rho, rho * vp * vp, rho * vs * vs)])
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -313,6 +313,19 @@ int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const float *d_taup
 int ifos_batch_run(ifos_batch *b, const ifos_shot *shots, int ns,
                    float *d_syn, float *const d_grad[3], double *l2,
                    int use_graph);
+/* grid levels of uneven size (no counterpart in the reference, whose
+ * coarse-to-fine levels run one after the other): forward solves of nlev
+ * batch contexts, each on its own grid, run together.  Level l runs ns[l]
+ * shots shots[l] (want_adjoint = 0) and, when d_syn and d_syn[l] are not
+ * NULL, writes their seismograms to d_syn[l] [ns[l]][3][ntr][nt+1] as
+ * ifos_batch_run on that context alone would, bit for bit.  The levels'
+ * time loops are enqueued on the contexts' own streams (use_graph = 0), or
+ * captured once into one graph with one branch per level and replayed
+ * (use_graph = 1; the graph is kept by lv[0] until a context's launch
+ * state changes).  Synchronous; every context appears once. */
+#define IFOS_MAX_LEVELS 8
+int ifos_levels_run(ifos_batch *const *lv, int nlev, const ifos_shot *const *shots, const int *ns,
+                    float *const *d_syn, int use_graph);
 /* tile shapes of the velocity and stress march launches, each
  * {TI, jc, group, split, pipe, kb, sb, lds}: TI wavefronts per workgroup (1, 2, 4, 8);
  * jc j-planes per tile (<= 0: automatic); group: tiles of one shot before
