@@ -390,6 +390,40 @@ __global__ void k_fill(float *d, int64_t n, float v)
     for (; t < n; t += stride) d[t] = v;
 }
 
+// time-domain gradient: running time integrals u += v dt of three
+// components over the whole device layout (halos stay 0: v is 0 there)
+__global__ void k_td_integrate(float *u0, float *u1, float *u2, const float *v0, const float *v1, const float *v2,
+                               int64_t n, float dt)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) {
+        u0[t] = u0[t] + v0[t] * dt;
+        u1[t] = u1[t] + v1[t] * dt;
+        u2[t] = u2[t] + v2[t] * dt;
+    }
+}
+
+// time-domain gradient: the lambda, mu, rho sums to the vp, vs, rho
+// gradients with the parametrisation of gradient_F.c:135-139, in FP64, added
+// to the FP32 gradients
+__global__ __launch_bounds__(256) void k_td_finish(DevGeom g, const double *al, const double *am, const double *ar,
+                                                   const float *rho, const float *pi, const float *u, float *g1,
+                                                   float *g2, float *g3)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = blockIdx.z + 1;
+    if (k > g.nz || i > g.nx) return;
+    const int64_t c = at(g, j, i, k);
+    const double r = rho[c], p = pi[c], m = u[c];
+    const double srp = sqrt(r * p), sru = sqrt(r * m);
+    const double l = al[c], mu = am[c];
+    g1[c] = (float)((double)g1[c] + 2.0 * srp * l);
+    g2[c] = (float)((double)g2[c] + (-4.0 * sru * l + 2.0 * sru * mu));
+    g3[c] = (float)((double)g3[c] + (ar[c] + m / r * mu + (p - 2.0 * m) / r * l));
+}
+
 static dim3 cell_grid(const ifos_geom *g, dim3 &block)
 {
     block = dim3(64, 4, 1);
@@ -671,6 +705,31 @@ extern "C" int ifos_residual(const ifos_geom *g, const float *d_obs, const float
     if (l2) *l2 += acc;
     return IFOS_OK;
 }
+
+namespace ifos {
+int launch_td_integrate(const ifos_geom *g, float *const u[3], const float *const v[3], float dt, hipStream_t s)
+{
+    const size_t n = ifos_field_elems(g);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    { ProfScope ps_(IFOS_K_GRADIENT, s);
+    hipLaunchKernelGGL(k_td_integrate, dim3(blocks), dim3(256), 0, s, u[0], u[1], u[2], v[0], v[1], v[2], (int64_t)n,
+                       dt);
+    }
+    return hip_check(hipGetLastError(), "k_td_integrate");
+}
+
+int launch_td_finish(const ifos_geom *g, double *const acc[3], const ifos_model *m, float *const d_grad[3],
+                     hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    dim3 block, grid = cell_grid(g, block);
+    { ProfScope ps_(IFOS_K_GRADIENT, s);
+    hipLaunchKernelGGL(k_td_finish, grid, block, 0, s, G, acc[0], acc[1], acc[2], m->rho, m->pi, m->u, d_grad[0],
+                       d_grad[1], d_grad[2]);
+    }
+    return hip_check(hipGetLastError(), "k_td_finish");
+}
+}  // namespace ifos
 
 extern "C" int ifos_fill(float *d, size_t n, float value, void *stream)
 {

@@ -1,6 +1,7 @@
 // host.cpp -- host side of libifos3d_hip.so: geometry, error reporting, the
 // CPML profiles and source wavelet (computed once per run on the CPU, with
 // the reference's float/double promotions), and the plain one-shot driver.
+#include <algorithm>
 #include <cmath>
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -358,7 +359,10 @@ struct ifos_ctx {
     float eta;
     float *r[6], *taus, *taup, *tavg[3];
     std::vector<void *> allocs;
+    struct TdBufs *td;  // buffers of the time-domain gradient (ifos_ctx_run_shot_td)
 };
+
+static void td_free(ifos_ctx *c);
 
 static int dalloc(ifos_ctx *c, void **p, size_t bytes)
 {
@@ -377,6 +381,7 @@ extern "C" int ifos_ctx_destroy(ifos_ctx *c)
 {
     if (!c) return IFOS_OK;
     hipDeviceSynchronize();
+    td_free(c);
     for (void *p : c->allocs) hipFree(p);
     delete c;
     return IFOS_OK;
@@ -514,68 +519,105 @@ extern "C" int ifos_ctx_set_freqs(ifos_ctx *c, const float *finv, int nf, int ta
     return IFOS_OK;
 }
 
-// one propagation of NT steps; back = 0 forward (sources), 1 adjoint
-static int propagate(ifos_ctx *c, int back, const ifos_shot *s, int ntast, float *d_syn, hipStream_t st)
+// the state one propagation advances: wavefields, CPML memory variables and
+// (viscoelastic) memory variables
+struct ShotState {
+    float *fields[9], *psi_s[9], *psi_v[9], *r[6];
+};
+
+static ShotState ctx_state(const ifos_ctx *c)
+{
+    ShotState S;
+    for (int n = 0; n < 9; ++n) {
+        S.fields[n] = c->fields[n];
+        S.psi_s[n] = c->psi_s[n];
+        S.psi_v[n] = c->psi_v[n];
+    }
+    for (int n = 0; n < 6; ++n) S.r[n] = c->r[n];
+    return S;
+}
+
+// time step nt of a propagation of state S in the order of ifos3d.c:754-814
+// (back = 0: forward with the source) and ifos3d.c:905-960 (back = 1:
+// adjoint with the receivers injecting): velocity update with its
+// injections, stress update, explosive source, the DFT sample (dft), free
+// surface, seismograms (d_syn, forward only)
+static int step(ifos_ctx *c, const ShotState &S, int back, const ifos_shot *s, int nt, bool dft, int ntast,
+                float *d_syn, hipStream_t st)
 {
     const ifos_geom *g = &c->g;
-    ifos_fields f = ctx_fields(c);
+    ifos_fields f = {S.fields[0], S.fields[1], S.fields[2], S.fields[3], S.fields[4],
+                     S.fields[5], S.fields[6], S.fields[7], S.fields[8]};
     ifos_model m = ctx_model(c);
     ifos_psi ps, pv;
     for (int n = 0; n < 9; ++n) {
-        ps.p[n] = c->psi_s[n];
-        pv.p[n] = c->psi_v[n];
+        ps.p[n] = S.psi_s[n];
+        pv.p[n] = S.psi_v[n];
     }
-    float *Fp[6], *Cp[6];
-    for (int n = 0; n < 6; ++n) {
-        Fp[n] = c->F[(back ? 6 : 0) + n];
-        Cp[n] = c->C[(back ? 6 : 0) + n];
+    int rc = ifos_update_v(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &ps, st);
+    if (!rc) {
+        if (!back)
+            rc = ifos_add_forces(g, &f, &m, c->src, c->sig, 1, nt, 0.0f, 0.0f, st);
+        else
+            rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
     }
-    const bool dft = s->want_adjoint != 0;
+    if (!rc) rc = ifos_update_v(g, IFOS_REGION_CPML, &f, &m, c->prof, &ps, st);
+    if (!rc && back) rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
+    if (c->L) {
+        ifos_visco V;
+        for (int n = 0; n < 6; ++n) V.r[n] = S.r[n];
+        V.taus = c->taus;
+        V.taup = c->taup;
+        V.tausipjp = c->tavg[0];
+        V.tausjpkp = c->tavg[1];
+        V.tausipkp = c->tavg[2];
+        V.eta = c->eta;
+        V.L = c->L;
+        if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_INTERIOR, &f, &m, &V, c->prof, &pv, st);
+        if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_CPML, &f, &m, &V, c->prof, &pv, st);
+    } else {
+        if (!rc) rc = ifos_update_s(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &pv, st);
+        if (!rc) rc = ifos_update_s(g, IFOS_REGION_CPML, &f, &m, c->prof, &pv, st);
+    }
+    if (!rc && !back) rc = ifos_psource(g, &f, c->src, c->sig, 1, nt, st);
+    if (!rc && dft && nt % ntast == 0) {
+        float *Fp[6], *Cp[6];
+        for (int n = 0; n < 6; ++n) {
+            Fp[n] = c->F[(back ? 6 : 0) + n];
+            Cp[n] = c->C[(back ? 6 : 0) + n];
+        }
+        rc = ifos_discfourier(g, &f, Fp, c->prec == IFOS_DFT_KAHAN ? Cp : nullptr, c->finv, c->nf, nt, ntast, back,
+                              c->prec, st);
+    }
+    if (!rc && g->free_surf) rc = ifos_surface(g, &f, &m, st);
+    if (!rc && !back && d_syn) rc = ifos_seismo(g, &f, c->rec, s->ntr, nt, g->nt, d_syn, st);
+    return rc;
+}
+
+// NaN probe of ifos3d.c:761 on vy of state S (the reference checks every
+// step on the host; here every 64 steps)
+static int probe_nan(ifos_ctx *c, const ShotState &S, int nt, hipStream_t st)
+{
+    const ifos_geom *g = &c->g;
     const size_t probe = ifos_offset(g, g->ny / 2, g->nx / 2, g->nz / 2);
-    for (int nt = 1; nt <= g->nt; ++nt) {
-        int rc;
-        // NaN probe every 64 steps (ifos3d.c:761 checks every step on the host)
-        if ((nt & 63) == 0) {
-            float v = 0.0f;
-            rc = hip_check(hipMemcpyAsync(&v, c->fields[1] + probe, 4, hipMemcpyDeviceToHost, st), "probe");
-            if (!rc) rc = hip_check(hipStreamSynchronize(st), "probe sync");
-            if (rc) return rc;
-            if (isnan(v)) {
-                set_error("simulation unstable: NaN at the probe cell (step %d)", nt);
-                return IFOS_ENAN;
-            }
-        }
-        rc = ifos_update_v(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &ps, st);
-        if (!rc) {
-            if (!back)
-                rc = ifos_add_forces(g, &f, &m, c->src, c->sig, 1, nt, 0.0f, 0.0f, st);
-            else
-                rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
-        }
-        if (!rc) rc = ifos_update_v(g, IFOS_REGION_CPML, &f, &m, c->prof, &ps, st);
-        if (!rc && back) rc = ifos_add_adjoint(g, &f, c->rec, c->chain, c->adj, s->ntr, nt, st);
-        if (c->L) {
-            ifos_visco V;
-            for (int n = 0; n < 6; ++n) V.r[n] = c->r[n];
-            V.taus = c->taus;
-            V.taup = c->taup;
-            V.tausipjp = c->tavg[0];
-            V.tausjpkp = c->tavg[1];
-            V.tausipkp = c->tavg[2];
-            V.eta = c->eta;
-            V.L = c->L;
-            if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_INTERIOR, &f, &m, &V, c->prof, &pv, st);
-            if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_CPML, &f, &m, &V, c->prof, &pv, st);
-        } else {
-            if (!rc) rc = ifos_update_s(g, IFOS_REGION_INTERIOR, &f, &m, c->prof, &pv, st);
-            if (!rc) rc = ifos_update_s(g, IFOS_REGION_CPML, &f, &m, c->prof, &pv, st);
-        }
-        if (!rc && !back) rc = ifos_psource(g, &f, c->src, c->sig, 1, nt, st);
-        if (!rc && dft && nt % ntast == 0)
-            rc = ifos_discfourier(g, &f, Fp, c->prec == IFOS_DFT_KAHAN ? Cp : nullptr, c->finv, c->nf, nt, ntast,
-                                  back, c->prec, st);
-        if (!rc && g->free_surf) rc = ifos_surface(g, &f, &m, st);
-        if (!rc && !back && d_syn) rc = ifos_seismo(g, &f, c->rec, s->ntr, nt, g->nt, d_syn, st);
+    float v = 0.0f;
+    int rc = hip_check(hipMemcpyAsync(&v, S.fields[1] + probe, 4, hipMemcpyDeviceToHost, st), "probe");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "probe sync");
+    if (!rc && isnan(v)) {
+        set_error("simulation unstable: NaN at the probe cell (step %d)", nt);
+        rc = IFOS_ENAN;
+    }
+    return rc;
+}
+
+// one propagation of NT steps; back = 0 forward (sources), 1 adjoint
+static int propagate(ifos_ctx *c, int back, const ifos_shot *s, int ntast, float *d_syn, hipStream_t st)
+{
+    const ShotState S = ctx_state(c);
+    const bool dft = s->want_adjoint != 0;
+    for (int nt = 1; nt <= c->g.nt; ++nt) {
+        int rc = (nt & 63) == 0 ? probe_nan(c, S, nt, st) : IFOS_OK;
+        if (!rc) rc = step(c, S, back, s, nt, dft, ntast, d_syn, st);
         if (rc) return rc;
     }
     return IFOS_OK;
@@ -592,29 +634,27 @@ static int zero_wavefield(ifos_ctx *c, hipStream_t st)
     return rc;
 }
 
-extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, float *const d_grad[3],
-                                 double *l2, void *stream)
+// checks a shot and uploads its wavelet, source and receivers
+static int prepare_shot(ifos_ctx *c, const ifos_shot *s, hipStream_t st, const char *who, bool need_freqs)
 {
-    if (!c || !s) return IFOS_EINVAL;
     const ifos_geom *g = &c->g;
-    hipStream_t st = (hipStream_t)stream;
     if (s->ntr > c->ntr_max) {
-        set_error("ifos_ctx_run_shot: ntr=%d > ntr_max=%d", s->ntr, c->ntr_max);
+        set_error("%s: ntr=%d > ntr_max=%d", who, s->ntr, c->ntr_max);
         return IFOS_EINVAL;
     }
-    if (s->want_adjoint && c->nf < 1) {
-        set_error("ifos_ctx_run_shot: frequencies not set");
+    if (need_freqs && s->want_adjoint && c->nf < 1) {
+        set_error("%s: frequencies not set", who);
         return IFOS_EINVAL;
     }
     // bounds of source and receivers
     if (s->src_i < 1 || s->src_i > g->nx || s->src_j < 1 || s->src_j > g->ny || s->src_k < 1 || s->src_k > g->nz) {
-        set_error("ifos_ctx_run_shot: source (%d,%d,%d) outside the grid", s->src_i, s->src_j, s->src_k);
+        set_error("%s: source (%d,%d,%d) outside the grid", who, s->src_i, s->src_j, s->src_k);
         return IFOS_EINVAL;
     }
     for (int r = 0; r < s->ntr; ++r) {
         const int *p = s->rec_ijk + 3 * r;
         if (p[0] < 1 || p[0] > g->nx || p[1] < 1 || p[1] > g->ny || p[2] < 1 || p[2] > g->nz) {
-            set_error("ifos_ctx_run_shot: receiver %d (%d,%d,%d) outside the grid", r, p[0], p[1], p[2]);
+            set_error("%s: receiver %d (%d,%d,%d) outside the grid", who, r, p[0], p[1], p[2]);
             return IFOS_EINVAL;
         }
     }
@@ -633,8 +673,17 @@ extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, 
         rc = hip_check(hipMemcpyAsync(c->chain, chain.data(), 2 * sizeof(int) * s->ntr, hipMemcpyHostToDevice, st),
                        "upload receiver chain");
     if (!rc) rc = hip_check(hipStreamSynchronize(st), "sync uploads");
-    if (rc) return rc;
+    return rc;
+}
 
+extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, float *const d_grad[3],
+                                 double *l2, void *stream)
+{
+    if (!c || !s) return IFOS_EINVAL;
+    const ifos_geom *g = &c->g;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = prepare_shot(c, s, st, "ifos_ctx_run_shot", true);
+    if (rc) return rc;
     const int ntast = s->want_adjoint ? ifos_ntast(c->finv[c->nf - 1], c->tast, g->dt) : 1;
     // ifos3d.c:738-739: zero wavefield and DFT accumulators
     rc = zero_wavefield(c, st);
@@ -659,6 +708,181 @@ extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, 
         Fb[n] = c->F[6 + n];
     }
     return ifos_gradient_F(g, Ff, Fb, &m, c->finv, c->nf, d_grad, st);
+}
+
+// ---------------------------------------------------------------------------
+// time-domain gradient with checkpoints (no counterpart in the reference,
+// whose gradient is the frequency-domain one of gradient_F.c).  The products
+// of gradient_F.c:122-132 are summed over the time samples of the shot
+// instead of over DFT frequencies: lambda and mu from the derivatives of the
+// displacements (the velocities integrated from the end of the record,
+// u(t) = int_t^T v dt', which is where the 1 / (4 pi^2 f^2) of gradient_F
+// comes from), rho from the velocities, each sample weighted by DT.  The
+// forward sample n pairs with the adjoint step m = NT - 1 - n, the pairing
+// of the reference's DFT times (disc_fourier.c:36-37: t = n DT forward,
+// TIME - m DT adjoint, TIME = (NT - 1) DT), for n = 1 .. NT - 2.
+// The forward pass stores its state every K steps; the adjoint pass runs
+// the segments from the last to the first, recomputing each segment's
+// forward velocities from its checkpoint (one extra forward propagation).
+struct TdBufs {
+    int K = 0, nck = 0;
+    std::vector<ShotState> ck;  // state after steps 0, K, 2K, ...
+    ShotState work;             // the recomputed forward state
+    std::vector<float *> seg;   // [K][3] forward velocities of one segment
+    float *uf[3], *ub[3];       // running displacements, forward and adjoint
+    double *acc[3];             // lambda, mu, rho sums
+    std::vector<void *> allocs;
+};
+
+static void td_free(ifos_ctx *c)
+{
+    if (!c->td) return;
+    for (void *p : c->td->allocs) (void)hipFree(p);
+    delete c->td;
+    c->td = nullptr;
+}
+
+static int td_alloc_one(TdBufs *T, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) for the time-domain gradient: %s", bytes, hipGetErrorString(e));
+        return IFOS_ENOMEM;
+    }
+    T->allocs.push_back(*p);
+    return IFOS_OK;
+}
+
+static int td_alloc_state(ifos_ctx *c, TdBufs *T, ShotState &S)
+{
+    int rc = IFOS_OK;
+    for (int n = 0; n < 9 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.fields[n], c->fe * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.psi_s[n], ifos_psi_elems(&c->g, n) * 4);
+    for (int n = 0; n < 9 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.psi_v[n], ifos_psi_elems(&c->g, n) * 4);
+    for (int n = 0; n < 6; ++n) S.r[n] = nullptr;
+    if (c->L)
+        for (int n = 0; n < 6 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.r[n], c->fe * 4);
+    return rc;
+}
+
+static int td_alloc(ifos_ctx *c, int K)
+{
+    const int nck = (c->g.nt + K - 1) / K;
+    if (c->td && c->td->K == K && (c->td->work.r[0] != nullptr) == (c->L != 0)) return IFOS_OK;
+    td_free(c);
+    TdBufs *T = new TdBufs();
+    c->td = T;
+    T->K = K;
+    T->nck = nck;
+    T->ck.resize(nck);
+    int rc = IFOS_OK;
+    for (int q = 0; q < nck && !rc; ++q) rc = td_alloc_state(c, T, T->ck[q]);
+    if (!rc) rc = td_alloc_state(c, T, T->work);
+    T->seg.assign(3 * (size_t)K, nullptr);
+    for (size_t q = 0; q < T->seg.size() && !rc; ++q) rc = td_alloc_one(T, (void **)&T->seg[q], c->fe * 4);
+    for (int n = 0; n < 3 && !rc; ++n) rc = td_alloc_one(T, (void **)&T->uf[n], c->fe * 4);
+    for (int n = 0; n < 3 && !rc; ++n) rc = td_alloc_one(T, (void **)&T->ub[n], c->fe * 4);
+    for (int n = 0; n < 3 && !rc; ++n) rc = td_alloc_one(T, (void **)&T->acc[n], c->fe * 8);
+    if (rc) td_free(c);
+    return rc;
+}
+
+static int copy_state(ifos_ctx *c, const ShotState &dst, const ShotState &src, hipStream_t st)
+{
+    int rc = IFOS_OK;
+    for (int n = 0; n < 9 && !rc; ++n)
+        rc = hip_check(hipMemcpyAsync(dst.fields[n], src.fields[n], c->fe * 4, hipMemcpyDeviceToDevice, st),
+                       "checkpoint copy");
+    for (int n = 0; n < 9 && !rc; ++n) {
+        const size_t b = ifos_psi_elems(&c->g, n) * 4;
+        rc = hip_check(hipMemcpyAsync(dst.psi_s[n], src.psi_s[n], b, hipMemcpyDeviceToDevice, st), "checkpoint copy");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(dst.psi_v[n], src.psi_v[n], b, hipMemcpyDeviceToDevice, st),
+                           "checkpoint copy");
+    }
+    if (c->L)
+        for (int n = 0; n < 6 && !rc; ++n)
+            rc = hip_check(hipMemcpyAsync(dst.r[n], src.r[n], c->fe * 4, hipMemcpyDeviceToDevice, st),
+                           "checkpoint copy");
+    return rc;
+}
+
+extern "C" size_t ifos_td_bytes(const ifos_geom *g, int every, int L)
+{
+    const int K = every > 0 ? std::min(every, g->nt) : std::max(1, (int)lround(sqrt((double)g->nt)));
+    const size_t nck = (size_t)(g->nt + K - 1) / K, fe = ifos_field_elems(g);
+    size_t psi = 0;
+    for (int n = 0; n < 9; ++n) psi += ifos_psi_elems(g, n);
+    const size_t state = (9 + (L ? 6 : 0)) * fe * 4 + 2 * psi * 4;
+    return (nck + 1) * state + 3 * (size_t)K * fe * 4 + 6 * fe * 4 + 3 * fe * 8;
+}
+
+extern "C" int ifos_ctx_run_shot_td(ifos_ctx *c, const ifos_shot *s, int every, float *d_syn,
+                                    float *const d_grad[3], double *l2, void *stream)
+{
+    if (!c || !s || !d_syn || !d_grad) {
+        set_error("ifos_ctx_run_shot_td: needs a shot, the seismogram buffer and the gradients");
+        return IFOS_EINVAL;
+    }
+    const ifos_geom *g = &c->g;
+    const int N = g->nt;
+    hipStream_t st = (hipStream_t)stream;
+    if (!s->want_adjoint || !s->d_obs || N < 3) {
+        set_error("ifos_ctx_run_shot_td: needs want_adjoint, observed data and NT >= 3");
+        return IFOS_EINVAL;
+    }
+    const int K = every > 0 ? std::min(every, N) : std::max(1, (int)lround(sqrt((double)N)));
+    int rc = prepare_shot(c, s, st, "ifos_ctx_run_shot_td", false);
+    if (!rc) rc = td_alloc(c, K);
+    if (rc) return rc;
+    TdBufs *T = c->td;
+    const ShotState S = ctx_state(c);
+    // forward (ifos3d.c:754-814) with the state stored after steps 0, K, 2K, ...
+    rc = zero_wavefield(c, st);
+    if (!rc) rc = ifos_fill(d_syn, 3 * (size_t)s->ntr * (N + 1), 0.0f, st);
+    if (!rc) rc = copy_state(c, T->ck[0], S, st);
+    for (int nt = 1; nt <= N && !rc; ++nt) {
+        if ((nt & 63) == 0) rc = probe_nan(c, S, nt, st);
+        if (!rc) rc = step(c, S, 0, s, nt, false, 1, d_syn, st);
+        if (!rc && nt % K == 0 && nt / K < T->nck) rc = copy_state(c, T->ck[nt / K], S, st);
+    }
+    // residual -> adjoint sources (ifos3d.c:837-866)
+    if (!rc) rc = ifos_fill(c->adj, 3 * (size_t)s->ntr * (N + 2), 0.0f, st);
+    if (!rc) rc = ifos_residual(g, s->d_obs, d_syn, c->adj, s->ntr, N, l2, st);
+    if (!rc) rc = zero_wavefield(c, st);
+    for (int n = 0; n < 3 && !rc; ++n) {
+        rc = ifos_fill(T->uf[n], c->fe, 0.0f, st);
+        if (!rc) rc = ifos_fill(T->ub[n], c->fe, 0.0f, st);
+        if (!rc) rc = ifos_fill((float *)T->acc[n], 2 * c->fe, 0.0f, st);
+    }
+    const float dt = g->dt;
+    const float *vb[3] = {S.fields[0], S.fields[1], S.fields[2]};
+    for (int q = T->nck - 1; q >= 0 && !rc; --q) {
+        const int a = q * K, b = std::min(a + K, N);
+        // recompute the segment's forward velocities from its checkpoint
+        rc = copy_state(c, T->work, T->ck[q], st);
+        for (int n = a + 1; n <= b && !rc; ++n) {
+            rc = step(c, T->work, 0, s, n, false, 1, nullptr, st);
+            for (int d = 0; d < 3 && !rc; ++d)
+                rc = hip_check(hipMemcpyAsync(T->seg[3 * (n - a - 1) + d], T->work.fields[d], c->fe * 4,
+                                              hipMemcpyDeviceToDevice, st),
+                               "segment copy");
+        }
+        // the segment backwards, the adjoint forwards (ifos3d.c:905-960)
+        for (int n = b; n > a && !rc; --n) {
+            const float *vf[3] = {T->seg[3 * (n - a - 1)], T->seg[3 * (n - a - 1) + 1], T->seg[3 * (n - a - 1) + 2]};
+            rc = launch_td_integrate(g, T->uf, vf, dt, st);
+            if (rc || n > N - 2) continue;
+            const int m = N - 1 - n;
+            if ((m & 63) == 0) rc = probe_nan(c, S, m, st);
+            if (!rc) rc = step(c, S, 1, s, m, false, 1, nullptr, st);
+            if (!rc) rc = launch_td_integrate(g, T->ub, vb, dt, st);
+            if (!rc) rc = launch_td_sample(g, T->uf, vf, T->ub, vb, (double)dt, T->acc, st);
+        }
+    }
+    ifos_model mod = ctx_model(c);
+    if (!rc) rc = launch_td_finish(g, T->acc, &mod, d_grad, st);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

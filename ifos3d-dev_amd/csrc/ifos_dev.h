@@ -202,6 +202,14 @@ int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof,
 // hess, hess_F for one frequency of one receiver-side set
 int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, const ifos_model *m,
                       float *const d_grad[3], hipStream_t s, bool hess = false);
+// time-domain gradient (stencil.hip, aux.hip): one time sample of the
+// lambda / mu / rho sums; u += v dt; the sums to the vp, vs, rho gradients
+int launch_td_sample(const ifos_geom *g, const float *const uf[3], const float *const vf[3],
+                     const float *const ub[3], const float *const vb[3], double w, double *const acc[3],
+                     hipStream_t s);
+int launch_td_integrate(const ifos_geom *g, float *const u[3], const float *const v[3], float dt, hipStream_t s);
+int launch_td_finish(const ifos_geom *g, double *const acc[3], const ifos_model *m, float *const d_grad[3],
+                     hipStream_t s);
 void set_error(const char *fmt, ...);
 int hip_check(hipError_t e, const char *what);
 

@@ -646,12 +646,21 @@ struct GradArgs {
     float fd;  // 1 / (4 pi^2 f^2) as gradient_F.c:70-72 rounds it
     const float *rho, *pi, *u;
     float *g1, *g2, *g3;
+    double *acc[3];  // GRAD_TD: lambda, mu and rho sums, one double per cell of the device layout
+    double w;        // GRAD_TD: weight of the time sample (the time step)
 };
 
-// HESS: hess_F.c:28-192 instead, the same fields read as the forward set
-// (f, fi) and one receiver-side set (b, bi), fd = 1 / (2 pi f) as
+// MODE GRAD_HESS: hess_F.c:28-192 instead, the same fields read as the
+// forward set (f, fi) and one receiver-side set (b, bi), fd = 1 / (2 pi f) as
 // hess_F.c:75-77 rounds it, the three Hessian diagonals in place of g1..g3.
-template <int TI, bool HESS>
+// MODE GRAD_TD: one time sample of the time-domain gradient: f[0..2] the
+// forward displacement, f[3..5] the forward velocity, f[6..8] the adjoint
+// displacement, f[9..11] the adjoint velocity; the products of
+// gradient_F.c:122-132 (displacement derivatives for lambda and mu, the
+// velocities for rho, no frequency factor) are added, times w, to acc[0..2].
+enum { GRAD_F = 0, GRAD_HESS = 1, GRAD_TD = 2 };
+
+template <int TI, int MODE>
 __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, GradArgs A)
 {
     int k, i, jb, je, shot;
@@ -683,6 +692,7 @@ __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, Gra
             qx[t][3] = LD(fx, 2, 0, 0);
             qy[t][3] = LD(fy, 1, 0, 0);
             qz[t][3] = LD(fz, 2, 0, 0);
+            if (MODE == GRAD_TD && (t & 1)) continue;  // velocities: cell values only
             const float x_c = qx[t][1], y_c = qy[t][2], z_c = qz[t][1];
             d[t][0] = b1 * (x_c - LD(fx, 0, -1, 0)) + b2 * (LD(fx, 0, 1, 0) - LD(fx, 0, -2, 0));
             d[t][1] = b1 * (qx[t][2] - x_c) + b2 * (qx[t][3] - qx[t][0]);
@@ -697,7 +707,18 @@ __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, Gra
         }
         // d[t] = xx xy xz yx yy yz zx zy zz of f (t 0), fi (1), b (2), bi (3)
         const float *f = d[0], *fi = d[1], *bb = d[2], *bi = d[3];
-        if (HESS) {
+        if (MODE == GRAD_TD) {
+            float gl = (f[0] + f[4] + f[8]) * (bb[0] + bb[4] + bb[8]);
+            gl = -gl;
+            float gm = 2.0f * f[0] * bb[0] + 2.0f * f[4] * bb[4] + 2.0f * f[8] * bb[8] + (f[1] + f[3]) * (bb[1] + bb[3]) +
+                       (f[2] + f[6]) * (bb[2] + bb[6]) + (f[5] + f[7]) * (bb[5] + bb[7]);
+            gm = -gm;
+            const float gr = qx[1][1] * qx[3][1] + qy[1][2] * qy[3][2] + qz[1][1] * qz[3][1];
+            const int64_t c = at(g, j, i, k);
+            A.acc[0][c] += (double)gl * A.w;
+            A.acc[1][c] += (double)gm * A.w;
+            A.acc[2][c] += (double)gr * A.w;
+        } else if (MODE == GRAD_HESS) {
             // hess_F.c:135-182, values of the cell: qx[t][1], qy[t][2], qz[t][1]
             const float fdiv = f[0] + f[4] + f[8], fidiv = fi[0] + fi[4] + fi[8];
             const float bdiv = bb[0] + bb[4] + bb[8], bidiv = bi[0] + bi[4] + bi[8];
@@ -1298,9 +1319,35 @@ int launch_grad_march(const ifos_geom *g, const float *const f[12], float fd, co
     A.g3 = d_grad[2];
     const Box all = region_box(g, IFOS_REGION_ALL);
     const MTiles T = make_tiles(&all, 1, 4, 0, 1, 0);
-    if (hess) hipLaunchKernelGGL((k_grad_march<4, true>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
-    else hipLaunchKernelGGL((k_grad_march<4, false>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
+    if (hess)
+        hipLaunchKernelGGL((k_grad_march<4, GRAD_HESS>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
+    else
+        hipLaunchKernelGGL((k_grad_march<4, GRAD_F>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
     return hip_check(hipGetLastError(), "k_grad_march");
+}
+
+// one time sample of the time-domain gradient over [1..N]^3: uf, vf forward
+// displacement and velocity, ub, vb adjoint displacement and velocity
+int launch_td_sample(const ifos_geom *g, const float *const uf[3], const float *const vf[3],
+                     const float *const ub[3], const float *const vb[3], double w, double *const acc[3],
+                     hipStream_t s)
+{
+    DevGeom G = make_dev_geom(g);
+    GradArgs A = {};
+    for (int n = 0; n < 3; ++n) {
+        A.f[n] = uf[n];
+        A.f[3 + n] = vf[n];
+        A.f[6 + n] = ub[n];
+        A.f[9 + n] = vb[n];
+        A.acc[n] = acc[n];
+    }
+    A.w = w;
+    const Box all = region_box(g, IFOS_REGION_ALL);
+    const MTiles T = make_tiles(&all, 1, 4, 0, 1, 0);
+    { ProfScope ps_(IFOS_K_GRADIENT, s);
+    hipLaunchKernelGGL((k_grad_march<4, GRAD_TD>), dim3((unsigned)(T.per * 8)), dim3(64, 4), 0, s, G, T, A);
+    }
+    return hip_check(hipGetLastError(), "k_grad_march td");
 }
 }  // namespace ifos
 

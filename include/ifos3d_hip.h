@@ -279,6 +279,24 @@ int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float *d_taup,
  * (device layout, accumulated); *l2 (host, accumulated) */
 int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn,
                       float *const d_grad[3], double *l2, void *stream);
+/* Time-domain gradient with checkpoints (no counterpart in the reference,
+ * whose gradient is the frequency-domain gradient_F.c): the gradient_F.c
+ * products summed over every time sample of the shot instead of over DFT
+ * frequencies -- lambda and mu from the displacements (velocities
+ * integrated from the end of the record), rho from the velocities, each
+ * sample weighted by DT, forward sample n paired with adjoint step
+ * NT - 1 - n (the DFT times of disc_fourier.c:36-37) for n = 1..NT-2; the
+ * lambda / mu / rho sums are formed in FP64 and added to d_grad (vp, vs, rho
+ * as gradient_F.c:135-139).  The forward state is stored every `every` steps
+ * (<= 0: round(sqrt(NT))) and each segment is recomputed from its
+ * checkpoint during the adjoint pass; the context keeps these buffers
+ * (ifos_td_bytes) until it is destroyed or `every` changes.  Needs
+ * want_adjoint, d_obs and d_syn; d_syn receives the forward seismograms,
+ * *l2 the misfit as ifos_ctx_run_shot. */
+int ifos_ctx_run_shot_td(ifos_ctx *c, const ifos_shot *s, int every, float *d_syn,
+                         float *const d_grad[3], double *l2, void *stream);
+/* device bytes the time-domain gradient buffers take (L: viscoelastic) */
+size_t ifos_td_bytes(const ifos_geom *g, int every, int L);
 /* device pointers owned by the context (fields, DFT accumulators) */
 int ifos_ctx_fields(ifos_ctx *c, ifos_fields *f);
 int ifos_ctx_dft(ifos_ctx *c, int back, float *d_F[6]);

@@ -924,6 +924,77 @@ int og_ntast(float fmax, int tast, float dt)
 
 /* ------------------------------------------------------------------ */
 /* one gradient shot, ifos3d.c:712-975 */
+void og_td_gradient(const og_grid *g, const oreal *vfwd, const oreal *vadj, const oreal *rho,
+                    const oreal *pi, const oreal *u, oreal *grad1, oreal *grad2, oreal *grad3)
+{
+    const size_t fs = og_field_size(g);
+    const int N = g->nt;
+    const oreal dt = (oreal)g->dt;
+    oreal b1, b2;
+    oreal *uf[3], *ub[3];
+    double *acc[3];
+    size_t c;
+    int n, m, q, i, j, k;
+    fd4_coeffs(g->fdcoeff, &b1, &b2);
+    for (q = 0; q < 3; q++) {
+        uf[q] = (oreal *)calloc(fs, sizeof(oreal));
+        ub[q] = (oreal *)calloc(fs, sizeof(oreal));
+        acc[q] = (double *)calloc(fs, sizeof(double));
+    }
+    /* forward samples N and N - 1 have no adjoint partner but enter uf */
+    for (n = N; n >= N - 1 && n >= 1; n--)
+        for (q = 0; q < 3; q++)
+            for (c = 0; c < fs; c++) uf[q][c] = uf[q][c] + vfwd[((size_t)(n - 1) * 3 + q) * fs + c] * dt;
+    for (m = 1; m <= N - 2; m++) {
+        const oreal *vf[3], *vb[3];
+        n = N - 1 - m;
+        for (q = 0; q < 3; q++) {
+            vf[q] = vfwd + ((size_t)(n - 1) * 3 + q) * fs;
+            vb[q] = vadj + ((size_t)(m - 1) * 3 + q) * fs;
+            for (c = 0; c < fs; c++) {
+                uf[q][c] = uf[q][c] + vf[q][c] * dt;
+                ub[q][c] = ub[q][c] + vb[q][c] * dt;
+            }
+        }
+        for (j = 1; j <= g->ny; j++)
+            for (i = 1; i <= g->nx; i++)
+                for (k = 1; k <= g->nz; k++) {
+                    grad9 a, b;
+                    oreal gl, gm, gr;
+                    c = og_idx(g, j, i, k);
+                    fgrad9(g, c, uf[0], uf[1], uf[2], b1, b2, &a);
+                    fgrad9(g, c, ub[0], ub[1], ub[2], b1, b2, &b);
+                    gl = (a.xx + a.yy + a.zz) * (b.xx + b.yy + b.zz);
+                    gl = -gl;
+                    gm = 2 * a.xx * b.xx + 2 * a.yy * b.yy + 2 * a.zz * b.zz + (a.xy + a.yx) * (b.xy + b.yx) +
+                         (a.xz + a.zx) * (b.xz + b.zx) + (a.yz + a.zy) * (b.yz + b.zy);
+                    gm = -gm;
+                    gr = vf[0][c] * vb[0][c] + vf[1][c] * vb[1][c] + vf[2][c] * vb[2][c];
+                    acc[0][c] += (double)gl * (double)dt;
+                    acc[1][c] += (double)gm * (double)dt;
+                    acc[2][c] += (double)gr * (double)dt;
+                }
+    }
+    for (j = 1; j <= g->ny; j++)
+        for (i = 1; i <= g->nx; i++)
+            for (k = 1; k <= g->nz; k++) {
+                const double r = rho[og_idx(g, j, i, k)], p = pi[og_idx(g, j, i, k)], mu = u[og_idx(g, j, i, k)];
+                const double srp = sqrt(r * p), sru = sqrt(r * mu);
+                double l, gm_;
+                c = og_idx(g, j, i, k);
+                l = acc[0][c];
+                gm_ = acc[1][c];
+                grad1[c] = (oreal)((double)grad1[c] + 2.0 * srp * l);
+                grad2[c] = (oreal)((double)grad2[c] + (-4.0 * sru * l + 2.0 * sru * gm_));
+                grad3[c] = (oreal)((double)grad3[c] + (acc[2][c] + mu / r * gm_ + (p - 2.0 * mu) / r * l));
+            }
+    for (q = 0; q < 3; q++) {
+        free(uf[q]);
+        free(ub[q]);
+        free(acc[q]);
+    }
+}
+
 int og_run_shot(const og_grid *g, og_shot_io *io)
 {
     const size_t fs = og_field_size(g);
@@ -1023,6 +1094,8 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
                                      pi, u, avg[0], avg[1], avg[2], prof, psi_v);
         }
         og_psource(g, nt, f[3], f[4], f[5], &io->src, sig, 1);
+        if (io->vfwd)
+            for (n = 0; n < 3; n++) memcpy(io->vfwd + ((size_t)(nt - 1) * 3 + n) * fs, f[n], fs * sizeof(oreal));
         if (io->want_adjoint && nt % ntast == 0)
             og_discfourier(g, nt, f[0], f[1], f[2], F[0], F[1], F[2], F[3], F[4], F[5],
                            io->finv, nf, ntast, 0);
@@ -1082,6 +1155,8 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
             og_update_s_cpml_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
                                      pi, u, avg[0], avg[1], avg[2], prof, psi_v);
         }
+        if (io->vadj)
+            for (n = 0; n < 3; n++) memcpy(io->vadj + ((size_t)(nt - 1) * 3 + n) * fs, f[n], fs * sizeof(oreal));
         if (nt % ntast == 0)
             og_discfourier(g, nt, f[0], f[1], f[2], F[6], F[7], F[8], F[9], F[10], F[11],
                            io->finv, nf, ntast, 1);

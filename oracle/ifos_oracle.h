@@ -242,9 +242,28 @@ typedef struct og_shot_io {
     const oreal *taus, *taup;
     oreal eta;
     int L;
+    /* optional: the velocities vx, vy, vz after every step at the DFT point
+     * of ifos3d.c (forward after psource, adjoint after the stress update),
+     * [nt][3] canonical fields each, NULL if not wanted */
+    oreal *vfwd, *vadj;
 } og_shot_io;
 
 int og_run_shot(const og_grid *g, og_shot_io *io);
+
+/*
+ * The time-domain gradient from the velocity histories of og_run_shot
+ * (vfwd, vadj, [nt][3] canonical fields): the products of
+ * gradient_F.c:122-132 summed over the time samples instead of the DFT
+ * frequencies.  Forward sample n pairs with adjoint step m = nt - 1 - n
+ * (the DFT times of disc_fourier.c:36-37), n = 1 .. nt - 2; lambda and mu
+ * use the displacements integrated from the end of the record (forward:
+ * uf(n) = sum over n' >= n of vf(n') DT; adjoint: ub(m) = sum over m' <= m
+ * of vb(m') DT, both summed in oreal), rho the velocities; each sample is
+ * weighted by DT and summed in double; the sums are added to grad1..3 with
+ * the vp, vs, rho parametrisation of gradient_F.c:135-139 in double.
+ */
+void og_td_gradient(const og_grid *g, const oreal *vfwd, const oreal *vadj, const oreal *rho,
+                    const oreal *pi, const oreal *u, oreal *grad1, oreal *grad2, oreal *grad3);
 
 #ifdef __cplusplus
 }

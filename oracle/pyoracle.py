@@ -243,7 +243,7 @@ class Oracle:
         return self.lib.og_ntast(C.c_float(fmax), tast, C.c_float(dt))
 
     def run_shot(self, g: Grid, rho, pi, u, src, rec_ijk, finv, tast=100, vppml=4000.0, fpml=50.0,
-                 shape=1, obs=None, want_adjoint=True, want_fields=False, visco=None):
+                 shape=1, obs=None, want_adjoint=True, want_fields=False, visco=None, want_history=False):
         """visco = (taus, taup, eta): the viscoelastic update with one
         relaxation mechanism instead of the elastic one"""
         class Io(C.Structure):
@@ -256,7 +256,8 @@ class Oracle:
                         ("l2", C.c_void_p), ("grad1", C.c_void_p), ("grad2", C.c_void_p),
                         ("grad3", C.c_void_p), ("ffwd", C.c_void_p), ("fadj", C.c_void_p),
                         ("taus", C.c_void_p), ("taup", C.c_void_p),
-                        ("eta", C.c_float if self.precision == "f32" else C.c_double), ("L", C.c_int)]
+                        ("eta", C.c_float if self.precision == "f32" else C.c_double), ("L", C.c_int),
+                        ("vfwd", C.c_void_p), ("vadj", C.c_void_p)]
         rec = np.ascontiguousarray(rec_ijk, np.int32)
         ntr = rec.shape[0]
         fv = np.asarray(finv, np.float32)
@@ -265,6 +266,8 @@ class Oracle:
         l2 = np.zeros(1, np.float32)
         ff = np.zeros((6, len(fv)) + g.shape, self.dtype) if want_fields else None
         fa = np.zeros((6, len(fv)) + g.shape, self.dtype) if want_fields else None
+        hf = np.zeros((g.nt, 3) + g.shape, self.dtype) if want_history else None
+        ha = np.zeros((g.nt, 3) + g.shape, self.dtype) if want_history else None
         o = [np.ascontiguousarray(a, np.float32) for a in obs] if obs is not None else [None] * 3
         io = Io(_p(rho).value, _p(pi).value, _p(u).value, OgSource(*src), shape, ntr, rec.ctypes.data,
                 o[0].ctypes.data if o[0] is not None else None,
@@ -274,6 +277,8 @@ class Oracle:
                 syn[0].ctypes.data, syn[1].ctypes.data, syn[2].ctypes.data, l2.ctypes.data,
                 grads[0].ctypes.data, grads[1].ctypes.data, grads[2].ctypes.data,
                 ff.ctypes.data if ff is not None else None, fa.ctypes.data if fa is not None else None)
+        if want_history:
+            io.vfwd, io.vadj = hf.ctypes.data, ha.ctypes.data
         if visco is not None:
             taus = np.ascontiguousarray(visco[0], self.dtype)
             taup = np.ascontiguousarray(visco[1], self.dtype)
@@ -286,7 +291,17 @@ class Oracle:
         if want_fields:
             out["ffwd"] = ff
             out["fadj"] = fa
+        if want_history:
+            out["vfwd"] = hf
+            out["vadj"] = ha
         return out
+
+    def td_gradient(self, g: Grid, vfwd, vadj, rho, pi, u, grads):
+        """og_td_gradient: the time-domain gradient from velocity histories
+        [nt][3] (run_shot(..., want_history=True)), added to grads"""
+        arr = [np.ascontiguousarray(a, self.dtype) for a in (vfwd, vadj, rho, pi, u)]
+        og = g.og()
+        self.lib.og_td_gradient(C.byref(og), *[_p(a) for a in arr], *[_p(t) for t in grads])
 
 
 # ----------------------------------------------------------------------
