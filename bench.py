@@ -338,6 +338,31 @@ def run_plain(w, G, d_start, obs, shots, precision):
     return dict(elapsed=elapsed, shots=shots, prof=prof)
 
 
+def run_td(w, G, d_start, obs, shots):
+    """the checkpointed time-domain (full-band) gradient, one shot at a time,
+    checkpoints every round(sqrt(NT)) steps"""
+    import ifosgpu as ig
+    ntr = len(w.receivers)
+    log(f"time-domain gradient: {ig.ShotContext.td_bytes(G) / 2**30:.1f} GiB of checkpoints and buffers")
+    ctx = ig.ShotContext(G, 1, ntr)
+    ctx.set_cpml(w.vppml, w.fpml)
+    ctx.set_model(*d_start)
+    grads = [G.empty() for _ in range(3)]
+    d_syn = ig.DBuf((3, ntr, w.nt + 1))
+    ctx.run_shot_td(w.shots[0], w.receivers, d_syn, grads, obs[0])  # allocates the checkpoints
+    ig.sync()
+    _prof_start(ig)
+    t0 = time.perf_counter()
+    for n in range(shots):
+        ctx.run_shot_td(w.shots[n], w.receivers, d_syn, grads, obs[n])
+        log(f"time-domain gradient shot {n + 1}/{shots}")
+    ig.sync()
+    elapsed = time.perf_counter() - t0
+    prof = _prof_stop(ig)
+    ctx.close()
+    return dict(elapsed=elapsed, shots=shots, prof=prof)
+
+
 def copy_probe(mib=1024, reps=5):
     """plain device copy of a known size: the practical HBM ceiling, and the
     reference kernel the rocprof FETCH_SIZE/WRITE_SIZE counters are scaled by"""
@@ -454,6 +479,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0,
                     help="shorter time loop (profiling runs only: the line then reports a smaller workload)")
     ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
+    ap.add_argument("--td-shots", type=int, default=1,
+                    help="time-domain (full-band, checkpointed) gradient shots timed beside the batch")
     ap.add_argument("--precision", default="compat", choices=["compat", "kahan"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -504,6 +531,12 @@ def main():
                               not args.no_autotune, args.fused)
         batched["value"] = batched["shots"] / batched["elapsed"]
         log(f"batched path: {batched['value']:.4f} shots/s")
+
+    td = None
+    if args.td_shots > 0:
+        td = run_td(w, G, d_start, obs, args.td_shots)
+        td["value"] = td["shots"] / td["elapsed"]
+        log(f"time-domain gradient: {td['value']:.4f} shots/s")
 
     res = batched if args.path == "batched" else plain
     el, value = res["elapsed"], res["value"]
@@ -575,6 +608,8 @@ def main():
         "hbm_frac_stencils": round(hbm_frac_stencils, 4) if hbm_frac_stencils else None,
         "plain_shots_per_s": round(plain["value"], 4) if plain else None,
         "batched_over_plain": round(batched["value"] / plain["value"], 3) if (batched and plain) else None,
+        "td_shots_per_s": round(td["value"], 4) if td else None,
+        "td_kernels_ms": {k: round(v[0], 1) for k, v in td["prof"].items() if v[1]} if td else None,
         "roofline": roof,
         "kernels": kernels,
         "plain_kernels": kernel_table(plain["prof"], algorithmic_bytes(w, nf, kahan))
