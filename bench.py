@@ -315,6 +315,85 @@ def run_levels(args):
     print(json.dumps(out), flush=True)
 
 
+def run_fwd(args):
+    """the forward-only bandwidth case: one shot of NT steps with seismograms,
+    through the plain path and the batched path (one shot, autotuned)"""
+    import ifosgpu as ig
+    import workloads as wl
+    w = wl.fwd536()
+    true, _ = wl.random_models(w)
+    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+    d_model = [G.to_dev(a) for a in true]
+    del true
+    ntr = len(w.receivers)
+    src = w.shots[0]
+    res = {}
+    ctx = ig.ShotContext(G, 1, ntr)
+    ctx.set_cpml(w.vppml, w.fpml)
+    ctx.set_freqs(w.finv, w.tast)
+    ctx.set_model(*d_model)
+    d_syn = ig.DBuf((3, ntr, w.nt + 1))
+    ctx.run_shot(src, w.receivers, d_syn, None, want_adjoint=False)
+    ig.sync()
+    _prof_start(ig)
+    t0 = time.perf_counter()
+    ctx.run_shot(src, w.receivers, d_syn, None, want_adjoint=False)
+    ig.sync()
+    res["plain"] = (time.perf_counter() - t0, _prof_stop(ig))
+    ctx.close()
+    log(f"plain forward: {res['plain'][0]:.2f} s")
+    b = ig.BatchContext(G, 1, 1, ntr, ig.DFT_COMPAT, 1)
+    b.set_cpml(w.vppml, w.fpml)
+    b.set_model(*d_model)
+    tune = None
+    if not args.no_autotune:
+        bv, bs, _ = b.autotune(reps=2)
+        tune = {"v": bv, "s": bs}
+    d_b = ig.DBuf((1, 3, ntr, w.nt + 1))
+    b.run([src], w.receivers, d_syn=d_b, want_adjoint=False)
+    ig.sync()
+    _prof_start(ig)
+    t0 = time.perf_counter()
+    b.run([src], w.receivers, d_syn=d_b, want_adjoint=False)
+    ig.sync()
+    res["batched"] = (time.perf_counter() - t0, _prof_stop(ig))
+    b.close()
+    log(f"batched forward (one shot): {res['batched'][0]:.2f} s, tiles {tune}")
+    cells = w.cells
+    ab_p = algorithmic_bytes(w, 1)
+    _, _, axes = slab_stats(w)
+    ab_b = {"v_fused": 48 * cells + 24 * axes + 12 * cells, "s_fused": 60 * cells + 24 * axes + 20 * cells,
+            "vs_fused": 72 * cells + 48 * axes + 32 * cells}
+    best = min(res, key=lambda k: res[k][0])
+    el, prof = res[best]
+    ab = ab_p if best == "plain" else ab_b
+    dom, avg_ms, achieved = roofline_of(prof, ab)
+    st_cats = [k for k in ("v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused") if k in ab]
+    st_bytes = sum(ab[k] * prof[k][1] for k in st_cats)
+    st_ms = sum(prof[k][0] for k in st_cats)
+    out = {
+        "metric": "forward Gcell-updates/s, 512x256x512 + CPML bandwidth case",
+        "value": round(cells * w.nt / el / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(1000.0 * el, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"fwd536: {w.nx}x{w.ny}x{w.nz} (512x256x512 + {w.fw}-cell CPML), dh {w.dh} m, "
+                               f"dt {w.dt:.4e} s, NT {w.nt}, seeded random model, one explosive 15 Hz source, "
+                               f"{ntr} receivers, elastic FP32 4th order, forward only",
+                   "path": f"{best} (fastest of plain and one-shot batched)"},
+        "plain_gcell_updates_per_s": round(cells * w.nt / res["plain"][0] / 1e9, 3),
+        "batched_gcell_updates_per_s": round(cells * w.nt / res["batched"][0] / 1e9, 3),
+        "hbm_frac_stencils": round(st_bytes / (st_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if st_ms else None,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(ab[dom]), "avg_launch_ms": round(avg_ms, 5)},
+        "kernels": kernel_table(prof, ab),
+        "autotune": tune,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_plain(w, G, d_start, obs, shots, precision):
     """the one-shot-per-launch path on the first `shots` shot positions"""
     import ifosgpu as ig
@@ -463,7 +542,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed batches (batched) or shots (plain)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84", "levels3"])
+    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84", "levels3", "fwd536"])
     ap.add_argument("--level-shots", type=int, default=32, help="shots per grid level (levels3)")
     ap.add_argument("--path", default="batched", choices=["batched", "plain"])
     ap.add_argument("--shots-per-batch", type=int, default=16)
@@ -505,6 +584,9 @@ def main():
     log(f"copy probe: {copy}")
     if args.config == "levels3":
         run_levels(args)
+        return
+    if args.config == "fwd536":
+        run_fwd(args)
         return
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
     if args.nt > 0:

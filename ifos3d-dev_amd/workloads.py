@@ -12,6 +12,10 @@ receivers, time step), in the canonical/device layouts of ifosgpu.
            16 explosive Ricker shots (10 Hz) on a 4x4 grid, 1024 receivers
            (32x32) near the top, 6 DFT frequencies 4-14 Hz.
 
+* fwd536:   536 x 280 x 536 (512 x 256 x 512 + 12-cell CPML), dh 5 m,
+           dt = 0.9 CFL, NT 2000, the seeded random field (seed 2), one
+           explosive 15 Hz Ricker source, 4096 receivers (64 x 64): the
+           forward-only bandwidth case.
 * levels3:  three coarse-to-fine grid levels of one 768 m cube, interiors
            96^3 at dh 8 m, 128^3 at 6 m and 192^3 at 4 m, each + 10-cell
            CPML, dt = 0.9 CFL per level and NT = ceil(0.6 s / dt); the
@@ -78,6 +82,20 @@ def grad276() -> Workload:
     finv = [4.0, 6.0, 8.0, 10.0, 12.0, 14.0]
     return Workload("grad276", n, n, n, fw, dh, float(np.float32(dt)), 1000, finv, 100, 5500.0, 10.0, shots, rec,
                     seed=5)
+
+
+def fwd536() -> Workload:
+    """the forward-only bandwidth case: 536 x 280 x 536 (512 x 256 x 512 + 12-cell
+    CPML), dh 5 m, dt = 0.9 CFL, NT 2000, one explosive Ricker source (15 Hz)
+    at the centre 10 cells below the top, 4096 receivers (64 x 64, every 8
+    cells) at the top"""
+    fw, dh = 12, 5.0
+    nx, ny, nz = 536, 280, 536
+    dt = cfl_dt(dh, 5500.0)
+    shots = [(fw + 1 + 256, fw + 10, fw + 1 + 256, 0.0, 15.0, 1.0, 1)]
+    rec = np.array([[fw + 1 + 8 * a, fw + 1, fw + 1 + 8 * b] for a in range(64) for b in range(64)], np.int32)
+    return Workload("fwd536", nx, ny, nz, fw, dh, float(np.float32(dt)), 2000, [15.0], 100, 5500.0, 15.0, shots,
+                    rec, seed=2)
 
 
 LEVELS3 = ((96, 8.0), (128, 6.0), (192, 4.0))  # (interior cells per axis, dh), one 768 m cube

@@ -152,3 +152,40 @@ def test_viscoelastic_gradient_shot(free_surf):
         assert max_ulp(got, g.interior(ref["grads"][n])) == 0, f"gradient {n}"
     assert abs(l2 - ref["l2"]) <= 1e-5 * abs(ref["l2"])
     ctx.close()
+
+
+@pytest.mark.parametrize("nt,nf", [(160, 3), (700, 8)])
+def test_errors_against_fp64_oracle(nt, nf):
+    """the FP32 device shot (compat and Kahan DFT) against the FP64 oracle
+    run of the same shot: the error the FP32 method carries, stated and held
+    to the path's tolerances (seismograms rel. L2 <= 1e-5, gradients <= 1e-4)"""
+    import ifosgpu as ig
+    g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=nt)
+    src = (18, 12, 19, 0.0, 50.0, 1.0, 1)
+    rec = np.array([[9 + 2 * n, 10, 19] for n in range(9)], np.int32)
+    finv = [40.0 + 10.0 * n for n in range(nf)]
+    _note(f"FP32 device vs FP64 oracle: NT={nt}, {nf} frequencies")
+    o32, o64 = Oracle("f32"), Oracle("f64")
+    start, true = _models(g, 1)
+    obs = o32.run_shot(g, *true, src, rec, finv, want_adjoint=False)["syn"]
+    ref = o64.run_shot(g, *[a.astype(np.float64) for a in start], src, rec, finv, obs=obs)
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt, g.fdcoeff, g.free_surf)
+    d_obs = ig.DBuf.from_numpy(np.stack(obs))
+    errs = {}
+    for name, prec in (("compat", ig.DFT_COMPAT), ("kahan", ig.DFT_KAHAN)):
+        ctx = ig.ShotContext(G, nf, len(rec), prec)
+        ctx.set_cpml(4000.0, 50.0)
+        ctx.set_freqs(finv, 100)
+        ctx.set_model(*[G.to_dev(a) for a in start])
+        d_syn = ig.DBuf((3, len(rec), g.nt + 1))
+        grads = [G.empty() for _ in range(3)]
+        ctx.run_shot(src, rec, d_syn, grads, d_obs=d_obs)
+        ig.sync()
+        syn = d_syn.numpy()
+        es = max(rel_l2(syn[c][:, 1:], ref["syn"][c][:, 1:]) for c in range(3))
+        eg = [rel_l2(g.interior(G.from_dev(grads[n])), g.interior(ref["grads"][n])) for n in range(3)]
+        ctx.close()
+        errs[name] = (es, eg)
+        _note(f"  {name}: seismograms rel-L2 {es:.2e}; gradients vp {eg[0]:.2e} vs {eg[1]:.2e} rho {eg[2]:.2e}")
+        assert es <= 1e-5
+        assert max(eg) <= 1e-4
