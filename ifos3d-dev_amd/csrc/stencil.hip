@@ -181,7 +181,7 @@ struct MTiles {
 };
 
 __device__ inline bool decode_tile(const MTiles &T, int TI, int &k, int &i, int &jb, int &je, int &shot,
-                                   Box &R)
+                                   Box &R, bool *krow = nullptr)
 {
     const int b = blockIdx.x;
     const int L = (b & 7) * T.per + (b >> 3);
@@ -214,7 +214,43 @@ __device__ inline bool decode_tile(const MTiles &T, int TI, int &k, int &i, int 
     if (k > R.k1 || i > R.i1) return false;
     jb = R.j0 + tj * T.jc;
     je = min(jb + T.jc - 1, R.j1);
+    // the wavefront spans 64 consecutive k of one (j, i) row, all inside the box
+    if (krow) *krow = __builtin_amdgcn_readfirstlane((int)(kw == 64 && k - dk + 63 <= R.k1)) != 0;
     return true;
+}
+
+// k-neighbours through the lanes of a wavefront that spans 64 consecutive k
+// of one row (DPP wavefront shifts instead of loads): lane l holds the
+// row's value c at k0 + l; h holds the values just outside the row in lanes
+// 0, 1 (k0 - 2, k0 - 1) and 62, 63 (k0 + 64, k0 + 65), loaded with the
+// per-lane offset khalo_off.  Lanes whose shift source lies outside the
+// wavefront keep the `old` operand of the DPP move.
+template <int CTRL>
+__device__ inline float dpp_move(float old, float src)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xF, 0xF, false));
+}
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+struct KNb {
+    float m2, m1, p1, p2;  // values at k-2, k-1, k+1, k+2
+};
+__device__ inline KNb kshift(float c, float h)
+{
+    KNb r;
+    const float hs = dpp_move<DPP_WAVE_SHL1>(h, h);  // lane 0: h[1]
+    const float hr = dpp_move<DPP_WAVE_SHR1>(h, h);  // lane 63: h[62]
+    r.m1 = dpp_move<DPP_WAVE_SHR1>(hs, c);
+    r.m2 = dpp_move<DPP_WAVE_SHR1>(h, r.m1);
+    r.p1 = dpp_move<DPP_WAVE_SHL1>(hr, c);
+    r.p2 = dpp_move<DPP_WAVE_SHL1>(h, r.p1);
+    return r;
+}
+// byte offset (within the LD addressing) of lane l's halo element: k - 2 for
+// lanes 0, 1, k + 2 for lanes 62, 63, the lane's own k otherwise
+__device__ inline uint32_t khalo_off()
+{
+    const int l = (int)threadIdx.x;
+    return (uint32_t)((l < 2 ? 0 : l >= 62 ? 4 : 2) * 4);
 }
 
 __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
@@ -228,6 +264,7 @@ __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
 // (j+dj, i+di, k+dk), |d| <= 2 (dj <= 3), lies at vo + (dk+2)*4 (immediate)
 // + (dj+2)*SJ + (di+2)*SI (scalar): all non-negative, all uniform but vo.
 #define LD(r, dj, di, dk) bload(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
+#define LDH(r, dj) bload(r, vo + khv, ((dj) + 2) * SJ + 2 * SI)
 #define ST(v, r) bstore(v, r, vo + 8, 2 * SJ + 2 * SI)
 
 // Operands of one plane j of the velocity march: the in-plane neighbours,
@@ -237,6 +274,7 @@ __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
 struct VPlane {
     float sxx_a, sxx_b, sxx_c, sxx_d, sxy_a, sxy_b, sxy_d, sxz_xa, sxz_xb, sxz_0, sxz_xd, sxz_za, sxz_zb, sxz_zd,
         syz_za, syz_zb, syz_zd, szz_a, szz_b, szz_c, szz_d, rip, rjp, rkp, vx0, vy0, vz0, sxy_n, syz_n, syy_n;
+    float sxz_h, syz_h, szz_h;  // k-row halos (krow wavefronts)
 };
 #define VLOAD(P, D)                                                                                            \
     do {                                                                                                       \
@@ -245,10 +283,15 @@ struct VPlane {
         P.sxy_a = LD(rsxy, D, -2, 0); P.sxy_b = LD(rsxy, D, -1, 0); P.sxy_d = LD(rsxy, D, 1, 0);                 \
         P.sxz_xa = LD(rsxz, D, -2, 0); P.sxz_xb = LD(rsxz, D, -1, 0); P.sxz_0 = LD(rsxz, D, 0, 0);               \
         P.sxz_xd = LD(rsxz, D, 1, 0);                                                                          \
-        P.sxz_za = LD(rsxz, D, 0, -2); P.sxz_zb = LD(rsxz, D, 0, -1); P.sxz_zd = LD(rsxz, D, 0, 1);              \
-        P.syz_za = LD(rsyz, D, 0, -2); P.syz_zb = LD(rsyz, D, 0, -1); P.syz_zd = LD(rsyz, D, 0, 1);              \
-        P.szz_a = LD(rszz, D, 0, -1); P.szz_b = LD(rszz, D, 0, 0); P.szz_c = LD(rszz, D, 0, 1);                  \
-        P.szz_d = LD(rszz, D, 0, 2);                                                                           \
+        if (krow) {                                                                                            \
+            P.sxz_h = LDH(rsxz, D); P.syz_h = LDH(rsyz, D); P.szz_h = LDH(rszz, D);                            \
+            P.szz_b = LD(rszz, D, 0, 0);                                                                       \
+        } else {                                                                                               \
+            P.sxz_za = LD(rsxz, D, 0, -2); P.sxz_zb = LD(rsxz, D, 0, -1); P.sxz_zd = LD(rsxz, D, 0, 1);          \
+            P.syz_za = LD(rsyz, D, 0, -2); P.syz_zb = LD(rsyz, D, 0, -1); P.syz_zd = LD(rsyz, D, 0, 1);          \
+            P.szz_a = LD(rszz, D, 0, -1); P.szz_b = LD(rszz, D, 0, 0); P.szz_c = LD(rszz, D, 0, 1);              \
+            P.szz_d = LD(rszz, D, 0, 2);                                                                       \
+        }                                                                                                      \
         P.rip = LD(rrip, D, 0, 0); P.rjp = LD(rrjp, D, 0, 0); P.rkp = LD(rrkp, D, 0, 0);                         \
         P.vx0 = LD(rvx, D, 0, 0); P.vy0 = LD(rvy, D, 0, 0); P.vz0 = LD(rvz, D, 0, 0);                            \
         P.sxy_n = LD(rsxy, (D) + 1, 0, 0); P.syz_n = LD(rsyz, (D) + 1, 0, 0); P.syy_n = LD(rsyy, (D) + 2, 0, 0); \
@@ -260,7 +303,9 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
 {
     int k, i, jb, je, shot;
     Box R;
-    if (!decode_tile(T, TI, k, i, jb, je, shot, R)) return;
+    bool krow;
+    if (!decode_tile(T, TI, k, i, jb, je, shot, R, &krow)) return;
+    const uint32_t khv = khalo_off();
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const int64_t so = (int64_t)shot * S.fstride;
@@ -288,6 +333,12 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
 
     for (int j = jb; j <= je; ++j, vo += SJ) {
         if (PIPE && j < je) VLOAD(nxt, 1);
+        if (krow) {
+            const KNb a = kshift(cur.sxz_0, cur.sxz_h), b = kshift(syz_0, cur.syz_h), c = kshift(cur.szz_b, cur.szz_h);
+            cur.sxz_za = a.m2; cur.sxz_zb = a.m1; cur.sxz_zd = a.p1;
+            cur.syz_za = b.m2; cur.syz_zb = b.m1; cur.syz_zd = b.p1;
+            cur.szz_a = c.m1; cur.szz_c = c.p1; cur.szz_d = c.p2;
+        }
         float sxx_x = g.dtdx * D4F(b1, b2, cur.sxx_a, cur.sxx_b, cur.sxx_c, cur.sxx_d);
         float sxy_y = g.dtdy * D4B(b1, b2, sxy_m2, sxy_m1, sxy_0, cur.sxy_n);
         float sxz_z = g.dtdz * D4B(b1, b2, cur.sxz_za, cur.sxz_zb, cur.sxz_0, cur.sxz_zd);
@@ -329,15 +380,20 @@ struct SPlane {
     float vx_xa, vx_xb, vx_xd, vx_zb, vx_zc, vx_zd, vy_xb, vy_xc, vy_xd, vy_zb, vy_zc, vy_zd, vz_xb, vz_xc, vz_xd,
         vz_za, vz_zb, vz_zd, uipjp, ujpkp, uipkp, pi, u, sxx0, syy0, szz0, sxy0, syz0, sxz0, vx_n, vz_n, vy_n;
     float r0[6], ts, tp, tsipjp, tsjpkp, tsipkp;
+    float vx_h, vy_h, vz_h;  // k-row halos (krow wavefronts)
 };
 #define SLOAD(P, D)                                                                                            \
     do {                                                                                                       \
         P.vx_xa = LD(rvx, D, -2, 0); P.vx_xb = LD(rvx, D, -1, 0); P.vx_xd = LD(rvx, D, 1, 0);                    \
-        P.vx_zb = LD(rvx, D, 0, -1); P.vx_zc = LD(rvx, D, 0, 1); P.vx_zd = LD(rvx, D, 0, 2);                     \
         P.vy_xb = LD(rvy, D, -1, 0); P.vy_xc = LD(rvy, D, 1, 0); P.vy_xd = LD(rvy, D, 2, 0);                     \
-        P.vy_zb = LD(rvy, D, 0, -1); P.vy_zc = LD(rvy, D, 0, 1); P.vy_zd = LD(rvy, D, 0, 2);                     \
         P.vz_xb = LD(rvz, D, -1, 0); P.vz_xc = LD(rvz, D, 1, 0); P.vz_xd = LD(rvz, D, 2, 0);                     \
-        P.vz_za = LD(rvz, D, 0, -2); P.vz_zb = LD(rvz, D, 0, -1); P.vz_zd = LD(rvz, D, 0, 1);                    \
+        if (krow) {                                                                                            \
+            P.vx_h = LDH(rvx, D); P.vy_h = LDH(rvy, D); P.vz_h = LDH(rvz, D);                                  \
+        } else {                                                                                               \
+            P.vx_zb = LD(rvx, D, 0, -1); P.vx_zc = LD(rvx, D, 0, 1); P.vx_zd = LD(rvx, D, 0, 2);                 \
+            P.vy_zb = LD(rvy, D, 0, -1); P.vy_zc = LD(rvy, D, 0, 1); P.vy_zd = LD(rvy, D, 0, 2);                 \
+            P.vz_za = LD(rvz, D, 0, -2); P.vz_zb = LD(rvz, D, 0, -1); P.vz_zd = LD(rvz, D, 0, 1);                \
+        }                                                                                                      \
         P.uipjp = LD(ruipjp, D, 0, 0); P.ujpkp = LD(rujpkp, D, 0, 0); P.uipkp = LD(ruipkp, D, 0, 0);             \
         P.pi = LD(rpi, D, 0, 0); P.u = LD(ru, D, 0, 0);                                                        \
         P.sxx0 = LD(rsxx, D, 0, 0); P.syy0 = LD(rsyy, D, 0, 0); P.szz0 = LD(rszz, D, 0, 0);                      \
@@ -356,7 +412,9 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 {
     int k, i, jb, je, shot;
     Box R;
-    if (!decode_tile(T, TI, k, i, jb, je, shot, R)) return;
+    bool krow;
+    if (!decode_tile(T, TI, k, i, jb, je, shot, R, &krow)) return;
+    const uint32_t khv = khalo_off();
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
@@ -393,6 +451,12 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 
     for (int j = jb; j <= je; ++j, vo += SJ) {
         if (PIPE && j < je) SLOAD(nxt, 1);
+        if (krow) {
+            const KNb a = kshift(vx_0, cur.vx_h), b = kshift(vy_0, cur.vy_h), c = kshift(vz_0, cur.vz_h);
+            cur.vx_zb = a.m1; cur.vx_zc = a.p1; cur.vx_zd = a.p2;
+            cur.vy_zb = b.m1; cur.vy_zc = b.p1; cur.vy_zd = b.p2;
+            cur.vz_za = c.m2; cur.vz_zb = c.m1; cur.vz_zd = c.p1;
+        }
         float vxx = D4B(b1, b2, cur.vx_xa, cur.vx_xb, vx_0, cur.vx_xd);
         float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, cur.vx_n);
         float vxz = D4F(b1, b2, cur.vx_zb, vx_0, cur.vx_zc, cur.vx_zd);

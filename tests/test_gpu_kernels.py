@@ -56,11 +56,13 @@ def _upload(ig, G, rho, pi, u, avg, f, psi, prof):
     return dmodel, df, dpsi, dprof
 
 
+# nz 150: k rows of 64 cells fill whole wavefronts (k-neighbours through lane shifts) next to a partial row
+@pytest.mark.parametrize("nz", [30, 150])
 @pytest.mark.parametrize("free_surf", [0, 1])
 @pytest.mark.parametrize("fdcoeff", [1, 2])
-def test_velocity_update_regions(env, free_surf, fdcoeff):
+def test_velocity_update_regions(env, free_surf, fdcoeff, nz):
     _, ig = env
-    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf)
+    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nz=nz)
     _note(f"velocity update parity fdcoeff={fdcoeff} free_surf={free_surf} grid={g.nx}x{g.ny}x{g.nz}")
     o, rng, rho, pi, u, avg, prof, f, psi = _setup(g, 11 + fdcoeff + 2 * free_surf)
     G = _geom(ig, g)
@@ -90,12 +92,13 @@ def test_velocity_update_regions(env, free_surf, fdcoeff):
         assert np.array_equal(dpsi2[n].numpy(), po[n]), f"fused psi_s[{n}]"
 
 
+@pytest.mark.parametrize("nz", [29, 150])
 @pytest.mark.parametrize("dh", [2.0, 2.5])  # power-of-two spacing: multiply by 1/dh; else IEEE division
 @pytest.mark.parametrize("free_surf", [0, 1])
 @pytest.mark.parametrize("fdcoeff", [1, 2])
-def test_stress_update_regions(env, free_surf, fdcoeff, dh):
+def test_stress_update_regions(env, free_surf, fdcoeff, dh, nz):
     _, ig = env
-    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=29, dh=dh)
+    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=nz, dh=dh)
     _note(f"stress update parity fdcoeff={fdcoeff} free_surf={free_surf} dh={dh}")
     o, rng, rho, pi, u, avg, prof, f, psi = _setup(g, 21 + fdcoeff + 2 * free_surf)
     G = _geom(ig, g)
@@ -121,12 +124,12 @@ def test_stress_update_regions(env, free_surf, fdcoeff, dh):
         assert np.array_equal(dpsi2[n].numpy(), po[n]), f"fused psi_v[{n}]"
 
 
-@pytest.mark.parametrize("free_surf,fdcoeff", [(0, 2), (1, 1)])
-def test_visco_stress_update_regions(env, free_surf, fdcoeff):
+@pytest.mark.parametrize("free_surf,fdcoeff,nz", [(0, 2, 29), (1, 1, 29), (0, 2, 150)])
+def test_visco_stress_update_regions(env, free_surf, fdcoeff, nz):
     """update_s / update_s_CPML (one relaxation mechanism), two steps so the
     memory variables carry over; interior+CPML launches and the fused one"""
     _, ig = env
-    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=29)
+    g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=nz)
     _note(f"viscoelastic stress update parity fdcoeff={fdcoeff} free_surf={free_surf}")
     o, rng, rho, pi, u, avg, prof, f, psi = _setup(g, 31 + fdcoeff + 2 * free_surf)
     taus = (0.01 + 0.05 * rng.random(g.shape)).astype(np.float32)
