@@ -16,6 +16,12 @@ is therefore on proportionality: along perturbations of vp (two places), vs
 and rho, the finite-difference derivative divided by <backward, delta> must
 be positive and the same within 25% (measured spread about 17%, ratio about
 0.01).  The misfit itself is compared with the device's FP64-reduced L2.
+
+The same check runs with the checkpointed time-domain gradient
+(ifos_ctx_run_shot_td), the full-band counterpart of those products: the
+deviation from the exact derivative comes from the adjoint itself (the
+reference's adjoint source and CPML), not from the frequency sampling, so
+its spread is of the same size.
 """
 import numpy as np
 import pytest
@@ -73,25 +79,28 @@ class _Problem:
             J += float((R * R).sum())
         return J
 
-    def misfit_and_gradient(self, m):
+    def misfit_and_gradient(self, m, mode="dft"):
         ig = self.ig
         self.ctx.set_model(*self.upload(m))
         grads = [self.G.empty() for _ in range(3)]
         l2 = 0.0
         for n, s in enumerate(self.srcs):
-            l2 += self.ctx.run_shot(s, self.rec, self.d_syn, grads, d_obs=self.d_obs[n])
+            if mode == "td":
+                l2 += self.ctx.run_shot_td(s, self.rec, self.d_syn, grads, self.d_obs[n])
+            else:
+                l2 += self.ctx.run_shot(s, self.rec, self.d_syn, grads, d_obs=self.d_obs[n])
         ig.sync()
         return l2, [self.g.interior(self.G.from_dev(t)).astype(np.float64) for t in grads]
 
 
-def _autograd_fn(problem):
+def _autograd_fn(problem, mode):
     import torch
 
     class IfosMisfit(torch.autograd.Function):
         @staticmethod
         def forward(ctx, vp, vs, rho):
             m = tuple(problem.full(t.detach().cpu().numpy()) for t in (vp, vs, rho))
-            l2, grads = problem.misfit_and_gradient(m)
+            l2, grads = problem.misfit_and_gradient(m, mode)
             ctx.save_for_backward(*[torch.from_numpy(-gq) for gq in grads])
             return torch.tensor(l2, dtype=torch.float64)
 
@@ -102,17 +111,20 @@ def _autograd_fn(problem):
     return IfosMisfit
 
 
-def test_autograd_gradient_is_proportional_to_finite_differences():
+@pytest.mark.parametrize("mode", ["dft", "td"])
+def test_autograd_gradient_is_proportional_to_finite_differences(mode):
+    """mode dft: gradient_F over 16 DFT frequencies (ifos_ctx_run_shot);
+    td: the checkpointed time-domain gradient (ifos_ctx_run_shot_td)"""
     import torch
     p = _Problem()
     g = p.g
-    fn = _autograd_fn(p)
+    fn = _autograd_fn(p, mode)
     interior = lambda a: torch.tensor(g.interior(a).copy(), dtype=torch.float64, requires_grad=True)
     vp, vs, rho = (interior(a) for a in p.base)
     J = fn.apply(vp, vs, rho)
     J.backward()
     J_host = p.misfit(p.base)
-    print(f"misfit: device {J.item():.6e}, host FP64 {J_host:.6e}", flush=True)
+    print(f"{mode}: misfit: device {J.item():.6e}, host FP64 {J_host:.6e}", flush=True)
     assert J.item() == pytest.approx(J_host, rel=1e-5)
     dirs = {"vp at the anomaly": (p.bump(24, 20, 20, 4), 0, 0), "vp elsewhere": (p.bump(28, 26, 14, 3), 0, 0),
             "vs at the anomaly": (0, p.bump(24, 20, 20, 4), 0), "rho at the anomaly": (0, 0, p.bump(24, 20, 20, 4))}
