@@ -6,12 +6,17 @@ A gradient shot is: forward NT=1000 steps with the on-the-fly DFT (6
 frequencies), residual against observed data, adjoint NT steps with the
 DFT, gradient_F.  A step of this benchmark is one batch: all 16 shot
 positions propagated together by the batched path (ifos_batch_run).  The
-plain one-shot path (ifos_ctx_run_shot, the drop-in's shape) is timed on one
-shot beside it.  Inputs (models, observed data) are resident in HBM before
-the timed region.  Prints one JSON line.
+plain one-shot path (ifos_ctx_run_shot, the drop-in's shape) and the
+checkpointed time-domain gradient (ifos_ctx_run_shot_td) are timed on one
+shot each beside it.  Inputs (models, observed data) are resident in HBM
+before the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad276|toy84]
                   [--path batched|plain] [--shots-per-batch S] [--chunk C] [--graph]
+
+Two other workloads print a line of their own:
+  --config fwd536   forward-only bandwidth case, 536 x 280 x 536, NT 2000 (Gcell-updates/s)
+  --config levels3  three uneven grid levels x 32 shots in one graph launch (forward solves/s)
 """
 from __future__ import annotations
 
