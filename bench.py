@@ -44,13 +44,13 @@ def kernel_names(path, nf, chunk, tiles=None):
                 "s_interior": "ifos::k_s_march<4, false, false, false>",
                 "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs<false>",
                 "dft": f"ifos::k_dft_nf<{nf}>", "dft_kahan": f"ifos::k_dft_kahan_nf<{nf}>",
-                "gradient": "ifos::k_grad_march<4>"}
+                "gradient": "ifos::k_grad_march<4, 0>"}
     tv, ts = tiles
     return {"v_fused": f"ifos::k_v_march<{tv[0]}, true, false, {b(tv[4])}>",
             "s_fused": f"ifos::k_s_march<{ts[0]}, true, false, {b(ts[4])}>",
             "vs_fused": "ifos::k_vs_fused<16, false>",
             "dft": f"ifos::k_dft_chunk<{nf}, {chunk}, false>", "dft_kahan": f"ifos::k_dft_chunk<{nf}, {chunk}, true>",
-            "gradient": "ifos::k_grad_march<4>"}
+            "gradient": "ifos::k_grad_march<4, 0>"}
 
 
 # tile shapes {TI, jc, group, split, pipe} used without the autotuner (its
