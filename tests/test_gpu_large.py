@@ -78,3 +78,58 @@ def test_bandwidth_case_shape_matches_oracle():
     for c in range(3):
         assert max_ulp(sb[c][:, 1:], ref["syn"][c][:, 1:]) == 0, f"batched path, seismogram {c}"
     b.close()
+
+
+@pytest.mark.timeout(900)
+def test_gradient_workload_shape_batched_matches_plain():
+    """the 276^3 gradient workload's grid, model, receivers and frequencies (a
+    200-step record): two shots through the batched path (autotuned tiles)
+    give the plain path's seismograms, misfits and accumulated gradient bit
+    for bit"""
+    import ifosgpu as ig
+    import workloads as wl
+    w = wl.grad276()
+    w.nt = 200
+    true, start = wl.random_models(w)
+    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+    d_true = [G.to_dev(a) for a in true]
+    d_start = [G.to_dev(a) for a in start]
+    del true, start
+    shots = [w.shots[0], w.shots[5]]
+    ntr = len(w.receivers)
+    _note(f"grad276 grid, NT={w.nt}: observed data")
+    ctx = ig.ShotContext(G, len(w.finv), ntr)
+    ctx.set_cpml(w.vppml, w.fpml)
+    ctx.set_freqs(w.finv, w.tast)
+    d_syn = ig.DBuf((3, ntr, w.nt + 1))
+    ctx.set_model(*d_true)
+    obs = []
+    for s in shots:
+        ctx.run_shot(s, w.receivers, d_syn, None, want_adjoint=False)
+        obs.append(d_syn.clone())
+    _note("plain path, two gradient shots")
+    ctx.set_model(*d_start)
+    gp = [G.empty() for _ in range(3)]
+    l2p, synp = [], []
+    for n, s in enumerate(shots):
+        l2p.append(ctx.run_shot(s, w.receivers, d_syn, gp, d_obs=obs[n]))
+        synp.append(d_syn.numpy())
+    ctx.close()
+    _note("batched path, the same two shots")
+    b = ig.BatchContext(G, 2, len(w.finv), ntr, ig.DFT_COMPAT, 8)
+    b.set_cpml(w.vppml, w.fpml)
+    b.set_freqs(w.finv, w.tast)
+    b.set_model(*d_start)
+    b.autotune(reps=1)
+    gb = [G.empty() for _ in range(3)]
+    d_b = ig.DBuf((2, 3, ntr, w.nt + 1))
+    l2b = b.run(shots, w.receivers, d_syn=d_b, grads=gb, d_obs=obs)
+    sb = d_b.numpy()
+    b.close()
+    for n in range(2):
+        assert max_ulp(sb[n][:, :, 1:], synp[n][:, :, 1:]) == 0, f"seismograms, shot {n}"
+        assert l2b[n] == l2p[n], f"misfit, shot {n}"
+    for q in range(3):
+        a, c = G.from_dev(gb[q]), G.from_dev(gp[q])
+        assert np.abs(c).max() > 0
+        assert max_ulp(a, c) == 0, f"gradient {q + 1}"
