@@ -8,6 +8,11 @@ device layout) through the plain path and the batched path and compares
 every velocity of the grid after the last step with the oracle's, bit for
 bit; the seismograms of those steps are compared too.  The full 2000-step
 run is measured by `bench.py --config fwd536`.
+
+The gradient workload's 276^3 grid is checked the same way for the batched
+path: two gradient shots of a 200-step record through the batched kernels
+(the tiles the autotuner picks on that grid) against the plain path, which
+the oracle pins, bit for bit.
 """
 import sys
 
