@@ -166,6 +166,20 @@ __device__ inline void bstore(float v, rsrc_t r, uint32_t voff, uint32_t soff)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
 }
+// the same with a cache policy CP: IFOS_CPOL_NT is the non-temporal
+// (streaming) hint, for words a kernel touches once, so that they do not
+// evict the neighbour lines other workgroups still read from L2
+constexpr int IFOS_CPOL_NT = 2;
+template <int CP>
+__device__ inline float bload_cp(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, CP));
+}
+template <int CP>
+__device__ inline void bstore_cp(float v, rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, CP);
+}
 
 DevGeom make_dev_geom(const ifos_geom *g);
 

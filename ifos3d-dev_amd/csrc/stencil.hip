@@ -265,6 +265,14 @@ __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
 // + (dj+2)*SJ + (di+2)*SI (scalar): all non-negative, all uniform but vo.
 #define LD(r, dj, di, dk) bload(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
 #define LDH(r, dj) bload(r, vo + khv, ((dj) + 2) * SJ + 2 * SI)
+// words a march touches once (its own cell's old value, its stores) carry
+// the policy CPN, the streaming hint in the whole-domain launches (CPML
+// template true: the batched path, whose fields are far larger than the
+// 256 MB Infinity Cache) and the default policy in the interior launches of
+// the plain path (measured: streaming there makes them 10-25 % slower, as
+// the next launches re-read those words from the Infinity Cache)
+#define LDN(r, dj, di, dk) bload_cp<CPN>(r, vo + ((dk) + 2) * 4, ((dj) + 2) * SJ + ((di) + 2) * SI)
+#define STN(v, r) bstore_cp<CPN>(v, r, vo + 8, 2 * SJ + 2 * SI)
 #define ST(v, r) bstore(v, r, vo + 8, 2 * SJ + 2 * SI)
 
 // Operands of one plane j of the velocity march: the in-plane neighbours,
@@ -293,7 +301,7 @@ struct VPlane {
             P.szz_d = LD(rszz, D, 0, 2);                                                                       \
         }                                                                                                      \
         P.rip = LD(rrip, D, 0, 0); P.rjp = LD(rrjp, D, 0, 0); P.rkp = LD(rrkp, D, 0, 0);                         \
-        P.vx0 = LD(rvx, D, 0, 0); P.vy0 = LD(rvy, D, 0, 0); P.vz0 = LD(rvz, D, 0, 0);                            \
+        P.vx0 = LDN(rvx, D, 0, 0); P.vy0 = LDN(rvy, D, 0, 0); P.vz0 = LDN(rvz, D, 0, 0);                         \
         P.sxy_n = LD(rsxy, (D) + 1, 0, 0); P.syz_n = LD(rsyz, (D) + 1, 0, 0); P.syy_n = LD(rsyy, (D) + 2, 0, 0); \
     } while (0)
 
@@ -306,6 +314,7 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
     bool krow;
     if (!decode_tile(T, TI, k, i, jb, je, shot, R, &krow)) return;
     const uint32_t khv = khalo_off();
+    constexpr int CPN = CPML ? IFOS_CPOL_NT : 0;
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const int64_t so = (int64_t)shot * S.fstride;
@@ -360,13 +369,13 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
         syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = cur.syz_n;
         syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = cur.syy_n;
         if (!PIPE && j < je) VLOAD(nxt, 1);  // next plane's operands, issued before this plane's stores
-        ST(nvx, rvx);
-        ST(nvy, rvy);
-        ST(nvz, rvz);
+        STN(nvx, rvx);
+        STN(nvy, rvy);
+        STN(nvz, rvz);
         if (SNAP) {
-            bstore(nvx, rring, vo + 8, 2 * SJ + 2 * SI + N.soff);
-            bstore(nvy, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + g.fbytes);
-            bstore(nvz, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + 2 * g.fbytes);
+            bstore_cp<CPN>(nvx, rring, vo + 8, 2 * SJ + 2 * SI + N.soff);
+            bstore_cp<CPN>(nvy, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + g.fbytes);
+            bstore_cp<CPN>(nvz, rring, vo + 8, 2 * SJ + 2 * SI + N.soff + 2 * g.fbytes);
         }
         cur = nxt;
     }
@@ -396,11 +405,11 @@ struct SPlane {
         }                                                                                                      \
         P.uipjp = LD(ruipjp, D, 0, 0); P.ujpkp = LD(rujpkp, D, 0, 0); P.uipkp = LD(ruipkp, D, 0, 0);             \
         P.pi = LD(rpi, D, 0, 0); P.u = LD(ru, D, 0, 0);                                                        \
-        P.sxx0 = LD(rsxx, D, 0, 0); P.syy0 = LD(rsyy, D, 0, 0); P.szz0 = LD(rszz, D, 0, 0);                      \
-        P.sxy0 = LD(rsxy, D, 0, 0); P.syz0 = LD(rsyz, D, 0, 0); P.sxz0 = LD(rsxz, D, 0, 0);                      \
+        P.sxx0 = LDN(rsxx, D, 0, 0); P.syy0 = LDN(rsyy, D, 0, 0); P.szz0 = LDN(rszz, D, 0, 0);                   \
+        P.sxy0 = LDN(rsxy, D, 0, 0); P.syz0 = LDN(rsyz, D, 0, 0); P.sxz0 = LDN(rsxz, D, 0, 0);                   \
         P.vx_n = LD(rvx, (D) + 2, 0, 0); P.vz_n = LD(rvz, (D) + 2, 0, 0); P.vy_n = LD(rvy, (D) + 1, 0, 0);       \
         if (VISCO) {                                                                                           \
-            for (int q_ = 0; q_ < 6; ++q_) P.r0[q_] = LD(rr[q_], D, 0, 0);                                     \
+            for (int q_ = 0; q_ < 6; ++q_) P.r0[q_] = LDN(rr[q_], D, 0, 0);                                    \
             P.ts = LD(rtaus, D, 0, 0); P.tp = LD(rtaup, D, 0, 0);                                              \
             P.tsipjp = LD(rtsipjp, D, 0, 0); P.tsjpkp = LD(rtsjpkp, D, 0, 0); P.tsipkp = LD(rtsipkp, D, 0, 0);  \
         }                                                                                                      \
@@ -415,6 +424,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
     bool krow;
     if (!decode_tile(T, TI, k, i, jb, je, shot, R, &krow)) return;
     const uint32_t khv = khalo_off();
+    constexpr int CPN = CPML ? IFOS_CPOL_NT : 0;
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
     const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
@@ -498,15 +508,15 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
         vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = cur.vz_n;
         vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = cur.vy_n;
         if (!PIPE && j < je) SLOAD(nxt, 1);
-        ST(sn[3], rsxy);
-        ST(sn[4], rsyz);
-        ST(sn[5], rsxz);
-        ST(sn[0], rsxx);
-        ST(sn[1], rsyy);
-        ST(sn[2], rszz);
+        STN(sn[3], rsxy);
+        STN(sn[4], rsyz);
+        STN(sn[5], rsxz);
+        STN(sn[0], rsxx);
+        STN(sn[1], rsyy);
+        STN(sn[2], rszz);
         if (VISCO) {
 #pragma unroll
-            for (int n = 0; n < 6; ++n) ST(rn[n], rr[n]);
+            for (int n = 0; n < 6; ++n) STN(rn[n], rr[n]);
         }
         cur = nxt;
     }
@@ -851,6 +861,9 @@ __global__ __launch_bounds__(64 * TI) void k_grad_march(DevGeom g, MTiles T, Gra
 
 #undef LD
 #undef ST
+#undef LDH
+#undef LDN
+#undef STN
 
 // ---------------------------------------------------------------------------
 // slab kernels: one thread per CPML cell, k fastest within each box
