@@ -551,7 +551,7 @@ def main():
     ap.add_argument("--level-shots", type=int, default=32, help="shots per grid level (levels3)")
     ap.add_argument("--path", default="batched", choices=["batched", "plain"])
     ap.add_argument("--shots-per-batch", type=int, default=16)
-    ap.add_argument("--chunk", type=int, default=8, help="DFT samples per accumulator pass")
+    ap.add_argument("--chunk", type=int, default=16, help="DFT samples per accumulator pass")
     ap.add_argument("--graph", action="store_true", help="replay the time loops from hipGraphs")
     ap.add_argument("--no-autotune", action="store_true", help="default tile shapes instead of the autotuner's")
     ap.add_argument("--fused", action="store_true",

@@ -192,9 +192,11 @@ __global__ void k_bsurface(DevGeom g, BFields F, float *sxy, float *syz, const f
 // chunked DFT: nsnap (<= CH) snapshots of one shot's v, every accumulator
 // of the cell updated nsnap times in registers.  trig[(c*NF + l)*2 + {0,1}]
 // are the cos/sin factors of snapshot c (disc_fourier.c:42-44).  One thread
-// per cell of [1..N]^3 and shot (blockIdx.z = shot * ny + j - 1); the
-// frequencies are walked in order with the next frequency's accumulators
-// loaded while the current one is updated.
+// per cell of [1..N]^3, shot and velocity component (blockIdx.z = shot * ny
+// + j - 1, blockIdx.y = 3 * (i row block) + component), so a thread holds
+// one component's snapshots and its two accumulators; the frequencies are
+// walked in order with the next frequency's accumulators loaded while the
+// current one is updated.
 struct FBase {
     float *p[6];
 };
@@ -204,30 +206,29 @@ __global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring,
                                                   FBase F, FBase Cc, int64_t fshot, uint32_t tot,
                                                   const double *__restrict__ trig)
 {
+    const int q = (int)(blockIdx.y % 3u);  // velocity component: accumulators q (Re) and 3 + q (Im)
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = (int)(blockIdx.y / 3u) * blockDim.y + threadIdx.y + 1;
     const int j = (int)(blockIdx.z % (unsigned)g.ny) + 1;
     const int s = (int)(blockIdx.z / (unsigned)g.ny);
     if (k > g.nz || i > g.nx) return;
     const int64_t cell = at(g, j, i, k);
     const uint32_t vo = (uint32_t)cell * 4u;
     const int64_t fe = g.fbytes / 4;
-    const float *rp = ring + (int64_t)s * rstride + cell;
-    float v[CH][3];
+    const float *rp = ring + (int64_t)s * rstride + (int64_t)q * fe + cell;
+    float v[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c)
-        if (c < nsnap)
+        if (c < nsnap) v[c] = __builtin_nontemporal_load(rp + 3 * c * fe);
+    rsrc_t r[2], rc[2];
 #pragma unroll
-            for (int n = 0; n < 3; ++n) v[c][n] = __builtin_nontemporal_load(rp + (3 * c + n) * fe);
-    rsrc_t r[6], rc[6];
-#pragma unroll
-    for (int n = 0; n < 6; ++n) {
-        r[n] = make_rsrc(F.p[n] + (int64_t)s * fshot, tot);
-        if (KAHAN) rc[n] = make_rsrc(Cc.p[n] + (int64_t)s * fshot, tot);
+    for (int n = 0; n < 2; ++n) {
+        r[n] = make_rsrc(F.p[q + 3 * n] + (int64_t)s * fshot, tot);
+        if (KAHAN) rc[n] = make_rsrc(Cc.p[q + 3 * n] + (int64_t)s * fshot, tot);
     }
-    float a[6], cm[6], an[6], cn[6];
+    float a[2], cm[2], an[2], cn[2];
 #pragma unroll
-    for (int n = 0; n < 6; ++n) {
+    for (int n = 0; n < 2; ++n) {
         a[n] = bload(r[n], vo, 0);
         cm[n] = KAHAN ? bload(rc[n], vo, 0) : 0.0f;
     }
@@ -235,18 +236,18 @@ __global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring,
     for (int l = 0; l < NF; ++l) {
         if (l + 1 < NF) {
 #pragma unroll
-            for (int n = 0; n < 6; ++n) {
+            for (int n = 0; n < 2; ++n) {
                 an[n] = bload(r[n], vo, (uint32_t)(l + 1) * g.fbytes);
                 cn[n] = KAHAN ? bload(rc[n], vo, (uint32_t)(l + 1) * g.fbytes) : 0.0f;
             }
         }
 #pragma unroll
-        for (int n = 0; n < 6; ++n) {
+        for (int n = 0; n < 2; ++n) {  // n = 0: cos (Re), 1: sin (Im)
             float acc = a[n], cmp = cm[n];
 #pragma unroll
             for (int c = 0; c < CH; ++c)
                 if (c < nsnap) {
-                    const double term = (double)v[c][n % 3] * trig[(c * NF + l) * 2 + (n < 3 ? 0 : 1)];
+                    const double term = (double)v[c] * trig[(c * NF + l) * 2 + n];
                     if (KAHAN) {
                         const float yv = (float)(term - (double)cmp);
                         const float t = acc + yv;
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(256) void k_dft_chunk(DevGeom g, const float *ring,
         }
         if (l + 1 < NF) {
 #pragma unroll
-            for (int n = 0; n < 6; ++n) {
+            for (int n = 0; n < 2; ++n) {
                 a[n] = an[n];
                 cm[n] = cn[n];
             }
@@ -597,7 +598,7 @@ static int dft_chunk(ifos_batch *b, int back, int ns, int q0, int nsnap, hipStre
     const int64_t fshot = (int64_t)b->nf_max * b->fe;
     const uint32_t tot = (uint32_t)((size_t)b->nf * b->fe * 4);
     const double *tr = b->trig[back] + (size_t)q0 * b->nf * 2;
-    dim3 block(64, 4), grid((unsigned)((g->nz + 63) / 64), (unsigned)((g->nx + 3) / 4), (unsigned)(ns * g->ny));
+    dim3 block(64, 4), grid((unsigned)((g->nz + 63) / 64), (unsigned)(3 * ((g->nx + 3) / 4)), (unsigned)(ns * g->ny));
     const bool kahan = b->prec == IFOS_DFT_KAHAN;
     ProfScope ps_(IFOS_K_DFT, st);
 #define IFOS_CHUNK_LAUNCH(NFV, CHV)                                                                             \
