@@ -552,14 +552,15 @@ void discfourier(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2, int nt, f
     const ifos_geom &g = geom();
     if (nx1 != 1 || ny1 != 1 || nz1 != 1 || nx2 != NX || ny2 != NY || nz2 != NZ)
         fatal("discfourier over a sub-box not supported (whole local grid only)");
-    if (pshot != 0) fatal("discfourier: Hessian wavefields (pshot %d) not supported", pshot);
+    if (pshot < 0) fatal("discfourier: pshot %d", pshot);
     ifos_fields f = {};
     f.vx = m3(vx, WAVE_TOP);
     f.vy = m3(vy, WAVE_TOP);
     f.vz = m3(vz, WAVE_TOP);
     float ****F[6] = {Fvx, Fvy, Fvz, Fvxi, Fvyi, Fvzi};
     float *d[6];
-    for (int n = 0; n < 6; ++n) d[n] = m4(F[n], nf);
+    // disc_fourier.c:40: frequency l of Hessian wavefield pshot is entry pshot*nf + l
+    for (int n = 0; n < 6; ++n) d[n] = m4(F[n], (pshot + 1) * nf) + (size_t)pshot * nf * S.fe;
     ck(ifos_discfourier(&g, &f, d, nullptr, finv, nf, nt, ntast, back, IFOS_DFT_COMPAT, nullptr), "discfourier");
 }
 
@@ -590,6 +591,35 @@ void gradient_F(int nx, int ny, int nz, float ****fvx, float ****fvy, float ****
     }
     ck(ifos_gradient_F(&g, df, db, &m, finv, nf, dg, nullptr), "gradient_F");
     for (int n = 0; n < 3; ++n) down3(G[n], AVG, dg[n]);
+}
+
+void hess_F(int nx, int ny, int nz, float ****fvx, float ****fvy, float ****fvz, float ****fivx, float ****fivy,
+            float ****fivz, float ****bvx, float ****bvy, float ****bvz, float ****bivx, float ****bivy,
+            float ****bivz, float ***hess1, float ***hess2, float ***hess3, int nt, float ***rho, float ***pi,
+            float ***u, float *finv, int nf, int ntr_hess)
+{
+    (void)nt;
+    const ifos_geom &g = geom();
+    if (nx != NX || ny != NY || nz != NZ) fatal("hess_F: grid %dx%dx%d != local grid", nx, ny, nz);
+    if (ntr_hess < 0) fatal("hess_F: ntr_hess %d", ntr_hess);
+    float ****Ff[6] = {fvx, fvy, fvz, fivx, fivy, fivz}, ****Fb[6] = {bvx, bvy, bvz, bivx, bivy, bivz};
+    float *df[6], *db[6];
+    for (int n = 0; n < 6; ++n) {
+        df[n] = m4(Ff[n], nf);
+        db[n] = m4(Fb[n], nf * (ntr_hess + 1));
+    }
+    ifos_model m = {};
+    m.rho = m3(rho, MODEL);
+    m.pi = m3(pi, MODEL);
+    m.u = m3(u, MODEL);
+    float ***H[3] = {hess1, hess2, hess3};
+    float *dh[3];
+    for (int n = 0; n < 3; ++n) {  // the host may have changed them (readhess, outgrad)
+        dh[n] = m3(H[n], AVG, false);
+        up3(H[n], AVG, dh[n]);
+    }
+    ck(ifos_hess_F(&g, df, db, ntr_hess, &m, finv, nf, dh, nullptr), "hess_F");
+    for (int n = 0; n < 3; ++n) down3(H[n], AVG, dh[n]);
 }
 
 void av_mat(float ***rho, float ***pi, float ***u, float ***taus, float ***taup, float ***uipjp, float ***ujpkp,
@@ -687,10 +717,13 @@ void zero_invers(int nx, int ny, int nz, float ****Ffvx, float ****Ffvy, float *
 {
     (void)nx, (void)ny, (void)nz;
     geom();
-    if (ntr_hess != 0) fatal("zero_invers: Hessian wavefields (ntr_hess %d) not supported", ntr_hess);
+    if (ntr_hess < 0) fatal("zero_invers: ntr_hess %d", ntr_hess);
     float ****F[12] = {Ffvx, Ffvy, Ffvz, Ffivx, Ffivy, Ffivz, Fbvx, Fbvy, Fbvz, Fbivx, Fbivy, Fbivz};
-    for (int n = 0; n < 12; ++n)
-        ck(ifos_fill(m4(F[n], nfmax, false), S.fe * (size_t)nfmax, 0.0f, nullptr), "zero_invers");
+    // zero_invers.c:48-79: the adjoint side holds nfmax * (ntr_hess + 1) entries
+    for (int n = 0; n < 12; ++n) {
+        const int m = n < 6 ? nfmax : nfmax * (ntr_hess + 1);
+        ck(ifos_fill(m4(F[n], m, false), S.fe * (size_t)m, 0.0f, nullptr), "zero_invers");
+    }
 }
 
 void zero_grad(int nx, int ny, int nz, float ***grad1, float ***grad2, float ***grad3)

@@ -184,6 +184,16 @@ void zero_wavefield(int NX, int NY, int NZ, float ***vx, float ***vy,
                     float ***psi_vzy, float ***psi_vxz, float ***psi_vyz,
                     float ***psi_vzz);
 
+/* fd.h:418 (hess_F.c:28): the receiver-side DFT fields of receiver n,
+ * frequency l are entry n*nf + l of fb* / fib* (written by discfourier with
+ * pshot = n) */
+void hess_F(int nx, int ny, int nz, float ****fvx, float ****fvy,
+            float ****fvz, float ****fivx, float ****fivy, float ****fivz,
+            float ****bvx, float ****bvy, float ****bvz, float ****bivx,
+            float ****bivy, float ****bivz, float ***hess1, float ***hess2,
+            float ***hess3, int nt, float ***rho, float ***pi, float ***u,
+            float *finv, int nf, int ntr_hess);
+
 /* fd.h:393 (zero_invers.c:26) */
 void zero_invers(int NX, int NY, int NZ, float ****Ffvx, float ****Ffvy,
                  float ****Ffvz, float ****Ffivx, float ****Ffivy,

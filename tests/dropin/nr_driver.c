@@ -349,3 +349,66 @@ done:
     (void)j;
     return rc;
 }
+
+/*
+ * The Hessian wavefields and hess_F through the drop-in, in the order of
+ * ifos3d.c:880-983: zero_invers with ntr_hess receiver-side sets, the
+ * forward DFT (pshot 0, back 0) of nsteps velocity snapshots vf
+ * [nsteps][3] (canonical), the receiver-side DFTs (pshot p = 1..ntr_hess,
+ * back 1) of vb [ntr_hess][nsteps][3], then hess_F into hess [3]
+ * (canonical, accumulated).  Each snapshot is a fresh tensor, as the drop-in
+ * uploads a wavefield when it first sees it (on the reference's path the
+ * velocities stay on the device between the updates).
+ */
+int dd_hess(int nx, int ny, int nz, int fw, float dx, float dy, float dz, float dt, int nt, int fdcoeff,
+            const float *rho_c, const float *pi_c, const float *u_c, const float *vf, const float *vb, int nsteps,
+            int ntr_hess, const float *finv, int nf, float *hess)
+{
+    const int lo = -2;
+    const size_t cs = (size_t)(ny + 5) * (nx + 5) * (nz + 5);
+    float ***rho, ***pi, ***u, ***h[3], ***v[3];
+    float ****F[12];
+    float ****snaps = (float ****)malloc(sizeof(float ***) * 3 * (size_t)(ntr_hess + 1) * nsteps);
+    int n, p, s, c, ns_ = 0;
+
+    NX = nx; NY = ny; NZ = nz; FW = fw; DX = dx; DY = dy; DZ = dz; DT = dt; NT = nt;
+    TIME = (NT - 1) * DT;
+    FDORDER = 4; FDCOEFF = fdcoeff; ABS_TYPE = 1; FREE_SURF = 0; L = 0; SEISMO = 1;
+    POS[1] = POS[2] = POS[3] = 0; NPROCX = NPROCY = NPROCZ = 1; ALPHA = BETA = 0.0f;
+
+    rho = nr3(0, NY + 1, 0, NX + 1, 0, NZ + 1); pi = nr3(0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    u = nr3(0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    from_canon(rho, rho_c, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    from_canon(pi, pi_c, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    from_canon(u, u_c, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    for (c = 0; c < 3; c++) h[c] = nr3(1, NY, 1, NX, 1, NZ);
+    for (n = 0; n < 12; n++) F[n] = nr4(1, n < 6 ? nf : nf * (ntr_hess + 1), lo, NY + 2, -1, NX + 2, -1, NZ + 2);
+
+    zero_invers(NX, NY, NZ, F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], F[11], nf, ntr_hess);
+    for (p = 0; p <= ntr_hess; p++)
+        for (s = 0; s < nsteps; s++) {
+            const float *src = p == 0 ? vf + (size_t)s * 3 * cs : vb + ((size_t)(p - 1) * nsteps + s) * 3 * cs;
+            for (c = 0; c < 3; c++) {
+                v[c] = snaps[ns_++] = nr3(lo, NY + 2, -1, NX + 2, -1, NZ + 2);
+                from_canon(v[c], src + (size_t)c * cs, lo, NY + 2, -1, NX + 2, -1, NZ + 2);
+            }
+            if (p == 0)
+                discfourier(1, NX, 1, NY, 1, NZ, s + 1, v[0], v[1], v[2], F[0], F[1], F[2], F[3], F[4], F[5],
+                            (float *)finv, nf, 1, 0, 0);
+            else
+                discfourier(1, NX, 1, NY, 1, NZ, s + 1, v[0], v[1], v[2], F[6], F[7], F[8], F[9], F[10], F[11],
+                            (float *)finv, nf, 1, p, 1);
+            /* the tensors stay allocated until the release: the drop-in keys its mirrors by the pointer */
+        }
+    for (c = 0; c < 3; c++) from_canon(h[c], hess + (size_t)c * cs, 1, NY, 1, NX, 1, NZ);
+    hess_F(NX, NY, NZ, F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], F[11], h[0], h[1], h[2],
+           NT, rho, pi, u, (float *)finv, nf, ntr_hess);
+    for (c = 0; c < 3; c++) to_canon(h[c], hess + (size_t)c * cs, 1, NY, 1, NX, 1, NZ);
+    ifos_dropin_release();
+    for (n = 0; n < ns_; n++) nr3_free(snaps[n], lo, -1, -1);
+    free(snaps);
+    for (n = 0; n < 12; n++) nr4_free(F[n], 1, n < 6 ? nf : nf * (ntr_hess + 1), lo, -1, -1);
+    for (c = 0; c < 3; c++) nr3_free(h[c], 1, 1, 1);
+    nr3_free(rho, 0, 0, 0); nr3_free(pi, 0, 0, 0); nr3_free(u, 0, 0, 0);
+    return 0;
+}

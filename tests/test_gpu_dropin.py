@@ -1,10 +1,11 @@
 """The drop-in boundary: the reference prototypes (update_v, update_v_CPML,
 update_s_elastic, update_s_CPML_elastic, psource, seismo, surface_elastic,
-discfourier, gradient_F, av_mat, CPML_coeff, CPML_ini_elastic,
+discfourier, gradient_F, hess_F, av_mat, CPML_coeff, CPML_ini_elastic,
 zero_wavefield, zero_invers) called on Numerical-Recipes arrays by an
 IFOS3D-shaped driver (tests/dropin/nr_driver.c), in the order of
-ifos3d.c:712-975.  Seismograms, misfit and gradients must equal the oracle's
-bit for bit."""
+ifos3d.c:712-975 (and of its Hessian loop, ifos3d.c:880-983, for the
+Hessian wavefields and hess_F).  Seismograms, misfit, gradients and
+Hessians must equal the oracle's bit for bit."""
 import ctypes as C
 import os
 import sys
@@ -100,3 +101,43 @@ def test_dropin_viscoelastic_shot():
     assert np.float32(l2) == np.float32(ref["l2"])
     for n in range(3):
         assert max_ulp(g.interior(grads[n]), g.interior(ref["grads"][n])) == 0, f"grad{n + 1}"
+
+
+def test_dropin_hessian_matches_oracle():
+    """discfourier with Hessian wavefields (pshot 1..2) and hess_F through the
+    drop-in prototypes, against og_discfourier and og_hess_F"""
+    g = cs.dft_grid()
+    print("drop-in Hessian wavefields and hess_F", flush=True)
+    lib = C.CDLL(DRIVER)
+    o = Oracle("f32")
+    rng = np.random.default_rng(17)
+    nsteps, ntr_hess, finv = 4, 2, [40.0, 70.0]
+    nf = len(finv)
+    vf = (rng.standard_normal((nsteps, 3) + g.shape)).astype(np.float32)
+    vb = (rng.standard_normal((ntr_hess, nsteps, 3) + g.shape)).astype(np.float32)
+    for a in (vf, vb):  # the halos the velocity fields carry are 0
+        a[..., :3, :, :] = 0; a[..., -2:, :, :] = 0
+        a[..., :, :3, :] = 0; a[..., :, -2:, :] = 0
+        a[..., :, :, :3] = 0; a[..., :, :, -2:] = 0
+    rho, pi, u = (np.ascontiguousarray(x, np.float32) for x in cs.shot_models(g)[1])
+    hess = np.zeros((3,) + g.shape, np.float32)
+    fv = np.asarray(finv, np.float32)
+    rc = lib.dd_hess(g.nx, g.ny, g.nz, g.fw, C.c_float(g.dx), C.c_float(g.dy), C.c_float(g.dz), C.c_float(g.dt),
+                     g.nt, g.fdcoeff, _p(rho), _p(pi), _p(u), _p(vf), _p(vb), nsteps, ntr_hess, _p(fv), nf, _p(hess))
+    assert rc == 0
+    Ff = [np.zeros((nf,) + g.shape, np.float32) for _ in range(6)]
+    Fb = [np.zeros((nf * (ntr_hess + 1),) + g.shape, np.float32) for _ in range(6)]
+    for s in range(nsteps):
+        o.discfourier(g, s + 1, *vf[s], Ff[:3], Ff[3:], finv, 1, 0)
+    for p in range(1, ntr_hess + 1):
+        sl = [np.ascontiguousarray(F[p * nf:(p + 1) * nf]) for F in Fb]
+        for s in range(nsteps):
+            o.discfourier(g, s + 1, *vb[p - 1][s], sl[:3], sl[3:], finv, 1, 1)
+        for F, x in zip(Fb, sl):
+            F[p * nf:(p + 1) * nf] = x
+    want = [o.zeros(g) for _ in range(3)]
+    o.hess_F(g, Ff, Fb, want, rho, pi, u, finv, ntr_hess)
+    for n in range(3):
+        w, got = g.interior(want[n]), g.interior(hess[n])
+        assert np.abs(w).max() > 0
+        assert max_ulp(got, w) == 0, f"hess{n + 1}"
