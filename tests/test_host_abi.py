@@ -97,3 +97,42 @@ def test_receiver_chain():
     assert ch[:, 0].tolist() == [1, 1, 0, 1, 0, 0]
     assert ch[:, 1].tolist() == [2, 5, 4, -1, -1, -1]
     assert ig.rec_chain(np.zeros((0, 3), np.int32)).shape == (0, 2)
+
+
+def test_entry_points_reject_bad_arguments_before_any_device_call():
+    """argument checks of the SU I/O, grid-level and time-domain entry points
+    return IFOS_EINVAL with a message and touch no device"""
+    import ifosgpu as ig
+    lib = ig.lib()
+    EINVAL = -1
+    last = lambda: lib.ifos_last_error().decode()
+    # SU I/O: ns out of range, a missing section, nothing to read
+    assert lib.ifos_su_write(b"/tmp/x.su", None, 2, 0, None, None, None, C.c_float(1), C.c_float(1), C.c_float(1),
+                             C.c_float(1e-3), 1) == EINVAL
+    assert "ifos_su_write" in last()
+    assert lib.ifos_su_write(b"/tmp/x.su", None, 2, 70000, None, None, None, C.c_float(1), C.c_float(1),
+                             C.c_float(1), C.c_float(1e-3), 1) == EINVAL
+    assert lib.ifos_su_read(b"/nonexistent/x.su", 3, 10, None) == EINVAL
+    assert lib.ifos_su_read(b"/nonexistent/x.su", 0, 10, None) == 0  # readseis.c:53: no traces, no file
+    # grid levels: none, too many
+    assert lib.ifos_levels_run(None, 0, None, None, None, 0) == EINVAL
+    assert "ifos_levels_run" in last()
+    # time-domain gradient: no context / no buffers
+    assert lib.ifos_ctx_run_shot_td(None, None, 0, None, None, None, None) == EINVAL
+    assert "ifos_ctx_run_shot_td" in last()
+
+
+def test_td_bytes_formula():
+    """ifos_td_bytes: (ceil(NT/K) + 1) states + K velocity triples + 6 running
+    displacement fields + 3 FP64 sums"""
+    import ifosgpu as ig
+    g = small_grid(nx=30, ny=28, nz=32, fw=6, nt=100)
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dy, g.dz, g.dt, g.nt)
+    fe = G.field_elems()
+    psi = sum(G.psi_elems(n) for n in range(9))
+    for every, L in ((10, 0), (7, 1), (0, 0)):
+        K = every if every > 0 else round(np.sqrt(g.nt))
+        nck = -(-g.nt // K)
+        state = (9 + (6 if L else 0)) * fe * 4 + 2 * psi * 4
+        want = (nck + 1) * state + 3 * K * fe * 4 + 6 * fe * 4 + 3 * fe * 8
+        assert ig.ShotContext.td_bytes(G, every, L) == want
