@@ -575,7 +575,7 @@ def main():
             ap.error("--tiles wants two groups of five to eight integers separated by '/'")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(shots would be sharded one rank per GPU; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(shots would be sharded one rank per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import ifosgpu as ig
