@@ -574,7 +574,7 @@ def main():
         if len(DEFAULT_TILES) != 2 or any(len(t) not in (5, 6, 7, 8) for t in DEFAULT_TILES):
             ap.error("--tiles wants two groups of five to eight integers separated by '/'")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(shots would be sharded one rank per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
