@@ -17,6 +17,7 @@ before the timed region.  Prints one JSON line.
 Two other workloads print a line of their own:
   --config fwd536   forward-only bandwidth case, 536 x 280 x 536, NT 2000 (Gcell-updates/s)
   --config levels3  three uneven grid levels x 32 shots in one graph launch (forward solves/s)
+  --config aux276   the optimiser, Hessian, residual and SU steps at 276^3 (ms per call)
 """
 from __future__ import annotations
 
@@ -399,6 +400,115 @@ def run_fwd(args):
     print(json.dumps(out), flush=True)
 
 
+def run_aux(args):
+    """the steps on either side of the propagation at the 276^3 gradient
+    workload's size, each timed over a few calls: the L-BFGS update (two
+    parameters, five stored pairs, FP64 dots), conjugate, precon_grad
+    (DAMPTYPE 2: tapers round the 16 sources and 1024 receivers), modelupdate,
+    hess_F (one receiver-side set, 6 frequencies), the residual of one shot's
+    traces, and an SU write and read of one trace component"""
+    import tempfile
+    import ifosgpu as ig
+    import workloads as wl
+    w = wl.grad276()
+    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+    n1 = (w.nx * w.ny * w.nz)
+    cells = n1
+    rng = np.random.default_rng(3)
+    fe = G.field_elems()
+
+    def field(scale=1.0):
+        return ig.DBuf.from_numpy((scale * rng.standard_normal(G.dshape)).astype(np.float32))
+
+    def timed(fn, reps):
+        fn()
+        ig.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ig.sync()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    res = {}
+    grads = [field() for _ in range(3)]
+    # L-BFGS: fill the five slots, then time updates that use all of them
+    lb = ig.Lbfgs(G, 2, 5)
+    lb.savegrad(grads[0], grads[1])
+    step = ig.DBuf.from_numpy((0.01 * rng.standard_normal(2 * n1)).astype(np.float32))
+    it = [1]
+
+    def lb_update():
+        it[0] += 1
+        ww = (it[0] - 1) % 5 or 5
+        lb.mod(ww).copy_from(step)
+        lb.update(grads[0], grads[1], it[0])
+
+    for _ in range(6):
+        lb_update()
+    ms = timed(lb_update, 3)
+    k, n = 5, 2 * n1
+    res["lbfgs_update"] = {"ms": round(ms, 3), "bytes": 4 * (10 + 10 * k) * n,
+                           "GBps": round(4 * (10 + 10 * k) * n / (ms * 1e-3) / 1e9, 1),
+                           "bytes_rule": "(10 + 10 k) n floats, k = 5 pairs, n = 2 N^3"}
+    lb.close()
+    priors = [field() for _ in range(6)]
+    ig.conjugate(G, grads, priors, 1)
+    ms = timed(lambda: ig.conjugate(G, grads, priors, 0), 3)
+    res["conjugate"] = {"ms": round(ms, 3)}
+    src = np.array([[s[0], s[1], s[2]] for s in w.shots], np.int32)
+    ms = timed(lambda: ig.precon_grad(G, grads, 2, src, w.receivers), 2)
+    res["precon_grad"] = {"ms": round(ms, 3), "bytes": 24 * cells,
+                          "note": f"{len(src)} source and {len(w.receivers)} receiver tapers per cell (FP64 exp)"}
+    true, _ = wl.random_models(w)
+    dm = [G.to_dev(a) for a in true]
+    mod = ig.DBuf((2 * n1,))
+    small = [field(1e-6) for _ in range(3)]
+    ms = timed(lambda: ig.modelupdate(G, small, *dm, 1e-6, (1.0, 1.0, 1.0), 4000.0, 2300.0, 2000.0, bfgsmod=mod,
+                                      numpar=2), 3)
+    res["modelupdate"] = {"ms": round(ms, 3), "bytes": 44 * cells,
+                          "GBps": round(44 * cells / (ms * 1e-3) / 1e9, 1),
+                          "bytes_rule": "3 gradients + 3 model fields read, 3 model fields + 2 step slots written"}
+    nf = len(w.finv)
+    Ff = [ig.DBuf((nf * fe,)) for _ in range(6)]  # zero-filled: the timing does not depend on the values
+    Fb = [ig.DBuf((2 * nf * fe,)) for _ in range(6)]
+    hess = [G.empty() for _ in range(3)]
+    ms = timed(lambda: ig.hess_F(G, Ff, Fb, 1, dm, w.finv, hess), 2)
+    hb = (12 * 4 * nf + 36) * cells
+    res["hess_F"] = {"ms": round(ms, 3), "bytes": hb, "GBps": round(hb / (ms * 1e-3) / 1e9, 1),
+                     "bytes_rule": "48 nf + 36 B per cell (12 DFT fields per frequency, 3 Hessians and the model)"}
+    ntr, ns = len(w.receivers), w.nt
+    obs = ig.DBuf.from_numpy(rng.standard_normal((3, ntr, ns + 1)).astype(np.float32))
+    syn = ig.DBuf.from_numpy(rng.standard_normal((3, ntr, ns + 1)).astype(np.float32))
+    adj = ig.DBuf((3, ntr, ns + 2))
+    ms = timed(lambda: ig.residual(G, obs, syn, adj, ntr, ns), 5)
+    res["residual"] = {"ms": round(ms, 3), "traces": 3 * ntr, "samples": ns}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "vy.su")
+        sec = ig.DBuf.from_numpy(rng.standard_normal((ntr, ns + 1)).astype(np.float32))
+        ms_w = timed(lambda: ig.su_write(path, sec, ntr, ns, w.receivers, w.dh, w.dh, w.dh, w.dt, 1,
+                                         src_xyz=(1.0, 2.0, 3.0)), 3)
+        ms_r = timed(lambda: ig.su_read(path, ntr, ns, sec), 3)
+        mb = ntr * (240 + 4 * ns) / 1e6
+    res["su_write"] = {"ms": round(ms_w, 3), "MB": round(mb, 2)}
+    res["su_read"] = {"ms": round(ms_r, 3), "MB": round(mb, 2)}
+    out = {
+        "metric": "auxiliary steps at 276^3 (ms per call)",
+        "value": res["lbfgs_update"]["ms"],
+        "unit": "ms per L-BFGS update",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": res["lbfgs_update"]["ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 sums)",
+        "data": "synthetic",
+        "config": {"workload": f"grad276 grid {w.nx}^3, 2 inversion parameters, 5 L-BFGS pairs, {nf} frequencies, "
+                               f"{ntr} receivers x {ns} samples"},
+        "steps_ms": res,
+        "roofline": {"kernel": "lbfgs_update", "bound": "hbm", "achieved": res["lbfgs_update"]["GBps"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(res["lbfgs_update"]["GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_plain(w, G, d_start, obs, shots, precision):
     """the one-shot-per-launch path on the first `shots` shot positions"""
     import ifosgpu as ig
@@ -547,7 +657,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed batches (batched) or shots (plain)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84", "levels3", "fwd536"])
+    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84", "levels3", "fwd536", "aux276"])
     ap.add_argument("--level-shots", type=int, default=32, help="shots per grid level (levels3)")
     ap.add_argument("--path", default="batched", choices=["batched", "plain"])
     ap.add_argument("--shots-per-batch", type=int, default=16)
@@ -592,6 +702,9 @@ def main():
         return
     if args.config == "fwd536":
         run_fwd(args)
+        return
+    if args.config == "aux276":
+        run_aux(args)
         return
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
     if args.nt > 0:
