@@ -27,6 +27,9 @@ extern "C" {
 extern float DT, DX, DY, DZ, TIME, ALPHA, BETA, VPPML, FPML;
 extern int NT, NX, NY, NZ, FDORDER, FDCOEFF, ABS_TYPE, FW, FREE_SURF, L, SEISMO;
 extern int POS[4], NPROCX, NPROCY, NPROCZ;
+// the optimiser's (globvar.h:60-62)
+extern int LBFGS, NUMPAR, BFGSNUM;
+extern float VP0, VS0, RHO0, WEIGHT[3];
 }
 
 namespace {
@@ -56,6 +59,12 @@ struct State {
     int src_n = -1, src_back = -1;
     const void *seis_key = nullptr;
     int seis_ntr = -1;
+    // the L-BFGS history (bfgsgrad and bfgsscale live here; bfgsmod stays the
+    // host's), keyed by the caller's bfgsgrad matrix
+    ifos_lbfgs *lb = nullptr;
+    const void *lb_key = nullptr;
+    float *d_step = nullptr;  // modelupdate's bfgsmod slot
+    size_t cap_step = 0;
     std::string err;
 } S;
 
@@ -622,6 +631,103 @@ void hess_F(int nx, int ny, int nz, float ****fvx, float ****fvy, float ****fvz,
     for (int n = 0; n < 3; ++n) down3(H[n], AVG, dh[n]);
 }
 
+// ---- optimiser steps (lbfgs_save.c, lbfgs.c, modelupdate.c, conjugategrad.c) on the device:
+// the gradients, priors and model are uploaded from the host arrays at each
+// call and written back (the host reads them in between: outgrad, matcopy,
+// outmod); bfgsgrad and bfgsscale are kept on the device in FP64 sums,
+// bfgsmod is the host's (modelupdate writes it there, lbfgs uploads it)
+static ifos_lbfgs *lb_for(float **bfgsgrad)
+{
+    const ifos_geom &g = geom();
+    if (!S.lb || S.lb_key != bfgsgrad) {
+        if (S.lb) ifos_lbfgs_destroy(S.lb);
+        S.lb = nullptr;
+        if (NUMPAR < 1 || NUMPAR > 2 || BFGSNUM < 1) fatal("L-BFGS: NUMPAR %d, BFGSNUM %d", NUMPAR, BFGSNUM);
+        ck(ifos_lbfgs_create(&g, NUMPAR, BFGSNUM, &S.lb), "lbfgs");
+        S.lb_key = bfgsgrad;
+    }
+    return S.lb;
+}
+
+static float *upload_avg(float ***t)
+{
+    float *d = m3(t, AVG, false);
+    up3(t, AVG, d);
+    return d;
+}
+
+void lbfgs_savegrad(float ***grad1, float ***grad2, float ***grad3, float **bfgsgrad1)
+{
+    (void)grad3;
+    ifos_lbfgs *b = lb_for(bfgsgrad1);
+    ck(ifos_lbfgs_savegrad(b, upload_avg(grad1), NUMPAR > 1 ? upload_avg(grad2) : nullptr), "lbfgs_savegrad");
+}
+
+void lbfgs(float ***grad1, float ***grad2, float ***grad3, float *bfgsscale, float **bfgsmod, float **bfgsgrad,
+           int iteration)
+{
+    (void)grad3;
+    ifos_lbfgs *b = lb_for(bfgsgrad);
+    const size_t n = (size_t)NUMPAR * NX * NY * NZ;
+    for (int w = 1; w <= BFGSNUM; ++w)  // the model steps modelupdate wrote
+        hck(hipMemcpy(ifos_lbfgs_mod(b, w), &bfgsmod[w][1], n * 4, hipMemcpyHostToDevice), "upload bfgsmod");
+    float *d1 = upload_avg(grad1), *d2 = NUMPAR > 1 ? upload_avg(grad2) : nullptr;
+    ck(ifos_lbfgs_update(b, d1, d2, iteration, nullptr), "lbfgs");
+    down3(grad1, AVG, d1);
+    if (d2) down3(grad2, AVG, d2);
+    std::vector<double> sc(BFGSNUM);
+    ck(ifos_lbfgs_scale(b, sc.data()), "lbfgs");
+    for (int w = 1; w <= BFGSNUM; ++w) bfgsscale[w] = (float)sc[w - 1];
+}
+
+void modelupdate(int nx, int ny, int nz, float ***gradvp, float ***gradvs, float ***gradrho, float ***rho,
+                 float ***pi, float ***u, float **bfgsmod1, float step, float *beta, int it_group)
+{
+    (void)beta;
+    const ifos_geom &g = geom();
+    if (nx != NX || ny != NY || nz != NZ) fatal("modelupdate: grid %dx%dx%d != local grid", nx, ny, nz);
+    float ***G[3] = {gradvp, gradvs, gradrho}, ***M[3] = {rho, pi, u};
+    float *dg[3], *dm[3];
+    for (int q = 0; q < 3; ++q) {
+        dg[q] = upload_avg(G[q]);
+        dm[q] = m3(M[q], MODEL, false);  // the caller's copy (cpmodel) or the model itself
+        up3(M[q], MODEL, dm[q]);
+    }
+    const size_t n = (size_t)NUMPAR * NX * NY * NZ;
+    if (LBFGS) grow((void **)&S.d_step, S.cap_step, n * 4);
+    ck(ifos_modelupdate(&g, dg, dm[0], dm[1], dm[2], LBFGS ? S.d_step : nullptr, NUMPAR, LBFGS, step, WEIGHT, VP0,
+                        VS0, RHO0, nullptr, nullptr),
+       "modelupdate");
+    for (int q = 0; q < 3; ++q) {
+        down3(M[q], MODEL, dm[q]);
+        down3(G[q], AVG, dg[q]);  // without L-BFGS the gradients were normalised in place
+    }
+    if (LBFGS) {
+        int w = it_group % BFGSNUM;
+        if (w == 0) w = BFGSNUM;
+        hck(hipMemcpy(&bfgsmod1[w][1], S.d_step, n * 4, hipMemcpyDeviceToHost), "download bfgsmod");
+    }
+}
+
+void conjugate(int nx, int ny, int nz, float ***grad1, float ***grad2, float ***grad3, float ***gradprior1,
+               float ***gradprior2, float ***gradprior3, float ***gradprior4, float ***gradprior5,
+               float ***gradprior6, float *beta, int iteration, int cdf)
+{
+    (void)iteration;
+    const ifos_geom &g = geom();
+    if (nx != NX || ny != NY || nz != NZ) fatal("conjugate: grid %dx%dx%d != local grid", nx, ny, nz);
+    float ***G[3] = {grad1, grad2, grad3};
+    float ***P[6] = {gradprior1, gradprior2, gradprior3, gradprior4, gradprior5, gradprior6};
+    float *dg[3], *dp[6];
+    for (int q = 0; q < 3; ++q) dg[q] = upload_avg(G[q]);
+    for (int q = 0; q < 6; ++q) dp[q] = upload_avg(P[q]);
+    float b3[3] = {0.0f, 0.0f, 0.0f};
+    ck(ifos_conjugate(&g, dg, dp, b3, cdf, nullptr), "conjugate");
+    for (int q = 0; q < 3; ++q) down3(G[q], AVG, dg[q]);
+    for (int q = 0; q < 6; ++q) down3(P[q], AVG, dp[q]);
+    for (int q = 0; q < 3; ++q) beta[q] = b3[q];
+}
+
 void av_mat(float ***rho, float ***pi, float ***u, float ***taus, float ***taup, float ***uipjp, float ***ujpkp,
             float ***uipkp, float ***tausipjp, float ***tausjpkp, float ***tausipkp, float ***rjp, float ***rkp,
             float ***rip)
@@ -742,6 +848,9 @@ const char *ifos_dropin_last_error(void) { return S.err.c_str(); }
 
 void ifos_dropin_release(void)
 {
+    if (S.lb) ifos_lbfgs_destroy(S.lb);
+    S.lb = nullptr;
+    S.lb_key = nullptr;
     (void)hipDeviceSynchronize();
     for (auto &kv : S.mir) (void)hipFree(kv.second.dev);
     S.mir.clear();

@@ -205,6 +205,28 @@ void zero_invers(int NX, int NY, int NZ, float ****Ffvx, float ****Ffvy,
 void zero_grad(int NX, int NY, int NZ, float ***grad1, float ***grad2,
                float ***grad3);
 
+/* Optimiser steps.  They read the globals LBFGS, NUMPAR, BFGSNUM, VP0, VS0,
+ * RHO0 and WEIGHT[3] (globvar.h:60-62), which the caller defines.  Dot
+ * products are reduced in FP64.  bfgsgrad and bfgsscale's history lives on
+ * the device, keyed by the bfgsgrad matrix: the host's bfgsgrad is not
+ * written; bfgsscale[1..BFGSNUM] is. */
+/* fd.h:425 (lbfgs_save.c:25) */
+void lbfgs_savegrad(float ***grad1, float ***grad2, float ***grad3,
+                    float **bfgsgrad1);
+/* fd.h:424 (lbfgs.c:26): bfgsmod[1..BFGSNUM] (modelupdate's steps) is read
+ * from the host */
+void lbfgs(float ***grad1, float ***grad2, float ***grad3, float *bfgsscale,
+           float **bfgsmod, float **bfgsgrad, int iteration);
+/* fd.h:391 (modelupdate.c:26) */
+void modelupdate(int nx, int ny, int nz, float ***gradvp, float ***gradvs,
+                 float ***gradrho, float ***rho, float ***pi, float ***u,
+                 float **bfgsmod1, float step, float *beta, int it_group);
+/* fd.h:400 (conjugategrad.c:26) */
+void conjugate(int nx, int ny, int nz, float ***grad1, float ***grad2,
+               float ***grad3, float ***gradprior1, float ***gradprior2,
+               float ***gradprior3, float ***gradprior4, float ***gradprior5,
+               float ***gradprior6, float *beta, int iteration, int cdf);
+
 /* --- extensions (no counterpart in the reference) --- */
 /* message of the last fatal error (the routines above stop the process) */
 const char *ifos_dropin_last_error(void);

@@ -25,6 +25,8 @@
 float DT, DX, DY, DZ, TIME, ALPHA, BETA, VPPML, FPML;
 int NT, NX, NY, NZ, FDORDER, FDCOEFF, ABS_TYPE, FW, FREE_SURF, L, SEISMO;
 int POS[4], NPROCX, NPROCY, NPROCZ;
+int LBFGS, NUMPAR, BFGSNUM;
+float VP0, VS0, RHO0, WEIGHT[3];
 
 /* ---- Numerical-Recipes style tensors -------------------------------- */
 static float ***nr3(int jl, int jh, int il, int ih, int kl, int kh)
@@ -410,5 +412,69 @@ int dd_hess(int nx, int ny, int nz, int fw, float dx, float dy, float dz, float 
     for (n = 0; n < 12; n++) nr4_free(F[n], 1, n < 6 ? nf : nf * (ntr_hess + 1), lo, -1, -1);
     for (c = 0; c < 3; c++) nr3_free(h[c], 1, 1, 1);
     nr3_free(rho, 0, 0, 0); nr3_free(pi, 0, 0, 0); nr3_free(u, 0, 0, 0);
+    return 0;
+}
+
+/*
+ * ifos3d.c:1005-1040 and :1138, iteration after iteration: the gradients
+ * grads [niter][3] (canonical) go through lbfgs_savegrad / lbfgs (use_lbfgs 1)
+ * or conjugate (use_lbfgs 0, cdf 1 on the first iteration), then modelupdate
+ * with steps[it] updates the model model_c [3] (rho, pi, u, canonical).
+ * Outputs: the model after the last update, the gradients as each iteration
+ * leaves them (the direction, normalised when use_lbfgs is 0), beta [niter][3]
+ * and bfgsscale [niter][bfgsnum].
+ */
+int dd_optim(int nx, int ny, int nz, int fw, float dx, float dt, int nt, const float *model_c, const float *grads_c,
+             int niter, int use_lbfgs, int numpar, int bfgsnum, const float *steps, const float *weight, float vp0,
+             float vs0, float rho0, float *model_out, float *dir_out, float *beta_out, float *scale_out)
+{
+    const size_t cs = (size_t)(ny + 5) * (nx + 5) * (nz + 5);
+    float ***m[3], ***gr[3], ***prior[6], **bfgsmod, **bfgsgrad, *bfgsscale, beta[3] = {0.0f, 0.0f, 0.0f};
+    int c, it, w;
+
+    NX = nx; NY = ny; NZ = nz; FW = fw; DX = DY = DZ = dx; DT = dt; NT = nt;
+    TIME = (NT - 1) * DT;
+    FDORDER = 4; FDCOEFF = 2; ABS_TYPE = 1; FREE_SURF = 0; L = 0; SEISMO = 1;
+    POS[1] = POS[2] = POS[3] = 0; NPROCX = NPROCY = NPROCZ = 1; ALPHA = BETA = 0.0f;
+    LBFGS = use_lbfgs; NUMPAR = numpar; BFGSNUM = bfgsnum;
+    VP0 = vp0; VS0 = vs0; RHO0 = rho0;
+    for (c = 0; c < 3; c++) WEIGHT[c] = weight[c];
+
+    for (c = 0; c < 3; c++) {
+        m[c] = nr3(0, NY + 1, 0, NX + 1, 0, NZ + 1);
+        from_canon(m[c], model_c + (size_t)c * cs, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+        gr[c] = nr3(1, NY, 1, NX, 1, NZ);
+    }
+    for (c = 0; c < 6; c++) prior[c] = nr3(1, NY, 1, NX, 1, NZ);
+    bfgsmod = nr2(1, BFGSNUM, 1, NUMPAR * NX * NY * NZ);
+    bfgsgrad = nr2(1, BFGSNUM, 1, NUMPAR * NX * NY * NZ);
+    bfgsscale = (float *)calloc((size_t)BFGSNUM + 1, sizeof(float));
+    for (it = 0; it < niter; it++) {
+        for (c = 0; c < 3; c++) from_canon(gr[c], grads_c + ((size_t)it * 3 + c) * cs, 1, NY, 1, NX, 1, NZ);
+        if (LBFGS) {
+            if (it > 0)
+                lbfgs(gr[0], gr[1], gr[2], bfgsscale, bfgsmod, bfgsgrad, it + 1);
+            else
+                lbfgs_savegrad(gr[0], gr[1], gr[2], bfgsgrad);
+        } else
+            conjugate(NX, NY, NZ, gr[0], gr[1], gr[2], prior[0], prior[1], prior[2], prior[3], prior[4], prior[5],
+                      beta, it + 1, it == 0);
+        modelupdate(NX, NY, NZ, gr[0], gr[1], gr[2], m[0], m[1], m[2], bfgsmod, steps[it], beta, it + 1);
+        for (c = 0; c < 3; c++) {
+            to_canon(gr[c], dir_out + ((size_t)it * 3 + c) * cs, 1, NY, 1, NX, 1, NZ);
+            beta_out[it * 3 + c] = beta[c];
+        }
+        for (w = 1; w <= BFGSNUM; w++) scale_out[it * BFGSNUM + w - 1] = bfgsscale[w];
+    }
+    for (c = 0; c < 3; c++) to_canon(m[c], model_out + (size_t)c * cs, 0, NY + 1, 0, NX + 1, 0, NZ + 1);
+    ifos_dropin_release();
+    for (c = 0; c < 3; c++) {
+        nr3_free(m[c], 0, 0, 0);
+        nr3_free(gr[c], 1, 1, 1);
+    }
+    for (c = 0; c < 6; c++) nr3_free(prior[c], 1, 1, 1);
+    nr2_free(bfgsmod, 1, 1);
+    nr2_free(bfgsgrad, 1, 1);
+    free(bfgsscale);
     return 0;
 }

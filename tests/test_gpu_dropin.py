@@ -1,7 +1,8 @@
 """The drop-in boundary: the reference prototypes (update_v, update_v_CPML,
 update_s_elastic, update_s_CPML_elastic, psource, seismo, surface_elastic,
 discfourier, gradient_F, hess_F, av_mat, CPML_coeff, CPML_ini_elastic,
-zero_wavefield, zero_invers) called on Numerical-Recipes arrays by an
+zero_wavefield, zero_invers, and the optimiser's lbfgs_savegrad, lbfgs,
+conjugate, modelupdate) called on Numerical-Recipes arrays by an
 IFOS3D-shaped driver (tests/dropin/nr_driver.c), in the order of
 ifos3d.c:712-975 (and of its Hessian loop, ifos3d.c:880-983, for the
 Hessian wavefields and hess_F).  Seismograms, misfit, gradients and
@@ -141,3 +142,78 @@ def test_dropin_hessian_matches_oracle():
         w, got = g.interior(want[n]), g.interior(hess[n])
         assert np.abs(w).max() > 0
         assert max_ulp(got, w) == 0, f"hess{n + 1}"
+
+
+@pytest.mark.parametrize("use_lbfgs,numpar", [(0, 2), (1, 1), (1, 2)])
+def test_dropin_optimiser_matches_native(use_lbfgs, numpar):
+    """lbfgs_savegrad / lbfgs or conjugate, then modelupdate, for four
+    iterations through the drop-in prototypes on Numerical-Recipes arrays
+    (the order of ifos3d.c:1005-1040): the model, the directions, beta and
+    bfgsscale equal those of the same sequence through the device entry
+    points (ifos_lbfgs_*, ifos_conjugate, ifos_modelupdate), which
+    tests/test_gpu_optim.py pins to the oracle, bit for bit"""
+    import ifosgpu as ig
+    from helpers import small_grid
+    g = small_grid(nx=18, ny=16, nz=20, fw=4)
+    print(f"drop-in optimiser: lbfgs {use_lbfgs}, numpar {numpar}", flush=True)
+    lib = C.CDLL(DRIVER)
+    rng = np.random.default_rng(90 + 2 * use_lbfgs + numpar)
+    niter, bfgsnum = 4, 3
+    vp = 4000.0 * (1 + 0.05 * rng.random(g.shape))
+    vs = vp / 1.8
+    rho = 2000.0 * (1 + 0.05 * rng.random(g.shape))
+    model = np.ascontiguousarray(np.stack([rho, rho * vp * vp, rho * vs * vs]), np.float32)
+    grads = np.zeros((niter, 3) + g.shape, np.float32)
+    inner = (slice(3, -2),) * 3  # cells 1..N of the canonical layout
+    prev = rng.standard_normal((3,) + g.shape)
+    for it in range(niter):
+        prev = 0.7 * prev + 0.3 * rng.standard_normal((3,) + g.shape)
+        for c in range(3):
+            grads[it, c][inner] = prev[c][inner]
+    steps = np.array([0.02, 0.015, 0.01, 0.012], np.float32)
+    weight = np.array([1.0, 0.8, 0.5], np.float32)
+    vp0, vs0, rho0 = 120.0, 70.0, 25.0
+    m_out = np.zeros_like(model)
+    d_out = np.zeros_like(grads)
+    beta_out = np.zeros((niter, 3), np.float32)
+    scale_out = np.zeros((niter, bfgsnum), np.float32)
+    rc = lib.dd_optim(g.nx, g.ny, g.nz, g.fw, C.c_float(g.dx), C.c_float(g.dt), g.nt, _p(model), _p(grads), niter,
+                      use_lbfgs, numpar, bfgsnum, _p(steps), _p(weight), C.c_float(vp0), C.c_float(vs0),
+                      C.c_float(rho0), _p(m_out), _p(d_out), _p(beta_out), _p(scale_out))
+    assert rc == 0
+
+    G = ig.Geom(g.nx, g.ny, g.nz, g.fw, g.dx, g.dx, g.dx, g.dt, g.nt, 2, 0)
+    dm = [G.to_dev(a) for a in model]
+    dp = [G.to_dev(np.zeros(g.shape, np.float32)) for _ in range(6)]
+    lb = ig.Lbfgs(G, numpar, bfgsnum) if use_lbfgs else None
+    try:
+        for it in range(niter):
+            dg = [G.to_dev(a) for a in grads[it]]
+            beta = np.zeros(3, np.float32)
+            if lb is not None:
+                g2 = dg[1] if numpar > 1 else None
+                if it > 0:
+                    lb.update(dg[0], g2, it + 1)
+                else:
+                    lb.savegrad(dg[0], g2)
+            else:
+                beta = np.asarray(ig.conjugate(G, dg, dp, 1 if it == 0 else 0)[0], np.float32)
+            w = (it + 1) % bfgsnum or bfgsnum
+            ig.modelupdate(G, dg, *dm, float(steps[it]), weight, vp0, vs0, rho0,
+                           bfgsmod=lb.mod(w) if lb is not None else None, numpar=numpar)
+            for c in range(3):
+                want = g.interior(G.from_dev(dg[c]))
+                assert np.abs(want).max() > 0
+                assert max_ulp(g.interior(d_out[it, c]), want) == 0, f"direction {c + 1}, iteration {it + 1}"
+            assert np.array_equal(beta_out[it], beta), f"beta, iteration {it + 1}"
+            if lb is not None:
+                assert np.array_equal(scale_out[it], lb.scale().astype(np.float32)), f"bfgsscale, iteration {it + 1}"
+        if lb is not None:
+            assert np.all(scale_out[-1] != 0)
+        for c in range(3):
+            want = g.interior(G.from_dev(dm[c]))
+            assert not np.array_equal(want, g.interior(model[c]))
+            assert max_ulp(g.interior(m_out[c]), want) == 0, f"model {c}"
+    finally:
+        if lb is not None:
+            lb.close()
