@@ -245,12 +245,19 @@ __device__ inline KNb kshift(float c, float h)
     r.p2 = dpp_move<DPP_WAVE_SHL1>(h, r.p1);
     return r;
 }
-// byte offset (within the LD addressing) of lane l's halo element: k - 2 for
-// lanes 0, 1, k + 2 for lanes 62, 63, the lane's own k otherwise
+// byte offset (within the LD addressing, where 4 x is the word at k - 2 + x)
+// of lane l's halo element, k = k0 + l: k0 - 2 and k0 - 1 for lanes 0, 1,
+// k0 + 64 and k0 + 65 for lanes 62, 63.  kshift reads no other lane of h, so
+// lanes 2-31 load lane 1's word and lanes 32-61 lane 62's: the instruction
+// touches only the two lines just outside the row, not the row's own lines
+// again.  Lanes 2-31 step back (1 - l words): vo + offset wraps as uint32 to
+// vo - 4 (l - 1), still positive, as vo counts at least k + IFOS_KOFF - 2 >=
+// l + IFOS_KOFF - 1 words (k >= k0 + l, k0 >= 1).
 __device__ inline uint32_t khalo_off()
 {
     const int l = (int)threadIdx.x;
-    return (uint32_t)((l < 2 ? 0 : l >= 62 ? 4 : 2) * 4);
+    const int x = l < 2 ? 0 : l < 32 ? 1 - l : l < 63 ? 66 - l : 4;
+    return (uint32_t)(x * 4);
 }
 
 __device__ inline void shift_psi(Psi9 &psi, const Shots &S, int shot)
