@@ -534,19 +534,29 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 // Stress march with the in-plane neighbours staged in LDS.  Tiles of 64
 // k-lanes (one row per wavefront, one shot per workgroup): at each plane j
 // every wavefront writes its row of vx, vy, vz (the centre of its j-queues)
-// to LDS, lanes 0-3 add the two k-neighbours on either side of the row, the
-// workgroup adds the rows just outside it (vx at i0-2, i0-1, i0+TI; vy and
-// vz at i0-1, i0+TI, i0+TI+1), and every i- and k-neighbour of the update is
-// then read from LDS instead of through L1: the only global loads left are
-// one stream per array.  Two LDS buffers alternate by plane, so one barrier
-// per plane suffices.  Every lane of the workgroup takes part in the loads
-// it is responsible for and in the barrier; only lanes inside the box update
-// and store.  Same arithmetic, in the same order, as k_s_march.
+// to LDS, the halo words of its row (k0-2, k0-1, k0+64, k0+65), and its share
+// of the rows just outside the workgroup (vx at i0-2, i0-1, i0+TI; vy and vz
+// at i0-1, i0+TI, i0+TI+1); every i- and k-neighbour of the update is then
+// read from LDS instead of through L1.  Software-pipelined one plane ahead:
+// the halo words, the outside rows, the own cell's operands and the next
+// j-queue entries of plane j+1 are loaded while plane j is computed, so the
+// LDS writes of plane j wait only for loads issued an iteration earlier.
+// Two LDS buffers alternate by plane, so one barrier per plane suffices (a
+// wavefront writes a buffer again only after the next barrier, which every
+// wavefront reaches after its reads of that buffer).  Every lane loads, at a
+// clamped cell (k <= nz+2, rows <= nx+2): an updating lane (k <= nz, i <= nx)
+// reads only k-2..k+2 and rows i-2..i+2, none of them clamped, so the clamps
+// only keep the loads of idle lanes in bounds; only lanes inside the box
+// update and store.  The barrier waits for LDS only (s_waitcnt lgkmcnt(0)):
+// the loads in flight for the next plane stay in flight across it.  Same
+// arithmetic, in the same order, as k_s_march.
 template <int TI, bool CPML, bool VISCO>
 __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S,
                                                   ViscoF V)
 {
-    constexpr int RW = TI + 4;  // LDS rows: i0-2 .. i0+TI+1
+    constexpr int RW = TI + 4;              // LDS rows: i0-2 .. i0+TI+1
+    constexpr int NH = (9 + TI - 1) / TI;   // outside rows per wavefront (9 per workgroup)
+    constexpr int CPN = CPML ? IFOS_CPOL_NT : 0;
     __shared__ float sh[2][3][RW][68];
     const int b = blockIdx.x;
     const int L = (b & 7) * T.per + (b >> 3);
@@ -563,12 +573,14 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
     const int tk = t % T.ntk[q];
     t /= T.ntk[q];
     const int ti = t % T.nti[q], tj = t / T.nti[q];
-    const int lane = (int)threadIdx.x, y = (int)threadIdx.y;
+    const int lane = (int)threadIdx.x;
+    const int y = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one row per wavefront
     const int k0 = R.k0 + tk * 64, i0 = R.i0 + ti * TI;
     const int k = k0 + lane, i = i0 + y;
     const int jb = R.j0 + tj * T.jc, je = min(jb + T.jc - 1, R.j1);
-    const bool act = k <= R.k1 && i <= R.i1;                 // updates and stores
-    const bool live = k <= g.nz + 2 && i <= g.nx + 2;        // its row values exist in memory
+    const bool act = k <= R.k1 && i <= R.i1;  // updates and stores
+    const int kc = min(k, g.nz + 2);          // clamped load cell
+    const int yc = min(i, g.nx + 2) - i0;     // clamped own row (wavefront-uniform)
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2, DT = g.dt;
     const int64_t so = (int64_t)shot * S.fstride;
@@ -590,85 +602,107 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
         rtsjpkp = make_rsrc(V.tausjpkp, g.fbytes);
         rtsipkp = make_rsrc(V.tausipkp, g.fbytes);
     }
-    // byte offset of (j-2, i0-2, k-2) at plane j: operand (j+dj, i0+r, k+dk)
-    // at vb + (dk+2)*4 + (dj+2)*SJ + (r+2)*SI, r in [-2, TI+1]
-    uint32_t vb = (uint32_t)((int64_t)jb * g.sj + (int64_t)i0 * g.si + k + IFOS_KOFF - 2) * 4u;
-    const uint32_t own = (uint32_t)(y + 2) * SI;
+    // byte offset of (j-2, i0-2, kc-2) at plane j: operand (j+dj, i0+r, kc)
+    // at vb + 8 + (dj+2)*SJ + (r+2)*SI
+    uint32_t vb = (uint32_t)((int64_t)jb * g.sj + (int64_t)i0 * g.si + kc + IFOS_KOFF - 2) * 4u;
+    const uint32_t own = (uint32_t)(yc + 2) * SI;
 #define LB(r, dj) bload(r, vb + 8u, (uint32_t)((dj) + 2) * SJ + own)
-    // lanes 0-3: k0-2, k0-1, k0+64, k0+65 of the own row (LDS columns 0, 1, 66, 67)
-    const int hk = lane < 2 ? k0 - 2 + lane : k0 + 62 + lane;
-    const bool hl = lane < 4 && hk <= g.nz + 2 && i <= g.nx + 2;
-    const int hcol = lane < 2 ? lane : lane + 64;
-    const uint32_t hoff = (uint32_t)((hk - k + 2) * 4);  // byte offset of hk from vb (>= 0)
+#define LBN(r, dj) bload_cp<CPN>(r, vb + 8u, (uint32_t)((dj) + 2) * SJ + own)
+    // halo words of the own row: lane l loads k0-2+l (lanes 0, 1), k0+64 and
+    // k0+65 (lanes 62, 63); lanes 2-31 repeat lane 1's word and lanes 32-61
+    // lane 62's (the load touches two lines).  Lanes 2-61 write theirs to
+    // their own column, which the row write after it overwrites, so the 64
+    // lanes write 64 different words.  hd is the word's byte offset from the
+    // lane's own word (>= -4 (lane + 2): the sum vb + 8 + hd stays >= 0).
+    const int kh = lane < 2 ? k0 - 2 + lane : lane < 32 ? k0 - 1 : lane < 63 ? k0 + 64 : k0 + 65;
+    const int hcol = lane < 62 ? (lane < 2 ? lane : lane + 2) : lane + 4;
+    const uint32_t hd = (uint32_t)((min(kh, g.nz + 2) - kc) * 4);
+    // outside rows of this wavefront: e = min(y + n TI, 8) is (component e / 3,
+    // row i0 + orow[e]); the load row is clamped to nx+2.  The 9 rows take
+    // NH TI slots; the spare slots repeat row 8 (the same words, written with
+    // the same values), so no wavefront branches around a load or a write.
+    int oc[NH], orow[NH];
+    uint32_t ooff[NH];
+#pragma unroll
+    for (int n = 0; n < NH; ++n) {
+        const int e = min(y + n * TI, 8);
+        const int c = e < 3 ? 0 : (e < 6 ? 1 : 2), m = e - 3 * c;
+        const int r = c == 0 ? (m == 0 ? -2 : (m == 1 ? -1 : TI)) : (m == 0 ? -1 : (m == 1 ? TI : TI + 1));
+        oc[n] = c;
+        orow[n] = r;
+        ooff[n] = (uint32_t)(min(i0 + r, g.nx + 2) - i0 + 2) * SI;
+    }
     int hx = 0;
     if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
     const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
 
-    // j queues of the own column: vx, vz at j-1..j+1, vy at j-2..j
-    float vx_m1 = 0.0f, vx_0 = 0.0f, vx_p1 = 0.0f, vz_m1 = 0.0f, vz_0 = 0.0f, vz_p1 = 0.0f;
-    float vy_m2 = 0.0f, vy_m1 = 0.0f, vy_0 = 0.0f;
-    if (live) {
-        vx_m1 = LB(rvx, -1); vx_0 = LB(rvx, 0); vx_p1 = LB(rvx, 1);
-        vz_m1 = LB(rvz, -1); vz_0 = LB(rvz, 0); vz_p1 = LB(rvz, 1);
-        vy_m2 = LB(rvy, -2); vy_m1 = LB(rvy, -1); vy_0 = LB(rvy, 0);
-    }
+    // operands of plane j + D: the own cell's, the halo words, the outside rows
+    struct Ops {
+        float s[6], pi, u, uipjp, ujpkp, uipkp, r[6], ts, tp, tsipjp, tsjpkp, tsipkp;
+        float hk[3], ho[NH];
+    };
+    auto load_ops = [&](Ops &O, int D) {
+        const uint32_t pj = (uint32_t)(D + 2) * SJ;
+        O.s[0] = LBN(rsxx, D); O.s[1] = LBN(rsyy, D); O.s[2] = LBN(rszz, D);
+        O.s[3] = LBN(rsxy, D); O.s[4] = LBN(rsyz, D); O.s[5] = LBN(rsxz, D);
+        O.uipjp = LB(ruipjp, D); O.ujpkp = LB(rujpkp, D); O.uipkp = LB(ruipkp, D);
+        O.pi = LB(rpi, D); O.u = LB(ru, D);
+        if (VISCO) {
+#pragma unroll
+            for (int n = 0; n < 6; ++n) O.r[n] = LBN(rr[n], D);
+            O.ts = LB(rtaus, D); O.tp = LB(rtaup, D);
+            O.tsipjp = LB(rtsipjp, D); O.tsjpkp = LB(rtsjpkp, D); O.tsipkp = LB(rtsipkp, D);
+        }
+        O.hk[0] = bload(rvx, vb + 8u + hd, pj + own);
+        O.hk[1] = bload(rvy, vb + 8u + hd, pj + own);
+        O.hk[2] = bload(rvz, vb + 8u + hd, pj + own);
+#pragma unroll
+        for (int n = 0; n < NH; ++n) {
+            const rsrc_t rc = oc[n] == 0 ? rvx : (oc[n] == 1 ? rvy : rvz);
+            O.ho[n] = bload(rc, vb + 8u, pj + ooff[n]);
+        }
+    };
+
+    // j queues of the own column: vx, vz at j-1..j+2, vy at j-2..j+1
+    float vx_m1 = LB(rvx, -1), vx_0 = LB(rvx, 0), vx_p1 = LB(rvx, 1), vx_p2 = LB(rvx, 2);
+    float vz_m1 = LB(rvz, -1), vz_0 = LB(rvz, 0), vz_p1 = LB(rvz, 1), vz_p2 = LB(rvz, 2);
+    float vy_m2 = LB(rvy, -2), vy_m1 = LB(rvy, -1), vy_0 = LB(rvy, 0), vy_p1 = LB(rvy, 1);
+    Ops cur, nxt{};
+    load_ops(cur, 0);
 
     for (int j = jb; j <= je; ++j, vb += SJ) {
         const int p = (j - jb) & 1;
-        // operands of the own cell (global, one stream per array)
-        float sxx0 = 0.0f, syy0 = 0.0f, szz0 = 0.0f, sxy0 = 0.0f, syz0 = 0.0f, sxz0 = 0.0f;
-        float c_pi = 0.0f, c_u = 0.0f, c_uipjp = 0.0f, c_ujpkp = 0.0f, c_uipkp = 0.0f;
-        float r0[6] = {0, 0, 0, 0, 0, 0}, ts = 0.0f, tp = 0.0f, tsipjp = 0.0f, tsjpkp = 0.0f, tsipkp = 0.0f;
-        if (act) {
-            sxx0 = LB(rsxx, 0); syy0 = LB(rsyy, 0); szz0 = LB(rszz, 0);
-            sxy0 = LB(rsxy, 0); syz0 = LB(rsyz, 0); sxz0 = LB(rsxz, 0);
-            c_uipjp = LB(ruipjp, 0); c_ujpkp = LB(rujpkp, 0); c_uipkp = LB(ruipkp, 0);
-            c_pi = LB(rpi, 0); c_u = LB(ru, 0);
-            if (VISCO) {
-#pragma unroll
-                for (int n = 0; n < 6; ++n) r0[n] = LB(rr[n], 0);
-                ts = LB(rtaus, 0); tp = LB(rtaup, 0);
-                tsipjp = LB(rtsipjp, 0); tsjpkp = LB(rtsjpkp, 0); tsipkp = LB(rtsipkp, 0);
-            }
-        }
-        // next queue entries: vx, vz at j+2, vy at j+1
-        float vx_n = 0.0f, vz_n = 0.0f, vy_n = 0.0f;
-        if (live) {
-            vx_n = LB(rvx, 2); vz_n = LB(rvz, 2); vy_n = LB(rvy, 1);
-        }
-        // stage plane j: own row, k-halo lanes, rows outside the workgroup
+        // plane j+1's operands and the queue entries after it (vx, vz at j+3,
+        // vy at j+2), loaded unconditionally (a branch here would make the
+        // waits for this iteration's operands wait for these loads too): at
+        // j = je they are unused, plane je+1 <= ny+1 exists and j+3 is
+        // clamped to the last plane in memory, ny+2
+        load_ops(nxt, 1);
+        const uint32_t q3 = (uint32_t)(min(j + 3, g.ny + 2) - j + 2) * SJ + own;
+        const float vx_n = bload(rvx, vb + 8u, q3), vz_n = bload(rvz, vb + 8u, q3), vy_n = LB(rvy, 2);
+        // stage plane j: halo words first, then the row over them
+        sh[p][0][y + 2][hcol] = cur.hk[0];
+        sh[p][1][y + 2][hcol] = cur.hk[1];
+        sh[p][2][y + 2][hcol] = cur.hk[2];
         sh[p][0][y + 2][lane + 2] = vx_0;
         sh[p][1][y + 2][lane + 2] = vy_0;
         sh[p][2][y + 2][lane + 2] = vz_0;
-        if (lane < 4) {
-            const uint32_t ho = 2u * SJ + own;
-            sh[p][0][y + 2][hcol] = hl ? bload(rvx, vb + hoff, ho) : 0.0f;
-            sh[p][1][y + 2][hcol] = hl ? bload(rvy, vb + hoff, ho) : 0.0f;
-            sh[p][2][y + 2][hcol] = hl ? bload(rvz, vb + hoff, ho) : 0.0f;
-        }
-        for (int e = y; e < 9; e += TI) {
-            // (component, row relative to i0): vx -2 -1 TI; vy, vz -1 TI TI+1
-            const int c = e < 3 ? 0 : (e < 6 ? 1 : 2);
-            const int m = e - 3 * c;
-            const int r = c == 0 ? (m == 0 ? -2 : (m == 1 ? -1 : TI)) : (m == 0 ? -1 : (m == 1 ? TI : TI + 1));
-            const rsrc_t rc = c == 0 ? rvx : (c == 1 ? rvy : rvz);
-            const bool ok = i0 + r <= g.nx + 2 && k <= g.nz + 2;
-            sh[p][c][r + 2][lane + 2] = ok ? bload(rc, vb + 8u, 2u * SJ + (uint32_t)(r + 2) * SI) : 0.0f;
-        }
-        __syncthreads();
+#pragma unroll
+        for (int n = 0; n < NH; ++n) sh[p][oc[n]][orow[n] + 2][lane + 2] = cur.ho[n];
+        __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         if (act) {
             const float(*X)[68] = sh[p][0];
             const float(*Y)[68] = sh[p][1];
             const float(*Z)[68] = sh[p][2];
             const int c0 = lane + 2, r0_ = y + 2;
             float vxx = D4B(b1, b2, X[r0_ - 2][c0], X[r0_ - 1][c0], vx_0, X[r0_ + 1][c0]);
-            float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, vx_n);
+            float vxy = D4F(b1, b2, vx_m1, vx_0, vx_p1, vx_p2);
             float vxz = D4F(b1, b2, X[r0_][c0 - 1], vx_0, X[r0_][c0 + 1], X[r0_][c0 + 2]);
             float vyx = D4F(b1, b2, Y[r0_ - 1][c0], vy_0, Y[r0_ + 1][c0], Y[r0_ + 2][c0]);
-            float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, vy_n);
+            float vyy = D4B(b1, b2, vy_m2, vy_m1, vy_0, vy_p1);
             float vyz = D4F(b1, b2, Y[r0_][c0 - 1], vy_0, Y[r0_][c0 + 1], Y[r0_][c0 + 2]);
             float vzx = D4F(b1, b2, Z[r0_ - 1][c0], vz_0, Z[r0_ + 1][c0], Z[r0_ + 2][c0]);
-            float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, vz_n);
+            float vzy = D4F(b1, b2, vz_m1, vz_0, vz_p1, vz_p2);
             float vzz = D4B(b1, b2, Z[r0_][c0 - 2], Z[r0_][c0 - 1], vz_0, Z[r0_][c0 + 1]);
             div_spacing(g, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
             if (CPML) {
@@ -681,39 +715,43 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
             float sn[6], rn[6];
             if (!VISCO) {
                 // update_s_ssg_elastic.c:117-138
-                const float fipjp = c_uipjp * DT, fjpkp = c_ujpkp * DT, fipkp = c_uipkp * DT;
-                const float f = 2.0f * c_u;
-                sn[3] = sxy0 + (fipjp * vxyyx);
-                sn[4] = syz0 + (fjpkp * vyzzy);
-                sn[5] = sxz0 + (fipkp * vxzzx);
-                sn[0] = sxx0 + DT * ((c_pi * vxxyyzz) - (f * vyyzz));
-                sn[1] = syy0 + DT * ((c_pi * vxxyyzz) - (f * vxxzz));
-                sn[2] = szz0 + DT * ((c_pi * vxxyyzz) - (f * vxxyy));
+                const float fipjp = cur.uipjp * DT, fjpkp = cur.ujpkp * DT, fipkp = cur.uipkp * DT;
+                const float f = 2.0f * cur.u;
+                sn[3] = cur.s[3] + (fipjp * vxyyx);
+                sn[4] = cur.s[4] + (fjpkp * vyzzy);
+                sn[5] = cur.s[5] + (fipkp * vxzzx);
+                sn[0] = cur.s[0] + DT * ((cur.pi * vxxyyzz) - (f * vyyzz));
+                sn[1] = cur.s[1] + DT * ((cur.pi * vxxyyzz) - (f * vxxzz));
+                sn[2] = cur.s[2] + DT * ((cur.pi * vxxyyzz) - (f * vxxyy));
             } else {
-                sn[0] = sxx0; sn[1] = syy0; sn[2] = szz0;
-                sn[3] = sxy0; sn[4] = syz0; sn[5] = sxz0;
 #pragma unroll
-                for (int n = 0; n < 6; ++n) rn[n] = r0[n];
-                stress_visco(DT, V.eta, V.L, c_pi, c_u, c_uipjp, c_ujpkp, c_uipkp, ts, tp, tsipjp, tsjpkp, tsipkp,
-                             vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn, rn);
+                for (int n = 0; n < 6; ++n) {
+                    sn[n] = cur.s[n];
+                    rn[n] = cur.r[n];
+                }
+                stress_visco(DT, V.eta, V.L, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
+                             cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy,
+                             sn, rn);
             }
             const uint32_t st = 2u * SJ + own;
-            bstore(sn[3], rsxy, vb + 8u, st);
-            bstore(sn[4], rsyz, vb + 8u, st);
-            bstore(sn[5], rsxz, vb + 8u, st);
-            bstore(sn[0], rsxx, vb + 8u, st);
-            bstore(sn[1], rsyy, vb + 8u, st);
-            bstore(sn[2], rszz, vb + 8u, st);
+            bstore_cp<CPN>(sn[3], rsxy, vb + 8u, st);
+            bstore_cp<CPN>(sn[4], rsyz, vb + 8u, st);
+            bstore_cp<CPN>(sn[5], rsxz, vb + 8u, st);
+            bstore_cp<CPN>(sn[0], rsxx, vb + 8u, st);
+            bstore_cp<CPN>(sn[1], rsyy, vb + 8u, st);
+            bstore_cp<CPN>(sn[2], rszz, vb + 8u, st);
             if (VISCO) {
 #pragma unroll
-                for (int n = 0; n < 6; ++n) bstore(rn[n], rr[n], vb + 8u, st);
+                for (int n = 0; n < 6; ++n) bstore_cp<CPN>(rn[n], rr[n], vb + 8u, st);
             }
         }
-        vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_n;
-        vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_n;
-        vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_n;
+        vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2; vx_p2 = vx_n;
+        vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = vz_p2; vz_p2 = vz_n;
+        vy_m2 = vy_m1; vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = vy_n;
+        cur = nxt;
     }
 #undef LB
+#undef LBN
 }
 // ---------------------------------------------------------------------------
 // gradient_F.c:79-139 for one frequency of one shot as a march over j: the
