@@ -47,11 +47,12 @@ def kernel_names(path, nf, chunk, tiles=None):
                 "dft": f"ifos::k_dft_nf<{nf}>", "dft_kahan": f"ifos::k_dft_kahan_nf<{nf}>",
                 "gradient": "ifos::k_grad_march<4, 0>"}
     tv, ts = tiles
-    # the LDS-staged stress march runs when asked for with TI >= 2, one k
-    # column and one shot per workgroup
-    s_lds = len(ts) > 7 and ts[7] and ts[0] >= 2 and ts[5] <= 1 and ts[6] == 1
-    return {"v_fused": f"ifos::k_v_march<{tv[0]}, true, false, {b(tv[4])}>",
-            "s_fused": f"ifos::k_s_lds<{ts[0]}, true, false>" if s_lds
+    # the LDS-staged marches run when asked for with TI >= 2, one k column
+    # and one shot per workgroup
+    lds = lambda t: len(t) > 7 and t[7] and t[0] >= 2 and t[5] <= 1 and t[6] == 1
+    return {"v_fused": f"ifos::k_v_lds<{tv[0]}, true, false>" if lds(tv)
+            else f"ifos::k_v_march<{tv[0]}, true, false, {b(tv[4])}>",
+            "s_fused": f"ifos::k_s_lds<{ts[0]}, true, false>" if lds(ts)
             else f"ifos::k_s_march<{ts[0]}, true, false, {b(ts[4])}>",
             "vs_fused": "ifos::k_vs_fused<16, false>",
             "dft": f"ifos::k_dft_chunk<{nf}, {chunk}, false>", "dft_kahan": f"ifos::k_dft_chunk<{nf}, {chunk}, true>",
@@ -60,7 +61,7 @@ def kernel_names(path, nf, chunk, tiles=None):
 
 # tile shapes {TI, jc, group, split, pipe} used without the autotuner (its
 # usual choice on grad276)
-DEFAULT_TILES = ((4, 32, 0, 0, 0, 1, 1, 0), (4, 32, 0, 0, 0, 1, 1, 1))
+DEFAULT_TILES = ((4, 32, 0, 0, 0, 1, 1, 1), (4, 32, 0, 0, 0, 1, 1, 1))
 CATS = ["v_interior", "v_cpml", "v_fused", "s_interior", "s_cpml", "s_fused", "dft", "gradient", "sources", "other",
         "vs_fused"]
 

@@ -389,6 +389,169 @@ __global__ __launch_bounds__(64 * TI) void k_v_march(DevGeom g, MTiles T, VField
 }
 #undef VLOAD
 
+// Velocity march with the in-plane neighbours staged in LDS, built like
+// k_s_lds (which see for the staging, the clamps and the pipelining): the
+// five stresses a velocity update differentiates along i or k go through
+// LDS -- sxx, sxy and sxz for their i-neighbours (rows just outside the
+// workgroup: sxx at i0-1, i0+TI, i0+TI+1; sxy and sxz at i0-2, i0-1,
+// i0+TI), sxz, syz and szz for their k-neighbours (the row's halo words).
+// sxx, sxz and szz are read at the plane itself, sxy and syz come from the
+// j-queues.  Same arithmetic, in the same order, as k_v_march.
+template <int TI, bool CPML, bool SNAP>
+__global__ __launch_bounds__(64 * TI) void k_v_lds(DevGeom g, MTiles T, VFields F, Prof P, Psi9 psi, Shots S,
+                                                  Snap N)
+{
+    constexpr int RW = TI + 4;
+    constexpr int NH = (9 + TI - 1) / TI;
+    constexpr int CPN = CPML ? IFOS_CPOL_NT : 0;
+    __shared__ float sh[2][5][RW][68];  // sxx, sxy, sxz, syz, szz
+    const int b = blockIdx.x;
+    const int L = (b & 7) * T.per + (b >> 3);
+    if (L >= T.total) return;  // whole workgroup
+    const int gs = T.group * T.nsg;
+    const int rem = L % gs;
+    const int shot = rem / T.group;
+    int t = (L / gs) * T.group + rem % T.group;
+    if (t >= T.start[T.nb] || shot >= T.ns) return;  // whole workgroup
+    int q = 0;
+    while (q + 1 < T.nb && t >= T.start[q + 1]) ++q;
+    t -= T.start[q];
+    const Box R = T.b[q];
+    const int tk = t % T.ntk[q];
+    t /= T.ntk[q];
+    const int ti = t % T.nti[q], tj = t / T.nti[q];
+    const int lane = (int)threadIdx.x;
+    const int y = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int k0 = R.k0 + tk * 64, i0 = R.i0 + ti * TI;
+    const int k = k0 + lane, i = i0 + y;
+    const int jb = R.j0 + tj * T.jc, je = min(jb + T.jc - 1, R.j1);
+    const bool act = k <= R.k1 && i <= R.i1;
+    const int kc = min(k, g.nz + 2);
+    const int yc = min(i, g.nx + 2) - i0;
+    const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
+    const float b1 = g.b1, b2 = g.b2;
+    const int64_t so = (int64_t)shot * S.fstride;
+    const rsrc_t rsxx = make_rsrc(F.sxx + so, g.fbytes), rsyy = make_rsrc(F.syy + so, g.fbytes),
+                 rszz = make_rsrc(F.szz + so, g.fbytes), rsxy = make_rsrc(F.sxy + so, g.fbytes),
+                 rsyz = make_rsrc(F.syz + so, g.fbytes), rsxz = make_rsrc(F.sxz + so, g.fbytes),
+                 rvx = make_rsrc(F.vx + so, g.fbytes), rvy = make_rsrc(F.vy + so, g.fbytes),
+                 rvz = make_rsrc(F.vz + so, g.fbytes), rrip = make_rsrc(F.rip, g.fbytes),
+                 rrjp = make_rsrc(F.rjp, g.fbytes), rrkp = make_rsrc(F.rkp, g.fbytes);
+    rsrc_t rring;
+    if (SNAP) rring = make_rsrc(N.ring + (int64_t)shot * N.rstride, N.rbytes);
+    if (CPML) shift_psi(psi, S, shot);
+    uint32_t vb = (uint32_t)((int64_t)jb * g.sj + (int64_t)i0 * g.si + kc + IFOS_KOFF - 2) * 4u;
+    const uint32_t own = (uint32_t)(yc + 2) * SI;
+#define LB(r, dj) bload(r, vb + 8u, (uint32_t)((dj) + 2) * SJ + own)
+#define LBN(r, dj) bload_cp<CPN>(r, vb + 8u, (uint32_t)((dj) + 2) * SJ + own)
+    const int kh = lane < 2 ? k0 - 2 + lane : lane < 32 ? k0 - 1 : lane < 63 ? k0 + 64 : k0 + 65;
+    const int hcol = lane < 62 ? (lane < 2 ? lane : lane + 2) : lane + 4;
+    const uint32_t hd = (uint32_t)((min(kh, g.nz + 2) - kc) * 4);
+    // outside rows: e = min(y + n TI, 8) is (stress e / 3: sxx, sxy, sxz; row i0 + orow)
+    int oc[NH], orow[NH];
+    uint32_t ooff[NH];
+#pragma unroll
+    for (int n = 0; n < NH; ++n) {
+        const int e = min(y + n * TI, 8);
+        const int c = e < 3 ? 0 : (e < 6 ? 1 : 2), m = e - 3 * c;
+        const int r = c == 0 ? (m == 0 ? -1 : (m == 1 ? TI : TI + 1)) : (m == 0 ? -2 : (m == 1 ? -1 : TI));
+        oc[n] = c;
+        orow[n] = r;
+        ooff[n] = (uint32_t)(min(i0 + r, g.nx + 2) - i0 + 2) * SI;
+    }
+    int hx = 0;
+    if (CPML) hx = slab_h(i, g.nx, g.fw, 1);
+    const int hz = CPML ? slab_h(k, g.nz, g.fw, 1) : 0;
+
+    struct Ops {
+        float sxx, sxz, szz, rip, rjp, rkp, vx0, vy0, vz0;
+        float hk[3], ho[NH];  // halo words of sxz, syz, szz; outside rows
+    };
+    auto load_ops = [&](Ops &O, int D) {
+        const uint32_t pj = (uint32_t)(D + 2) * SJ;
+        O.sxx = LB(rsxx, D); O.sxz = LB(rsxz, D); O.szz = LB(rszz, D);
+        O.rip = LB(rrip, D); O.rjp = LB(rrjp, D); O.rkp = LB(rrkp, D);
+        O.vx0 = LBN(rvx, D); O.vy0 = LBN(rvy, D); O.vz0 = LBN(rvz, D);
+        O.hk[0] = bload(rsxz, vb + 8u + hd, pj + own);
+        O.hk[1] = bload(rsyz, vb + 8u + hd, pj + own);
+        O.hk[2] = bload(rszz, vb + 8u + hd, pj + own);
+#pragma unroll
+        for (int n = 0; n < NH; ++n) {
+            const rsrc_t rc = oc[n] == 0 ? rsxx : (oc[n] == 1 ? rsxy : rsxz);
+            O.ho[n] = bload(rc, vb + 8u, pj + ooff[n]);
+        }
+    };
+
+    // j queues: sxy and syz at j-2..j+1, syy at j-1..j+2
+    float sxy_m2 = LB(rsxy, -2), sxy_m1 = LB(rsxy, -1), sxy_0 = LB(rsxy, 0), sxy_p1 = LB(rsxy, 1);
+    float syz_m2 = LB(rsyz, -2), syz_m1 = LB(rsyz, -1), syz_0 = LB(rsyz, 0), syz_p1 = LB(rsyz, 1);
+    float syy_m1 = LB(rsyy, -1), syy_0 = LB(rsyy, 0), syy_p1 = LB(rsyy, 1), syy_p2 = LB(rsyy, 2);
+    Ops cur, nxt{};
+    load_ops(cur, 0);
+
+    for (int j = jb; j <= je; ++j, vb += SJ) {
+        const int p = (j - jb) & 1;
+        // plane j+1's operands and the queue entries after it (sxy, syz at
+        // j+2, syy at j+3 clamped to the last plane in memory), unconditional
+        // as in k_s_lds
+        load_ops(nxt, 1);
+        const uint32_t q3 = (uint32_t)(min(j + 3, g.ny + 2) - j + 2) * SJ + own;
+        const float sxy_n = LB(rsxy, 2), syz_n = LB(rsyz, 2), syy_n = bload(rsyy, vb + 8u, q3);
+        // stage plane j: halo words first, then the rows over them
+        sh[p][2][y + 2][hcol] = cur.hk[0];
+        sh[p][3][y + 2][hcol] = cur.hk[1];
+        sh[p][4][y + 2][hcol] = cur.hk[2];
+        sh[p][0][y + 2][lane + 2] = cur.sxx;
+        sh[p][1][y + 2][lane + 2] = sxy_0;
+        sh[p][2][y + 2][lane + 2] = cur.sxz;
+        sh[p][3][y + 2][lane + 2] = syz_0;
+        sh[p][4][y + 2][lane + 2] = cur.szz;
+#pragma unroll
+        for (int n = 0; n < NH; ++n) sh[p][oc[n]][orow[n] + 2][lane + 2] = cur.ho[n];
+        __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if (act) {
+            const float(*XX)[68] = sh[p][0];
+            const float(*XY)[68] = sh[p][1];
+            const float(*XZ)[68] = sh[p][2];
+            const float(*YZ)[68] = sh[p][3];
+            const float(*ZZ)[68] = sh[p][4];
+            const int c0 = lane + 2, r0_ = y + 2;
+            float sxx_x = g.dtdx * D4F(b1, b2, XX[r0_ - 1][c0], cur.sxx, XX[r0_ + 1][c0], XX[r0_ + 2][c0]);
+            float sxy_y = g.dtdy * D4B(b1, b2, sxy_m2, sxy_m1, sxy_0, sxy_p1);
+            float sxz_z = g.dtdz * D4B(b1, b2, XZ[r0_][c0 - 2], XZ[r0_][c0 - 1], cur.sxz, XZ[r0_][c0 + 1]);
+            float syy_y = g.dtdy * D4F(b1, b2, syy_m1, syy_0, syy_p1, syy_p2);
+            float sxy_x = g.dtdx * D4B(b1, b2, XY[r0_ - 2][c0], XY[r0_ - 1][c0], sxy_0, XY[r0_ + 1][c0]);
+            float syz_z = g.dtdz * D4B(b1, b2, YZ[r0_][c0 - 2], YZ[r0_][c0 - 1], syz_0, YZ[r0_][c0 + 1]);
+            float szz_z = g.dtdz * D4F(b1, b2, ZZ[r0_][c0 - 1], cur.szz, ZZ[r0_][c0 + 1], ZZ[r0_][c0 + 2]);
+            float sxz_x = g.dtdx * D4B(b1, b2, XZ[r0_ - 2][c0], XZ[r0_ - 1][c0], cur.sxz, XZ[r0_ + 1][c0]);
+            float syz_y = g.dtdy * D4B(b1, b2, syz_m2, syz_m1, syz_0, syz_p1);
+            if (CPML) {
+                const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
+                v_cpml(g, P, psi, j, i, k, hx, hy, hz, sxx_x, sxy_x, sxz_x, sxy_y, syy_y, syz_y, sxz_z, syz_z,
+                       szz_z);
+            }
+            const float nvx = cur.vx0 + (sxx_x + sxy_y + sxz_z) / cur.rip;
+            const float nvy = cur.vy0 + (syy_y + sxy_x + syz_z) / cur.rjp;
+            const float nvz = cur.vz0 + (szz_z + sxz_x + syz_y) / cur.rkp;
+            const uint32_t st = 2u * SJ + own;
+            bstore_cp<CPN>(nvx, rvx, vb + 8u, st);
+            bstore_cp<CPN>(nvy, rvy, vb + 8u, st);
+            bstore_cp<CPN>(nvz, rvz, vb + 8u, st);
+            if (SNAP) {
+                bstore_cp<CPN>(nvx, rring, vb + 8u, st + N.soff);
+                bstore_cp<CPN>(nvy, rring, vb + 8u, st + N.soff + g.fbytes);
+                bstore_cp<CPN>(nvz, rring, vb + 8u, st + N.soff + 2 * g.fbytes);
+            }
+        }
+        sxy_m2 = sxy_m1; sxy_m1 = sxy_0; sxy_0 = sxy_p1; sxy_p1 = sxy_n;
+        syz_m2 = syz_m1; syz_m1 = syz_0; syz_0 = syz_p1; syz_p1 = syz_n;
+        syy_m1 = syy_0; syy_0 = syy_p1; syy_p1 = syy_p2; syy_p2 = syy_n;
+        cur = nxt;
+    }
+#undef LB
+#undef LBN
+}
+
 // Operands of one plane j of the stress march (plus the j-queue values first
 // needed at j: vx, vz at j+2, vy at j+1), and for the viscoelastic update
 // the memory variables and relaxation ratios of the cell.
@@ -1118,6 +1281,27 @@ static void launch_v_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, co
     }
 }
 
+// the LDS-staged marches need 64-lane rows (every box at least 64 cells deep
+// in k), one k column and one shot per workgroup, and TI >= 2
+static bool lds_ok(const MarchCfg &c, const MTiles &T)
+{
+    if (!c.lds || c.ti < 2 || T.sb != 1) return false;
+    for (int q = 0; q < T.nb; ++q)
+        if (T.kw[q] != 64 || T.kb[q] != 1) return false;
+    return true;
+}
+
+template <bool CPML, bool SNAP>
+static void launch_v_lds_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
+                            const Psi9 &psi, const Shots &S, const Snap &N, hipStream_t s)
+{
+    switch (ti) {
+    case 2: hipLaunchKernelGGL((k_v_lds<2, CPML, SNAP>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, N); break;
+    case 4: hipLaunchKernelGGL((k_v_lds<4, CPML, SNAP>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, N); break;
+    default: hipLaunchKernelGGL((k_v_lds<8, CPML, SNAP>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, N); break;
+    }
+}
+
 template <bool CPML, bool SNAP>
 static int launch_v(const MarchCfg &c, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
                     const Psi9 &psi, const Shots &S, const Snap &N, hipStream_t s)
@@ -1128,6 +1312,10 @@ static int launch_v(const MarchCfg &c, const MTiles &T, const DevGeom &G, const 
         return IFOS_EINVAL;
     }
     const dim3 grid((unsigned)(T.per * 8));
+    if (lds_ok(c, T)) {
+        launch_v_lds_ti<CPML, SNAP>(c.ti, grid, T, G, F, P, psi, S, N, s);
+        return hip_check(hipGetLastError(), "k_v_lds");
+    }
     if (c.pipe) launch_v_ti<CPML, SNAP, true>(c.ti, grid, T, G, F, P, psi, S, N, s);
     else launch_v_ti<CPML, SNAP, false>(c.ti, grid, T, G, F, P, psi, S, N, s);
     return hip_check(hipGetLastError(), "k_v_march");
@@ -1143,16 +1331,6 @@ static void launch_s_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, co
     case 4: hipLaunchKernelGGL((k_s_march<4, CPML, VISCO, PIPE>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
     default: hipLaunchKernelGGL((k_s_march<8, CPML, VISCO, PIPE>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
     }
-}
-
-// the LDS-staged marches need 64-lane rows (every box at least 64 cells deep
-// in k), one k column and one shot per workgroup, and TI >= 2
-static bool lds_ok(const MarchCfg &c, const MTiles &T)
-{
-    if (!c.lds || c.ti < 2 || T.sb != 1) return false;
-    for (int q = 0; q < T.nb; ++q)
-        if (T.kw[q] != 64 || T.kb[q] != 1) return false;
-    return true;
 }
 
 template <bool CPML, bool VISCO>
