@@ -41,8 +41,9 @@ HBM_PEAK_GBS = 8000.0
 def kernel_names(path, nf, chunk, tiles=None):
     b = lambda x: "true" if x else "false"
     if path == "plain":
-        return {"v_interior": "ifos::k_v_march<4, false, false, false>",
-                "s_interior": "ifos::k_s_march<4, false, false, false>",
+        # interior boxes at least 64 cells deep in k take the LDS marches
+        return {"v_interior": "ifos::k_v_lds<4, false, false>",
+                "s_interior": "ifos::k_s_lds<4, false, false>",
                 "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs<false>",
                 "dft": f"ifos::k_dft_nf<{nf}>", "dft_kahan": f"ifos::k_dft_kahan_nf<{nf}>",
                 "gradient": "ifos::k_grad_march<4, 0>"}

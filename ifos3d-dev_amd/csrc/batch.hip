@@ -442,7 +442,7 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->nf = 0;
     b->tast = 100;
     b->ntr = 0;
-    b->cfg_v = b->cfg_s = MarchCfg{4, 0, 0, 0, 0, 1, 1, 0};
+    b->cfg_v = b->cfg_s = MarchCfg{4, 32, 0, 0, 0, 1, 1, 1};  // the autotuner's usual pick
     b->fused = 0;  // opt-in; the autotuner turns it on where it beats the separate marches
     b->fz_ti = 16;
     b->fz_jc = 0;

@@ -355,7 +355,7 @@ int ifos_levels_run(ifos_batch *const *lv, int nlev, const ifos_shot *const *sho
  * remaining TI / (kb sb) wavefronts go along i (kb sb divides TI); lds: 1 =
  * the in-plane neighbours are staged in LDS (used where every row is 64 k
  * wide and kb = sb = 1, TI >= 2; elsewhere ignored).  NULL leaves a shape
- * unchanged. */
+ * unchanged.  A new batch starts with {4, 32, 0, 0, 0, 1, 1, 1} for both. */
 #define IFOS_TILE_FIELDS 8
 int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[IFOS_TILE_FIELDS], const int cfg_s[IFOS_TILE_FIELDS]);
 /* the autotuner's candidate shapes (IFOS_TILE_FIELDS ints each, at most
