@@ -70,9 +70,11 @@ def _run_case(g, src, rec, finv, tast, vppml, fpml, precision="compat"):
     return ref, obs, out
 
 
-@pytest.mark.parametrize("fdcoeff,free_surf", [(2, 0), (1, 0), (2, 1)])
-def test_gradient_shot_matches_oracle(fdcoeff, free_surf):
-    g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=160, fdcoeff=fdcoeff, free_surf=free_surf)
+# nz 90: the interior box is 74 cells deep in k, so its launches take the
+# LDS-staged marches (a full 64-lane k tile and a 10-lane one)
+@pytest.mark.parametrize("fdcoeff,free_surf,nz", [(2, 0, 38), (1, 0, 38), (2, 1, 38), (2, 1, 90)])
+def test_gradient_shot_matches_oracle(fdcoeff, free_surf, nz):
+    g = small_grid(nx=36, ny=34, nz=nz, fw=8, nt=160 if nz < 64 else 90, fdcoeff=fdcoeff, free_surf=free_surf)
     src = (18, 12, 19, 0.0, 50.0, 1.0, 1)
     rec = np.array([[9 + 2 * n, 10, 19] for n in range(9)], np.int32)
     finv = [40.0, 60.0, 80.0]
@@ -114,12 +116,13 @@ def test_force_source_shot():
         assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0
 
 
-@pytest.mark.parametrize("free_surf", [0, 1])
-def test_viscoelastic_gradient_shot(free_surf):
+@pytest.mark.parametrize("free_surf,nz", [(0, 34), (1, 34), (0, 84)])
+def test_viscoelastic_gradient_shot(free_surf, nz):
     """the plain path with the viscoelastic update (one relaxation
-    mechanism) against the oracle's viscoelastic shot, bit for bit"""
+    mechanism) against the oracle's viscoelastic shot, bit for bit; at nz 84
+    the interior launches take the LDS-staged marches (70 cells deep in k)"""
     import ifosgpu as ig
-    g = small_grid(nx=32, ny=30, nz=34, fw=7, nt=120, free_surf=free_surf)
+    g = small_grid(nx=32, ny=30, nz=nz, fw=7, nt=120 if nz < 64 else 80, free_surf=free_surf)
     src = (16, 11, 17, 0.0, 50.0, 1.0, 1)
     rec = np.array([[8 + 2 * n, 9, 17] for n in range(8)], np.int32)
     finv = [40.0, 70.0]
