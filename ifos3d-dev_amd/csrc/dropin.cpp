@@ -854,9 +854,11 @@ void ifos_dropin_release(void)
     (void)hipDeviceSynchronize();
     for (auto &kv : S.mir) (void)hipFree(kv.second.dev);
     S.mir.clear();
-    void *bufs[] = {S.d_prof, S.d_src, S.d_rec, S.d_sig, S.d_adj, S.d_seisrec};
+    void *bufs[] = {S.d_prof, S.d_src, S.d_rec, S.d_sig, S.d_adj, S.d_seisrec, S.d_step};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    S.d_step = nullptr;
+    S.cap_step = 0;
     S.d_prof = nullptr;
     S.d_src = S.d_rec = S.d_chain = S.d_seisrec = nullptr;
     S.d_sig = S.d_adj = S.d_sec = nullptr;

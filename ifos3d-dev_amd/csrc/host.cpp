@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -886,26 +887,38 @@ extern "C" int ifos_ctx_run_shot_td(ifos_ctx *c, const ifos_shot *s, int every, 
 }
 
 // ---------------------------------------------------------------------------
-// launch timer
+// launch timer: each timed launch is bracketed by two events on its own
+// stream.  Events come from a free list and go back to it once a pair has
+// been read, so a long run (thousands of launches per propagation) holds at
+// most 2 * (kMaxClosed + open scopes) events: when kMaxClosed pairs are
+// waiting to be read, the oldest are waited for and folded into the sums.
 namespace ifos {
+struct ProfMark {
+    int cat;
+    hipEvent_t a, b;
+};
 struct ProfState {
     bool on = false;
-    std::vector<hipEvent_t> pool;
-    size_t used = 0;
-    std::vector<std::pair<int, size_t>> marks;  // (category, start event index)
+    long next_id = 0;
+    std::map<long, ProfMark> open;     // begun, end not yet recorded
+    std::vector<ProfMark> closed;      // both events recorded, not yet read
+    std::vector<hipEvent_t> free_ev;   // events ready for reuse
     double ms[IFOS_K_NCAT] = {};
     long count[IFOS_K_NCAT] = {};
 };
 static ProfState g_prof;
+static const size_t kMaxClosed = 1024;
 
 static hipEvent_t prof_event()
 {
-    if (g_prof.used == g_prof.pool.size()) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        g_prof.pool.push_back(e);
+    if (!g_prof.free_ev.empty()) {
+        hipEvent_t e = g_prof.free_ev.back();
+        g_prof.free_ev.pop_back();
+        return e;
     }
-    return g_prof.pool[g_prof.used++];
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
 }
 
 // launches captured into a graph are not timed one by one
@@ -915,36 +928,61 @@ static bool capturing(hipStream_t s)
     return s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
-void prof_begin(int cat, hipStream_t s)
+// reads every closed pair (waiting for its end event) and recycles its events
+static void prof_fold_closed()
 {
-    if (!g_prof.on || capturing(s)) return;
-    size_t idx = g_prof.used;
-    hipEvent_t e = prof_event();
-    if (!e) return;
-    (void)hipEventRecord(e, s);
-    g_prof.marks.push_back({cat, idx});
-}
-
-void prof_end(int cat, hipStream_t s)
-{
-    if (!g_prof.on || g_prof.marks.empty() || capturing(s)) return;
-    hipEvent_t e = prof_event();
-    if (!e) return;
-    (void)hipEventRecord(e, s);
-    (void)cat;
-}
-
-static void prof_collect()
-{
-    for (auto &m : g_prof.marks) {
+    for (auto &m : g_prof.closed) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, g_prof.pool[m.second], g_prof.pool[m.second + 1]) == hipSuccess) {
-            g_prof.ms[m.first] += ms;
-            g_prof.count[m.first] += 1;
+        if (hipEventSynchronize(m.b) == hipSuccess && hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+            g_prof.ms[m.cat] += ms;
+            g_prof.count[m.cat] += 1;
         }
+        g_prof.free_ev.push_back(m.a);
+        g_prof.free_ev.push_back(m.b);
     }
-    g_prof.marks.clear();
-    g_prof.used = 0;
+    g_prof.closed.clear();
+}
+
+long prof_begin(int cat, hipStream_t s)
+{
+    if (!g_prof.on || capturing(s)) return -1;
+    hipEvent_t a = prof_event();
+    if (!a) return -1;
+    hipEvent_t b = prof_event();
+    if (!b) {
+        g_prof.free_ev.push_back(a);
+        return -1;
+    }
+    (void)hipEventRecord(a, s);
+    const long id = g_prof.next_id++;
+    g_prof.open[id] = ProfMark{cat, a, b};
+    return id;
+}
+
+void prof_end(long id, hipStream_t s)
+{
+    if (id < 0) return;
+    auto it = g_prof.open.find(id);
+    if (it == g_prof.open.end()) return;  // dropped by a reset in between
+    ProfMark m = it->second;
+    g_prof.open.erase(it);
+    (void)hipEventRecord(m.b, s);
+    g_prof.closed.push_back(m);
+    if (g_prof.closed.size() >= kMaxClosed) prof_fold_closed();
+}
+
+static void prof_drop_all()
+{
+    for (auto &kv : g_prof.open) {
+        g_prof.free_ev.push_back(kv.second.a);
+        g_prof.free_ev.push_back(kv.second.b);
+    }
+    g_prof.open.clear();
+    for (auto &m : g_prof.closed) {
+        g_prof.free_ev.push_back(m.a);
+        g_prof.free_ev.push_back(m.b);
+    }
+    g_prof.closed.clear();
 }
 }  // namespace ifos
 
@@ -957,8 +995,7 @@ extern "C" int ifos_prof_enable(int on)
 extern "C" int ifos_prof_reset(void)
 {
     (void)hipDeviceSynchronize();
-    g_prof.marks.clear();
-    g_prof.used = 0;
+    prof_drop_all();
     for (int n = 0; n < IFOS_K_NCAT; ++n) {
         g_prof.ms[n] = 0.0;
         g_prof.count[n] = 0;
@@ -970,7 +1007,7 @@ extern "C" int ifos_prof_read(double *ms, long *count, int ncat)
 {
     int rc = hip_check(hipDeviceSynchronize(), "ifos_prof_read sync");
     if (rc) return rc;
-    prof_collect();
+    prof_fold_closed();
     for (int n = 0; n < ncat && n < IFOS_K_NCAT; ++n) {
         if (ms) ms[n] = g_prof.ms[n];
         if (count) count[n] = g_prof.count[n];

@@ -228,14 +228,15 @@ void set_error(const char *fmt, ...);
 int hip_check(hipError_t e, const char *what);
 
 // Optional per-kernel timing (ifos_prof_enable): a pair of HIP events is
-// recorded around each launch on the launch's own stream.
-void prof_begin(int cat, hipStream_t s);
-void prof_end(int cat, hipStream_t s);
+// recorded around each launch on the launch's own stream; prof_begin returns
+// the pair's id (-1 when timing is off or the stream is being captured).
+long prof_begin(int cat, hipStream_t s);
+void prof_end(long id, hipStream_t s);
 struct ProfScope {
-    int cat;
+    long id;
     hipStream_t s;
-    ProfScope(int c, hipStream_t st) : cat(c), s(st) { prof_begin(cat, s); }
-    ~ProfScope() { prof_end(cat, s); }
+    ProfScope(int c, hipStream_t st) : id(prof_begin(c, st)), s(st) {}
+    ~ProfScope() { prof_end(id, s); }
 };
 
 }  // namespace ifos

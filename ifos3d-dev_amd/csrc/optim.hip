@@ -303,7 +303,7 @@ extern "C" int ifos_lbfgs_savegrad(ifos_lbfgs *b, const float *d_grad1, const fl
     const Pack P = {d_grad1, d_grad2, b->numpar, b->n1};
     hipLaunchKernelGGL(k_lb_pack, dim3(grid_for(b->n)), dim3(256), 0, 0, b->G, P, nullptr, b->grad, 0);
     int rc = hip_check(hipGetLastError(), "k_lb_pack");
-    if (!rc) rc = hip_check(hipDeviceSynchronize(), "lbfgs_savegrad");
+    if (!rc) rc = hip_check(hipStreamSynchronize(0), "lbfgs_savegrad");
     return rc;
 }
 
@@ -363,8 +363,36 @@ extern "C" int ifos_lbfgs_update(ifos_lbfgs *b, float *d_grad1, float *d_grad2, 
                            b->grad + (size_t)w * n);
         rc = hip_check(hipGetLastError(), "k_lb_unpack");
     }
-    if (!rc) rc = hip_check(hipDeviceSynchronize(), "ifos_lbfgs_update");
+    if (!rc) rc = hip_check(hipStreamSynchronize(0), "ifos_lbfgs_update");
     return rc;
+}
+
+// conjugate's FP64 partial sums: allocated once per device on first use and
+// kept for the life of the process (calls on one device share them; the
+// optimiser's calls are serialised on the null stream)
+static int cg_buffers(double **part, double **out)
+{
+    static double *s_part[64] = {}, *s_out[64] = {};
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) {
+        set_error("ifos_conjugate: device %d out of range", dev);
+        return IFOS_EINVAL;
+    }
+    if (!s_part[dev]) {
+        rc = hip_check(hipMalloc(&s_part[dev], kRedBlocks * 6 * sizeof(double)), "conjugate partials");
+        if (!rc) rc = hip_check(hipMalloc(&s_out[dev], 6 * sizeof(double)), "conjugate sums");
+        if (rc) {
+            if (s_part[dev]) (void)hipFree(s_part[dev]);
+            s_part[dev] = nullptr;
+            s_out[dev] = nullptr;
+            return rc;
+        }
+    }
+    *part = s_part[dev];
+    *out = s_out[dev];
+    return IFOS_OK;
 }
 
 extern "C" int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float *const d_prior[6], float beta[3],
@@ -381,12 +409,11 @@ extern "C" int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float 
     if (cdf == 1) {
         hipLaunchKernelGGL(k_cg_copy, dim3(gr), dim3(256), 0, 0, G, A);
         int rc = hip_check(hipGetLastError(), "k_cg_copy");
-        if (!rc) rc = hip_check(hipDeviceSynchronize(), "ifos_conjugate");
+        if (!rc) rc = hip_check(hipStreamSynchronize(0), "ifos_conjugate");
         return rc;
     }
     double *part = nullptr, *out = nullptr;
-    int rc = hip_check(hipMalloc(&part, kRedBlocks * 6 * sizeof(double)), "conjugate partials");
-    if (!rc) rc = hip_check(hipMalloc(&out, 6 * sizeof(double)), "conjugate sums");
+    int rc = cg_buffers(&part, &out);
     double s[6] = {0, 0, 0, 0, 0, 0};
     if (!rc) {
         hipLaunchKernelGGL(k_cg_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, 0, G, A, part);
@@ -394,8 +421,6 @@ extern "C" int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float 
         rc = hip_check(hipGetLastError(), "k_cg_dots");
     }
     if (!rc) rc = hip_check(hipMemcpy(s, out, sizeof(s), hipMemcpyDeviceToHost), "conjugate sums copy");
-    (void)hipFree(part);
-    (void)hipFree(out);
     if (rc) return rc;
     if (sums)
         for (int m = 0; m < 6; ++m) sums[m] = s[m];
@@ -414,7 +439,7 @@ extern "C" int ifos_conjugate(const ifos_geom *g, float *const d_grad[3], float 
     if (beta[0] > 0.5f) beta[2] = 0.5f;
     hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(256), 0, 0, G, A, beta[0], beta[1], beta[2]);
     rc = hip_check(hipGetLastError(), "k_cg_update");
-    if (!rc) rc = hip_check(hipDeviceSynchronize(), "ifos_conjugate");
+    if (!rc) rc = hip_check(hipStreamSynchronize(0), "ifos_conjugate");
     return rc;
 }
 

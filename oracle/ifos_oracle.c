@@ -17,9 +17,22 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define HALO_LO 2
 #define HALO_HI 2
+
+/* threads the OpenMP cell loops use (1 in a build without OpenMP) */
+int og_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 size_t og_field_size(const og_grid *g)
 {
@@ -334,6 +347,7 @@ void og_update_v(const og_grid *g, const int box[6], int nt, oreal *vx,
     int i, j, k;
     sdiff d;
     fd4_coeffs(g->fdcoeff, &b1, &b2);
+    #pragma omp parallel for private(i, k, d) schedule(static)
     for (j = box[2]; j <= box[3]; j++)
         for (i = box[0]; i <= box[1]; i++)
             for (k = box[4]; k <= box[5]; k++) {
@@ -415,6 +429,7 @@ void og_update_v_cpml(const og_grid *g, const int box[6], int nt, oreal *vx,
     oreal *p_sxy_y = psi[3], *p_syy_y = psi[4], *p_syz_y = psi[5];
     oreal *p_sxz_z = psi[6], *p_syz_z = psi[7], *p_szz_z = psi[8];
     fd4_coeffs(g->fdcoeff, &b1, &b2);
+    #pragma omp parallel for private(i, k, d) schedule(static)
     for (j = 1; j <= g->ny; j++)
         for (i = 1; i <= g->nx; i++)
             for (k = 1; k <= g->nz; k++) {
@@ -507,6 +522,7 @@ void og_update_s_elastic(const og_grid *g, const int box[6], const oreal *vx,
     int i, j, k;
     vdiff d;
     fd4_coeffs(g->fdcoeff, &b1, &b2);
+    #pragma omp parallel for private(i, k, d) schedule(static)
     for (j = box[2]; j <= box[3]; j++)
         for (i = box[0]; i <= box[1]; i++)
             for (k = box[4]; k <= box[5]; k++) {
@@ -561,6 +577,7 @@ void og_update_s_cpml_elastic(const og_grid *g, const int box[6],
     int i, j, k;
     vdiff d;
     fd4_coeffs(g->fdcoeff, &b1, &b2);
+    #pragma omp parallel for private(i, k, d) schedule(static)
     for (j = 1; j <= g->ny; j++)
         for (i = 1; i <= g->nx; i++)
             for (k = 1; k <= g->nz; k++) {
@@ -637,6 +654,7 @@ void og_update_s_visco(const og_grid *g, const int box[6], const oreal *vx,
     int i, j, k;
     vdiff d;
     fd4_coeffs(g->fdcoeff, &b1, &b2);
+    #pragma omp parallel for private(i, k, d) schedule(static)
     for (j = box[2]; j <= box[3]; j++)
         for (i = box[0]; i <= box[1]; i++)
             for (k = box[4]; k <= box[5]; k++) {
@@ -659,6 +677,7 @@ void og_update_s_cpml_visco(const og_grid *g, const int box[6],
     int i, j, k;
     vdiff d;
     fd4_coeffs(g->fdcoeff, &b1, &b2);
+    #pragma omp parallel for private(i, k, d) schedule(static)
     for (j = 1; j <= g->ny; j++)
         for (i = 1; i <= g->nx; i++)
             for (k = 1; k <= g->nz; k++) {
@@ -772,6 +791,7 @@ void og_discfourier(const og_grid *g, int nt, const oreal *vx,
         double trig1 = cos(2.0 * t * finv[l - 1] * M_PI) * g->dt * ntast;
         double trig2 = sin(2.0 * t * finv[l - 1] * M_PI) * g->dt * ntast;
         size_t off = (size_t)(l - 1) * fs;
+        #pragma omp parallel for private(i, k) schedule(static)
         for (j = 1; j <= g->ny; j++)
             for (i = 1; i <= g->nx; i++)
                 for (k = 1; k <= g->nz; k++) {
@@ -816,6 +836,7 @@ void og_gradient_F(const og_grid *g, const oreal *fvx, const oreal *fvy,
         size_t off = (size_t)(l - 1) * fs;
         oreal fdummy = (oreal)(2.0 * 2.0 * finv[l - 1] * finv[l - 1] * M_PI * M_PI);
         fdummy = 1 / fdummy;
+        #pragma omp parallel for private(i, k) schedule(static)
         for (j = 1; j <= g->ny; j++)
             for (i = 1; i <= g->nx; i++)
                 for (k = 1; k <= g->nz; k++) {
@@ -868,6 +889,7 @@ void og_hess_F(const og_grid *g, const oreal *fvx, const oreal *fvy,
             const size_t off = (size_t)(l - 1) * fs, offb = (size_t)(m - 1) * fs;
             oreal fdummy = (oreal)(finv[l - 1] * M_PI * 2); /* hess_F.c:75-77 */
             fdummy = 1 / fdummy;
+            #pragma omp parallel for private(i, k) schedule(static)
             for (j = 1; j <= g->ny; j++)
                 for (i = 1; i <= g->nx; i++)
                     for (k = 1; k <= g->nz; k++) {
@@ -956,6 +978,7 @@ void og_td_gradient(const og_grid *g, const oreal *vfwd, const oreal *vadj, cons
                 ub[q][c] = ub[q][c] + vb[q][c] * dt;
             }
         }
+        #pragma omp parallel for private(i, k, c) schedule(static)
         for (j = 1; j <= g->ny; j++)
             for (i = 1; i <= g->nx; i++)
                 for (k = 1; k <= g->nz; k++) {
@@ -975,6 +998,7 @@ void og_td_gradient(const og_grid *g, const oreal *vfwd, const oreal *vadj, cons
                     acc[2][c] += (double)gr * (double)dt;
                 }
     }
+    #pragma omp parallel for private(i, k, c) schedule(static)
     for (j = 1; j <= g->ny; j++)
         for (i = 1; i <= g->nx; i++)
             for (k = 1; k <= g->nz; k++) {

@@ -49,6 +49,7 @@ typedef struct og_grid {
 } og_grid;
 
 /* number of elements of one canonical 3-D field */
+int og_threads(void);
 size_t og_field_size(const og_grid *g);
 /* flat offset of (j,i,k) in a canonical field */
 size_t og_idx(const og_grid *g, int j, int i, int k);

@@ -100,6 +100,11 @@ class Oracle:
         self.dtype = np.float32 if precision == "f32" else np.float64
         self.precision = precision
 
+    def threads(self) -> int:
+        """threads of the oracle's OpenMP cell loops"""
+        self.lib.og_threads.restype = C.c_int
+        return int(self.lib.og_threads())
+
     def zeros(self, g: Grid, n=None):
         return np.zeros(g.shape if n is None else (n,) + g.shape, dtype=self.dtype)
 
@@ -207,6 +212,13 @@ class Oracle:
         self.lib.og_surface_elastic(C.byref(og), _p(u), _p(pi), _p(sxx), _p(syy), _p(szz), _p(sxy),
                                     _p(syz), _p(vx), _p(vy), _p(vz))
 
+    def seismo(self, g, ins, rec_ijk, sec, f):
+        """seismo_ssg.c: sample ins of sec[3] (ntr, ns+1) from vx, vy, vz"""
+        og = g.og()
+        rec = np.ascontiguousarray(rec_ijk, np.int32)
+        self.lib.og_seismo(C.byref(og), ins, rec.shape[0], rec.ctypes.data_as(C.c_void_p), _p(sec[0]), _p(sec[1]),
+                           _p(sec[2]), sec[0].shape[1] - 1, _p(f[0]), _p(f[1]), _p(f[2]))
+
     def residual(self, g, obs, syn):
         ntr, ns1 = obs.shape
         diff = np.zeros_like(obs)
@@ -243,9 +255,12 @@ class Oracle:
         return self.lib.og_ntast(C.c_float(fmax), tast, C.c_float(dt))
 
     def run_shot(self, g: Grid, rho, pi, u, src, rec_ijk, finv, tast=100, vppml=4000.0, fpml=50.0,
-                 shape=1, obs=None, want_adjoint=True, want_fields=False, visco=None, want_history=False):
+                 shape=1, obs=None, want_adjoint=True, want_fields=False, visco=None, want_history=False,
+                 grads=None):
         """visco = (taus, taup, eta): the viscoelastic update with one
-        relaxation mechanism instead of the elastic one"""
+        relaxation mechanism instead of the elastic one.  grads: three
+        canonical arrays the shot's gradient is added into in place (the
+        reference accumulates over shots, ifos3d.c:974); fresh zeros if None"""
         class Io(C.Structure):
             _fields_ = [("rho", C.c_void_p), ("pi", C.c_void_p), ("u", C.c_void_p), ("src", OgSource),
                         ("source_shape", C.c_int), ("ntr", C.c_int), ("rec_ijk", C.c_void_p),
@@ -262,7 +277,10 @@ class Oracle:
         ntr = rec.shape[0]
         fv = np.asarray(finv, np.float32)
         syn = [np.zeros((ntr, g.nt + 1), np.float32) for _ in range(3)]
-        grads = [self.zeros(g) for _ in range(3)]
+        if grads is None:
+            grads = [self.zeros(g) for _ in range(3)]
+        for a in grads:
+            assert a.dtype == self.dtype and a.shape == g.shape and a.flags.c_contiguous
         l2 = np.zeros(1, np.float32)
         ff = np.zeros((6, len(fv)) + g.shape, self.dtype) if want_fields else None
         fa = np.zeros((6, len(fv)) + g.shape, self.dtype) if want_fields else None

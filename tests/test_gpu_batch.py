@@ -18,7 +18,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import max_ulp, small_grid
+from helpers import max_ulp, rel_l2, small_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -97,6 +97,21 @@ def _batched(ig, G, g, true, start, srcs, rec, finv, tast, prec, chunk, use_grap
     return out
 
 
+def _oracle(g, true, start, srcs, rec, finv, tast):
+    """the oracle's observed data, synthetics, misfits and the gradient
+    accumulated over the shots in order (ifos3d.c:712-975)"""
+    from pyoracle import Oracle
+    o = Oracle("f32")
+    obs = [o.run_shot(g, *true, s, rec, finv, tast=tast, want_adjoint=False)["syn"] for s in srcs]
+    grads = [o.zeros(g) for _ in range(3)]
+    syn, l2 = [], []
+    for n, s in enumerate(srcs):
+        r = o.run_shot(g, *start, s, rec, finv, tast=tast, obs=obs[n], grads=grads)
+        syn.append(r["syn"])
+        l2.append(r["l2"])
+    return dict(obs=obs, syn=syn, l2=l2, grads=grads)
+
+
 CASES = {
     # explosive sources in the interior, receivers in the interior and in the y slab, two sharing a cell
     "interior": dict(free_surf=0, srcs=[(18, 12, 19, 0.0, 50.0, 1.0, 1), (12, 14, 22, 0.0, 45.0, 1.0, 1),
@@ -131,6 +146,21 @@ def test_batch_matches_plain(case, prec_name, chunk, use_graph):
     for n in range(3):
         assert max_ulp(out["grads"][n], ref["grads"][n]) == 0, f"gradient {n}"
     assert np.abs(ref["grads"][0]).max() > 0
+    # the batched outputs against the oracle directly (not only through the
+    # plain path): compat bit for bit; Kahan changes the DFT's rounding only
+    orc = _oracle(g, true, start, c["srcs"], rec, finv, tast)
+    kahan = prec_name == "kahan"
+    for n in range(len(c["srcs"])):
+        for q in range(3):
+            assert max_ulp(out["obs"][n][q][:, 1:], orc["obs"][n][q][:, 1:]) == 0, f"oracle: observed, shot {n}"
+            assert max_ulp(out["syn"][n][q][:, 1:], orc["syn"][n][q][:, 1:]) == 0, f"oracle: synthetics, shot {n}"
+        assert abs(out["l2"][n] - orc["l2"][n]) <= 1e-5 * abs(orc["l2"][n]), f"oracle: misfit, shot {n}"
+    for n in range(3):
+        a, want = g.interior(out["grads"][n]), g.interior(orc["grads"][n])
+        if kahan:
+            assert rel_l2(a, want) <= 1e-4, f"oracle: gradient {n} (Kahan DFT)"
+        else:
+            assert max_ulp(a, want) == 0, f"oracle: gradient {n}"
     if use_graph:
         # replays: the captured loops run again with eager work in between
         b = out["ctx"]
