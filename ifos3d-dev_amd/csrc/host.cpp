@@ -360,10 +360,12 @@ struct ifos_ctx {
     float eta;
     float *r[6], *taus, *taup, *tavg[3];
     std::vector<void *> allocs;
-    struct TdBufs *td;  // buffers of the time-domain gradient (ifos_ctx_run_shot_td)
+    struct TdBufs *td;    // buffers of the time-domain gradient (ifos_ctx_run_shot_td)
+    struct AdjBufs *cadj;  // buffers of the consistent adjoint (ifos_ctx_run_shot_adj)
 };
 
 static void td_free(ifos_ctx *c);
+static void adj_free(ifos_ctx *c);
 
 static int dalloc(ifos_ctx *c, void **p, size_t bytes)
 {
@@ -383,6 +385,7 @@ extern "C" int ifos_ctx_destroy(ifos_ctx *c)
     if (!c) return IFOS_OK;
     hipDeviceSynchronize();
     td_free(c);
+    adj_free(c);
     for (void *p : c->allocs) hipFree(p);
     delete c;
     return IFOS_OK;
@@ -395,6 +398,8 @@ extern "C" int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max, i
         return IFOS_EINVAL;
     }
     ifos_ctx *c = new ifos_ctx();
+    c->td = nullptr;
+    c->cadj = nullptr;
     c->g = *g;
     c->nf_max = nf_max;
     c->ntr_max = ntr_max;
@@ -883,6 +888,196 @@ extern "C" int ifos_ctx_run_shot_td(ifos_ctx *c, const ifos_shot *s, int every, 
     }
     ifos_model mod = ctx_model(c);
     if (!rc) rc = launch_td_finish(g, T->acc, &mod, d_grad, st);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// consistent (exact discrete) adjoint (adjoint.hip).  The forward pass is
+// ifos_ctx_run_shot's, with the state stored every K steps; the reverse pass
+// walks the segments from the last to the first, recomputes each segment's
+// forward steps from its checkpoint into a K-deep ring of what the
+// transposed steps read (v(n), sigma(n-1) and both CPML memories after step
+// n), then runs the reverse steps of the segment.
+struct AdjBufs {
+    int K = 0, nck = 0, ntr = 0;
+    std::vector<ShotState> ck;
+    ShotState work;
+    std::vector<float *> rv, rs, rps, rpv;  // ring: [K][3] v, [K][6] sigma, [K][9] psi_s, [K][9] psi_v
+    float *L[9], *scr[9], *mu_s[9], *mu_v[9];
+    double *G[8], *Gr, *Gp, *Gu, *J;
+    float *res;
+    std::vector<void *> allocs;
+};
+
+static void adj_free(ifos_ctx *c)
+{
+    if (!c->cadj) return;
+    for (void *p : c->cadj->allocs) (void)hipFree(p);
+    delete c->cadj;
+    c->cadj = nullptr;
+}
+
+static int adj_one(AdjBufs *A, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) for the consistent adjoint: %s", bytes, hipGetErrorString(e));
+        return IFOS_ENOMEM;
+    }
+    A->allocs.push_back(*p);
+    return hip_check(hipMemset(*p, 0, bytes ? bytes : 4), "hipMemset");
+}
+
+static int adj_alloc(ifos_ctx *c, int K, int ntr)
+{
+    if (c->cadj && c->cadj->K == K && c->cadj->ntr == ntr) return IFOS_OK;
+    adj_free(c);
+    AdjBufs *A = new AdjBufs();
+    c->cadj = A;
+    A->K = K;
+    A->ntr = ntr;
+    A->nck = (c->g.nt + K - 1) / K;
+    const size_t fe = c->fe;
+    int rc = IFOS_OK;
+    auto state = [&](ShotState &S) {
+        for (int n = 0; n < 9 && !rc; ++n) rc = adj_one(A, (void **)&S.fields[n], fe * 4);
+        for (int n = 0; n < 9 && !rc; ++n) rc = adj_one(A, (void **)&S.psi_s[n], ifos_psi_elems(&c->g, n) * 4);
+        for (int n = 0; n < 9 && !rc; ++n) rc = adj_one(A, (void **)&S.psi_v[n], ifos_psi_elems(&c->g, n) * 4);
+        for (int n = 0; n < 6; ++n) S.r[n] = nullptr;
+    };
+    A->ck.resize(A->nck);
+    for (int q = 0; q < A->nck && !rc; ++q) state(A->ck[q]);
+    if (!rc) state(A->work);
+    A->rv.assign(3 * (size_t)K, nullptr);
+    A->rs.assign(6 * (size_t)K, nullptr);
+    A->rps.assign(9 * (size_t)K, nullptr);
+    A->rpv.assign(9 * (size_t)K, nullptr);
+    for (auto &p : A->rv)
+        if (!rc) rc = adj_one(A, (void **)&p, fe * 4);
+    for (auto &p : A->rs)
+        if (!rc) rc = adj_one(A, (void **)&p, fe * 4);
+    for (size_t q = 0; q < A->rps.size() && !rc; ++q) {
+        rc = adj_one(A, (void **)&A->rps[q], ifos_psi_elems(&c->g, (int)(q % 9)) * 4);
+        if (!rc) rc = adj_one(A, (void **)&A->rpv[q], ifos_psi_elems(&c->g, (int)(q % 9)) * 4);
+    }
+    for (int n = 0; n < 9 && !rc; ++n) {
+        rc = adj_one(A, (void **)&A->L[n], fe * 4);
+        if (!rc) rc = adj_one(A, (void **)&A->scr[n], fe * 4);
+        if (!rc) rc = adj_one(A, (void **)&A->mu_s[n], ifos_psi_elems(&c->g, n) * 4);
+        if (!rc) rc = adj_one(A, (void **)&A->mu_v[n], ifos_psi_elems(&c->g, n) * 4);
+    }
+    for (int n = 0; n < 8 && !rc; ++n) rc = adj_one(A, (void **)&A->G[n], fe * 8);
+    if (!rc) rc = adj_one(A, (void **)&A->Gr, fe * 8);
+    if (!rc) rc = adj_one(A, (void **)&A->Gp, fe * 8);
+    if (!rc) rc = adj_one(A, (void **)&A->Gu, fe * 8);
+    if (!rc) rc = adj_one(A, (void **)&A->J, 8);
+    if (!rc) rc = adj_one(A, (void **)&A->res, 3 * (size_t)(ntr > 0 ? ntr : 1) * (c->g.nt + 1) * 4);
+    if (rc) adj_free(c);
+    return rc;
+}
+
+static int adj_K(const ifos_geom *g, int every)
+{
+    return every > 0 ? std::min(every, g->nt) : std::max(1, (int)lround(sqrt((double)g->nt)));
+}
+
+extern "C" size_t ifos_adj_bytes(const ifos_geom *g, int every, int ntr)
+{
+    const int K = adj_K(g, every);
+    const size_t nck = (size_t)(g->nt + K - 1) / K, fe = ifos_field_elems(g);
+    size_t psi = 0;
+    for (int n = 0; n < 9; ++n) psi += ifos_psi_elems(g, n);
+    const size_t state = 9 * fe * 4 + 2 * psi * 4;
+    return (nck + 1) * state + (size_t)K * state + 18 * fe * 4 + 2 * psi * 4 + 11 * fe * 8 +
+           3 * (size_t)(ntr > 0 ? ntr : 1) * (g->nt + 1) * 4 + 8;
+}
+
+extern "C" int ifos_ctx_run_shot_adj(ifos_ctx *c, const ifos_shot *s, int every, float *d_syn,
+                                     float *const d_grad[3], double *J, void *stream)
+{
+    if (!c || !s || !d_syn || !d_grad || !s->d_obs) {
+        set_error("ifos_ctx_run_shot_adj: needs a shot with observed data, the seismogram buffer and the gradients");
+        return IFOS_EINVAL;
+    }
+    const ifos_geom *g = &c->g;
+    if (c->L || g->free_surf || s->src_type != 1) {
+        set_error("ifos_ctx_run_shot_adj: the elastic update, no free surface and an explosive source only");
+        return IFOS_EINVAL;
+    }
+    const int N = g->nt, K = adj_K(g, every);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = prepare_shot(c, s, st, "ifos_ctx_run_shot_adj", false);
+    if (!rc) rc = adj_alloc(c, K, s->ntr);
+    if (rc) return rc;
+    AdjBufs *A = c->cadj;
+    const ShotState S = ctx_state(c);
+    // forward (ifos3d.c:754-814) with the state stored after steps 0, K, 2K, ...
+    rc = zero_wavefield(c, st);
+    if (!rc) rc = ifos_fill(d_syn, 3 * (size_t)s->ntr * (N + 1), 0.0f, st);
+    if (!rc) rc = copy_state(c, A->ck[0], S, st);
+    for (int nt = 1; nt <= N && !rc; ++nt) {
+        if ((nt & 63) == 0) rc = probe_nan(c, S, nt, st);
+        if (!rc) rc = step(c, S, 0, s, nt, false, 1, d_syn, st);
+        if (!rc && nt % K == 0 && nt / K < A->nck) rc = copy_state(c, A->ck[nt / K], S, st);
+    }
+    if (!rc) rc = probe_nan(c, S, N, st);
+    if (!rc) rc = launch_ca_residual(d_syn, s->d_obs, A->res, s->ntr, N, A->J, st);
+    for (int n = 0; n < 9 && !rc; ++n) {
+        rc = ifos_fill(A->L[n], c->fe, 0.0f, st);
+        if (!rc) rc = ifos_fill(A->mu_s[n], ifos_psi_elems(g, n), 0.0f, st);
+        if (!rc) rc = ifos_fill(A->mu_v[n], ifos_psi_elems(g, n), 0.0f, st);
+    }
+    for (int n = 0; n < 8 && !rc; ++n) rc = ifos_fill((float *)A->G[n], 2 * c->fe, 0.0f, st);
+    const ifos_model mod = ctx_model(c);
+    for (int q = A->nck - 1; q >= 0 && !rc; --q) {
+        const int a = q * K, b = std::min(a + K, N);
+        rc = copy_state(c, A->work, A->ck[q], st);
+        for (int n = a + 1; n <= b && !rc; ++n) {
+            const size_t e = (size_t)(n - a - 1);
+            for (int m = 0; m < 6 && !rc; ++m)
+                rc = hip_check(hipMemcpyAsync(A->rs[6 * e + m], A->work.fields[3 + m], c->fe * 4,
+                                              hipMemcpyDeviceToDevice, st), "ring copy");
+            if (!rc) rc = step(c, A->work, 0, s, n, false, 1, nullptr, st);
+            for (int m = 0; m < 3 && !rc; ++m)
+                rc = hip_check(hipMemcpyAsync(A->rv[3 * e + m], A->work.fields[m], c->fe * 4, hipMemcpyDeviceToDevice,
+                                              st), "ring copy");
+            for (int m = 0; m < 9 && !rc; ++m) {
+                const size_t pb = ifos_psi_elems(g, m) * 4;
+                rc = hip_check(hipMemcpyAsync(A->rps[9 * e + m], A->work.psi_s[m], pb, hipMemcpyDeviceToDevice, st),
+                               "ring copy");
+                if (!rc)
+                    rc = hip_check(hipMemcpyAsync(A->rpv[9 * e + m], A->work.psi_v[m], pb, hipMemcpyDeviceToDevice,
+                                                  st), "ring copy");
+            }
+        }
+        for (int n = b; n > a && !rc; --n) {
+            const size_t e = (size_t)(n - a - 1);
+            CaStep C;
+            for (int m = 0; m < 9; ++m) {
+                C.L[m] = A->L[m];
+                C.scr[m] = A->scr[m];
+                C.pv[m] = A->rpv[9 * e + m];
+                C.ps[m] = A->rps[9 * e + m];
+                C.mu_v[m] = A->mu_v[m];
+                C.mu_s[m] = A->mu_s[m];
+            }
+            for (int m = 0; m < 3; ++m) C.v[m] = A->rv[3 * e + m];
+            for (int m = 0; m < 6; ++m) C.s[m] = A->rs[6 * e + m];
+            const float *mp[9] = {mod.rho, mod.pi, mod.u, mod.uipjp, mod.ujpkp, mod.uipkp, mod.rjp, mod.rkp, mod.rip};
+            for (int m = 0; m < 9; ++m) C.model[m] = mp[m];
+            for (int m = 0; m < 8; ++m) C.G[m] = A->G[m];
+            C.rec = c->rec;
+            C.ntr = s->ntr;
+            C.res = A->res;
+            C.n = n;
+            rc = launch_ca_step(g, c->prof, C, st);
+        }
+    }
+    if (!rc) rc = launch_ca_finish(g, A->G, &mod, A->Gr, A->Gp, A->Gu, d_grad, st);
+    double hJ = 0.0;
+    if (!rc) rc = hip_check(hipMemcpyAsync(&hJ, A->J, 8, hipMemcpyDeviceToHost, st), "misfit copy");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "consistent adjoint");
+    if (!rc && J) *J = hJ;
     return rc;
 }
 

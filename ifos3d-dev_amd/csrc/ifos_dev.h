@@ -222,6 +222,26 @@ int launch_td_sample(const ifos_geom *g, const float *const uf[3], const float *
                      const float *const ub[3], const float *const vb[3], double w, double *const acc[3],
                      hipStream_t s);
 int launch_td_integrate(const ifos_geom *g, float *const u[3], const float *const v[3], float dt, hipStream_t s);
+
+// one reverse step n of the consistent adjoint (adjoint.hip)
+struct CaStep {
+    float *L[9], *scr[9];        // adjoint fields, scratch (device layout, zero halos)
+    const float *v[3];           // forward v(n)
+    const float *s[6];           // forward sigma(n-1): sxx syy szz sxy syz sxz
+    float *pv[9], *ps[9];        // forward CPML memory after step n (read only)
+    float *mu_v[9], *mu_s[9];    // their adjoints
+    const float *model[9];       // rho pi u uipjp ujpkp uipkp rjp rkp rip
+    double *G[8];                // dJ / d rip rjp rkp pi u uipjp ujpkp uipkp
+    const int *rec;
+    int ntr;
+    const float *res;            // [3][ntr][nt+1]
+    int n;
+};
+int launch_ca_step(const ifos_geom *g, const float *d_prof, const CaStep &S, hipStream_t st);
+int launch_ca_residual(const float *d_syn, const float *d_obs, float *d_res, int ntr, int nt, double *d_J,
+                       hipStream_t st);
+int launch_ca_finish(const ifos_geom *g, double *const G[8], const ifos_model *m, double *Gr, double *Gp,
+                     double *Gu, float *const d_grad[3], hipStream_t st);
 int launch_td_finish(const ifos_geom *g, double *const acc[3], const ifos_model *m, float *const d_grad[3],
                      hipStream_t s);
 void set_error(const char *fmt, ...);

@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
         _lib.ifos_psi_elems.restype = C.c_size_t
         _lib.ifos_batch_bytes.restype = C.c_size_t
         _lib.ifos_td_bytes.restype = C.c_size_t
+        _lib.ifos_adj_bytes.restype = C.c_size_t
     return _lib
 
 
@@ -449,6 +450,22 @@ class ShotContext:
         check(lib().ifos_ctx_run_shot_td(self.h, C.byref(s), every, _ptr(d_syn), Gp, C.byref(l2), stream),
               "ifos_ctx_run_shot_td")
         return l2.value
+
+    def run_shot_adj(self, src, rec_ijk, d_syn, grads, d_obs, every=0, shape=1, stream=None):
+        """ifos_ctx_run_shot_adj: the consistent (exact discrete) adjoint;
+        returns J = 1/2 sum (syn - obs)^2 and adds dJ/d(vp, vs, rho) to grads"""
+        rec = np.ascontiguousarray(rec_ijk, np.int32)
+        s = IfosShot(src[0], src[1], src[2], src[6], src[3], src[4], src[5], shape, rec.shape[0],
+                     rec.ctypes.data, d_obs.data_ptr(), 1)
+        J = C.c_double(0.0)
+        Gp = (C.c_void_p * 3)(*[t.data_ptr() for t in grads])
+        check(lib().ifos_ctx_run_shot_adj(self.h, C.byref(s), every, _ptr(d_syn), Gp, C.byref(J), stream),
+              "ifos_ctx_run_shot_adj")
+        return J.value
+
+    @staticmethod
+    def adj_bytes(g, every=0, ntr=1) -> int:
+        return int(lib().ifos_adj_bytes(g.ref, every, ntr))
 
     @staticmethod
     def td_bytes(g, every=0, L=0) -> int:

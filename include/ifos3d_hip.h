@@ -297,6 +297,27 @@ int ifos_ctx_run_shot_td(ifos_ctx *c, const ifos_shot *s, int every, float *d_sy
                          float *const d_grad[3], double *l2, void *stream);
 /* device bytes the time-domain gradient buffers take (L: viscoelastic) */
 size_t ifos_td_bytes(const ifos_geom *g, int every, int L);
+/* Consistent (exact discrete) adjoint gradient -- no counterpart in the
+ * reference, whose adjoint re-runs the forward scheme with the reversed,
+ * integrated residuals (residual.c:43-53, injected twice:
+ * update_v_ssg.c:512-522, update_v_ssg_CPML.c:384-393) and whose gradient
+ * is the frequency-domain estimate of gradient_F.c.  The misfit is
+ * J = 1/2 sum over samples 1..NT, receivers and components of
+ * (syn - obs)^2 of the plain path's own forward scheme (ifos_ctx_run_shot's
+ * forward, bit-identical), and d_grad receives (added to it) its exact
+ * derivative with respect to vp, vs and rho per cell (pi = rho vp^2,
+ * u = rho vs^2, through av_mat's averages and matcopy's halo copies): the
+ * transpose of every time step run backwards (hand-written adjoint
+ * stencils), the model derivatives summed in FP64.  The forward state is
+ * stored every `every` steps (<= 0: round(sqrt(NT))) and each segment is
+ * recomputed once during the reverse pass; the context keeps these buffers
+ * (ifos_adj_bytes) until it is destroyed or `every` or ntr changes.  Elastic
+ * update, no free surface and an explosive source (type 1) only; needs
+ * d_obs, d_syn (receives the forward seismograms) and d_grad; *J the misfit. */
+int ifos_ctx_run_shot_adj(ifos_ctx *c, const ifos_shot *s, int every, float *d_syn,
+                          float *const d_grad[3], double *J, void *stream);
+/* device bytes the consistent-adjoint buffers take */
+size_t ifos_adj_bytes(const ifos_geom *g, int every, int ntr);
 /* device pointers owned by the context (fields, DFT accumulators) */
 int ifos_ctx_fields(ifos_ctx *c, ifos_fields *f);
 int ifos_ctx_dft(ifos_ctx *c, int back, float *d_F[6]);

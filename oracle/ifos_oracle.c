@@ -1206,3 +1206,339 @@ done:
     for (n = 0; n < 6; n++) free(rmem[n]);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* The consistent adjoint (ifos_oracle.h): the transpose of og_run_shot's
+ * forward step, run backwards over a stored forward history. */
+
+/* derivative components of the velocity update, in sdiff order
+ * (sxx_x sxy_y sxz_z syy_y sxy_x syz_z szz_z sxz_x syz_y): differentiated
+ * field, axis, forward (1) or backward (0) difference, half-grid CPML
+ * profile, CPML memory array (og_update_v_cpml) */
+static const int VQ_FIELD[9] = {3, 6, 8, 4, 6, 7, 5, 8, 7};
+static const int VQ_AXIS[9] = {0, 1, 2, 1, 0, 2, 2, 0, 1};
+static const int VQ_FWD[9] = {1, 0, 0, 1, 0, 0, 1, 0, 0};
+static const int VQ_HALF[9] = {1, 0, 0, 1, 0, 0, 1, 0, 0};
+static const int VQ_PSI[9] = {0, 3, 6, 4, 1, 7, 8, 2, 5};
+/* the same for the stress update, vdiff order (vxx vxy vxz vyx vyy vyz vzx
+ * vzy vzz; s_cpml) */
+static const int SQ_FIELD[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2};
+static const int SQ_AXIS[9] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+static const int SQ_FWD[9] = {0, 1, 1, 1, 0, 1, 1, 1, 0};
+static const int SQ_HALF[9] = {0, 1, 1, 1, 0, 1, 1, 1, 0};
+static const int SQ_PSI[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};
+
+/* CPML profile values and memory-variable slot of one derivative component
+ * at (j, i, k); 0 where the component has no memory variable.  stress: the
+ * z layers take K from the y axis at the low side (s_cpml). */
+static int q_cpml(const og_grid *g, oreal *const prof[18], int stress, int q, int j, int i, int k, oreal *K,
+                  oreal *a, oreal *b, int *arr, size_t *at)
+{
+    const int fw = g->fw;
+    const int ax = stress ? SQ_AXIS[q] : VQ_AXIS[q];
+    const int half = stress ? SQ_HALF[q] : VQ_HALF[q];
+    const int h = ax == 0 ? slab_h(i, g->nx, fw, 1) : ax == 1 ? slab_h(j, g->ny, fw, !g->free_surf)
+                                                                : slab_h(k, g->nz, fw, 1);
+    int kax = ax;
+    if (!h) return 0;
+    if (stress && ax == 2) kax = (k <= fw) ? 1 : 2;
+    *K = half ? PKH(kax, h) : PK(kax, h);
+    *a = half ? PAH(ax, h) : PA(ax, h);
+    *b = half ? PBH(ax, h) : PB(ax, h);
+    *arr = stress ? SQ_PSI[q] : VQ_PSI[q];
+    *at = ax == 0 ? psi_x_at(g, j, h, k) : ax == 1 ? psi_y_at(g, h, i, k) : psi_z_at(g, j, i, h);
+    return 1;
+}
+
+/* transposed 4th-order differences at x (L zero outside [1..N]^3):
+ * DFWD^T L = b1 (L[x-s] - L[x]) + b2 (L[x-2s] - L[x+s]),
+ * DBWD^T L = b1 (L[x] - L[x+s]) + b2 (L[x-s] - L[x+2s]) */
+static inline oreal dT(const oreal *L, size_t x, ptrdiff_t s, int fwd, oreal b1, oreal b2)
+{
+    if (fwd) return b1 * (L[x - s] - L[x]) + b2 * (L[x - 2 * s] - L[x + s]);
+    return b1 * (L[x] - L[x + s]) + b2 * (L[x - s] - L[x + 2 * s]);
+}
+
+static inline ptrdiff_t axis_stride(const og_grid *g, int ax)
+{
+    return ax == 0 ? SI(g) : ax == 1 ? SJ(g) : 1;
+}
+
+int og_run_shot_adj(const og_grid *g, og_adj_io *io)
+{
+    const size_t fs = og_field_size(g);
+    const int N = g->nt, ntr = io->ntr, fw = g->fw;
+    const size_t npx = (size_t)g->ny * 2 * fw * g->nz, npy = (size_t)2 * fw * g->nx * g->nz,
+                 npz = (size_t)g->ny * g->nx * 2 * fw;
+    const size_t np[9] = {npx, npx, npx, npy, npy, npy, npz, npz, npz};
+    const ptrdiff_t sj = SJ(g), si = SI(g);
+    const oreal DT = (oreal)g->dt;
+    const oreal dtd[3] = {(oreal)g->dt / (oreal)g->dx, (oreal)g->dt / (oreal)g->dy, (oreal)g->dt / (oreal)g->dz};
+    const oreal rh[3] = {(oreal)1 / (oreal)g->dx, (oreal)1 / (oreal)g->dy, (oreal)1 / (oreal)g->dz};
+    oreal b1, b2;
+    int box[6], n, q, nt, i, j, k, rc = 0;
+    oreal *rho, *pi, *u, *avg[6], *prof[18], *alpha[6];
+    oreal **hf, **hps, **hpv;   /* history: [(N+1)*9] fields, psi_s, psi_v */
+    oreal *L[9], *mus[9], *muv[9], *scr[9];
+    double *G[8];               /* rip rjp rkp pi u uipjp ujpkp uipkp */
+    double *Gr, *Gp, *Gu;
+    float *sig;
+    float *syn[3] = {io->syn_x, io->syn_y, io->syn_z};
+    const float *obs[3] = {io->obs_x, io->obs_y, io->obs_z};
+    oreal *sr[3];  /* the seismograms in the oracle's own precision: J and the residuals come from them */
+    double J = 0.0;
+
+    if (g->free_surf || io->src.type != 1 || g->fdcoeff < 1) return -2;
+    fd4_coeffs(g->fdcoeff, &b1, &b2);
+    og_interior_box(g, box);
+    rho = (oreal *)calloc(fs, sizeof(oreal));
+    pi = (oreal *)calloc(fs, sizeof(oreal));
+    u = (oreal *)calloc(fs, sizeof(oreal));
+    memcpy(rho, io->rho, fs * sizeof(oreal));
+    memcpy(pi, io->pi, fs * sizeof(oreal));
+    memcpy(u, io->u, fs * sizeof(oreal));
+    og_matcopy_wrap(g, rho);
+    og_matcopy_wrap(g, pi);
+    og_matcopy_wrap(g, u);
+    for (n = 0; n < 6; n++) avg[n] = (oreal *)calloc(fs, sizeof(oreal));
+    og_av_mat(g, rho, u, NULL, avg[0], avg[1], avg[2], NULL, NULL, NULL, avg[3], avg[4], avg[5]);
+    for (n = 0; n < 18; n++) prof[n] = (oreal *)calloc((size_t)(2 * fw + 1), sizeof(oreal));
+    for (n = 0; n < 6; n++) alpha[n] = (oreal *)calloc((size_t)(2 * fw + 1), sizeof(oreal));
+    og_cpml_coeff(g, io->vppml, io->fpml, prof, alpha);
+    sig = (float *)calloc((size_t)(N + 2), sizeof(float));
+    og_wavelet(g, &io->src, 1, io->source_shape, sig);
+
+    for (q = 0; q < 3; q++) sr[q] = (oreal *)calloc((size_t)(ntr > 0 ? ntr : 1) * (size_t)(N + 1), sizeof(oreal));
+    hf = (oreal **)calloc((size_t)(N + 1) * 9, sizeof(oreal *));
+    hps = (oreal **)calloc((size_t)(N + 1) * 9, sizeof(oreal *));
+    hpv = (oreal **)calloc((size_t)(N + 1) * 9, sizeof(oreal *));
+    for (n = 0; n < 9; n++) {
+        hf[n] = (oreal *)calloc(fs, sizeof(oreal));
+        hps[n] = (oreal *)calloc(np[n], sizeof(oreal));
+        hpv[n] = (oreal *)calloc(np[n], sizeof(oreal));
+    }
+    /* forward (og_run_shot's steps without the DFT), state kept after every step */
+    for (nt = 1; nt <= N; nt++) {
+        oreal **f = hf + (size_t)nt * 9, **ps = hps + (size_t)nt * 9, **pv = hpv + (size_t)nt * 9;
+        for (n = 0; n < 9; n++) {
+            f[n] = (oreal *)malloc(fs * sizeof(oreal));
+            ps[n] = (oreal *)malloc(np[n] * sizeof(oreal));
+            pv[n] = (oreal *)malloc(np[n] * sizeof(oreal));
+            memcpy(f[n], hf[(size_t)(nt - 1) * 9 + n], fs * sizeof(oreal));
+            memcpy(ps[n], hps[(size_t)(nt - 1) * 9 + n], np[n] * sizeof(oreal));
+            memcpy(pv[n], hpv[(size_t)(nt - 1) * 9 + n], np[n] * sizeof(oreal));
+        }
+        if (isnan((double)f[1][og_idx(g, g->ny / 2, g->nx / 2, g->nz / 2)])) { rc = -1; break; }
+        og_update_v(g, box, nt, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                    avg[3], avg[4], avg[5], &io->src, sig, sig, sig, 1, 0);
+        og_update_v_cpml(g, box, nt, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                         avg[3], avg[4], avg[5], &io->src, sig, sig, sig, 1, 0, prof, ps);
+        og_update_s_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                            pi, u, avg[0], avg[1], avg[2]);
+        og_update_s_cpml_elastic(g, box, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8],
+                                 pi, u, avg[0], avg[1], avg[2], prof, pv);
+        og_psource(g, nt, f[3], f[4], f[5], &io->src, sig, 1);
+        og_seismo(g, nt, ntr, io->rec_ijk, io->syn_x, io->syn_y, io->syn_z, N, f[0], f[1], f[2]);
+        for (n = 0; n < ntr; n++) {
+            const int *p = io->rec_ijk + 3 * n;
+            const size_t c = og_idx(g, p[1], p[0], p[2]), o = (size_t)n * (size_t)(N + 1) + (size_t)nt;
+            for (q = 0; q < 3; q++) sr[q][o] = f[q][c];
+        }
+    }
+    if (rc) goto done;
+    /* misfit */
+    for (q = 0; q < 3; q++)
+        for (n = 0; n < ntr; n++)
+            for (nt = 1; nt <= N; nt++) {
+                const size_t o = (size_t)n * (size_t)(N + 1) + (size_t)nt;
+                const double r = (double)sr[q][o] - (double)obs[q][o];
+                J += 0.5 * r * r;
+            }
+    *io->J = J;
+
+    for (n = 0; n < 9; n++) {
+        L[n] = (oreal *)calloc(fs, sizeof(oreal));
+        scr[n] = (oreal *)calloc(fs, sizeof(oreal));
+        mus[n] = (oreal *)calloc(np[n], sizeof(oreal));
+        muv[n] = (oreal *)calloc(np[n], sizeof(oreal));
+    }
+    for (n = 0; n < 8; n++) G[n] = (double *)calloc(fs, sizeof(double));
+
+    for (nt = N; nt >= 1; nt--) {
+        oreal *const *f = hf + (size_t)nt * 9, *const *fp = hf + (size_t)(nt - 1) * 9;
+        oreal *const *ps = hps + (size_t)nt * 9, *const *pv = hpv + (size_t)nt * 9;
+        /* receivers: d syn(nt) / d v(nt) */
+        for (n = 0; n < ntr; n++) {
+            const int *p = io->rec_ijk + 3 * n;
+            const size_t c = og_idx(g, p[1], p[0], p[2]), o = (size_t)n * (size_t)(N + 1) + (size_t)nt;
+            for (q = 0; q < 3; q++) L[q][c] += (oreal)((double)sr[q][o] - (double)obs[q][o]);
+        }
+        /* stress update of step nt: sigma(nt) = sigma(nt-1) + M(model) e'(v(nt)) */
+#pragma omp parallel for private(i, k, q) schedule(static)
+        for (j = 1; j <= g->ny; j++)
+            for (i = 1; i <= g->nx; i++)
+                for (k = 1; k <= g->nz; k++) {
+                    const size_t c = og_idx(g, j, i, k);
+                    const oreal Lxx = L[3][c], Lyy = L[4][c], Lzz = L[5][c], Lxy = L[6][c], Lyz = L[7][c],
+                                Lxz = L[8][c];
+                    const oreal gg = pi[c], f2 = (oreal)(2.0 * u[c]), sumL = Lxx + Lyy + Lzz;
+                    oreal gq[9], e[9];
+                    vdiff d;
+                    gq[0] = DT * (gg * sumL - f2 * (Lyy + Lzz));
+                    gq[4] = DT * (gg * sumL - f2 * (Lxx + Lzz));
+                    gq[8] = DT * (gg * sumL - f2 * (Lxx + Lyy));
+                    gq[1] = gq[3] = avg[0][c] * DT * Lxy;
+                    gq[5] = gq[7] = avg[1][c] * DT * Lyz;
+                    gq[2] = gq[6] = avg[2][c] * DT * Lxz;
+                    /* the forward's CPML-corrected derivatives of v(nt) */
+                    velocity_diffs(g, c, f[0], f[1], f[2], b1, b2, &d);
+                    e[0] = d.vxx; e[1] = d.vxy; e[2] = d.vxz; e[3] = d.vyx; e[4] = d.vyy;
+                    e[5] = d.vyz; e[6] = d.vzx; e[7] = d.vzy; e[8] = d.vzz;
+                    for (q = 0; q < 9; q++) {
+                        oreal K, a, b;
+                        int arr;
+                        size_t at;
+                        if (q_cpml(g, prof, 1, q, j, i, k, &K, &a, &b, &arr, &at)) {
+                            const oreal m = muv[arr][at] + gq[q];
+                            e[q] = e[q] / K + pv[arr][at];
+                            muv[arr][at] = b * m;
+                            scr[q][c] = gq[q] / K + a * m;
+                        } else {
+                            scr[q][c] = gq[q];
+                        }
+                    }
+                    G[3][c] += (double)DT * ((double)e[0] + e[4] + e[8]) * sumL;
+                    G[4][c] += -2.0 * (double)DT * (((double)e[4] + e[8]) * Lxx + ((double)e[0] + e[8]) * Lyy +
+                                                    ((double)e[0] + e[4]) * Lzz);
+                    G[5][c] += (double)DT * ((double)e[1] + e[3]) * Lxy;
+                    G[6][c] += (double)DT * ((double)e[5] + e[7]) * Lyz;
+                    G[7][c] += (double)DT * ((double)e[2] + e[6]) * Lxz;
+                }
+#pragma omp parallel for private(i, k, q) schedule(static)
+        for (j = 1; j <= g->ny; j++)
+            for (i = 1; i <= g->nx; i++)
+                for (k = 1; k <= g->nz; k++) {
+                    const size_t c = og_idx(g, j, i, k);
+                    for (q = 0; q < 9; q++)
+                        L[SQ_FIELD[q]][c] += rh[SQ_AXIS[q]] * dT(scr[q], c, axis_stride(g, SQ_AXIS[q]), SQ_FWD[q], b1, b2);
+                }
+        /* velocity update of step nt: v(nt) = v(nt-1) + d'(sigma(nt-1)) / rho-bar */
+#pragma omp parallel for private(i, k, q) schedule(static)
+        for (j = 1; j <= g->ny; j++)
+            for (i = 1; i <= g->nx; i++)
+                for (k = 1; k <= g->nz; k++) {
+                    const size_t c = og_idx(g, j, i, k);
+                    const oreal hv[3] = {L[0][c] / avg[5][c], L[1][c] / avg[3][c], L[2][c] / avg[4][c]};
+                    oreal dd[9];
+                    sdiff d;
+                    stress_diffs(g, c, fp[3], fp[4], fp[5], fp[6], fp[7], fp[8], b1, b2, &d);
+                    dd[0] = d.sxx_x; dd[1] = d.sxy_y; dd[2] = d.sxz_z; dd[3] = d.syy_y; dd[4] = d.sxy_x;
+                    dd[5] = d.syz_z; dd[6] = d.szz_z; dd[7] = d.sxz_x; dd[8] = d.syz_y;
+                    for (q = 0; q < 9; q++) {
+                        const oreal gqq = hv[q / 3];
+                        oreal K, a, b;
+                        int arr;
+                        size_t at;
+                        if (q_cpml(g, prof, 0, q, j, i, k, &K, &a, &b, &arr, &at)) {
+                            const oreal m = mus[arr][at] + gqq;
+                            dd[q] = dd[q] / K + ps[arr][at];
+                            mus[arr][at] = b * m;
+                            scr[q][c] = gqq / K + a * m;
+                        } else {
+                            scr[q][c] = gqq;
+                        }
+                    }
+                    G[0][c] -= (double)L[0][c] * ((double)dd[0] + dd[1] + dd[2]) / ((double)avg[5][c] * avg[5][c]);
+                    G[1][c] -= (double)L[1][c] * ((double)dd[3] + dd[4] + dd[5]) / ((double)avg[3][c] * avg[3][c]);
+                    G[2][c] -= (double)L[2][c] * ((double)dd[6] + dd[7] + dd[8]) / ((double)avg[4][c] * avg[4][c]);
+                }
+#pragma omp parallel for private(i, k, q) schedule(static)
+        for (j = 1; j <= g->ny; j++)
+            for (i = 1; i <= g->nx; i++)
+                for (k = 1; k <= g->nz; k++) {
+                    const size_t c = og_idx(g, j, i, k);
+                    for (q = 0; q < 9; q++)
+                        L[VQ_FIELD[q]][c] += dtd[VQ_AXIS[q]] * dT(scr[q], c, axis_stride(g, VQ_AXIS[q]), VQ_FWD[q], b1, b2);
+                }
+    }
+
+    /* av_mat's averages -> cells (rip = (rho + rho(i+1)) / 2, ..., uipjp =
+     * 4 / sum 1/u over 4 cells), matcopy's halo copies -> their source
+     * cells, then (pi, u, rho) -> (vp, vs, rho) */
+    Gr = (double *)calloc(fs, sizeof(double));
+    Gp = (double *)calloc(fs, sizeof(double));
+    Gu = (double *)calloc(fs, sizeof(double));
+    for (j = 1; j <= g->ny; j++)
+        for (i = 1; i <= g->nx; i++)
+            for (k = 1; k <= g->nz; k++) {
+                const size_t c = og_idx(g, j, i, k);
+                const ptrdiff_t nb[3][4] = {{0, si, sj + si, sj}, {0, 1, sj + 1, sj}, {0, 1, si + 1, si}};
+                int m;
+                Gp[c] += G[3][c];
+                Gu[c] += G[4][c];
+                Gr[c] += 0.5 * (G[0][c] + G[1][c] + G[2][c]);
+                Gr[c + si] += 0.5 * G[0][c];
+                Gr[c + sj] += 0.5 * G[1][c];
+                Gr[c + 1] += 0.5 * G[2][c];
+                for (m = 0; m < 3; m++) {
+                    const double w = (double)avg[m][c] * (double)avg[m][c] * 0.25;
+                    int t;
+                    for (t = 0; t < 4; t++) {
+                        const double ux = u[c + nb[m][t]];
+                        Gu[c + nb[m][t]] += G[5 + m][c] * w / (ux * ux);
+                    }
+                }
+            }
+    for (j = 0; j <= g->ny + 1; j++)
+        for (i = 0; i <= g->nx + 1; i++)
+            for (k = 0; k <= g->nz + 1; k++) {
+                const int wj = j == 0 ? g->ny : (j == g->ny + 1 ? 1 : j);
+                const int wi = i == 0 ? g->nx : (i == g->nx + 1 ? 1 : i);
+                const int wk = k == 0 ? g->nz : (k == g->nz + 1 ? 1 : k);
+                const size_t c = og_idx(g, j, i, k), t = og_idx(g, wj, wi, wk);
+                if (c == t) continue;
+                Gr[t] += Gr[c];
+                Gp[t] += Gp[c];
+                Gu[t] += Gu[c];
+                Gr[c] = Gp[c] = Gu[c] = 0.0;
+            }
+    for (j = 1; j <= g->ny; j++)
+        for (i = 1; i <= g->nx; i++)
+            for (k = 1; k <= g->nz; k++) {
+                const size_t c = og_idx(g, j, i, k);
+                const double r = rho[c], p = pi[c], m = u[c];
+                io->grad1[c] = (oreal)((double)io->grad1[c] + 2.0 * sqrt(r * p) * Gp[c]);
+                io->grad2[c] = (oreal)((double)io->grad2[c] + 2.0 * sqrt(r * m) * Gu[c]);
+                io->grad3[c] = (oreal)((double)io->grad3[c] + Gr[c] + Gp[c] * p / r + Gu[c] * m / r);
+            }
+    free(Gr);
+    free(Gp);
+    free(Gu);
+    for (n = 0; n < 9; n++) {
+        free(L[n]);
+        free(scr[n]);
+        free(mus[n]);
+        free(muv[n]);
+    }
+    for (n = 0; n < 8; n++) free(G[n]);
+done:
+    for (q = 0; q < 3; q++) free(sr[q]);
+    for (n = 0; n < (N + 1) * 9; n++) {
+        free(hf[n]);
+        free(hps[n]);
+        free(hpv[n]);
+    }
+    free(hf);
+    free(hps);
+    free(hpv);
+    free(rho);
+    free(pi);
+    free(u);
+    for (n = 0; n < 6; n++) {
+        free(avg[n]);
+        free(alpha[n]);
+    }
+    for (n = 0; n < 18; n++) free(prof[n]);
+    free(sig);
+    return rc;
+}

@@ -266,6 +266,38 @@ int og_run_shot(const og_grid *g, og_shot_io *io);
 void og_td_gradient(const og_grid *g, const oreal *vfwd, const oreal *vadj, const oreal *rho,
                     const oreal *pi, const oreal *u, oreal *grad1, oreal *grad2, oreal *grad3);
 
+/*
+ * The consistent (exact discrete) adjoint -- no counterpart in the reference,
+ * whose adjoint is the forward scheme run with time-reversed, integrated
+ * residuals (residual.c:43-53, injected twice: update_v_ssg.c:512-522 and
+ * update_v_ssg_CPML.c:384-393) and whose gradient is the frequency-domain
+ * estimate of gradient_F.c.  Here the misfit is
+ *     J = 1/2 sum_{n=1..NT} sum_r sum_{x,y,z} (syn - obs)^2
+ * of the very forward scheme of og_run_shot (elastic, 4th order, CPML, one
+ * explosive source, no free surface), and the gradient is its exact
+ * derivative: the transpose of every step (velocity update with its CPML
+ * recursion, stress update with its CPML recursion, receiver sampling) run
+ * backwards, the model derivatives of each update summed over the steps,
+ * then carried through av_mat's averages and matcopy's halo copies to the
+ * cells and to the (vp, vs, rho) parametrisation of gradient_F.c:135-139
+ * (pi = rho vp^2, u = rho vs^2).  Returns 0, -1 on a NaN at the probe cell,
+ * -2 for a configuration it does not cover.
+ */
+typedef struct og_adj_io {
+    const oreal *rho, *pi, *u;              /* canonical; halos rebuilt by matcopy */
+    og_source src;                          /* type 1 (explosive) */
+    int source_shape;
+    int ntr;
+    const int *rec_ijk;
+    const float *obs_x, *obs_y, *obs_z;     /* [ntr][nt+1] */
+    float vppml, fpml;
+    float *syn_x, *syn_y, *syn_z;           /* out [ntr][nt+1] */
+    double *J;                              /* out */
+    oreal *grad1, *grad2, *grad3;           /* dJ/dvp, dJ/dvs, dJ/drho, canonical, accumulated */
+} og_adj_io;
+
+int og_run_shot_adj(const og_grid *g, og_adj_io *io);
+
 #ifdef __cplusplus
 }
 #endif

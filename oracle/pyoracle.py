@@ -314,6 +314,39 @@ class Oracle:
             out["vadj"] = ha
         return out
 
+    def run_shot_adj(self, g: Grid, rho, pi, u, src, rec_ijk, obs, vppml=4000.0, fpml=50.0, shape=1,
+                     grads=None):
+        """og_run_shot_adj: the consistent (exact discrete) adjoint of the
+        elastic forward scheme.  obs: 3 arrays (ntr, nt+1).  Returns dict(syn,
+        J, grads) with J = 1/2 sum (syn - obs)^2 and grads = dJ/d(vp, vs, rho)
+        on the canonical grid (added into `grads` when given)."""
+        class AdjIo(C.Structure):
+            _fields_ = [("rho", C.c_void_p), ("pi", C.c_void_p), ("u", C.c_void_p), ("src", OgSource),
+                        ("source_shape", C.c_int), ("ntr", C.c_int), ("rec_ijk", C.c_void_p),
+                        ("obs_x", C.c_void_p), ("obs_y", C.c_void_p), ("obs_z", C.c_void_p),
+                        ("vppml", C.c_float), ("fpml", C.c_float),
+                        ("syn_x", C.c_void_p), ("syn_y", C.c_void_p), ("syn_z", C.c_void_p),
+                        ("J", C.c_void_p), ("grad1", C.c_void_p), ("grad2", C.c_void_p), ("grad3", C.c_void_p)]
+        rec = np.ascontiguousarray(rec_ijk, np.int32)
+        ntr = rec.shape[0]
+        o = [np.ascontiguousarray(a, np.float32) for a in obs]
+        syn = [np.zeros((ntr, g.nt + 1), np.float32) for _ in range(3)]
+        if grads is None:
+            grads = [self.zeros(g) for _ in range(3)]
+        m = [np.ascontiguousarray(a, self.dtype) for a in (rho, pi, u)]
+        J = np.zeros(1, np.float64)
+        io = AdjIo(_p(m[0]).value, _p(m[1]).value, _p(m[2]).value, OgSource(*src), shape, ntr, rec.ctypes.data,
+                   o[0].ctypes.data, o[1].ctypes.data, o[2].ctypes.data, vppml, fpml,
+                   syn[0].ctypes.data, syn[1].ctypes.data, syn[2].ctypes.data, J.ctypes.data,
+                   grads[0].ctypes.data, grads[1].ctypes.data, grads[2].ctypes.data)
+        og = g.og()
+        rc = self.lib.og_run_shot_adj(C.byref(og), C.byref(io))
+        if rc == -1:
+            raise FloatingPointError("oracle: simulation unstable (NaN probe)")
+        if rc:
+            raise ValueError("og_run_shot_adj: elastic, no free surface and an explosive source only")
+        return dict(syn=syn, J=float(J[0]), grads=grads)
+
     def td_gradient(self, g: Grid, vfwd, vadj, rho, pi, u, grads):
         """og_td_gradient: the time-domain gradient from velocity histories
         [nt][3] (run_shot(..., want_history=True)), added to grads"""
