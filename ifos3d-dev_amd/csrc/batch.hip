@@ -312,9 +312,10 @@ struct ifos_batch {
     int ntr;                   // receivers of the current batch
     bool any_slab_src, any_slab_rec;
     MarchCfg cfg_v, cfg_s;     // tile shapes of the two march launches
-    // viscoelastic (L = 1): memory variables [S][fe] x 6, shared ratios
-    int L;
-    float eta;
+    // viscoelastic (L mechanisms): memory variables [S][L][fe] x 6, shared
+    // relaxation ratios and their averages
+    int L, L_alloc;
+    float eta[IFOS_MAX_L];
     float *r[6], *taus, *taup, *tavg[3];
     hipStream_t st;
     std::vector<void *> allocs;
@@ -371,7 +372,7 @@ extern "C" size_t ifos_batch_bytes(const ifos_geom *g, int nshots, int nf_max, i
                                    int chunk)
 {
     const size_t fe = ifos_field_elems(g);
-    size_t psi = 0;  // (a viscoelastic batch adds 6 * nshots + 5 fields)
+    size_t psi = 0;  // (a viscoelastic batch adds 6 * L * nshots + 5 fields)
     for (int n = 0; n < 9; ++n) psi += ifos_psi_elems(g, n);
     size_t per_shot = 18 * fe + 3 * psi + 12 * (size_t)nf_max * fe * (dft_precision == IFOS_DFT_KAHAN ? 2 : 1) +
                       (size_t)chunk * 3 * fe + (size_t)(g->nt + 2) * (1 + 6 * (size_t)ntr_max);
@@ -453,7 +454,8 @@ extern "C" int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshot
     b->ver = 0;
     b->lv_exec = nullptr;
     b->L = 0;
-    b->eta = 0.0f;
+    b->L_alloc = 0;
+    for (int l = 0; l < IFOS_MAX_L; ++l) b->eta[l] = 0.0f;
     for (int n = 0; n < 6; ++n) b->r[n] = nullptr;
     b->taus = b->taup = nullptr;
     for (int n = 0; n < 3; ++n) b->tavg[n] = nullptr;
@@ -476,25 +478,33 @@ extern "C" int ifos_batch_set_model(ifos_batch *b, const float *d_rho, const flo
     return rc;
 }
 
-// the viscoelastic update with one relaxation mechanism (update_s_ssg.c)
-// for every shot of the batch; L = 0 switches back to the elastic one
-extern "C" int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const float *d_taup, float eta, int L)
+// the viscoelastic update (update_s_ssg.c) with L relaxation mechanisms
+// for every shot of the batch (the reference has L = 1); L = 0 switches back
+// to the elastic one
+extern "C" int ifos_batch_set_visco_n(ifos_batch *b, const float *d_taus, const float *d_taup, const float *eta,
+                                      int L)
 {
-    if (!b || L < 0 || L > 1 || (L == 1 && (!d_taus || !d_taup))) {
-        set_error("ifos_batch_set_visco: L must be 0 or 1 (with taus and taup)");
+    if (!b || L < 0 || L > IFOS_MAX_L || (L >= 1 && (!d_taus || !d_taup || !eta)) ||
+        (double)L * (double)b->fe * 4.0 >= 4294967296.0) {
+        set_error("ifos_batch_set_visco_n: L must be 0..%d (with taus, taup and L etas; L fields under 4 GiB)",
+                  IFOS_MAX_L);
         return IFOS_EINVAL;
     }
     drop_graphs(b);
     b->L = L;
-    b->eta = eta;
+    for (int l = 0; l < IFOS_MAX_L; ++l) b->eta[l] = l < L ? eta[l] : 0.0f;
     if (!L) return IFOS_OK;
     int rc = 0;
     if (!b->taus) {
-        for (int n = 0; n < 6 && !rc; ++n) rc = balloc(b, (void **)&b->r[n], (size_t)b->S * b->fe * 4);
         if (!rc) rc = balloc(b, (void **)&b->taus, b->fe * 4);
         if (!rc) rc = balloc(b, (void **)&b->taup, b->fe * 4);
         for (int n = 0; n < 3 && !rc; ++n) rc = balloc(b, (void **)&b->tavg[n], b->fe * 4);
         if (rc) return rc;
+    }
+    if (b->L_alloc < L) {  // [S][L][fe] per component (the old arrays stay with the context)
+        for (int n = 0; n < 6 && !rc; ++n) rc = balloc(b, (void **)&b->r[n], (size_t)b->S * L * b->fe * 4);
+        if (rc) return rc;
+        b->L_alloc = L;
     }
     rc = hip_check(hipMemcpyAsync(b->taus, d_taus, b->fe * 4, hipMemcpyDeviceToDevice, b->st), "copy taus");
     if (!rc) rc = hip_check(hipMemcpyAsync(b->taup, d_taup, b->fe * 4, hipMemcpyDeviceToDevice, b->st), "copy taup");
@@ -503,6 +513,16 @@ extern "C" int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const fl
     if (!rc) rc = ifos_av_tau(&b->g, b->taus, b->tavg[0], b->tavg[1], b->tavg[2], b->st);
     if (!rc) rc = hip_check(hipStreamSynchronize(b->st), "set_visco");
     return rc;
+}
+
+// one relaxation mechanism, as update_s_ssg.c (L = 0 or 1)
+extern "C" int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const float *d_taup, float eta, int L)
+{
+    if (!b || L < 0 || L > 1) {
+        set_error("ifos_batch_set_visco: L must be 0 or 1 (ifos_batch_set_visco_n takes more mechanisms)");
+        return IFOS_EINVAL;
+    }
+    return ifos_batch_set_visco_n(b, d_taus, d_taup, &eta, L);
 }
 
 extern "C" int ifos_batch_set_cpml(ifos_batch *b, float vppml, float fpml)
@@ -658,8 +678,9 @@ static int bpass(ifos_batch *b, int back, int ns, int dft, int seis, hipStream_t
         V.tausipjp = b->tavg[0];
         V.tausjpkp = b->tavg[1];
         V.tausipkp = b->tavg[2];
-        V.eta = b->eta;
+        for (int l = 0; l < IFOS_MAX_L; ++l) V.eta[l] = b->eta[l];
         V.L = b->L;
+        V.lstride = (int64_t)b->fe;
     }
     int nsnap = 0, q0 = 0;
     for (int nt = 1; nt <= g->nt; ++nt) {
@@ -830,7 +851,7 @@ static int zero_state(ifos_batch *b, int ns, hipStream_t st)
     for (int n = 0; n < 9 && !rc; ++n) rc = fill(b->psi_v_alt[n], (size_t)ns * b->pe[n / 3], st);
     b->cur_set = 0;
     if (b->L)  // zero_wavefield.c:80-95
-        for (int n = 0; n < 6 && !rc; ++n) rc = fill(b->r[n], (size_t)ns * b->fe, st);
+        for (int n = 0; n < 6 && !rc; ++n) rc = fill(b->r[n], (size_t)ns * b->L * b->fe, st);
     return rc;
 }
 
@@ -1187,8 +1208,9 @@ extern "C" int ifos_batch_autotune(ifos_batch *b, int ns, int reps, int *best_v,
         V.tausipjp = b->tavg[0];
         V.tausjpkp = b->tavg[1];
         V.tausipkp = b->tavg[2];
-        V.eta = b->eta;
+        for (int l = 0; l < IFOS_MAX_L; ++l) V.eta[l] = b->eta[l];
         V.L = b->L;
+        V.lstride = (int64_t)b->fe;
     }
     hipEvent_t e0, e1;
     int rc = hip_check(hipEventCreate(&e0), "event");

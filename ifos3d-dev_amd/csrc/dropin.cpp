@@ -422,7 +422,7 @@ ifos_visco visco_args(float ***rxx, float ***ryy, float ***rzz, float ***rxy, fl
                       float *eta)
 {
     if (L != 1) fatal("update_s / update_s_CPML need L = 1 (L = %d)", L);
-    ifos_visco V;
+    ifos_visco V = {};
     float ***r[6] = {rxx, ryy, rzz, rxy, ryz, rxz};
     for (int n = 0; n < 6; ++n) V.r[n] = m3(r[n], AVG);
     V.taus = m3(taus, MODEL);

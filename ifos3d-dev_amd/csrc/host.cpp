@@ -355,9 +355,10 @@ struct ifos_ctx {
     float *adj;            // [3][ntr][nt+2]
     float finv[IFOS_MAX_NF];
     int nf, tast;
-    // viscoelastic (L = 1): memory variables, relaxation ratios, averages
-    int L;
-    float eta;
+    // viscoelastic (L mechanisms): memory variables (L fields per component),
+    // relaxation ratios, their averages
+    int L, L_alloc;
+    float eta[IFOS_MAX_L];
     float *r[6], *taus, *taup, *tavg[3];
     std::vector<void *> allocs;
     struct TdBufs *td;    // buffers of the time-domain gradient (ifos_ctx_run_shot_td)
@@ -427,7 +428,8 @@ extern "C" int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max, i
     c->nf = 0;
     c->tast = 100;
     c->L = 0;
-    c->eta = 0.0f;
+    c->L_alloc = 0;
+    for (int l = 0; l < IFOS_MAX_L; ++l) c->eta[l] = 0.0f;
     for (int n = 0; n < 6; ++n) c->r[n] = nullptr;
     c->taus = c->taup = nullptr;
     for (int n = 0; n < 3; ++n) c->tavg[n] = nullptr;
@@ -435,26 +437,34 @@ extern "C" int ifos_ctx_create(ifos_ctx **out, const ifos_geom *g, int nf_max, i
     return IFOS_OK;
 }
 
-// the viscoelastic update with one relaxation mechanism (update_s_ssg.c)
-// instead of the elastic one; L = 0 switches back.  taus/taup in device
+// the viscoelastic update (update_s_ssg.c) instead of the elastic one with
+// L relaxation mechanisms, eta[l] = DT / relaxation time of mechanism l
+// (the reference has L = 1); L = 0 switches back.  taus/taup in device
 // layout (cell values), copied and halo-wrapped here.
-extern "C" int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float *d_taup, float eta, int L,
-                                  void *stream)
+extern "C" int ifos_ctx_set_visco_n(ifos_ctx *c, const float *d_taus, const float *d_taup, const float *eta,
+                                    int L, void *stream)
 {
-    if (!c || L < 0 || L > 1 || (L == 1 && (!d_taus || !d_taup))) {
-        set_error("ifos_ctx_set_visco: L must be 0 or 1 (with taus and taup)");
+    if (!c || L < 0 || L > IFOS_MAX_L || (L >= 1 && (!d_taus || !d_taup || !eta)) ||
+        (double)L * (double)c->fe * 4.0 >= 4294967296.0) {
+        set_error("ifos_ctx_set_visco_n: L must be 0..%d (with taus, taup and L etas; L fields under 4 GiB)",
+                  IFOS_MAX_L);
         return IFOS_EINVAL;
     }
     c->L = L;
-    c->eta = eta;
+    for (int l = 0; l < IFOS_MAX_L; ++l) c->eta[l] = l < L ? eta[l] : 0.0f;
     if (!L) return IFOS_OK;
     int rc = 0;
     if (!c->taus) {
-        for (int n = 0; n < 6 && !rc; ++n) rc = dalloc(c, (void **)&c->r[n], c->fe * 4);
         if (!rc) rc = dalloc(c, (void **)&c->taus, c->fe * 4);
         if (!rc) rc = dalloc(c, (void **)&c->taup, c->fe * 4);
         for (int n = 0; n < 3 && !rc; ++n) rc = dalloc(c, (void **)&c->tavg[n], c->fe * 4);
         if (rc) return rc;
+    }
+    if (c->L_alloc < L) {  // memory variables: L fields per component (the old ones stay with the context)
+        for (int n = 0; n < 6 && !rc; ++n) rc = dalloc(c, (void **)&c->r[n], c->fe * 4 * (size_t)L);
+        if (rc) return rc;
+        c->L_alloc = L;
+        td_free(c);  // its checkpoint states hold memory variables of the old count
     }
     hipStream_t s = (hipStream_t)stream;
     rc = hip_check(hipMemcpyAsync(c->taus, d_taus, c->fe * 4, hipMemcpyDeviceToDevice, s), "copy taus");
@@ -463,6 +473,17 @@ extern "C" int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float 
     if (!rc) rc = ifos_matcopy_wrap(&c->g, c->taup, stream);
     if (!rc) rc = ifos_av_tau(&c->g, c->taus, c->tavg[0], c->tavg[1], c->tavg[2], stream);
     return rc;
+}
+
+// one relaxation mechanism, as update_s_ssg.c (L = 0 or 1)
+extern "C" int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float *d_taup, float eta, int L,
+                                  void *stream)
+{
+    if (!c || L < 0 || L > 1) {
+        set_error("ifos_ctx_set_visco: L must be 0 or 1 (ifos_ctx_set_visco_n takes more mechanisms)");
+        return IFOS_EINVAL;
+    }
+    return ifos_ctx_set_visco_n(c, d_taus, d_taup, &eta, L, stream);
 }
 
 static ifos_model ctx_model(const ifos_ctx *c)
@@ -577,7 +598,8 @@ static int step(ifos_ctx *c, const ShotState &S, int back, const ifos_shot *s, i
         V.tausipjp = c->tavg[0];
         V.tausjpkp = c->tavg[1];
         V.tausipkp = c->tavg[2];
-        V.eta = c->eta;
+        V.eta = c->eta[0];
+        for (int l = 1; l < IFOS_MAX_L; ++l) V.eta_more[l - 1] = c->eta[l];
         V.L = c->L;
         if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_INTERIOR, &f, &m, &V, c->prof, &pv, st);
         if (!rc) rc = ifos_update_s_visco(g, IFOS_REGION_CPML, &f, &m, &V, c->prof, &pv, st);
@@ -636,7 +658,7 @@ static int zero_wavefield(ifos_ctx *c, hipStream_t st)
     for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->psi_s[n], ifos_psi_elems(&c->g, n), 0.0f, st);
     for (int n = 0; n < 9 && !rc; ++n) rc = ifos_fill(c->psi_v[n], ifos_psi_elems(&c->g, n), 0.0f, st);
     if (c->L)  // zero_wavefield.c:80-95
-        for (int n = 0; n < 6 && !rc; ++n) rc = ifos_fill(c->r[n], c->fe, 0.0f, st);
+        for (int n = 0; n < 6 && !rc; ++n) rc = ifos_fill(c->r[n], c->fe * (size_t)c->L, 0.0f, st);
     return rc;
 }
 
@@ -731,7 +753,7 @@ extern "C" int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn, 
 // the segments from the last to the first, recomputing each segment's
 // forward velocities from its checkpoint (one extra forward propagation).
 struct TdBufs {
-    int K = 0, nck = 0;
+    int K = 0, nck = 0, L = 0;  // L: memory-variable mechanisms of the states
     std::vector<ShotState> ck;  // state after steps 0, K, 2K, ...
     ShotState work;             // the recomputed forward state
     std::vector<float *> seg;   // [K][3] forward velocities of one segment
@@ -767,18 +789,19 @@ static int td_alloc_state(ifos_ctx *c, TdBufs *T, ShotState &S)
     for (int n = 0; n < 9 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.psi_v[n], ifos_psi_elems(&c->g, n) * 4);
     for (int n = 0; n < 6; ++n) S.r[n] = nullptr;
     if (c->L)
-        for (int n = 0; n < 6 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.r[n], c->fe * 4);
+        for (int n = 0; n < 6 && !rc; ++n) rc = td_alloc_one(T, (void **)&S.r[n], c->fe * 4 * (size_t)c->L);
     return rc;
 }
 
 static int td_alloc(ifos_ctx *c, int K)
 {
     const int nck = (c->g.nt + K - 1) / K;
-    if (c->td && c->td->K == K && (c->td->work.r[0] != nullptr) == (c->L != 0)) return IFOS_OK;
+    if (c->td && c->td->K == K && c->td->L == c->L) return IFOS_OK;
     td_free(c);
     TdBufs *T = new TdBufs();
     c->td = T;
     T->K = K;
+    T->L = c->L;
     T->nck = nck;
     T->ck.resize(nck);
     int rc = IFOS_OK;
@@ -808,7 +831,7 @@ static int copy_state(ifos_ctx *c, const ShotState &dst, const ShotState &src, h
     }
     if (c->L)
         for (int n = 0; n < 6 && !rc; ++n)
-            rc = hip_check(hipMemcpyAsync(dst.r[n], src.r[n], c->fe * 4, hipMemcpyDeviceToDevice, st),
+            rc = hip_check(hipMemcpyAsync(dst.r[n], src.r[n], c->fe * 4 * (size_t)c->L, hipMemcpyDeviceToDevice, st),
                            "checkpoint copy");
     return rc;
 }
@@ -819,7 +842,7 @@ extern "C" size_t ifos_td_bytes(const ifos_geom *g, int every, int L)
     const size_t nck = (size_t)(g->nt + K - 1) / K, fe = ifos_field_elems(g);
     size_t psi = 0;
     for (int n = 0; n < 9; ++n) psi += ifos_psi_elems(g, n);
-    const size_t state = (9 + (L ? 6 : 0)) * fe * 4 + 2 * psi * 4;
+    const size_t state = (9 + 6 * (size_t)L) * fe * 4 + 2 * psi * 4;
     return (nck + 1) * state + 3 * (size_t)K * fe * 4 + 6 * fe * 4 + 3 * fe * 8;
 }
 

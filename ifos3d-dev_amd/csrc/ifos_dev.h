@@ -117,11 +117,15 @@ struct Snap {
 
 // viscoelastic arrays of the stress march: memory variables r (per shot,
 // wavefield layout), relaxation-time ratios (shared model), eta, L
+// with L mechanisms each component's memory variables are L fields back to
+// back (mechanism l at r[n] + l * lstride elements; a batch's shot s at
+// r[n] + s * L * fstride); eta[l] = DT / relaxation time of mechanism l
 struct ViscoF {
     float *r[6];
     const float *taus, *taup, *tausipjp, *tausjpkp, *tausipkp;
-    float eta;
+    float eta[IFOS_MAX_L];
     int L;
+    int64_t lstride;
 };
 
 // Index box (inclusive), and a list of them for the slab launches.

@@ -164,6 +164,76 @@ __device__ inline void stress_visco(float DT, float eta, int L, float pi, float 
     for (int n = 0; n < 6; ++n) s[n] = s[n] + (dthalbe * r[n]);
 }
 
+// the memory-variable recursion of one mechanism (update_s_ssg.c:211-230)
+__device__ inline void visco_mech(float eta, float pi, float u, float uipjp, float ujpkp, float uipkp, float ts,
+                                  float tp, float tsipjp, float tsjpkp, float tsipkp, float vxyyx, float vyzzy,
+                                  float vxzzx, float vxxyyzz, float vyyzz, float vxxzz, float vxxyy, float r[6])
+{
+    const float b = (float)(1.0 / (1.0 + ((double)eta * 0.5)));
+    const float cc = (float)(1.0 - ((double)eta * 0.5));
+    const float dipjp = uipjp * eta * tsipjp, djpkp = ujpkp * eta * tsjpkp, dipkp = uipkp * eta * tsipkp;
+    const float dd = (float)(((2.0 * (double)u) * (double)eta) * (double)ts);
+    const float e = pi * eta * tp;
+    r[3] = b * ((r[3] * cc) - (dipjp * vxyyx));
+    r[4] = b * ((r[4] * cc) - (djpkp * vyzzy));
+    r[5] = b * ((r[5] * cc) - (dipkp * vxzzx));
+    r[0] = b * (((r[0] * cc) - (e * vxxyyzz)) + (dd * vyyzz));
+    r[1] = b * (((r[1] * cc) - (e * vxxyyzz)) + (dd * vxxzz));
+    r[2] = b * (((r[2] * cc) - (e * vxxyyzz)) + (dd * vxxyy));
+}
+
+// the viscoelastic stress step with V.L >= 1 mechanisms: where the one
+// mechanism of update_s_ssg.c:174-238 enters, the L memory variables of the
+// component are summed in mechanism order (the tau-method's SLS sum), before
+// and after each mechanism is updated with its own eta.  r0 holds mechanism
+// 0 (in registers, updated in place); ld(l, n) / st(l, n, v) read and write
+// component n of mechanism l >= 1 at the cell.
+template <class LDF, class STF>
+__device__ inline void stress_visco_n(float DT, const ViscoF &V, float pi, float u, float uipjp, float ujpkp,
+                                      float uipkp, float ts, float tp, float tsipjp, float tsjpkp, float tsipkp,
+                                      float vxyyx, float vyzzy, float vxzzx, float vxxyyzz, float vyyzz,
+                                      float vxxzz, float vxxyy, float s[6], float r0[6], LDF ld, STF st)
+{
+    const int L = V.L;
+    const float Lf = (float)L;
+    const float dthalbe = (float)((double)DT / 2.0);
+    const float fipjp = (float)((double)(uipjp * DT) * (1.0 + (double)(Lf * tsipjp)));
+    const float fjpkp = (float)((double)(ujpkp * DT) * (1.0 + (double)(Lf * tsjpkp)));
+    const float fipkp = (float)((double)(uipkp * DT) * (1.0 + (double)(Lf * tsipkp)));
+    const float gg = (float)((double)pi * (1.0 + (double)(Lf * tp)));
+    const float f = (float)((2.0 * (double)u) * (1.0 + (double)(Lf * ts)));
+    float sr[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) sr[n] = r0[n];
+    for (int l = 1; l < L; ++l)
+#pragma unroll
+        for (int n = 0; n < 6; ++n) sr[n] = sr[n] + ld(l, n);
+    s[3] = s[3] + ((fipjp * vxyyx) + (dthalbe * sr[3]));
+    s[4] = s[4] + ((fjpkp * vyzzy) + (dthalbe * sr[4]));
+    s[5] = s[5] + ((fipkp * vxzzx) + (dthalbe * sr[5]));
+    s[0] = s[0] + (DT * ((gg * vxxyyzz) - (f * vyyzz)) + (dthalbe * sr[0]));
+    s[1] = s[1] + (DT * ((gg * vxxyyzz) - (f * vxxzz)) + (dthalbe * sr[1]));
+    s[2] = s[2] + (DT * ((gg * vxxyyzz) - (f * vxxyy)) + (dthalbe * sr[2]));
+    visco_mech(V.eta[0], pi, u, uipjp, ujpkp, uipkp, ts, tp, tsipjp, tsjpkp, tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz,
+               vyyzz, vxxzz, vxxyy, r0);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) sr[n] = r0[n];
+    for (int l = 1; l < L; ++l) {
+        float rl[6];
+#pragma unroll
+        for (int n = 0; n < 6; ++n) rl[n] = ld(l, n);
+        visco_mech(V.eta[l], pi, u, uipjp, ujpkp, uipkp, ts, tp, tsipjp, tsjpkp, tsipkp, vxyyx, vyzzy, vxzzx,
+                   vxxyyzz, vyyzz, vxxzz, vxxyy, rl);
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+            st(l, n, rl[n]);
+            sr[n] = sr[n] + rl[n];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 6; ++n) s[n] = s[n] + (dthalbe * sr[n]);
+}
+
 // ---------------------------------------------------------------------------
 // Tiles.  A launch covers up to six boxes (the interior box, the CPML slabs,
 // or all of [1..N]^3), each cut into tiles of kw k-columns x (TI * rpw)
@@ -578,14 +648,16 @@ struct SPlane {
         P.sxx0 = LDN(rsxx, D, 0, 0); P.syy0 = LDN(rsyy, D, 0, 0); P.szz0 = LDN(rszz, D, 0, 0);                   \
         P.sxy0 = LDN(rsxy, D, 0, 0); P.syz0 = LDN(rsyz, D, 0, 0); P.sxz0 = LDN(rsxz, D, 0, 0);                   \
         P.vx_n = LD(rvx, (D) + 2, 0, 0); P.vz_n = LD(rvz, (D) + 2, 0, 0); P.vy_n = LD(rvy, (D) + 1, 0, 0);       \
-        if (VISCO) {                                                                                           \
+        if (VM) {                                                                                              \
             for (int q_ = 0; q_ < 6; ++q_) P.r0[q_] = LDN(rr[q_], D, 0, 0);                                    \
             P.ts = LD(rtaus, D, 0, 0); P.tp = LD(rtaup, D, 0, 0);                                              \
             P.tsipjp = LD(rtsipjp, D, 0, 0); P.tsjpkp = LD(rtsjpkp, D, 0, 0); P.tsipkp = LD(rtsipkp, D, 0, 0);  \
         }                                                                                                      \
     } while (0)
 
-template <int TI, bool CPML, bool VISCO, bool PIPE>
+// VM: 0 elastic, 1 one relaxation mechanism (the reference's update_s),
+// 2 V.L >= 1 mechanisms (stress_visco_n)
+template <int TI, bool CPML, int VM, bool PIPE>
 __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S,
                                                     ViscoF V)
 {
@@ -608,9 +680,10 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
                  ruipkp = make_rsrc(F.uipkp, g.fbytes);
     if (CPML) shift_psi(psi, S, shot);
     rsrc_t rr[6], rtaus, rtaup, rtsipjp, rtsjpkp, rtsipkp;
-    if (VISCO) {
+    if (VM) {
+        const int nm = VM == 1 ? 1 : V.L;
 #pragma unroll
-        for (int n = 0; n < 6; ++n) rr[n] = make_rsrc(V.r[n] + so, g.fbytes);
+        for (int n = 0; n < 6; ++n) rr[n] = make_rsrc(V.r[n] + so * nm, g.fbytes * (uint32_t)nm);
         rtaus = make_rsrc(V.taus, g.fbytes);
         rtaup = make_rsrc(V.taup, g.fbytes);
         rtsipjp = make_rsrc(V.tausipjp, g.fbytes);
@@ -655,7 +728,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
         const float vxxyyzz = vxx + vyy + vzz;
         const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
         float sn[6], rn[6];
-        if (!VISCO) {
+        if (!VM) {
             // update_s_ssg_elastic.c:117-138
             const float fipjp = cur.uipjp * DT, fjpkp = cur.ujpkp * DT, fipkp = cur.uipkp * DT;
             const float f = 2.0f * cur.u;
@@ -670,9 +743,20 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
             sn[3] = cur.sxy0; sn[4] = cur.syz0; sn[5] = cur.sxz0;
 #pragma unroll
             for (int n = 0; n < 6; ++n) rn[n] = cur.r0[n];
-            stress_visco(DT, V.eta, V.L, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
-                         cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn,
-                         rn);
+            if (VM == 1) {
+                stress_visco(DT, V.eta[0], V.L, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
+                             cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy,
+                             sn, rn);
+            } else {
+                const uint32_t own = 2 * SJ + 2 * SI;
+                auto ldm = [&](int l, int n) { return bload_cp<CPN>(rr[n], vo + 8, own + (uint32_t)l * g.fbytes); };
+                auto stm = [&](int l, int n, float v) {
+                    bstore_cp<CPN>(v, rr[n], vo + 8, own + (uint32_t)l * g.fbytes);
+                };
+                stress_visco_n(DT, V, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp, cur.tsipjp,
+                               cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy, sn, rn,
+                               ldm, stm);
+            }
         }
         vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = cur.vx_n;
         vz_m1 = vz_0; vz_0 = vz_p1; vz_p1 = cur.vz_n;
@@ -684,7 +768,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
         STN(sn[0], rsxx);
         STN(sn[1], rsyy);
         STN(sn[2], rszz);
-        if (VISCO) {
+        if (VM) {
 #pragma unroll
             for (int n = 0; n < 6; ++n) STN(rn[n], rr[n]);
         }
@@ -713,7 +797,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 // update and store.  The barrier waits for LDS only (s_waitcnt lgkmcnt(0)):
 // the loads in flight for the next plane stay in flight across it.  Same
 // arithmetic, in the same order, as k_s_march.
-template <int TI, bool CPML, bool VISCO>
+template <int TI, bool CPML, int VM>
 __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields F, Prof P, Psi9 psi, Shots S,
                                                   ViscoF V)
 {
@@ -756,9 +840,10 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
                  ruipkp = make_rsrc(F.uipkp, g.fbytes);
     if (CPML) shift_psi(psi, S, shot);
     rsrc_t rr[6], rtaus, rtaup, rtsipjp, rtsjpkp, rtsipkp;
-    if (VISCO) {
+    if (VM) {
+        const int nm = VM == 1 ? 1 : V.L;
 #pragma unroll
-        for (int n = 0; n < 6; ++n) rr[n] = make_rsrc(V.r[n] + so, g.fbytes);
+        for (int n = 0; n < 6; ++n) rr[n] = make_rsrc(V.r[n] + so * nm, g.fbytes * (uint32_t)nm);
         rtaus = make_rsrc(V.taus, g.fbytes);
         rtaup = make_rsrc(V.taup, g.fbytes);
         rtsipjp = make_rsrc(V.tausipjp, g.fbytes);
@@ -810,7 +895,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
         O.s[3] = LBN(rsxy, D); O.s[4] = LBN(rsyz, D); O.s[5] = LBN(rsxz, D);
         O.uipjp = LB(ruipjp, D); O.ujpkp = LB(rujpkp, D); O.uipkp = LB(ruipkp, D);
         O.pi = LB(rpi, D); O.u = LB(ru, D);
-        if (VISCO) {
+        if (VM) {
 #pragma unroll
             for (int n = 0; n < 6; ++n) O.r[n] = LBN(rr[n], D);
             O.ts = LB(rtaus, D); O.tp = LB(rtaup, D);
@@ -876,7 +961,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
             const float vxxyyzz = vxx + vyy + vzz;
             const float vyyzz = vyy + vzz, vxxzz = vxx + vzz, vxxyy = vxx + vyy;
             float sn[6], rn[6];
-            if (!VISCO) {
+            if (!VM) {
                 // update_s_ssg_elastic.c:117-138
                 const float fipjp = cur.uipjp * DT, fjpkp = cur.ujpkp * DT, fipkp = cur.uipkp * DT;
                 const float f = 2.0f * cur.u;
@@ -892,9 +977,22 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
                     sn[n] = cur.s[n];
                     rn[n] = cur.r[n];
                 }
-                stress_visco(DT, V.eta, V.L, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
-                             cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz, vxxyy,
-                             sn, rn);
+                if (VM == 1) {
+                    stress_visco(DT, V.eta[0], V.L, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
+                                 cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz,
+                                 vxxyy, sn, rn);
+                } else {
+                    const uint32_t sto = 2u * SJ + own;
+                    auto ldm = [&](int l, int n) {
+                        return bload_cp<CPN>(rr[n], vb + 8u, sto + (uint32_t)l * g.fbytes);
+                    };
+                    auto stm = [&](int l, int n, float v) {
+                        bstore_cp<CPN>(v, rr[n], vb + 8u, sto + (uint32_t)l * g.fbytes);
+                    };
+                    stress_visco_n(DT, V, cur.pi, cur.u, cur.uipjp, cur.ujpkp, cur.uipkp, cur.ts, cur.tp,
+                                   cur.tsipjp, cur.tsjpkp, cur.tsipkp, vxyyx, vyzzy, vxzzx, vxxyyzz, vyyzz, vxxzz,
+                                   vxxyy, sn, rn, ldm, stm);
+                }
             }
             const uint32_t st = 2u * SJ + own;
             bstore_cp<CPN>(sn[3], rsxy, vb + 8u, st);
@@ -903,7 +1001,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
             bstore_cp<CPN>(sn[0], rsxx, vb + 8u, st);
             bstore_cp<CPN>(sn[1], rsyy, vb + 8u, st);
             bstore_cp<CPN>(sn[2], rszz, vb + 8u, st);
-            if (VISCO) {
+            if (VM) {
 #pragma unroll
                 for (int n = 0; n < 6; ++n) bstore_cp<CPN>(rn[n], rr[n], vb + 8u, st);
             }
@@ -1115,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_v_slabs(DevGeom g, BoxList L, VFields F
     F.vz[c] += (szz_z + sxz_x + syz_y) / F.rkp[c];
 }
 
-template <bool VISCO>
+template <int VM>
 __global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F, Prof P, Psi9 psi, ViscoF V)
 {
     int j, i, k;
@@ -1138,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F
     const int hy = slab_h(j, g.ny, g.fw, !g.free_surf);
     const int hz = slab_h(k, g.nz, g.fw, 1);
     s_cpml(g, P, psi, j, i, k, hx, hy, hz, vxx, vyx, vzx, vxy, vyy, vzy, vxz, vyz, vzz);
-    if (!VISCO) {
+    if (!VM) {
         s_apply(g, F, c, vxx, vxy, vxz, vyx, vyy, vyz, vzx, vzy, vzz);
         return;
     }
@@ -1146,9 +1244,17 @@ __global__ __launch_bounds__(256) void k_s_slabs(DevGeom g, BoxList L, SFields F
     float rn[6];
 #pragma unroll
     for (int n = 0; n < 6; ++n) rn[n] = V.r[n][c];
-    stress_visco(g.dt, V.eta, V.L, F.pi[c], F.u[c], F.uipjp[c], F.ujpkp[c], F.uipkp[c], V.taus[c], V.taup[c],
-                 V.tausipjp[c], V.tausjpkp[c], V.tausipkp[c], vxy + vyx, vyz + vzy, vxz + vzx, vxx + vyy + vzz,
-                 vyy + vzz, vxx + vzz, vxx + vyy, sn, rn);
+    if (VM == 1) {
+        stress_visco(g.dt, V.eta[0], V.L, F.pi[c], F.u[c], F.uipjp[c], F.ujpkp[c], F.uipkp[c], V.taus[c],
+                     V.taup[c], V.tausipjp[c], V.tausjpkp[c], V.tausipkp[c], vxy + vyx, vyz + vzy, vxz + vzx,
+                     vxx + vyy + vzz, vyy + vzz, vxx + vzz, vxx + vyy, sn, rn);
+    } else {
+        auto ldm = [&](int l, int n) { return V.r[n][c + l * V.lstride]; };
+        auto stm = [&](int l, int n, float v) { V.r[n][c + l * V.lstride] = v; };
+        stress_visco_n(g.dt, V, F.pi[c], F.u[c], F.uipjp[c], F.ujpkp[c], F.uipkp[c], V.taus[c], V.taup[c],
+                       V.tausipjp[c], V.tausjpkp[c], V.tausipkp[c], vxy + vyx, vyz + vzy, vxz + vzx, vxx + vyy + vzz,
+                       vyy + vzz, vxx + vzz, vxx + vyy, sn, rn, ldm, stm);
+    }
     F.sxx[c] = sn[0];
     F.syy[c] = sn[1];
     F.szz[c] = sn[2];
@@ -1324,30 +1430,30 @@ static int launch_v(const MarchCfg &c, const MTiles &T, const DevGeom &G, const 
     return hip_check(hipGetLastError(), "k_v_march");
 }
 
-template <bool CPML, bool VISCO, bool PIPE>
+template <bool CPML, int VM, bool PIPE>
 static void launch_s_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
                         const Psi9 &psi, const Shots &S, const ViscoF &V, hipStream_t s)
 {
     switch (ti) {
-    case 1: hipLaunchKernelGGL((k_s_march<1, CPML, VISCO, PIPE>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, V); break;
-    case 2: hipLaunchKernelGGL((k_s_march<2, CPML, VISCO, PIPE>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
-    case 4: hipLaunchKernelGGL((k_s_march<4, CPML, VISCO, PIPE>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
-    default: hipLaunchKernelGGL((k_s_march<8, CPML, VISCO, PIPE>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
+    case 1: hipLaunchKernelGGL((k_s_march<1, CPML, VM, PIPE>), grid, dim3(64, 1), 0, s, G, T, F, P, psi, S, V); break;
+    case 2: hipLaunchKernelGGL((k_s_march<2, CPML, VM, PIPE>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
+    case 4: hipLaunchKernelGGL((k_s_march<4, CPML, VM, PIPE>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
+    default: hipLaunchKernelGGL((k_s_march<8, CPML, VM, PIPE>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
     }
 }
 
-template <bool CPML, bool VISCO>
+template <bool CPML, int VM>
 static void launch_s_lds_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
                             const Psi9 &psi, const Shots &S, const ViscoF &V, hipStream_t s)
 {
     switch (ti) {
-    case 2: hipLaunchKernelGGL((k_s_lds<2, CPML, VISCO>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
-    case 4: hipLaunchKernelGGL((k_s_lds<4, CPML, VISCO>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
-    default: hipLaunchKernelGGL((k_s_lds<8, CPML, VISCO>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
+    case 2: hipLaunchKernelGGL((k_s_lds<2, CPML, VM>), grid, dim3(64, 2), 0, s, G, T, F, P, psi, S, V); break;
+    case 4: hipLaunchKernelGGL((k_s_lds<4, CPML, VM>), grid, dim3(64, 4), 0, s, G, T, F, P, psi, S, V); break;
+    default: hipLaunchKernelGGL((k_s_lds<8, CPML, VM>), grid, dim3(64, 8), 0, s, G, T, F, P, psi, S, V); break;
     }
 }
 
-template <bool CPML, bool VISCO = false>
+template <bool CPML, int VM = 0>
 static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const SFields &F, const Prof &P,
                     const Psi9 &psi, const Shots &S, hipStream_t s, const ViscoF &V = ViscoF{})
 {
@@ -1358,11 +1464,11 @@ static int launch_s(const MarchCfg &c, const MTiles &T, const DevGeom &G, const 
     }
     const dim3 grid((unsigned)(T.per * 8));
     if (lds_ok(c, T)) {
-        launch_s_lds_ti<CPML, VISCO>(c.ti, grid, T, G, F, P, psi, S, V, s);
+        launch_s_lds_ti<CPML, VM>(c.ti, grid, T, G, F, P, psi, S, V, s);
         return hip_check(hipGetLastError(), "k_s_lds");
     }
-    if (c.pipe) launch_s_ti<CPML, VISCO, true>(c.ti, grid, T, G, F, P, psi, S, V, s);
-    else launch_s_ti<CPML, VISCO, false>(c.ti, grid, T, G, F, P, psi, S, V, s);
+    if (c.pipe) launch_s_ti<CPML, VM, true>(c.ti, grid, T, G, F, P, psi, S, V, s);
+    else launch_s_ti<CPML, VM, false>(c.ti, grid, T, G, F, P, psi, S, V, s);
     return hip_check(hipGetLastError(), "k_s_march");
 }
 
@@ -1377,7 +1483,7 @@ static SFields sfields(const ifos_fields *f, const ifos_model *m)
                  m->pi, m->u, m->uipjp, m->ujpkp, m->uipkp};
     return F;
 }
-static ViscoF viscof(const ifos_visco *v)
+static ViscoF viscof(const ifos_geom *g, const ifos_visco *v)
 {
     ViscoF V = {};
     for (int n = 0; n < 6; ++n) V.r[n] = v->r[n];
@@ -1386,8 +1492,10 @@ static ViscoF viscof(const ifos_visco *v)
     V.tausipjp = v->tausipjp;
     V.tausjpkp = v->tausjpkp;
     V.tausipkp = v->tausipkp;
-    V.eta = v->eta;
     V.L = v->L;
+    V.eta[0] = v->eta;
+    for (int l = 1; l < IFOS_MAX_L; ++l) V.eta[l] = v->eta_more[l - 1];
+    V.lstride = (int64_t)ifos_field_elems(g);
     return V;
 }
 
@@ -1444,10 +1552,12 @@ static int update_s_region(const ifos_geom *g, int region, const Box *box, const
         if (n == 0) return IFOS_OK;
         ProfScope ps_(IFOS_K_S_CPML, s);
         const dim3 grid((unsigned)((n + 255) / 256));
-        if (visco)
-            hipLaunchKernelGGL(k_s_slabs<true>, grid, dim3(256), 0, s, G, L, F, P, psi, viscof(visco));
+        if (visco && visco->L == 1)
+            hipLaunchKernelGGL(k_s_slabs<1>, grid, dim3(256), 0, s, G, L, F, P, psi, viscof(g, visco));
+        else if (visco)
+            hipLaunchKernelGGL(k_s_slabs<2>, grid, dim3(256), 0, s, G, L, F, P, psi, viscof(g, visco));
         else
-            hipLaunchKernelGGL(k_s_slabs<false>, grid, dim3(256), 0, s, G, L, F, P, psi, ViscoF{});
+            hipLaunchKernelGGL(k_s_slabs<0>, grid, dim3(256), 0, s, G, L, F, P, psi, ViscoF{});
         return hip_check(hipGetLastError(), "k_s_slabs");
     }
     Box boxes[1];
@@ -1456,9 +1566,13 @@ static int update_s_region(const ifos_geom *g, int region, const Box *box, const
     const int cat = region == IFOS_REGION_ALL ? IFOS_K_S_FUSED : IFOS_K_S_INTERIOR;
     ProfScope ps_(cat, s);
     if (visco) {
-        const ViscoF V = viscof(visco);
-        if (region == IFOS_REGION_INTERIOR) return launch_s<false, true>(c, T, G, F, P, psi, one, s, V);
-        return launch_s<true, true>(c, T, G, F, P, psi, one, s, V);
+        const ViscoF V = viscof(g, visco);
+        if (visco->L == 1) {
+            if (region == IFOS_REGION_INTERIOR) return launch_s<false, 1>(c, T, G, F, P, psi, one, s, V);
+            return launch_s<true, 1>(c, T, G, F, P, psi, one, s, V);
+        }
+        if (region == IFOS_REGION_INTERIOR) return launch_s<false, 2>(c, T, G, F, P, psi, one, s, V);
+        return launch_s<true, 2>(c, T, G, F, P, psi, one, s, V);
     }
     if (region == IFOS_REGION_INTERIOR) return launch_s<false>(c, T, G, F, P, psi, one, s);
     return launch_s<true>(c, T, G, F, P, psi, one, s);
@@ -1504,9 +1618,10 @@ extern "C" int ifos_update_s_visco(const ifos_geom *g, int region, const ifos_fi
         set_error("ifos_update_s_visco: null argument");
         return IFOS_EINVAL;
     }
-    if (visco->L != 1) {
-        set_error("ifos_update_s_visco: L=%d (one relaxation mechanism is supported, as in update_s_ssg.c)",
-                  visco->L);
+    if (visco->L < 1 || visco->L > IFOS_MAX_L ||
+        (double)visco->L * (double)ifos_field_elems(g) * 4.0 >= 4294967296.0) {
+        set_error("ifos_update_s_visco: L=%d (1..%d relaxation mechanisms, L fields of memory variables per "
+                  "component under 4 GiB)", visco->L, IFOS_MAX_L);
         return IFOS_EINVAL;
     }
     if (region != IFOS_REGION_INTERIOR && (!d_prof || !psi_v)) {
@@ -1586,19 +1701,22 @@ int launch_s_shots(const ifos_geom *g, const ifos_fields *f, const ifos_model *m
     const Prof P = {d_prof, 2 * g->fw + 1};
     const Psi9 psi = psi9(psi_v);
     ProfScope ps_(IFOS_K_S_FUSED, s);
+    auto go = [&](const MTiles &T, bool cpml) {
+        if (!V) return cpml ? launch_s<true>(c, T, G, F, P, psi, S, s) : launch_s<false>(c, T, G, F, P, psi, S, s);
+        if (V->L == 1)
+            return cpml ? launch_s<true, 1>(c, T, G, F, P, psi, S, s, *V)
+                        : launch_s<false, 1>(c, T, G, F, P, psi, S, s, *V);
+        return cpml ? launch_s<true, 2>(c, T, G, F, P, psi, S, s, *V) : launch_s<false, 2>(c, T, G, F, P, psi, S, s, *V);
+    };
     if (!c.split) {
         const Box all = region_box(g, IFOS_REGION_ALL);
-        const MTiles T = make_tiles(&all, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
-        return V ? launch_s<true, true>(c, T, G, F, P, psi, S, s, *V) : launch_s<true>(c, T, G, F, P, psi, S, s);
+        return go(make_tiles(&all, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb), true);
     }
     const Box in = region_box(g, IFOS_REGION_INTERIOR);
     Box sl[6];
     const int nb = slab_boxes(g, sl);
-    const MTiles Ti = make_tiles(&in, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
-    const MTiles Ts = make_tiles(sl, nb, c.ti, c.jc, S.n, c.group, c.kb, c.sb);
-    int rc = V ? launch_s<false, true>(c, Ti, G, F, P, psi, S, s, *V) : launch_s<false>(c, Ti, G, F, P, psi, S, s);
-    if (!rc)
-        rc = V ? launch_s<true, true>(c, Ts, G, F, P, psi, S, s, *V) : launch_s<true>(c, Ts, G, F, P, psi, S, s);
+    int rc = go(make_tiles(&in, 1, c.ti, c.jc, S.n, c.group, c.kb, c.sb), false);
+    if (!rc) rc = go(make_tiles(sl, nb, c.ti, c.jc, S.n, c.group, c.kb, c.sb), true);
     return rc;
 }
 

@@ -44,19 +44,36 @@ class IfosPsi(C.Structure):
     _fields_ = [("p", C.c_void_p * 9)]
 
 
+MAX_L = 4  # IFOS_MAX_L: relaxation mechanisms
+
+
 class IfosVisco(C.Structure):
     _fields_ = [("r", C.c_void_p * 6), ("taus", C.c_void_p), ("taup", C.c_void_p), ("tausipjp", C.c_void_p),
-                ("tausjpkp", C.c_void_p), ("tausipkp", C.c_void_p), ("eta", C.c_float), ("L", C.c_int)]
+                ("tausjpkp", C.c_void_p), ("tausipkp", C.c_void_p), ("eta", C.c_float), ("L", C.c_int),
+                ("eta_more", C.c_float * (MAX_L - 1))]
 
 
-def visco_struct(r, taus, taup, tausipjp, tausjpkp, tausipkp, eta, L=1):
+def _etas(eta):
+    """a scalar eta (one mechanism) or one eta per mechanism, as float32"""
+    e = np.atleast_1d(np.asarray(eta, np.float32))
+    if not 1 <= e.size <= MAX_L:
+        raise ValueError(f"1..{MAX_L} relaxation mechanisms")
+    return e
+
+
+def visco_struct(r, taus, taup, tausipjp, tausjpkp, tausipkp, eta, L=None):
+    """eta: scalar (one mechanism) or one per mechanism; r[n] holds the
+    mechanisms' fields back to back"""
     v = IfosVisco()
+    e = _etas(eta)
     for n in range(6):
         v.r[n] = r[n].data_ptr()
     v.taus, v.taup = taus.data_ptr(), taup.data_ptr()
     v.tausipjp, v.tausjpkp, v.tausipkp = tausipjp.data_ptr(), tausjpkp.data_ptr(), tausipkp.data_ptr()
-    v.eta = float(eta)
-    v.L = L
+    v.eta = float(e[0])
+    for l in range(1, e.size):
+        v.eta_more[l - 1] = float(e[l])
+    v.L = e.size if L is None else L
     return v
 
 
@@ -424,9 +441,13 @@ class ShotContext:
         check(lib().ifos_ctx_set_freqs(self.h, fv.ctypes.data_as(C.c_void_p), len(fv), tast),
               "ifos_ctx_set_freqs")
 
-    def set_visco(self, d_taus, d_taup, eta, L=1, stream=None):
-        check(lib().ifos_ctx_set_visco(self.h, _ptr(d_taus), _ptr(d_taup), C.c_float(eta), L, stream),
-              "ifos_ctx_set_visco")
+    def set_visco(self, d_taus, d_taup, eta, L=None, stream=None):
+        """eta: scalar (one relaxation mechanism, update_s_ssg.c) or one per
+        mechanism (ifos_ctx_set_visco_n); L = 0 switches back to elastic"""
+        e = _etas(eta)
+        n = e.size if L is None else L
+        check(lib().ifos_ctx_set_visco_n(self.h, _ptr(d_taus), _ptr(d_taup), e.ctypes.data_as(C.c_void_p), n,
+                                         stream), "ifos_ctx_set_visco_n")
 
     def run_shot(self, src, rec_ijk, d_syn, grads, d_obs=None, shape=1, want_adjoint=True, stream=None):
         """src = (i, j, k, tshift, fc, amp, type); returns the L2 misfit (FP64 sum)."""
@@ -607,9 +628,12 @@ class BatchContext:
         check(lib().ifos_batch_set_freqs(self.h, fv.ctypes.data_as(C.c_void_p), len(fv), tast),
               "ifos_batch_set_freqs")
 
-    def set_visco(self, d_taus, d_taup, eta, L=1):
-        check(lib().ifos_batch_set_visco(self.h, _ptr(d_taus), _ptr(d_taup), C.c_float(eta), L),
-              "ifos_batch_set_visco")
+    def set_visco(self, d_taus, d_taup, eta, L=None):
+        """as ShotContext.set_visco, for every shot of the batch"""
+        e = _etas(eta)
+        n = e.size if L is None else L
+        check(lib().ifos_batch_set_visco_n(self.h, _ptr(d_taus), _ptr(d_taup), e.ctypes.data_as(C.c_void_p), n),
+              "ifos_batch_set_visco_n")
 
     def run(self, srcs, rec_ijk, d_syn=None, grads=None, d_obs=None, shape=1, want_adjoint=True, use_graph=False):
         """srcs: list of (i, j, k, tshift, fc, amp, type), one per shot; rec_ijk:

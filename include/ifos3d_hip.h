@@ -147,14 +147,22 @@ int ifos_update_s(const ifos_geom *g, int region, const ifos_fields *f,
  * memory variables; taus, taup relaxation-time ratios (cell values, halos
  * [0..N+1] valid); tausipjp, tausjpkp, tausipkp their averages
  * (ifos_av_tau); eta = DT / relaxation time of the mechanism; L = 1 */
+#define IFOS_MAX_L 4
 typedef struct ifos_visco {
     float *r[6];
     const float *taus, *taup, *tausipjp, *tausjpkp, *tausipkp;
     float eta;
     int L;
+    /* L > 1 (beyond the reference, which hard-codes one mechanism,
+     * read_par_json.c:268-282): eta of mechanisms 2..L; each r[n] then
+     * holds L fields of the device layout back to back (mechanism l at
+     * r[n] + l * ifos_field_elems) */
+    float eta_more[IFOS_MAX_L - 1];
 } ifos_visco;
 /* update_s_ssg.c:28-245 (INTERIOR) / update_s_ssg_CPML.c:30-772 (CPML): the
- * viscoelastic stress update with one relaxation mechanism */
+ * viscoelastic stress update; L = 1 is the reference's update exactly, and
+ * L mechanisms (up to IFOS_MAX_L) sum their memory variables where the
+ * reference uses its one, each mechanism with its own eta */
 int ifos_update_s_visco(const ifos_geom *g, int region, const ifos_fields *f,
                         const ifos_model *m, const ifos_visco *visco,
                         const float *d_prof, const ifos_psi *psi_v,
@@ -275,6 +283,12 @@ int ifos_ctx_set_freqs(ifos_ctx *c, const float *finv, int nf, int tast);
  * averaged), eta = DT / relaxation time; L = 0: elastic (the default) */
 int ifos_ctx_set_visco(ifos_ctx *c, const float *d_taus, const float *d_taup,
                        float eta, int L, void *stream);
+/* L = 1..IFOS_MAX_L relaxation mechanisms, eta[l] = DT / relaxation time
+ * of mechanism l (the reference reads one, read_par_json.c:268-282; L > 1
+ * sums the mechanisms' memory variables where update_s_ssg.c uses its one);
+ * L = 0: elastic.  ifos_ctx_set_visco is this with L <= 1. */
+int ifos_ctx_set_visco_n(ifos_ctx *c, const float *d_taus, const float *d_taup,
+                         const float *eta, int L, void *stream);
 /* runs one shot; d_syn [3][ntr][nt+1] (device, written); d_grad[3]
  * (device layout, accumulated); *l2 (host, accumulated) */
 int ifos_ctx_run_shot(ifos_ctx *c, const ifos_shot *s, float *d_syn,
@@ -344,6 +358,9 @@ int ifos_batch_set_freqs(ifos_batch *b, const float *finv, int nf, int tast);
 /* as ifos_ctx_set_visco, for every shot of the batch */
 int ifos_batch_set_visco(ifos_batch *b, const float *d_taus, const float *d_taup,
                          float eta, int L);
+/* as ifos_ctx_set_visco_n, for every shot of the batch */
+int ifos_batch_set_visco_n(ifos_batch *b, const float *d_taus, const float *d_taup,
+                           const float *eta, int L);
 /* runs ns (<= nshots) shots with equal ntr and want_adjoint, synchronously.
  * d_syn: device [ns][3][ntr][nt+1] or NULL; d_grad[3]: device layout,
  * accumulated over the shots in order (NULL: no gradient); l2: host [ns]

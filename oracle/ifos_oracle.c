@@ -593,7 +593,11 @@ void og_update_s_cpml_elastic(const og_grid *g, const int box[6],
 /* update_s_ssg.c:174-238: the relaxed stress step, the memory-variable
  * recursion, and the second half of the memory-variable term.  Written with
  * the reference's own float/double mix (1.0 + L*tau, 2.0*u, eta*0.5 are
- * double expressions rounded to float on assignment). */
+ * double expressions rounded to float on assignment).  With L > 1 mechanisms
+ * the memory variables of each component are summed in mechanism order
+ * where the reference uses its one (the SLS sum of the tau-method), and
+ * each mechanism is updated with its own eta; L = 1 is the reference's
+ * arithmetic exactly. */
 static inline void stress_apply_visco(const og_grid *g, size_t c, const vdiff *d,
                                       oreal *sxx, oreal *syy, oreal *szz,
                                       oreal *sxy, oreal *syz, oreal *sxz,
@@ -601,46 +605,60 @@ static inline void stress_apply_visco(const og_grid *g, size_t c, const vdiff *d
                                       const oreal *uipjp, const oreal *ujpkp,
                                       const oreal *uipkp, const og_visco *V)
 {
+    const size_t fs = og_field_size(g);
     const oreal DT = g->dt;
     const oreal dthalbe = DT / 2.0;
     const int L = V->L;
-    oreal *rxx = V->r[0], *ryy = V->r[1], *rzz = V->r[2];
-    oreal *rxy = V->r[3], *ryz = V->r[4], *rxz = V->r[5];
-    const oreal eta = V->eta;
     oreal fipjp, fjpkp, fipkp, gg, f, b, cc, dipjp, djpkp, dipkp, dd, e;
+    oreal sr[6], *rc[6];
+    int l, n;
     oreal vxyyx = d->vxy + d->vyx, vyzzy = d->vyz + d->vzy, vxzzx = d->vxz + d->vzx;
     oreal vxxyyzz = d->vxx + d->vyy + d->vzz;
     oreal vyyzz = d->vyy + d->vzz, vxxzz = d->vxx + d->vzz, vxxyy = d->vxx + d->vyy;
+    for (n = 0; n < 6; n++) {
+        rc[n] = V->r[n] + c;
+        sr[n] = rc[n][0];
+        for (l = 1; l < L; l++) sr[n] += rc[n][(size_t)l * fs];
+    }
     fipjp = uipjp[c] * DT * (1.0 + L * V->tausipjp[c]);
     fjpkp = ujpkp[c] * DT * (1.0 + L * V->tausjpkp[c]);
     fipkp = uipkp[c] * DT * (1.0 + L * V->tausipkp[c]);
     gg = pi[c] * (1.0 + L * V->taup[c]);
     f = 2.0 * u[c] * (1.0 + L * V->taus[c]);
-    sxy[c] += (fipjp * vxyyx) + (dthalbe * rxy[c]);
-    syz[c] += (fjpkp * vyzzy) + (dthalbe * ryz[c]);
-    sxz[c] += (fipkp * vxzzx) + (dthalbe * rxz[c]);
-    sxx[c] += DT * ((gg * vxxyyzz) - (f * vyyzz)) + (dthalbe * rxx[c]);
-    syy[c] += DT * ((gg * vxxyyzz) - (f * vxxzz)) + (dthalbe * ryy[c]);
-    szz[c] += DT * ((gg * vxxyyzz) - (f * vxxyy)) + (dthalbe * rzz[c]);
-    b = 1.0 / (1.0 + (eta * 0.5));
-    cc = 1.0 - (eta * 0.5);
-    dipjp = uipjp[c] * eta * V->tausipjp[c];
-    djpkp = ujpkp[c] * eta * V->tausjpkp[c];
-    dipkp = uipkp[c] * eta * V->tausipkp[c];
-    dd = 2.0 * u[c] * eta * V->taus[c];
-    e = pi[c] * eta * V->taup[c];
-    rxy[c] = b * (rxy[c] * cc - (dipjp * vxyyx));
-    ryz[c] = b * (ryz[c] * cc - (djpkp * vyzzy));
-    rxz[c] = b * (rxz[c] * cc - (dipkp * vxzzx));
-    rxx[c] = b * (rxx[c] * cc - (e * vxxyyzz) + (dd * vyyzz));
-    ryy[c] = b * (ryy[c] * cc - (e * vxxyyzz) + (dd * vxxzz));
-    rzz[c] = b * (rzz[c] * cc - (e * vxxyyzz) + (dd * vxxyy));
-    sxy[c] += (dthalbe * rxy[c]);
-    syz[c] += (dthalbe * ryz[c]);
-    sxz[c] += (dthalbe * rxz[c]);
-    sxx[c] += (dthalbe * rxx[c]);
-    syy[c] += (dthalbe * ryy[c]);
-    szz[c] += (dthalbe * rzz[c]);
+    /* sr: rxx ryy rzz rxy ryz rxz */
+    sxy[c] += (fipjp * vxyyx) + (dthalbe * sr[3]);
+    syz[c] += (fjpkp * vyzzy) + (dthalbe * sr[4]);
+    sxz[c] += (fipkp * vxzzx) + (dthalbe * sr[5]);
+    sxx[c] += DT * ((gg * vxxyyzz) - (f * vyyzz)) + (dthalbe * sr[0]);
+    syy[c] += DT * ((gg * vxxyyzz) - (f * vxxzz)) + (dthalbe * sr[1]);
+    szz[c] += DT * ((gg * vxxyyzz) - (f * vxxyy)) + (dthalbe * sr[2]);
+    for (l = 0; l < L; l++) {
+        const oreal eta = V->eta[l];
+        const size_t o = (size_t)l * fs;
+        b = 1.0 / (1.0 + (eta * 0.5));
+        cc = 1.0 - (eta * 0.5);
+        dipjp = uipjp[c] * eta * V->tausipjp[c];
+        djpkp = ujpkp[c] * eta * V->tausjpkp[c];
+        dipkp = uipkp[c] * eta * V->tausipkp[c];
+        dd = 2.0 * u[c] * eta * V->taus[c];
+        e = pi[c] * eta * V->taup[c];
+        rc[3][o] = b * (rc[3][o] * cc - (dipjp * vxyyx));
+        rc[4][o] = b * (rc[4][o] * cc - (djpkp * vyzzy));
+        rc[5][o] = b * (rc[5][o] * cc - (dipkp * vxzzx));
+        rc[0][o] = b * (rc[0][o] * cc - (e * vxxyyzz) + (dd * vyyzz));
+        rc[1][o] = b * (rc[1][o] * cc - (e * vxxyyzz) + (dd * vxxzz));
+        rc[2][o] = b * (rc[2][o] * cc - (e * vxxyyzz) + (dd * vxxyy));
+    }
+    for (n = 0; n < 6; n++) {
+        sr[n] = rc[n][0];
+        for (l = 1; l < L; l++) sr[n] += rc[n][(size_t)l * fs];
+    }
+    sxy[c] += (dthalbe * sr[3]);
+    syz[c] += (dthalbe * sr[4]);
+    sxz[c] += (dthalbe * sr[5]);
+    sxx[c] += (dthalbe * sr[0]);
+    syy[c] += (dthalbe * sr[1]);
+    szz[c] += (dthalbe * sr[2]);
 }
 
 void og_update_s_visco(const og_grid *g, const int box[6], const oreal *vx,
@@ -1035,7 +1053,7 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
     int ntast;
     const size_t sl = (size_t)(nt_all + 2);
 
-    const int visco = io->L > 0 && io->taus && io->taup;
+    const int visco = io->L > 0 && io->L <= OG_MAX_L && io->taus && io->taup;
     oreal *taus = NULL, *taup = NULL, *tavg[3] = {NULL, NULL, NULL}, *rmem[6] = {NULL};
     og_visco V;
     memset(&V, 0, sizeof V);
@@ -1045,7 +1063,7 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
         taus = (oreal *)calloc(fs, sizeof(oreal));
         taup = (oreal *)calloc(fs, sizeof(oreal));
         for (n = 0; n < 3; n++) tavg[n] = (oreal *)calloc(fs, sizeof(oreal));
-        for (n = 0; n < 6; n++) rmem[n] = (oreal *)calloc(fs, sizeof(oreal));
+        for (n = 0; n < 6; n++) rmem[n] = (oreal *)calloc(fs * (size_t)io->L, sizeof(oreal));
     }
 
     rho = (oreal *)calloc(fs, sizeof(oreal));
@@ -1092,7 +1110,7 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
         V.tausipjp = tavg[0];
         V.tausjpkp = tavg[1];
         V.tausipkp = tavg[2];
-        V.eta = io->eta;
+        for (n = 0; n < OG_MAX_L; n++) V.eta[n] = io->eta[n];
         V.L = io->L;
         for (n = 0; n < 6; n++) V.r[n] = rmem[n];
     }
@@ -1156,7 +1174,7 @@ int og_run_shot(const og_grid *g, og_shot_io *io)
     /* adjoint, ifos3d.c:905-960: fresh wavefield, receivers inject */
     for (n = 0; n < 9; n++) memset(f[n], 0, fs * sizeof(oreal));
     if (visco)
-        for (n = 0; n < 6; n++) memset(rmem[n], 0, fs * sizeof(oreal));
+        for (n = 0; n < 6; n++) memset(rmem[n], 0, fs * (size_t)io->L * sizeof(oreal));
     for (n = 0; n < 9; n++) {
         size_t sz = (n < 3) ? npx : (n < 6) ? npy : npz;
         memset(psi_s[n], 0, sz * sizeof(oreal));

@@ -132,16 +132,22 @@ void og_update_s_cpml_elastic(const og_grid *g, const int box[6],
                               const oreal *uipkp, oreal *const prof[18],
                               oreal *psi[9]);
 
-/* Viscoelastic stress update with one relaxation mechanism, 4th order
- * (update_s_ssg.c:152-245 interior box, update_s_ssg_CPML.c:30-772 slabs;
- * the reference hard-codes the mechanism l = 1, update_s_ssg.c:110,211).
- * r[6] = rxx ryy rzz rxy ryz rxz (memory variables, [1..N]^3 used);
- * taus, taup cell values; tausipjp/tausjpkp/tausipkp from og_av_mat;
- * eta = eta[1] of the reference (DT / relaxation time); L the global L. */
+/* Viscoelastic stress update, 4th order (update_s_ssg.c:152-245 interior
+ * box, update_s_ssg_CPML.c:30-772 slabs).  The reference hard-codes one
+ * mechanism (update_s_ssg.c:110,211; read_par_json.c:268-282); L > 1
+ * sums the L memory variables of each stress component where the
+ * reference uses its one (before and after their update) and updates each
+ * mechanism with its own eta, in mechanism order -- for L = 1 it is the
+ * reference's arithmetic exactly.
+ * r[6] = rxx ryy rzz rxy ryz rxz, each L canonical fields back to back
+ * (mechanism l at r[n] + l * og_field_size); taus, taup cell values;
+ * tausipjp/tausjpkp/tausipkp from og_av_mat; eta[l] = DT / relaxation time
+ * of mechanism l (eta[1] of the reference for L = 1); L in 1..OG_MAX_L. */
+#define OG_MAX_L 4
 typedef struct og_visco {
     oreal *r[6];
     const oreal *taus, *taup, *tausipjp, *tausjpkp, *tausipkp;
-    oreal eta;
+    oreal eta[OG_MAX_L];
     int L;
 } og_visco;
 
@@ -238,10 +244,11 @@ typedef struct og_shot_io {
     /* optional: the frequency fields after the shot (6*nf canonical each),
      * NULL if not wanted */
     oreal *ffwd, *fadj;
-    /* optional viscoelastic run (L = 1): taus, taup canonical (cell values),
-     * eta; L = 0 (or NULL taus) runs the elastic update of ifos3d.c */
+    /* optional viscoelastic run (L = 1..OG_MAX_L mechanisms): taus, taup
+     * canonical (cell values), eta[l] per mechanism; L = 0 (or NULL taus)
+     * runs the elastic update of ifos3d.c */
     const oreal *taus, *taup;
-    oreal eta;
+    oreal eta[OG_MAX_L];
     int L;
     /* optional: the velocities vx, vy, vz after every step at the DFT point
      * of ifos3d.c (forward after psource, adjoint after the stress update),

@@ -28,12 +28,23 @@ class OgGrid(C.Structure):
                 ("nt", C.c_int), ("time", C.c_float)]
 
 
+MAX_L = 4  # OG_MAX_L: relaxation mechanisms
+
+
 def _visco_struct(real):
     class OgVisco(C.Structure):
         _fields_ = [("r", C.c_void_p * 6), ("taus", C.c_void_p), ("taup", C.c_void_p),
                     ("tausipjp", C.c_void_p), ("tausjpkp", C.c_void_p), ("tausipkp", C.c_void_p),
-                    ("eta", real), ("L", C.c_int)]
+                    ("eta", real * MAX_L), ("L", C.c_int)]
     return OgVisco
+
+
+def _etas(eta):
+    """a scalar eta (one mechanism) or a sequence of one eta per mechanism"""
+    e = [float(x) for x in np.atleast_1d(np.asarray(eta, np.float64))]
+    if not 1 <= len(e) <= MAX_L:
+        raise ValueError(f"1..{MAX_L} relaxation mechanisms")
+    return e
 
 
 OgVisco32 = _visco_struct(C.c_float)
@@ -140,15 +151,21 @@ class Oracle:
         self.lib.og_av_mat(C.byref(og), _p(rho), _p(u), _p(taus), *[_p(a) for a in tmp])
         return tmp[3:6]
 
-    def visco(self, r, taus, taup, tausipjp, tausjpkp, tausipkp, eta, L=1):
-        """an og_visco for update_s_visco / update_s_cpml_visco (keeps the arrays alive)"""
+    def visco(self, r, taus, taup, tausipjp, tausjpkp, tausipkp, eta, L=None):
+        """an og_visco for update_s_visco / update_s_cpml_visco (keeps the
+        arrays alive).  eta: scalar (one mechanism) or one per mechanism;
+        r[n]: the L mechanisms' fields back to back (shape (L,) + grid or
+        the grid's shape for L = 1)"""
         v = (OgVisco32 if self.precision == "f32" else OgVisco64)()
+        e = _etas(eta)
         for n in range(6):
+            assert r[n].size == len(e) * taus.size and r[n].flags.c_contiguous
             v.r[n] = r[n].ctypes.data
         v.taus, v.taup = taus.ctypes.data, taup.ctypes.data
         v.tausipjp, v.tausjpkp, v.tausipkp = tausipjp.ctypes.data, tausjpkp.ctypes.data, tausipkp.ctypes.data
-        v.eta = eta
-        v.L = L
+        for l, x in enumerate(e):
+            v.eta[l] = x
+        v.L = len(e) if L is None else L
         v._keep = (r, taus, taup, tausipjp, tausjpkp, tausipkp)
         return v
 
@@ -257,8 +274,9 @@ class Oracle:
     def run_shot(self, g: Grid, rho, pi, u, src, rec_ijk, finv, tast=100, vppml=4000.0, fpml=50.0,
                  shape=1, obs=None, want_adjoint=True, want_fields=False, visco=None, want_history=False,
                  grads=None):
-        """visco = (taus, taup, eta): the viscoelastic update with one
-        relaxation mechanism instead of the elastic one.  grads: three
+        """visco = (taus, taup, eta): the viscoelastic update instead of the
+        elastic one, eta a scalar (one relaxation mechanism) or one per
+        mechanism.  grads: three
         canonical arrays the shot's gradient is added into in place (the
         reference accumulates over shots, ifos3d.c:974); fresh zeros if None"""
         class Io(C.Structure):
@@ -271,7 +289,7 @@ class Oracle:
                         ("l2", C.c_void_p), ("grad1", C.c_void_p), ("grad2", C.c_void_p),
                         ("grad3", C.c_void_p), ("ffwd", C.c_void_p), ("fadj", C.c_void_p),
                         ("taus", C.c_void_p), ("taup", C.c_void_p),
-                        ("eta", C.c_float if self.precision == "f32" else C.c_double), ("L", C.c_int),
+                        ("eta", (C.c_float if self.precision == "f32" else C.c_double) * MAX_L), ("L", C.c_int),
                         ("vfwd", C.c_void_p), ("vadj", C.c_void_p)]
         rec = np.ascontiguousarray(rec_ijk, np.int32)
         ntr = rec.shape[0]
@@ -300,7 +318,10 @@ class Oracle:
         if visco is not None:
             taus = np.ascontiguousarray(visco[0], self.dtype)
             taup = np.ascontiguousarray(visco[1], self.dtype)
-            io.taus, io.taup, io.eta, io.L = taus.ctypes.data, taup.ctypes.data, visco[2], 1
+            e = _etas(visco[2])
+            io.taus, io.taup, io.L = taus.ctypes.data, taup.ctypes.data, len(e)
+            for l, x in enumerate(e):
+                io.eta[l] = x
         og = g.og()
         rc = self.lib.og_run_shot(C.byref(og), C.byref(io))
         if rc != 0:

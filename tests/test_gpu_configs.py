@@ -54,9 +54,14 @@ def _check_toy(F, G, w, obs, syns, l2s, grads, tag):
         if sha(syns[n][:, :, 1:]) != str(F[f"toy84_syn_sha_{n}"]):
             detail = f" (max ulp {max_ulp(syns[n], F['toy84_syn_0'])})" if n == 0 else ""
             raise AssertionError(f"{tag}: synthetics of shot {n} differ from the oracle{detail}")
-        # the reference sums the misfit in float (residual.c:53); the library in FP64
+        # the misfit: the library sums the squares of the integrated residual
+        # (residual.c:43-53) in FP64, the reference in float, one term after
+        # the other (57 600 terms here: the float sum is off by ~3e-5)
         want = float(F[f"toy84_l2_{n}"])
-        assert abs(l2s[n] - want) <= 1e-5 * abs(want), f"{tag}: misfit of shot {n}: {l2s[n]} vs {want}"
+        assert abs(l2s[n] - want) <= 2e-4 * abs(want), f"{tag}: misfit of shot {n}: {l2s[n]} vs {want}"
+        R = np.cumsum(syns[n][:, :, 1:] - obs[n][:, :, 1:], axis=2, dtype=np.float32) * np.float32(w.dt)
+        l2_64 = float((R.astype(np.float64) ** 2).sum())
+        assert abs(l2s[n] - l2_64) <= 1e-6 * l2_64, f"{tag}: misfit of shot {n}: {l2s[n]} vs FP64 sum {l2_64}"
     jc = g.shape[0] // 2
     for q in range(3):
         a = G.from_dev(grads[q])
