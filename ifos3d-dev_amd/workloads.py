@@ -16,6 +16,15 @@ receivers, time step), in the canonical/device layouts of ifosgpu.
            dt = 0.9 CFL, NT 2000, the seeded random field (seed 2), one
            explosive 15 Hz Ricker source, 4096 receivers (64 x 64): the
            forward-only bandwidth case.
+* visco404: 404 x 212 x 404 (384 x 192 x 384 + 10-cell CPML), dh 5 m, the
+           seeded random field (seed 3) as the model, 3 relaxation mechanisms
+           whose frequencies are fitted to constant Q over 5-40 Hz (the
+           tau-method, the reference's mfiles/qapprox.m), Q uniform in
+           20-100 per cell (seed 3) -> tau per cell, moduli relaxed so that
+           the phase velocity at the first relaxation frequency is the
+           model's (readmod.c:50-64,110-111), dt = 0.9 CFL of the unrelaxed
+           P velocity (checkfd_ssg.c), NT 2000, one explosive 15 Hz source,
+           4096 receivers, 8 DFT frequencies 5, 10, ..., 40 Hz.
 * levels3:  three coarse-to-fine grid levels of one 768 m cube, interiors
            96^3 at dh 8 m, 128^3 at 6 m and 192^3 at 4 m, each + 10-cell
            CPML, dt = 0.9 CFL per level and NT = ceil(0.6 s / dt); the
@@ -96,6 +105,72 @@ def fwd536() -> Workload:
     rec = np.array([[fw + 1 + 8 * a, fw + 1, fw + 1 + 8 * b] for a in range(64) for b in range(64)], np.int32)
     return Workload("fwd536", nx, ny, nz, fw, dh, float(np.float32(dt)), 2000, [15.0], 100, 5500.0, 15.0, shots,
                     rec, seed=2)
+
+
+def visco404() -> Workload:
+    """the viscoelastic + DFT case (see the module docstring); dt is set by
+    visco_models, which needs the model"""
+    fw, dh = 10, 5.0
+    nx, ny, nz = 404, 212, 404
+    shots = [(fw + 1 + 192, fw + 10, fw + 1 + 192, 0.0, 15.0, 1.0, 1)]
+    rec = np.array([[fw + 1 + 6 * a, fw + 1, fw + 1 + 6 * b] for a in range(64) for b in range(64)], np.int32)
+    finv = [5.0 * (n + 1) for n in range(8)]
+    return Workload("visco404", nx, ny, nz, fw, dh, 0.0, 2000, finv, 100, 5500.0, 15.0, shots, rec, seed=3,
+                    extra={"L": 3, "qrange": (20.0, 100.0), "band": (5.0, 40.0)})
+
+
+def q_of(ts, tau, f):
+    """Q(f) of the L-fold standard linear solid of the tau-method
+    (mfiles/qgsls.m: relaxation times ts, strain retardation ts (1 + tau))"""
+    w = 2.0 * np.pi * np.asarray(f, np.float64)
+    num = 1.0 - len(ts) + sum((1.0 + w * w * t * t * (1.0 + tau)) / (1.0 + w * w * t * t) for t in ts)
+    den = sum(w * t * tau / (1.0 + w * w * t * t) for t in ts)
+    return num / den
+
+
+def fit_relaxation(L=3, band=(5.0, 40.0), q0=50.0):
+    """relaxation frequencies and tau of L mechanisms for constant Q = q0
+    over the band: least squares on 1/Q (the tau-method, mfiles/qapprox.m),
+    the frequencies kept within a factor 3 of the band"""
+    from scipy.optimize import least_squares
+    f = np.linspace(band[0], band[1], 64)
+
+    def res(x):
+        ts = 1.0 / (2.0 * np.pi * np.exp(x[:L]))
+        return 1.0 / q_of(ts, x[L], f) - 1.0 / q0
+
+    x0 = np.concatenate([np.log(np.geomspace(band[0], band[1], L)), [2.0 / q0]])
+    lo = np.concatenate([np.full(L, np.log(band[0] / 3.0)), [1e-4]])
+    hi = np.concatenate([np.full(L, np.log(band[1] * 3.0)), [1.0]])
+    x = least_squares(res, x0, bounds=(lo, hi), x_scale=np.concatenate([np.ones(L), [0.01]])).x
+    return np.exp(x[:L]), float(x[L])
+
+
+def visco_models(w: Workload, corr=20.0):
+    """(rho, pi, u), (taus, taup), etas of the viscoelastic workload, and
+    its dt (sets w.dt): the random field model, Q uniform in qrange per
+    cell, tau = tau(q0 = 50) * 50 / Q (1/Q is proportional to tau in the
+    tau-method), moduli relaxed as readmod.c:50-64,110-111 does with the
+    first relaxation frequency as reference"""
+    L = w.extra["L"]
+    fl, tau50 = fit_relaxation(L, w.extra["band"])
+    shape = (w.ny + 5, w.nx + 5, w.nz + 5)
+    vp = _minmax(_grf(shape, corr, w.seed), 2500.0, 5500.0)
+    vs = vp / _minmax(_grf(shape, corr, w.seed + 1000), 1.7, 2.0)
+    rho = 310.0 * vp ** 0.25
+    q = np.random.default_rng(w.seed).uniform(*w.extra["qrange"], size=shape)
+    tau = tau50 * 50.0 / q
+    ts = 1.0 / (2.0 * np.pi * fl)
+    ws = 2.0 * np.pi * fl[0]
+    fac = 1.0 + sum(ws * ws * t * t * tau / (1.0 + ws * ws * t * t) for t in ts)  # sumu = sumpi (ts = tp)
+    pi = rho * vp * vp / fac
+    u = rho * vs * vs / fac
+    vmax = float(np.sqrt(pi * (1.0 + L * tau) / rho).max())
+    w.dt = float(np.float32(cfl_dt(w.dh, vmax)))
+    etas = np.array([w.dt / t for t in ts], np.float32)
+    w.extra.update(fl=[float(x) for x in fl], tau50=tau50, vmax_unrelaxed=vmax)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    return [f32(rho), f32(pi), f32(u)], [f32(tau), f32(tau)], etas
 
 
 LEVELS3 = ((96, 8.0), (128, 6.0), (192, 4.0))  # (interior cells per axis, dh), one 768 m cube

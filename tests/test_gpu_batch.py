@@ -267,9 +267,11 @@ def test_batch_autotune_keeps_results():
     b.close()
 
 
-@pytest.mark.parametrize("tiles", [None, ((4, 0, 1, 1, 1), (2, 0, 4, 1, 1))])
-def test_batch_viscoelastic(tiles):
-    """the batched path with the viscoelastic update against the plain one"""
+@pytest.mark.parametrize("tiles,L", [(None, 1), (((4, 0, 1, 1, 1), (2, 0, 4, 1, 1)), 1), (None, 3),
+                                     (((4, 0, 1, 1, 1), (2, 0, 4, 1, 1)), 3)])
+def test_batch_viscoelastic(tiles, L):
+    """the batched path with the viscoelastic update (L mechanisms) against
+    the plain one"""
     import ifosgpu as ig
     c = CASES["slab_sources"]
     g = small_grid(nx=36, ny=34, nz=38, fw=8, nt=100)
@@ -277,8 +279,9 @@ def test_batch_viscoelastic(tiles):
     finv, tast = [40.0, 60.0], 100
     rng = np.random.default_rng(21)
     visco = ((0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
-             (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32), np.float32(0.019))
-    _note(f"batched viscoelastic, tiles {tiles}")
+             (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
+             np.array([0.019, 0.0071, 0.043][:L], np.float32))
+    _note(f"batched viscoelastic, tiles {tiles}, L={L}")
     true, start = _models(g)
     G = _setup(ig, g, finv, tast, ig.DFT_COMPAT)
     ref = _plain(ig, G, g, true, start, c["srcs"], rec, finv, tast, ig.DFT_COMPAT, visco=visco)
@@ -296,7 +299,9 @@ def test_batch_viscoelastic(tiles):
     (70, 1, False, ((2, 5, 0, 0, 0, 1, 1, 1), (8, 0, 1, 0, 0, 1, 1, 1))),
     (70, 0, True, ((4, 0, 0, 0, 1, 1, 1, 1), (2, 7, 0, 0, 0, 1, 1, 1))),
     (140, 0, False, ((4, 0, 0, 1, 0, 1, 1, 1), (4, 0, 0, 1, 0, 1, 1, 1))),
-    (140, 1, True, ((8, 9, 3, 0, 0, 1, 1, 1), (8, 9, 3, 0, 0, 1, 1, 1)))])
+    (140, 1, True, ((8, 9, 3, 0, 0, 1, 1, 1), (8, 9, 3, 0, 0, 1, 1, 1))),
+    (70, 0, 3, ((4, 0, 0, 0, 1, 1, 1, 1), (4, 32, 0, 0, 0, 1, 1, 1))),
+    (140, 0, 3, ((4, 0, 0, 1, 0, 1, 1, 1), (8, 9, 3, 0, 0, 1, 1, 1)))])
 def test_batch_lds_march(nz, free_surf, visco, tiles):
     """the LDS-staged march (64-lane rows: nz >= 64; partial k and i tiles,
     CPML slabs in the same launch or split off, free surface, viscoelastic)
@@ -309,8 +314,10 @@ def test_batch_lds_march(nz, free_surf, visco, tiles):
     vis = None
     if visco:
         rng = np.random.default_rng(5)
+        # visco: the number of relaxation mechanisms (True: one)
         vis = ((0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
-               (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32), np.float32(0.019))
+               (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
+               np.array([0.019, 0.0071, 0.043][:int(visco)], np.float32))
     _note(f"lds march nz={nz} free_surf={free_surf} visco={visco} tiles {tiles}")
     true, start = _models(g)
     G = _setup(ig, g, finv, tast, ig.DFT_COMPAT)

@@ -168,3 +168,67 @@ def test_level_shapes_match_oracle():
         assert np.abs(syn).max() > 0
     for b in ctxs:
         b.close()
+
+
+@pytest.mark.timeout(900)
+def test_visco404_shape_matches_oracle():
+    """the viscoelastic + DFT case's own grid, model, relaxation mechanisms
+    (3, per-cell Q) and 8 frequencies: a 4-step gradient shot (a source next
+    to the receivers so the short record is not empty) through the plain path
+    and the batched path (compat DFT) against the oracle bit for bit; the
+    Kahan DFT within 1e-4 of it"""
+    import ifosgpu as ig
+    import workloads as wl
+    from pyoracle import Grid, Oracle
+    w = wl.visco404()
+    model, (taus, taup), etas = wl.visco_models(w)
+    w.nt = 4
+    src = (w.fw + 1 + 192, w.fw + 2, w.fw + 1 + 192, 0.0, 15.0, 1e6, 1)
+    rec = np.array([[w.fw + 1 + 192 + d, w.fw + 1, w.fw + 1 + 192 + e] for d in (-2, 0, 2) for e in (-2, 0, 2)],
+                   np.int32)
+    g = Grid(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff)
+    obs = [np.zeros((len(rec), w.nt + 1), np.float32) for _ in range(3)]
+    _note(f"oracle: visco404 ({w.nx}x{w.ny}x{w.nz}), L=3, {len(w.finv)} frequencies, NT={w.nt}")
+    ref = Oracle("f32").run_shot(g, *model, src, rec, w.finv, tast=w.tast, vppml=w.vppml, fpml=w.fpml, obs=obs,
+                                 visco=(taus, taup, etas))
+    G = _geom(ig, w)
+    d_model = [G.to_dev(a) for a in model]
+    d_tau = [G.to_dev(a) for a in (taus, taup)]
+    d_obs = ig.DBuf.from_numpy(np.stack(obs))
+    for prec_name, prec in (("compat", ig.DFT_COMPAT), ("kahan", ig.DFT_KAHAN)):
+        _note(f"plain path, {prec_name} DFT")
+        ctx = ig.ShotContext(G, len(w.finv), len(rec), prec)
+        ctx.set_cpml(w.vppml, w.fpml)
+        ctx.set_freqs(w.finv, w.tast)
+        ctx.set_visco(*d_tau, etas)
+        ctx.set_model(*d_model)
+        d_syn = ig.DBuf((3, len(rec), w.nt + 1))
+        grads = [G.empty() for _ in range(3)]
+        ctx.run_shot(src, rec, d_syn, grads, d_obs=d_obs)
+        ig.sync()
+        syn = d_syn.numpy()
+        for c in range(3):
+            assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0, f"{prec_name}: seismogram {c}"
+        assert np.abs(syn).max() > 0
+        for q in range(3):
+            a, want = g.interior(G.from_dev(grads[q])), g.interior(ref["grads"][q])
+            assert np.abs(want).max() > 0
+            if prec_name == "compat":
+                assert max_ulp(a, want) == 0, f"plain compat: gradient {q + 1}"
+            else:
+                assert rel_l2(a, want) <= 1e-4, f"plain kahan: gradient {q + 1}"
+        ctx.close()
+        del grads
+    _note("batched path, one shot, compat DFT")
+    b = ig.BatchContext(G, 1, len(w.finv), len(rec), ig.DFT_COMPAT, 4)
+    b.set_cpml(w.vppml, w.fpml)
+    b.set_freqs(w.finv, w.tast)
+    b.set_visco(*d_tau, etas)
+    b.set_model(*d_model)
+    grads = [G.empty() for _ in range(3)]
+    d_b = ig.DBuf((1, 3, len(rec), w.nt + 1))
+    b.run([src], rec, d_syn=d_b, grads=grads, d_obs=[d_obs])
+    ig.sync()
+    for q in range(3):
+        assert max_ulp(g.interior(G.from_dev(grads[q])), g.interior(ref["grads"][q])) == 0, f"batched: gradient {q + 1}"
+    b.close()

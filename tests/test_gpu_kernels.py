@@ -124,21 +124,23 @@ def test_stress_update_regions(env, free_surf, fdcoeff, dh, nz):
         assert np.array_equal(dpsi2[n].numpy(), po[n]), f"fused psi_v[{n}]"
 
 
-@pytest.mark.parametrize("free_surf,fdcoeff,nz", [(0, 2, 29), (1, 1, 29), (0, 2, 150)])
-def test_visco_stress_update_regions(env, free_surf, fdcoeff, nz):
-    """update_s / update_s_CPML (one relaxation mechanism), two steps so the
-    memory variables carry over; interior+CPML launches and the fused one"""
+@pytest.mark.parametrize("free_surf,fdcoeff,nz,L", [(0, 2, 29, 1), (1, 1, 29, 1), (0, 2, 150, 1), (0, 2, 29, 3),
+                                                     (1, 1, 29, 2), (0, 2, 150, 3), (0, 1, 70, 4)])
+def test_visco_stress_update_regions(env, free_surf, fdcoeff, nz, L):
+    """update_s / update_s_CPML with L relaxation mechanisms (L = 1: the
+    reference's), two steps so the memory variables carry over; interior+CPML
+    launches and the fused one"""
     _, ig = env
     g = small_grid(fdcoeff=fdcoeff, free_surf=free_surf, nx=30, ny=27, nz=nz)
-    _note(f"viscoelastic stress update parity fdcoeff={fdcoeff} free_surf={free_surf}")
+    _note(f"viscoelastic stress update parity fdcoeff={fdcoeff} free_surf={free_surf} L={L}")
     o, rng, rho, pi, u, avg, prof, f, psi = _setup(g, 31 + fdcoeff + 2 * free_surf)
     taus = (0.01 + 0.05 * rng.random(g.shape)).astype(np.float32)
     taup = (0.01 + 0.05 * rng.random(g.shape)).astype(np.float32)
     for a in (taus, taup):
         o.matcopy_wrap(g, a)
     tsa = o.av_tau(g, rho, u, taus)
-    r6 = [(1e-2 * rng.standard_normal(g.shape)).astype(np.float32) for _ in range(6)]
-    eta = np.float32(0.0137)
+    r6 = [(1e-2 * rng.standard_normal((L,) + g.shape)).astype(np.float32) for _ in range(6)]
+    eta = np.array([0.0137, 0.0051, 0.031, 0.0089][:L], np.float32)
     G = _geom(ig, g)
     # averages on the device
     dtaus = G.to_dev(taus)

@@ -116,23 +116,24 @@ def test_force_source_shot():
         assert max_ulp(syn[c][:, 1:], ref["syn"][c][:, 1:]) == 0
 
 
-@pytest.mark.parametrize("free_surf,nz", [(0, 34), (1, 34), (0, 84)])
-def test_viscoelastic_gradient_shot(free_surf, nz):
-    """the plain path with the viscoelastic update (one relaxation
-    mechanism) against the oracle's viscoelastic shot, bit for bit; at nz 84
-    the interior launches take the LDS-staged marches (70 cells deep in k)"""
+@pytest.mark.parametrize("free_surf,nz,L", [(0, 34, 1), (1, 34, 1), (0, 84, 1), (0, 34, 3), (1, 84, 3)])
+def test_viscoelastic_gradient_shot(free_surf, nz, L):
+    """the plain path with the viscoelastic update (L relaxation mechanisms;
+    the reference's L = 1) against the oracle's viscoelastic shot, bit for
+    bit; at nz 84 the interior launches take the LDS-staged marches (70
+    cells deep in k)"""
     import ifosgpu as ig
     g = small_grid(nx=32, ny=30, nz=nz, fw=7, nt=120 if nz < 64 else 80, free_surf=free_surf)
     src = (16, 11, 17, 0.0, 50.0, 1.0, 1)
     rec = np.array([[8 + 2 * n, 9, 17] for n in range(8)], np.int32)
     finv = [40.0, 70.0]
-    _note(f"viscoelastic gradient shot free_surf={free_surf}")
+    _note(f"viscoelastic gradient shot free_surf={free_surf} nz={nz} L={L}")
     o = Oracle("f32")
     start, true = _models(g, 3)
     rng = np.random.default_rng(7)
     taus = (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32)
     taup = (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32)
-    eta = np.float32(0.021)
+    eta = np.array([0.021, 0.0063, 0.047][:L], np.float32)
     vis = (taus, taup, eta)
     obs = o.run_shot(g, *true, src, rec, finv, want_adjoint=False, visco=vis)["syn"]
     ref = o.run_shot(g, *start, src, rec, finv, obs=obs, visco=vis)

@@ -43,7 +43,8 @@ def _models(g, seed):
 CASES = {
     "elastic": dict(free_surf=0, visco=False, every=(0, 1, 6, 7, 48)),
     "free_surface": dict(free_surf=1, visco=False, every=(5,)),
-    "viscoelastic": dict(free_surf=0, visco=True, every=(0, 11)),
+    "viscoelastic": dict(free_surf=0, visco=1, every=(0, 11)),
+    "viscoelastic_3": dict(free_surf=0, visco=3, every=(0, 7)),
 }
 
 
@@ -60,7 +61,8 @@ def test_td_gradient_matches_oracle(case):
     if c["visco"]:
         rng = np.random.default_rng(9)
         vis = ((0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
-               (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32), np.float32(0.021))
+               (0.02 + 0.03 * rng.random(g.shape)).astype(np.float32),
+               np.array([0.021, 0.0063, 0.047][:c["visco"]], np.float32))
     obs = o.run_shot(g, *true, src, rec, [40.0], want_adjoint=False, visco=vis)["syn"]
     ref = o.run_shot(g, *start, src, rec, [40.0], obs=obs, visco=vis, want_history=True)
     want = [o.zeros(g) for _ in range(3)]
