@@ -695,17 +695,21 @@ extern "C" int ifos_modelupdate(const ifos_geom *g, float *const d_grad[3], floa
     return rc;
 }
 
-// steplength.c:26-91 on the host, in the reference's float arithmetic
-extern "C" int ifos_steplength(float L2[4], float step[5], int lbfgs, int it_group, float teststep)
+// steplength.c:26-91: the parabola through (step[0..2], L2[0..2]) in T.
+// The literals are double as in the reference, so T = float reproduces its
+// float/double promotions (0.1 * step[1] formed in double, stored to float);
+// T = double is the FP64 fit.
+namespace ifos {
+template <class T>
+__host__ __device__ inline void steplength_t(T L2[4], T step[5], int lbfgs, int it_group, T teststep)
 {
-    if (!L2 || !step) return IFOS_EINVAL;
-    float a = ((step[0] - step[1]) * (L2[0] - L2[2]) - (step[0] - step[2]) * (L2[0] - L2[1])) /
-              (step[0] * step[2] * (step[0] - step[2]) + step[0] * step[1] * (step[1] - step[0]) +
-               step[1] * step[2] * (step[2] - step[1]));
-    float b = (L2[0] - L2[1]) / (step[0] - step[1]);
+    T a = ((step[0] - step[1]) * (L2[0] - L2[2]) - (step[0] - step[2]) * (L2[0] - L2[1])) /
+          (step[0] * step[2] * (step[0] - step[2]) + step[0] * step[1] * (step[1] - step[0]) +
+           step[1] * step[2] * (step[2] - step[1]));
+    T b = (L2[0] - L2[1]) / (step[0] - step[1]);
     b = b - a * (step[0] + step[1]);
-    const float c = L2[0] - a * step[0] * step[0] - b * step[0];
-    step[3] = 0.0f;
+    const T c = L2[0] - a * step[0] * step[0] - b * step[0];
+    step[3] = 0;
     if (a > 0) step[3] = -b / (2 * a);
     if (a < 0) {
         if (L2[0] < L2[1] && L2[0] < L2[2]) step[3] = 0.1 * step[1];
@@ -721,5 +725,31 @@ extern "C" int ifos_steplength(float L2[4], float step[5], int lbfgs, int it_gro
     if (step[4] < 0.2 * teststep) step[4] = 0.2 * teststep;
     if (step[4] > teststep) step[4] = teststep;
     if (lbfgs && it_group > 1) step[4] = 0.5;
+}
+
+// one thread: the FP64 fit where the FP64 misfits already are
+__global__ void k_steplength64(double *L2, double *step, int lbfgs, int it_group, double teststep)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) steplength_t<double>(L2, step, lbfgs, it_group, teststep);
+}
+}  // namespace ifos
+
+// steplength.c:26-91 on the host, in the reference's float arithmetic
+extern "C" int ifos_steplength(float L2[4], float step[5], int lbfgs, int it_group, float teststep)
+{
+    if (!L2 || !step) return IFOS_EINVAL;
+    ifos::steplength_t<float>(L2, step, lbfgs, it_group, teststep);
     return IFOS_OK;
+}
+
+extern "C" int ifos_steplength_f64(double *d_L2, double *d_step, int lbfgs, int it_group, double teststep,
+                                   void *stream)
+{
+    if (!d_L2 || !d_step) {
+        set_error("ifos_steplength_f64: null argument");
+        return IFOS_EINVAL;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(ifos::k_steplength64, dim3(1), dim3(64), 0, st, d_L2, d_step, lbfgs, it_group, teststep);
+    return hip_check(hipGetLastError(), "k_steplength64");
 }

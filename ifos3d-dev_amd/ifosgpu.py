@@ -592,6 +592,22 @@ def steplength(L2, step, lbfgs, it_group, teststep):
     return np.array(l2[:], np.float32), np.array(st[:], np.float32)
 
 
+def steplength_f64(L2, step, lbfgs, it_group, teststep, stream=None):
+    """the FP64 parabola fit on the device (ifos_steplength_f64): L2[0..2],
+    step[0..4] in; returns (L2[4], step[5]) as float64 after the call"""
+    l2 = np.zeros(4, np.float64)
+    l2[:min(len(L2), 4)] = np.asarray(L2, np.float64)[:4]
+    st = np.zeros(5, np.float64)
+    st[:min(len(step), 5)] = np.asarray(step, np.float64)[:5]
+    d_l2, d_st = DBuf.from_numpy(l2), DBuf.from_numpy(st)
+    check(lib().ifos_steplength_f64(C.c_void_p(d_l2.ptr()), C.c_void_p(d_st.ptr()), int(lbfgs), int(it_group), C.c_double(teststep), stream),
+          "ifos_steplength_f64")
+    out = d_l2.numpy(), d_st.numpy()
+    d_l2.free()
+    d_st.free()
+    return out
+
+
 class BatchContext:
     """The batched path: several shots of one model per launch (ifos_batch_*)."""
 

@@ -470,6 +470,11 @@ int ifos_modelupdate(const ifos_geom *g, float *const d_grad[3], float *d_rho, f
  * step[3] and L2[3] receive its clamped minimum, step[4] the next test
  * step (TESTSTEP = teststep) */
 int ifos_steplength(float L2[4], float step[5], int lbfgs, int it_group, float teststep);
+/* the same parabola fit in FP64 on the device (no counterpart in the
+ * reference, whose steplength.c:26-91 is float): d_L2[4] and d_step[5] are
+ * device doubles, typically the FP64 misfits of the test steps; one thread on
+ * `stream`, no host round trip */
+int ifos_steplength_f64(double *d_L2, double *d_step, int lbfgs, int it_group, double teststep, void *stream);
 
 #ifdef __cplusplus
 }

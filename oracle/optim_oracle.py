@@ -221,9 +221,11 @@ def modelupdate(grads, rho, pi, u, step, weight, vp0, vs0, rho0, lbfgs, numpar=2
     return maxima, mod
 
 
-def steplength(L2, step, lbfgs, it_group, teststep):
-    """float32 scalars as steplength.c computes them; returns (L2[4], step[5])"""
-    f = np.float32
+def steplength(L2, step, lbfgs, it_group, teststep, dtype=np.float32):
+    """steplength.c:26-91; with float32 (the reference's arithmetic: the
+    double literals promote, the results are stored back to float) or
+    float64 throughout (the FP64 fit); returns (L2[4], step[5])"""
+    f = dtype
     L2 = [f(x) for x in L2[:3]] + [f(0)]
     s = [f(x) for x in step[:5]]
     a = ((s[0] - s[1]) * (L2[0] - L2[2]) - (s[0] - s[2]) * (L2[0] - L2[1])) / \
@@ -257,4 +259,4 @@ def steplength(L2, step, lbfgs, it_group, teststep):
         s[4] = f(teststep)
     if lbfgs and it_group > 1:
         s[4] = f(0.5)
-    return np.array(L2, f), np.array(s, f)
+    return np.array(L2, dtype), np.array(s, dtype)

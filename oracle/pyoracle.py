@@ -518,3 +518,74 @@ class Ref:
         o = [np.ascontiguousarray(x, np.float32) for x in obs] if obs is not None else None
         ptrs = [_p(x) for x in o] if o is not None else [None, None, None]
         return float(self.lib.rg_run_shot(shape, tast, *ptrs, int(want_adjoint)))
+
+
+class RefOptim:
+    """The reference's optimiser routines (lbfgs.c, lbfgs_save.c,
+    conjugategrad.c, modelupdate.c, precongrad.c, steplength.c) compiled
+    from its sources (oracle/_ref), on compact interior arrays [ny][nx][nz]
+    (float32, the reference's l order)."""
+
+    def __init__(self, ny, nx, nz):
+        self.lib = C.CDLL(REFLIB)
+        self.shape = (ny, nx, nz)
+
+    def _c(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.shape == self.shape, a.shape
+        return a.copy()
+
+    def savegrad(self, g1, g2, numpar):
+        ny, nx, nz = self.shape
+        out = np.zeros(numpar * ny * nx * nz, np.float32)
+        self.lib.rg_lbfgs_savegrad(nx, ny, nz, numpar, _p(self._c(g1)), _p(self._c(g2)), _p(out))
+        return out
+
+    def lbfgs(self, g1, g2, scale, mod, hist, iteration, numpar):
+        """returns (direction (g1, g2), scale, hist) after lbfgs.c"""
+        ny, nx, nz = self.shape
+        a, b = self._c(g1), self._c(g2)
+        sc = np.ascontiguousarray(scale, np.float32).copy()
+        md = np.ascontiguousarray(mod, np.float32)
+        hs = np.ascontiguousarray(hist, np.float32).copy()
+        self.lib.rg_lbfgs(nx, ny, nz, numpar, len(sc), _p(a), _p(b), _p(sc), _p(md), _p(hs), iteration)
+        return (a, b), sc, hs
+
+    def conjugate(self, grads, priors, iteration, cdf):
+        ny, nx, nz = self.shape
+        g = [self._c(x) for x in grads]
+        p = [self._c(x) for x in priors]
+        pp = (C.c_void_p * 6)(*[x.ctypes.data for x in p])
+        beta = np.zeros(3, np.float32)
+        self.lib.rg_conjugate(nx, ny, nz, _p(g[0]), _p(g[1]), _p(g[2]), pp, _p(beta), iteration, cdf)
+        return g, p, beta
+
+    def modelupdate(self, grads, rho, pi, u, mod, step, it_group, lbfgs, numpar, weight, vp0, vs0, rho0):
+        ny, nx, nz = self.shape
+        g = [self._c(x) for x in grads]
+        m = [self._c(x) for x in (rho, pi, u)]
+        bfgsnum = mod.shape[0] if mod is not None else 1
+        md = np.ascontiguousarray(mod if mod is not None else np.zeros((1, 1)), np.float32).copy()
+        w = np.asarray(weight, np.float32)
+        self.lib.rg_modelupdate(nx, ny, nz, _p(g[0]), _p(g[1]), _p(g[2]), _p(m[0]), _p(m[1]), _p(m[2]), _p(md),
+                                C.c_float(step), it_group, int(lbfgs), numpar, bfgsnum, _p(w), C.c_float(vp0),
+                                C.c_float(vs0), C.c_float(rho0))
+        return g, m, (md if mod is not None else None)
+
+    def precon_grad(self, grads, src_m, rec_xyz, dh, fw, damptype, free_surf):
+        """src_m: [nsrc][3] source positions in metres; rec_xyz: [ntr][3] grid indices"""
+        ny, nx, nz = self.shape
+        g = [self._c(x) for x in grads]
+        s = np.ascontiguousarray(np.asarray(src_m, np.float32).reshape(-1, 3))
+        r = np.ascontiguousarray(np.asarray(rec_xyz, np.int32).reshape(-1, 3))
+        self.lib.rg_precon_grad(nx, ny, nz, _p(g[0]), _p(g[1]), _p(g[2]), len(s), _p(s), len(r), _p(r),
+                                C.c_float(dh[0]), C.c_float(dh[1]), C.c_float(dh[2]), fw, damptype, free_surf)
+        return g
+
+    def steplength(self, L2, step, iteration, it_group, lbfgs, teststep):
+        l2 = np.zeros(4, np.float32)
+        l2[:len(L2)] = L2
+        st = np.zeros(5, np.float32)
+        st[:len(step)] = step
+        self.lib.rg_steplength(_p(l2), _p(st), iteration, it_group, int(lbfgs), C.c_float(teststep))
+        return l2, st

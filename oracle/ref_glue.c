@@ -592,3 +592,181 @@ void rg_readseis(const char *prefix, int ishot, int comp, int ntr, int ns, float
     free_matrix(section, 1, ntr, 1, ns);
     free_matrix(sectionf, 1, ntr, 1, ns);
 }
+
+
+/* ---- optimiser routines (lbfgs.c, lbfgs_save.c, conjugategrad.c,
+ * modelupdate.c, precongrad.c, steplength.c) on compact interior arrays
+ * [ny][nx][nz] (the reference's l order: j outer, i, k inner) ---------- */
+
+static float ***cube_in(int ny, int nx, int nz, const float *a)
+{
+    float ***t = f3tensor(1, ny, 1, nx, 1, nz);
+    int j, i, k;
+    size_t l = 0;
+    for (j = 1; j <= ny; j++)
+        for (i = 1; i <= nx; i++)
+            for (k = 1; k <= nz; k++) t[j][i][k] = a ? a[l++] : 0.0f;
+    return t;
+}
+
+/* copies the cube to a (when a is not NULL) and frees it */
+static void cube_out(float ***t, int ny, int nx, int nz, float *a)
+{
+    int j, i, k;
+    size_t l = 0;
+    if (a)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                for (k = 1; k <= nz; k++) a[l++] = t[j][i][k];
+    free_f3tensor(t, 1, ny, 1, nx, 1, nz);
+}
+
+static void opt_globals(int nx, int ny, int nz)
+{
+    int flag = 0;
+    MPI_Initialized(&flag);
+    if (!flag) MPI_Init(NULL, NULL);
+    NX = nx; NY = ny; NZ = nz; NXG = nx; NYG = ny; NZG = nz;
+    POS[1] = POS[2] = POS[3] = 0;
+    MYID = 0;
+    FP = stderr;
+    FI = stderr;
+}
+
+/* lbfgs_savegrad: out [numpar * n] <- bfgsgrad1[1][1..numpar * n] */
+void rg_lbfgs_savegrad(int nx, int ny, int nz, int numpar, const float *g1, const float *g2, float *out)
+{
+    const int n = nx * ny * nz;
+    float ***a, ***b, ***c, **bg;
+    int l;
+    opt_globals(nx, ny, nz);
+    NUMPAR = numpar;
+    a = cube_in(ny, nx, nz, g1);
+    b = cube_in(ny, nx, nz, g2);
+    c = cube_in(ny, nx, nz, NULL);
+    bg = fmatrix(1, 1, 1, numpar * n);
+    lbfgs_savegrad(a, b, c, bg);
+    for (l = 0; l < numpar * n; l++) out[l] = bg[1][l + 1];
+    free_matrix(bg, 1, 1, 1, numpar * n);
+    cube_out(a, ny, nx, nz, NULL);
+    cube_out(b, ny, nx, nz, NULL);
+    cube_out(c, ny, nx, nz, NULL);
+}
+
+/* lbfgs: grad1 (, grad2) in/out (the direction on return); scale
+ * [bfgsnum] in/out (bfgsscale[1..bfgsnum]); mod [bfgsnum][numpar * n] in
+ * (bfgsmod[1..bfgsnum]); hist [bfgsnum][numpar * n] in/out (bfgsgrad) */
+void rg_lbfgs(int nx, int ny, int nz, int numpar, int bfgsnum, float *g1, float *g2, float *scale,
+              const float *mod, float *hist, int iteration)
+{
+    const int n = nx * ny * nz, m = numpar * n;
+    float ***a, ***b, ***c, **bm, **bg, *bs;
+    int w, l;
+    opt_globals(nx, ny, nz);
+    NUMPAR = numpar;
+    BFGSNUM = bfgsnum;
+    a = cube_in(ny, nx, nz, g1);
+    b = cube_in(ny, nx, nz, g2);
+    c = cube_in(ny, nx, nz, NULL);
+    bm = fmatrix(1, bfgsnum, 1, m);
+    bg = fmatrix(1, bfgsnum, 1, m);
+    bs = vector(1, bfgsnum);
+    for (w = 1; w <= bfgsnum; w++) {
+        bs[w] = scale[w - 1];
+        for (l = 1; l <= m; l++) {
+            bm[w][l] = mod[(size_t)(w - 1) * m + l - 1];
+            bg[w][l] = hist[(size_t)(w - 1) * m + l - 1];
+        }
+    }
+    lbfgs(a, b, c, bs, bm, bg, iteration);
+    for (w = 1; w <= bfgsnum; w++) {
+        scale[w - 1] = bs[w];
+        for (l = 1; l <= m; l++) hist[(size_t)(w - 1) * m + l - 1] = bg[w][l];
+    }
+    free_vector(bs, 1, bfgsnum);
+    free_matrix(bm, 1, bfgsnum, 1, m);
+    free_matrix(bg, 1, bfgsnum, 1, m);
+    cube_out(a, ny, nx, nz, g1);
+    cube_out(b, ny, nx, nz, numpar > 1 ? g2 : NULL);
+    cube_out(c, ny, nx, nz, NULL);
+}
+
+/* conjugate: g[3] in/out, prior[6] in/out, beta[3] out */
+void rg_conjugate(int nx, int ny, int nz, float *g1, float *g2, float *g3, float *const prior[6], float *beta,
+                  int iteration, int cdf)
+{
+    float ***G[3], ***P[6];
+    float *gp[3] = {g1, g2, g3};
+    int q;
+    opt_globals(nx, ny, nz);
+    for (q = 0; q < 3; q++) G[q] = cube_in(ny, nx, nz, gp[q]);
+    for (q = 0; q < 6; q++) P[q] = cube_in(ny, nx, nz, prior[q]);
+    conjugate(nx, ny, nz, G[0], G[1], G[2], P[0], P[1], P[2], P[3], P[4], P[5], beta, iteration, cdf);
+    for (q = 0; q < 3; q++) cube_out(G[q], ny, nx, nz, gp[q]);
+    for (q = 0; q < 6; q++) cube_out(P[q], ny, nx, nz, prior[q]);
+}
+
+/* modelupdate: grads (gradvp, gradvs, gradrho) in/out (normalised unless
+ * lbfgs), rho/pi/u in/out; with lbfgs, mod [bfgsnum][numpar * n] in/out
+ * (bfgsmod1[1..bfgsnum], slot it_group % bfgsnum written) */
+void rg_modelupdate(int nx, int ny, int nz, float *g1, float *g2, float *g3, float *rho, float *pi, float *u,
+                    float *mod, float step, int it_group, int lbfgs, int numpar, int bfgsnum, const float *weight,
+                    float vp0, float vs0, float rho0)
+{
+    const int n = nx * ny * nz, m = numpar * n;
+    float ***G[3], ***M[3], **bm = NULL, beta[3] = {0.0f, 0.0f, 0.0f};
+    float *gp[3] = {g1, g2, g3}, *mp[3] = {rho, pi, u};
+    int q, w, l;
+    opt_globals(nx, ny, nz);
+    LBFGS = lbfgs;
+    NUMPAR = numpar;
+    BFGSNUM = bfgsnum;
+    WEIGHT[0] = weight[0]; WEIGHT[1] = weight[1]; WEIGHT[2] = weight[2];
+    VP0 = vp0; VS0 = vs0; RHO0 = rho0;
+    for (q = 0; q < 3; q++) G[q] = cube_in(ny, nx, nz, gp[q]);
+    for (q = 0; q < 3; q++) M[q] = cube_in(ny, nx, nz, mp[q]);
+    if (lbfgs) {
+        bm = fmatrix(1, bfgsnum, 1, 2 * n);
+        for (w = 1; w <= bfgsnum; w++)
+            for (l = 1; l <= m; l++) bm[w][l] = mod[(size_t)(w - 1) * m + l - 1];
+    }
+    modelupdate(nx, ny, nz, G[0], G[1], G[2], M[0], M[1], M[2], bm, step, beta, it_group);
+    if (lbfgs) {
+        for (w = 1; w <= bfgsnum; w++)
+            for (l = 1; l <= m; l++) mod[(size_t)(w - 1) * m + l - 1] = bm[w][l];
+        free_matrix(bm, 1, bfgsnum, 1, 2 * n);
+    }
+    for (q = 0; q < 3; q++) cube_out(G[q], ny, nx, nz, gp[q]);
+    for (q = 0; q < 3; q++) cube_out(M[q], ny, nx, nz, mp[q]);
+}
+
+/* precon_grad: grads in/out; src [nsrc][3] positions in metres (x, y, z,
+ * srcpos[1..3]), rec [ntr][3] grid indices (x, y, z, recpos[1..3]) */
+void rg_precon_grad(int nx, int ny, int nz, float *g1, float *g2, float *g3, int nsrc, const float *src, int ntr,
+                    const int *rec, float dx, float dy, float dz, int fw, int damptype, int free_surf)
+{
+    float ***G[3], **srcpos = fmatrix(1, 7, 1, nsrc > 0 ? nsrc : 1);
+    int **recpos = imatrix(1, 3, 1, ntr > 0 ? ntr : 1);
+    float *gp[3] = {g1, g2, g3};
+    int q, t;
+    opt_globals(nx, ny, nz);
+    DX = dx; DY = dy; DZ = dz; FW = fw; DAMPTYPE = damptype; FREE_SURF = free_surf;
+    for (t = 1; t <= nsrc; t++)
+        for (q = 1; q <= 3; q++) srcpos[q][t] = src[3 * (t - 1) + q - 1];
+    for (t = 1; t <= ntr; t++)
+        for (q = 1; q <= 3; q++) recpos[q][t] = rec[3 * (t - 1) + q - 1];
+    for (q = 0; q < 3; q++) G[q] = cube_in(ny, nx, nz, gp[q]);
+    precon_grad(nx, ny, nz, G[0], G[1], G[2], nsrc, srcpos, ntr, recpos, 10.0f, 1, 0);
+    for (q = 0; q < 3; q++) cube_out(G[q], ny, nx, nz, gp[q]);
+    free_matrix(srcpos, 1, 7, 1, nsrc > 0 ? nsrc : 1);
+    free_imatrix(recpos, 1, 3, 1, ntr > 0 ? ntr : 1);
+}
+
+/* steplength: L2[4] and step[5] in/out */
+void rg_steplength(float *L2, float *step, int iteration, int it_group, int lbfgs, float teststep)
+{
+    opt_globals(1, 1, 1);
+    LBFGS = lbfgs;
+    TESTSTEP = teststep;
+    steplength(L2, step, iteration, it_group);
+}

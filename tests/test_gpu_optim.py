@@ -172,3 +172,28 @@ def test_modelupdate_matches_oracle(lbfgs, numpar):
         assert _max_ulp(g.interior(G.from_dev(d[n])), og[n]) == 0, f"grad{n + 1}"
     if lbfgs:
         assert _max_ulp(mod.numpy(), mod_o) == 0, "bfgsmod"
+
+
+def test_steplength_f64_matches_oracle():
+    """ifos_steplength_f64 (one device thread, FP64) against the FP64 oracle
+    bit for bit, and the float entry point against the reference's own
+    steplength.c outputs (tests/golden/optim_golden.npz)"""
+    import ifosgpu as ig
+    from optim_oracle import steplength
+    cases = [([3.0, 1.0, 2.0], [0, 0.5, 1.0, 0, 1.0], 0, 1),
+             ([1.0, 3.0, 2.0], [0, 0.5, 1.0, 0, 1.0], 0, 1),
+             ([2.0, 1.0, 3.5], [0, 0.5, 1.0, 0, 0.1], 0, 1),
+             ([5.0, 2.0, 1.0], [0, 0.5, 1.0, 0, 4.0], 1, 3),
+             ([2.0, 3.0, 1.0], [0, 0.5, 1.0, 0, 1.0], 0, 1),
+             ([1.2345678901, 0.98765432, 1.1111111], [0, 0.013, 0.026, 0, 0.02], 0, 1)]
+    for L2, st, lb, it in cases:
+        want = steplength(L2, st, lb, it, 0.8, dtype=np.float64)
+        got = ig.steplength_f64(L2, st, lb, it, 0.8)
+        np.testing.assert_array_equal(got[1], want[1])
+        np.testing.assert_array_equal(got[0][3:], want[0][3:])
+    F = np.load(os.path.join(os.path.dirname(__file__), "golden", "optim_golden.npz"), allow_pickle=False)
+    for c in range(5):
+        x = F[f"step_{c}_in"]
+        l2, s = ig.steplength(x[:3], x[3:8], int(x[8]), int(x[9]), float(x[10]))
+        np.testing.assert_array_equal(s, F[f"step_{c}_step"])
+        np.testing.assert_array_equal(l2[3:], F[f"step_{c}_L2"][3:])

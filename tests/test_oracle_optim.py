@@ -132,6 +132,33 @@ def test_steplength_branches_match_the_library():
         np.testing.assert_array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("alpha,smin", [(2.0, 0.3), (0.5, 0.9), (7.0, 0.05), (1e-3, 0.7)])
+def test_steplength_fp64_parabola_minimum(alpha, smin):
+    """the FP64 fit: three points of L2 = alpha (s - smin)^2 + 3 (exact in
+    double at these steps) give the minimum smin and L2[3] = 3 to the double
+    rounding of the fit, where the float fit is off by float rounding"""
+    st = [0.0, 0.5, 1.0, 0.0, 1.0]
+    L2 = [alpha * (s - smin) ** 2 + 3.0 for s in st[:3]]
+    l2, s = steplength(L2, st, 0, 1, 1.0, dtype=np.float64)
+    assert s[3] == pytest.approx(smin, rel=1e-12, abs=1e-15)
+    assert l2[3] == pytest.approx(3.0, rel=1e-13)
+    l2f, sf = steplength(L2, st, 0, 1, 1.0)
+    assert abs(float(sf[3]) - smin) <= 1e-5 * max(smin, 1.0) * (1.0 + 1.0 / alpha)
+
+
+def test_steplength_fp64_branches_follow_the_float_fit():
+    """the branch structure is the reference's: on the clamp cases the FP64
+    fit lands where the float fit does, to float rounding"""
+    cases = [([1.0, 3.0, 2.0], [0, 0.5, 1.0, 0, 1.0], 0, 1),
+             ([2.0, 1.0, 3.5], [0, 0.5, 1.0, 0, 0.1], 0, 1),
+             ([5.0, 2.0, 1.0], [0, 0.5, 1.0, 0, 4.0], 1, 3),
+             ([2.0, 3.0, 1.0], [0, 0.5, 1.0, 0, 1.0], 0, 1)]
+    for L2, st, lb, it in cases:
+        a = steplength(L2, st, lb, it, 0.8)
+        b = steplength(L2, st, lb, it, 0.8, dtype=np.float64)
+        np.testing.assert_allclose(b[1], a[1], rtol=1e-6)
+
+
 def _grads(shape, seed=3):
     rng = np.random.default_rng(seed)
     return [rng.standard_normal(shape).astype(np.float32) for _ in range(3)]
