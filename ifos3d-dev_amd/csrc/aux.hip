@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <vector>
+#include <cstring>
 
 #include "ifos_dev.h"
 
@@ -681,6 +682,87 @@ extern "C" int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *
                        d_rjp, d_rkp, d_rip);
     }
     return hip_check(hipGetLastError(), "k_av_mat");
+}
+
+namespace ifos {
+// largest phase velocity at infinite frequency (checkfd_ssg.c:79-122): per
+// cell c = sqrt(u (1.0 + L taus) / rho) and sqrt(pi (1.0 + L taup) / rho)
+// formed as the C does (L * tau in float, the rest in double, c stored to
+// float); cells with rho == 0 are skipped (the reference prints them and
+// keeps the previous c).  Maxima of non-negative floats compare as their bit
+// patterns, so one unsigned max per workgroup goes to part[blockIdx.x].
+__global__ __launch_bounds__(256) void k_cmax(DevGeom g, const float *rho, const float *pi, const float *u,
+                                              const float *taup, const float *taus, int L, unsigned *part)
+{
+    __shared__ unsigned red[256];
+    unsigned m = 0u;
+    for (int64_t r = blockIdx.x; r < (int64_t)g.nx * g.ny; r += gridDim.x) {
+        const int j = (int)(r / g.nx) + 1, i = (int)(r % g.nx) + 1;
+        for (int k = 1 + (int)threadIdx.x; k <= g.nz; k += blockDim.x) {
+            const int64_t c = at(g, j, i, k);
+            const float rr = rho[c];
+            if (rr == 0.0f) continue;
+            float cs, cp;
+            if (L) {
+                cs = (float)sqrt((double)u[c] * (1.0 + (double)((float)L * taus[c])) / (double)rr);
+                cp = (float)sqrt((double)pi[c] * (1.0 + (double)((float)L * taup[c])) / (double)rr);
+            } else {
+                cs = (float)sqrt((double)(u[c] / rr));
+                cp = (float)sqrt((double)(pi[c] / rr));
+            }
+            m = max(m, max(__float_as_uint(cs), __float_as_uint(cp)));
+        }
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+}  // namespace ifos
+
+extern "C" int ifos_check_stability(const ifos_geom *g, const float *d_rho, const float *d_pi, const float *d_u,
+                                    const float *d_taup, const float *d_taus, int L, float *cmax, float *dt_stab,
+                                    void *stream)
+{
+    if (!g || !d_rho || !d_pi || !d_u || (L && (!d_taup || !d_taus)) || L < 0) {
+        set_error("ifos_check_stability: null model array (taup, taus needed when L > 0)");
+        return IFOS_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t rows = (int64_t)g->nx * g->ny;
+    const unsigned blocks = (unsigned)std::min<int64_t>(rows, 1024);
+    unsigned *part = nullptr;
+    if (hip_check(hipMallocAsync((void **)&part, sizeof(unsigned) * blocks, s), "hipMallocAsync")) return IFOS_EHIP;
+    hipLaunchKernelGGL(k_cmax, dim3(blocks), dim3(256), 0, s, make_dev_geom(g), d_rho, d_pi, d_u, d_taup, d_taus,
+                       L, part);
+    std::vector<unsigned> h(blocks);
+    int rc = hip_check(hipGetLastError(), "k_cmax");
+    if (!rc) rc = hip_check(hipMemcpyAsync(h.data(), part, sizeof(unsigned) * blocks, hipMemcpyDeviceToHost, s),
+                            "hipMemcpyAsync");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    (void)hipFreeAsync(part, s);
+    if (rc) return rc;
+    unsigned mb = 0u;
+    for (unsigned v : h) mb = std::max(mb, v);
+    float cm;
+    memcpy(&cm, &mb, 4);
+    // checkfd_ssg.c:475-478 (FDORDER 4), :492-494, :505
+    const float gamma = 1.184614f;
+    float dhmin = g->dx;
+    if (g->dy < dhmin) dhmin = g->dy;
+    if (g->dz < dhmin) dhmin = g->dz;
+    const float dtstab = (float)(dhmin / (gamma * sqrt(3.0) * cm));
+    if (cmax) *cmax = cm;
+    if (dt_stab) *dt_stab = dtstab;
+    if (g->dt > dtstab) {
+        set_error("the simulation will get unstable, choose a smaller DT: DT = %e s, stability limit %e s "
+                  "(cmax %.2f m/s)", (double)g->dt, (double)dtstab, (double)cm);
+        return IFOS_EUNSTABLE;
+    }
+    return IFOS_OK;
 }
 
 extern "C" int ifos_residual(const ifos_geom *g, const float *d_obs, const float *d_syn, float *d_adj, int ntr,

@@ -84,7 +84,15 @@ class IfosShot(C.Structure):
 
 
 class IfosError(RuntimeError):
-    pass
+    """an IFOS_E* return code (`code`) with the library's message"""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+# return codes of include/ifos3d_hip.h
+OK, EINVAL, EHIP, ENAN, ENOMEM, EUNSTABLE = 0, -1, -2, -3, -4, -5
 
 
 _lib = None
@@ -111,7 +119,7 @@ def lib() -> C.CDLL:
 def check(rc: int, what: str = ""):
     if rc != 0:
         msg = lib().ifos_last_error().decode()
-        raise IfosError(f"{what}: error {rc}: {msg}")
+        raise IfosError(f"{what}: error {rc}: {msg}", rc)
 
 
 def device_count() -> int:
@@ -590,6 +598,17 @@ def steplength(L2, step, lbfgs, it_group, teststep):
     st = (C.c_float * 5)(*[float(x) for x in step[:5]] + [0.0] * (5 - len(step[:5])))
     check(lib().ifos_steplength(l2, st, int(lbfgs), int(it_group), C.c_float(teststep)), "ifos_steplength")
     return np.array(l2[:], np.float32), np.array(st[:], np.float32)
+
+
+def check_stability(g: "Geom", rho, pi, u, taup=None, taus=None, L=0, stream=None):
+    """checkfd_ssg.c's stability test (ifos_check_stability) on device model
+    arrays: returns (code, cmax, dt_stab) -- code OK or EUNSTABLE"""
+    cm, ds = C.c_float(0.0), C.c_float(0.0)
+    rc = lib().ifos_check_stability(g.ref, _ptr(rho), _ptr(pi), _ptr(u), _ptr(taup), _ptr(taus), int(L),
+                                    C.byref(cm), C.byref(ds), stream)
+    if rc not in (OK, EUNSTABLE):
+        check(rc, "ifos_check_stability")
+    return rc, cm.value, ds.value
 
 
 def steplength_f64(L2, step, lbfgs, it_group, teststep, stream=None):

@@ -41,6 +41,7 @@ enum {
     IFOS_EHIP = -2,     /* a HIP runtime call failed */
     IFOS_ENAN = -3,     /* simulation unstable: NaN at the probe cell */
     IFOS_ENOMEM = -4,
+    IFOS_EUNSTABLE = -5, /* DT above the stability limit (checkfd_ssg.c:505-508) */
 };
 
 /* Regions of the velocity/stress updates.  INTERIOR is the box of
@@ -239,6 +240,16 @@ int ifos_su_read(const char *path, int ntr, int ns, float *d_section);
  * 0. */
 int ifos_su_write(const char *path, const float *d_section, int ntr, int ns, const int *rec_xyz,
                   const int *rec_id, const float *src_xyz, float dx, float dy, float dz, float dt, int ndt);
+/* checkfd_ssg.c:77-158 and :473-510: *cmax = the largest phase velocity at
+ * infinite frequency over [1..N]^3, max of sqrt(pi (1 + L taup) / rho) and
+ * sqrt(u (1 + L taus) / rho) (without the L terms when L = 0; d_taup and
+ * d_taus may then be NULL), in the reference's float/double mix; *dt_stab =
+ * min(DX, DY, DZ) / (gamma sqrt(3) cmax) with the 4th-order Holberg gamma
+ * 1.184614.  Returns IFOS_EUNSTABLE when g->dt > *dt_stab (the reference
+ * aborts there), IFOS_OK otherwise; cmax / dt_stab may be NULL. */
+int ifos_check_stability(const ifos_geom *g, const float *d_rho, const float *d_pi, const float *d_u,
+                         const float *d_taup, const float *d_taus, int L, float *cmax, float *dt_stab,
+                         void *stream);
 /* av_mat.c:28-80: writes the six averaged arrays of *out */
 int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *d_u,
                 float *d_uipjp, float *d_ujpkp, float *d_uipkp, float *d_rjp,
