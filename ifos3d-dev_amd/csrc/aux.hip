@@ -9,8 +9,9 @@
 #include <stdio.h>
 
 #include <cmath>
-#include <vector>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "ifos_dev.h"
 
@@ -780,11 +781,86 @@ extern "C" int ifos_residual(const ifos_geom *g, const float *d_obs, const float
     int rc = hip_check(hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * ntr, hipMemcpyDeviceToHost, s),
                        "hipMemcpyAsync");
     if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    hipFreeAsync(part, s);
+    (void)hipFreeAsync(part, s);
     if (rc) return rc;
     double acc = 0.0;
     for (double v : h) acc += v;
     if (l2) *l2 += acc;
+    return IFOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ifos_residual without device allocation.  The per-trace FP64 partial sums
+// and their total live in a file-scope device array: part[0 .. 3 ntr) and the
+// total at the fixed slot kNoallocTotal.  k_residual writes every partial that
+// k_residual_total then reads, so nothing has to start at zero.  Each device
+// holds its own copy for the life of the process; every call on a device
+// shares it, so calls that overlap on one device race.
+constexpr int kNoallocTotal = 3 * IFOS_RESIDUAL_NOALLOC_MAX_TRACES;
+static __device__ __align__(256) double scratch[kNoallocTotal + 1];
+
+namespace {
+
+// the partials added in index order, the order ifos_residual's host loop uses
+__global__ void k_residual_total(const double *part, int n, double *total)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (int t = 0; t < n; ++t) acc += part[t];
+    *total = acc;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// the device address of this device's `scratch`, looked up once per device
+int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("ifos_residual_noalloc: device %d out of range", dev);
+        return IFOS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return IFOS_OK;
+}
+}  // namespace
+
+extern "C" int ifos_residual_noalloc(const ifos_geom *g, const float *d_obs, const float *d_syn, float *d_adj,
+                                     int ntr, int ns, double *l2, void *stream)
+{
+    if (!g) return IFOS_EINVAL;
+    if (ntr > IFOS_RESIDUAL_NOALLOC_MAX_TRACES) {
+        set_error("ifos_residual_noalloc: %d traces exceed the scratch maximum of %d", ntr,
+                  IFOS_RESIDUAL_NOALLOC_MAX_TRACES);
+        return IFOS_EINVAL;
+    }
+    if (ntr <= 0) return IFOS_OK;
+    double *part = nullptr;
+    int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)((3 * ntr + 127) / 128)), dim3(128), 0, s, d_obs, d_syn, d_adj,
+                       ntr, ns, ns + 1, ns + 2, g->dt, part);
+    hipLaunchKernelGGL(k_residual_total, dim3(1), dim3(64), 0, s, (const double *)part, 3 * ntr,
+                       part + kNoallocTotal);
+    rc = hip_check(hipGetLastError(), "k_residual");
+    double total = 0.0;
+    if (!rc) rc = hip_check(hipMemcpyAsync(&total, part + kNoallocTotal, sizeof(double), hipMemcpyDeviceToHost, s),
+                            "hipMemcpyAsync");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (rc) return rc;
+    if (l2) *l2 += total;
     return IFOS_OK;
 }
 

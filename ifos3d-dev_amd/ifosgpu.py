@@ -93,6 +93,8 @@ class IfosError(RuntimeError):
 
 # return codes of include/ifos3d_hip.h
 OK, EINVAL, EHIP, ENAN, ENOMEM, EUNSTABLE = 0, -1, -2, -3, -4, -5
+# trace limit of ifos_residual_noalloc's static partial-sum array (ifos3d_hip.h)
+RESIDUAL_NOALLOC_MAX_TRACES = 16384
 
 
 _lib = None
@@ -417,6 +419,14 @@ def residual(g, d_obs, d_syn, d_adj, ntr, ns, stream=None) -> float:
     check(lib().ifos_residual(g.ref, _ptr(d_obs), _ptr(d_syn), _ptr(d_adj), ntr, ns, C.byref(l2), stream),
           "ifos_residual")
     return l2.value
+
+
+def residual_noalloc(g, d_obs, d_syn, d_adj, ntr, ns, l2=0.0, stream=None) -> float:
+    """ifos_residual_noalloc: returns l2 + the misfit, as the C call accumulates"""
+    acc = C.c_double(l2)
+    check(lib().ifos_residual_noalloc(g.ref, _ptr(d_obs), _ptr(d_syn), _ptr(d_adj), ntr, ns, C.byref(acc),
+                                      stream), "ifos_residual_noalloc")
+    return acc.value
 
 
 class ShotContext:

@@ -258,6 +258,17 @@ int ifos_av_mat(const ifos_geom *g, const float *d_rho, const float *d_u,
  * *l2 (host) += sum of squares, reduced in FP64 */
 int ifos_residual(const ifos_geom *g, const float *d_obs, const float *d_syn,
                   float *d_adj, int ntr, int ns, double *l2, void *stream);
+/* ifos_residual without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as ifos_residual, bit for bit.  The FP64 partial
+ * sums live in a static device array of 3 * IFOS_RESIDUAL_NOALLOC_MAX_TRACES
+ * + 1 doubles (384 KiB), one per device, held until the process ends;
+ * ntr above IFOS_RESIDUAL_NOALLOC_MAX_TRACES returns IFOS_EINVAL.  Calls on
+ * one device share that array: calls that overlap on one device race. */
+#define IFOS_RESIDUAL_NOALLOC_MAX_TRACES 16384
+int ifos_residual_noalloc(const ifos_geom *g, const float *d_obs, const float *d_syn,
+                          float *d_adj, int ntr, int ns, double *l2, void *stream);
 /* fill n floats with a value (graph-capture safe zeroing) */
 int ifos_fill(float *d, size_t n, float value, void *stream);
 /* plain streaming copy of n floats (n % 4 == 0, 16-byte aligned) */
