@@ -6,8 +6,10 @@ A gradient shot is: forward NT=1000 steps with the on-the-fly DFT (6
 frequencies), residual against observed data, adjoint NT steps with the
 DFT, gradient_F.  A step of this benchmark is one batch: all 16 shot
 positions propagated together by the batched path (ifos_batch_run).  The
-plain one-shot path (ifos_ctx_run_shot, the drop-in's shape) and the
-checkpointed time-domain gradient (ifos_ctx_run_shot_td) are timed on one
+plain one-shot path (ifos_ctx_run_shot, the drop-in's shape), the
+checkpointed time-domain gradient (ifos_ctx_run_shot_td) and the drop-in
+library itself (the reference prototypes called on Numerical-Recipes host
+arrays by an IFOS3D-shaped driver, host copies included) are timed on one
 shot each beside it.  Inputs (models, observed data) are resident in HBM
 before the timed region.  Prints one JSON line.
 
@@ -563,6 +565,38 @@ def run_td(w, G, d_start, obs, shots):
     return dict(elapsed=elapsed, shots=shots, prof=prof)
 
 
+def run_dropin(w, host_start, obs0):
+    """one gradient shot through the reference prototypes of
+    libifos3d_dropin.so, called on Numerical-Recipes host arrays by an
+    IFOS3D-shaped driver (tests/dropin/nr_driver.c) in the order of
+    ifos3d.c:712-975; the driver's host set-up (NR tensors, matcopy, CPML
+    profiles), the per-step NaN probe and seismogram copies, the host
+    residual and the gradient download are inside the time"""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "tests", "dropin", "lib", "libnr_driver.so"))
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    src = w.shots[0]
+    si = np.array([src[0], src[1], src[2], src[6]], np.int32)
+    sf = np.array([src[3], src[4], src[5]], np.float32)
+    recs = np.ascontiguousarray(w.receivers, np.int32)
+    ntr = len(recs)
+    fv = np.asarray(w.finv, np.float32)
+    rho, pi, u = [np.ascontiguousarray(a, np.float32) for a in host_start]
+    obs = np.ascontiguousarray(obs0, np.float32)
+    syn = np.zeros((3, ntr, w.nt + 1), np.float32)
+    grads = np.zeros((3,) + rho.shape, np.float32)
+    l2 = C.c_float(0.0)
+    t0 = time.perf_counter()
+    rc = lib.dd_run_shot(w.nx, w.ny, w.nz, w.fw, C.c_float(w.dh), C.c_float(w.dh), C.c_float(w.dh),
+                         C.c_float(w.dt), w.nt, w.fdcoeff, 0, p(rho), p(pi), p(u), p(si), p(sf), 1, ntr, p(recs),
+                         p(obs), p(fv), len(fv), w.tast, C.c_float(w.vppml), C.c_float(w.fpml), p(syn), p(grads),
+                         C.byref(l2))
+    elapsed = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"dd_run_shot returned {rc}")
+    return dict(elapsed=elapsed, shots=1, l2=l2.value)
+
+
 def copy_probe(mib=1024, reps=5):
     """plain device copy of a known size: the practical HBM ceiling, and the
     reference kernel the rocprof FETCH_SIZE/WRITE_SIZE counters are scaled by"""
@@ -697,6 +731,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0,
                     help="shorter time loop (profiling runs only: the line then reports a smaller workload)")
     ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
+    ap.add_argument("--dropin-shots", type=int, default=1,
+                    help="gradient shots timed through the drop-in library on NR host arrays (grad276)")
     ap.add_argument("--td-shots", type=int, default=1,
                     help="time-domain (full-band, checkpointed) gradient shots timed beside the batch")
     ap.add_argument("--precision", default="compat", choices=["compat", "kahan"])
@@ -761,6 +797,12 @@ def main():
         td = run_td(w, G, d_start, obs, args.td_shots)
         td["value"] = td["shots"] / td["elapsed"]
         log(f"time-domain gradient: {td['value']:.4f} shots/s")
+
+    dropin = None
+    if args.dropin_shots > 0 and args.config == "grad276":
+        dropin = run_dropin(w, host_start, obs[0].numpy())
+        dropin["value"] = dropin["shots"] / dropin["elapsed"]
+        log(f"drop-in (reference prototypes on NR host arrays): {dropin['value']:.4f} shots/s")
 
     res = batched if args.path == "batched" else plain
     el, value = res["elapsed"], res["value"]
@@ -833,6 +875,7 @@ def main():
         "plain_shots_per_s": round(plain["value"], 4) if plain else None,
         "batched_over_plain": round(batched["value"] / plain["value"], 3) if (batched and plain) else None,
         "td_shots_per_s": round(td["value"], 4) if td else None,
+        "dropin_shots_per_s": round(dropin["value"], 4) if dropin else None,
         "td_kernels_ms": {k: round(v[0], 1) for k, v in td["prof"].items() if v[1]} if td else None,
         "roofline": roof,
         "kernels": kernels,

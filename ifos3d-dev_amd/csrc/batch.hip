@@ -1322,16 +1322,8 @@ extern "C" int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[IFOS_TILE_FIE
                                     const int cfg_s[IFOS_TILE_FIELDS])
 {
     if (!b) return IFOS_EINVAL;
-    for (const int *c : {cfg_v, cfg_s}) {
-        if (c && c[0] != 1 && c[0] != 2 && c[0] != 4 && c[0] != 8) {
-            set_error("ifos_batch_set_tiles: TI must be 1, 2, 4 or 8");
-            return IFOS_EINVAL;
-        }
-        if (c && (c[5] < 1 || c[6] < 1 || c[5] * c[6] > c[0] || c[0] % (c[5] * c[6]))) {
-            set_error("ifos_batch_set_tiles: kb=%d times sb=%d must divide TI=%d", c[5], c[6], c[0]);
-            return IFOS_EINVAL;
-        }
-    }
+    for (const int *c : {cfg_v, cfg_s})
+        if (c && check_tile_cfg(c, "ifos_batch_set_tiles")) return IFOS_EINVAL;
     if (cfg_v) b->cfg_v = MarchCfg{cfg_v[0], cfg_v[1], cfg_v[2], cfg_v[3], cfg_v[4], cfg_v[5], cfg_v[6], cfg_v[7]};
     if (cfg_s) b->cfg_s = MarchCfg{cfg_s[0], cfg_s[1], cfg_s[2], cfg_s[3], cfg_s[4], cfg_s[5], cfg_s[6], cfg_s[7]};
     drop_graphs(b);

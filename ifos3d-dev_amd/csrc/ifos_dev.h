@@ -151,6 +151,9 @@ struct MarchCfg {
     int sb;  // shots of a workgroup, one wavefront row each (kb * sb divides ti)
     int lds; // in-plane neighbours staged in LDS (64-lane rows, kb = sb = 1, ti >= 2)
 };
+// TI / kb / sb checks of a tile shape {TI, jc, group, split, pipe, kb, sb, lds}
+int check_tile_cfg(const int *c, const char *who);
+
 
 // Raw buffer access: one 32-bit byte offset per thread, the neighbour
 // displacements folded into the scalar offset and the instruction's

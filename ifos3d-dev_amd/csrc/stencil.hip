@@ -1378,6 +1378,20 @@ static MTiles make_tiles(const Box *boxes, int nb, int ti, int jc, int ns, int g
 // faster than the register marches over the whole j extent.
 static MarchCfg g_default_cfg = {4, 32, 0, 0, 0, 1, 1, 1};
 
+// the shape checks of ifos_batch_set_tiles and ifos_set_plain_tiles
+int check_tile_cfg(const int *c, const char *who)
+{
+    if (c[0] != 1 && c[0] != 2 && c[0] != 4 && c[0] != 8) {
+        set_error("%s: TI must be 1, 2, 4 or 8", who);
+        return IFOS_EINVAL;
+    }
+    if (c[5] < 1 || c[6] < 1 || c[5] * c[6] > c[0] || c[0] % (c[5] * c[6])) {
+        set_error("%s: kb=%d times sb=%d must divide TI=%d", who, c[5], c[6], c[0]);
+        return IFOS_EINVAL;
+    }
+    return IFOS_OK;
+}
+
 template <bool CPML, bool SNAP, bool PIPE>
 static void launch_v_ti(int ti, dim3 grid, const MTiles &T, const DevGeom &G, const VFields &F, const Prof &P,
                         const Psi9 &psi, const Shots &S, const Snap &N, hipStream_t s)
@@ -2069,3 +2083,20 @@ int launch_vs_fused(const ifos_geom *g, const FusedArgs &A, const float *d_prof,
     return hip_check(hipGetLastError(), "k_vs_fused");
 }
 }  // namespace ifos
+
+extern "C" int ifos_set_plain_tiles(const int cfg[IFOS_TILE_FIELDS])
+{
+    if (!cfg) return IFOS_EINVAL;
+    if (ifos::check_tile_cfg(cfg, "ifos_set_plain_tiles")) return IFOS_EINVAL;
+    ifos::g_default_cfg = ifos::MarchCfg{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6], cfg[7]};
+    return IFOS_OK;
+}
+
+extern "C" int ifos_get_plain_tiles(int cfg[IFOS_TILE_FIELDS])
+{
+    if (!cfg) return IFOS_EINVAL;
+    const ifos::MarchCfg &c = ifos::g_default_cfg;
+    const int v[IFOS_TILE_FIELDS] = {c.ti, c.jc, c.group, c.split, c.pipe, c.kb, c.sb, c.lds};
+    for (int n = 0; n < IFOS_TILE_FIELDS; ++n) cfg[n] = v[n];
+    return IFOS_OK;
+}

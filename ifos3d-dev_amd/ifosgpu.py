@@ -414,6 +414,19 @@ def av_mat(g, rho, u, out6, stream=None):
     check(lib().ifos_av_mat(g.ref, _ptr(rho), _ptr(u), *[_ptr(t) for t in out6], stream), "ifos_av_mat")
 
 
+def set_plain_tiles(cfg):
+    """tile shape {TI, jc, group, split, pipe, kb, sb, lds} of the one-shot
+    march launches (process-wide)"""
+    a = (C.c_int * 8)(*[int(x) for x in cfg])
+    check(lib().ifos_set_plain_tiles(a), "ifos_set_plain_tiles")
+
+
+def get_plain_tiles():
+    a = (C.c_int * 8)()
+    check(lib().ifos_get_plain_tiles(a), "ifos_get_plain_tiles")
+    return tuple(a)
+
+
 def residual(g, d_obs, d_syn, d_adj, ntr, ns, stream=None) -> float:
     l2 = C.c_double(0.0)
     check(lib().ifos_residual(g.ref, _ptr(d_obs), _ptr(d_syn), _ptr(d_adj), ntr, ns, C.byref(l2), stream),

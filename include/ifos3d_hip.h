@@ -418,6 +418,13 @@ int ifos_levels_run(ifos_batch *const *lv, int nlev, const ifos_shot *const *sho
  * unchanged.  A new batch starts with {4, 32, 0, 0, 0, 1, 1, 1} for both. */
 #define IFOS_TILE_FIELDS 8
 int ifos_batch_set_tiles(ifos_batch *b, const int cfg_v[IFOS_TILE_FIELDS], const int cfg_s[IFOS_TILE_FIELDS]);
+/* the tile shape (same fields; group, split and sb have no effect) of the
+ * one-shot march launches: ifos_update_v / ifos_update_s(_visco) on the
+ * interior or the whole domain, the shot contexts and the drop-in.
+ * Process-wide, default {4, 32, 0, 0, 0, 1, 1, 1}; set it between launches,
+ * not while another thread launches. */
+int ifos_set_plain_tiles(const int cfg[IFOS_TILE_FIELDS]);
+int ifos_get_plain_tiles(int cfg[IFOS_TILE_FIELDS]);
 /* the autotuner's candidate shapes (IFOS_TILE_FIELDS ints each, at most
  * max); returns their number */
 int ifos_batch_tune_candidates(int *cfg, int max);

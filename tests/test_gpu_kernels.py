@@ -124,6 +124,23 @@ def test_stress_update_regions(env, free_surf, fdcoeff, dh, nz):
         assert np.array_equal(dpsi2[n].numpy(), po[n]), f"fused psi_v[{n}]"
 
 
+@pytest.mark.parametrize("tiles", [(8, 0, 0, 0, 0, 1, 1, 1), (2, 7, 0, 0, 1, 1, 1, 0), (8, 16, 0, 0, 0, 2, 1, 0)])
+def test_plain_tile_shapes_match_oracle(env, tiles):
+    """the one-shot updates with other tile shapes (ifos_set_plain_tiles:
+    LDS and register marches, several rows or k columns per workgroup)
+    against the oracle, as the default shape above"""
+    _, ig = env
+    old = ig.get_plain_tiles()
+    _note(f"plain tile shape {tiles}")
+    ig.set_plain_tiles(tiles)
+    try:
+        for nz in (30, 150):
+            test_velocity_update_regions(env, 1, 2, nz)
+            test_stress_update_regions(env, 0, 2, 2.0, nz - 1)
+    finally:
+        ig.set_plain_tiles(old)
+
+
 @pytest.mark.parametrize("free_surf,fdcoeff,nz,L", [(0, 2, 29, 1), (1, 1, 29, 1), (0, 2, 150, 1), (0, 2, 29, 3),
                                                      (1, 1, 29, 2), (0, 2, 150, 3), (0, 1, 70, 4)])
 def test_visco_stress_update_regions(env, free_surf, fdcoeff, nz, L):

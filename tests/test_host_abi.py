@@ -122,6 +122,27 @@ def test_entry_points_reject_bad_arguments_before_any_device_call():
     assert "ifos_ctx_run_shot_td" in last()
 
 
+def test_plain_tile_shapes():
+    """the one-shot march shape: set and read back; bad shapes rejected with
+    the previous shape kept"""
+    import ifosgpu as ig
+    lib = ig.lib()
+    old = ig.get_plain_tiles()
+    assert old == (4, 32, 0, 0, 0, 1, 1, 1)
+    try:
+        for cfg in [(8, 32, 0, 0, 0, 1, 1, 1), (2, 0, 0, 0, 0, 1, 1, 1), (8, 16, 0, 0, 1, 2, 1, 0)]:
+            ig.set_plain_tiles(cfg)
+            assert ig.get_plain_tiles() == cfg
+        for bad in [(10, 32, 0, 0, 0, 1, 1, 1), (5, 32, 0, 0, 0, 1, 1, 1), (3, 32, 0, 0, 0, 1, 1, 1),
+                    (4, 32, 0, 0, 0, 3, 1, 0), (4, 32, 0, 0, 0, 2, 4, 0)]:
+            with pytest.raises(ig.IfosError):
+                ig.set_plain_tiles(bad)
+            assert "ifos_set_plain_tiles" in lib.ifos_last_error().decode()
+            assert ig.get_plain_tiles() == (8, 16, 0, 0, 1, 2, 1, 0)
+    finally:
+        ig.set_plain_tiles(old)
+
+
 def test_td_bytes_formula():
     """ifos_td_bytes: (ceil(NT/K) + 1) states + K velocity triples + 6 running
     displacement fields + 3 FP64 sums"""
