@@ -10,7 +10,9 @@ plain one-shot path (ifos_ctx_run_shot, the drop-in's shape), the
 checkpointed time-domain gradient (ifos_ctx_run_shot_td) and the drop-in
 library itself (the reference prototypes called on Numerical-Recipes host
 arrays by an IFOS3D-shaped driver, host copies included) are timed on one
-shot each beside it.  Inputs (models, observed data) are resident in HBM
+shot each beside it, and the batched path once more with compensated (Kahan)
+DFT sums and hipGraph replay of the time loops (two batches of 8 shots: the
+second set of accumulators does not fit 16).  Inputs (models, observed data) are resident in HBM
 before the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad276|toy84]
@@ -731,6 +733,8 @@ def main():
     ap.add_argument("--nt", type=int, default=0,
                     help="shorter time loop (profiling runs only: the line then reports a smaller workload)")
     ap.add_argument("--plain-shots", type=int, default=1, help="plain-path shots timed beside the batch")
+    ap.add_argument("--no-kahan-graph", action="store_true",
+                    help="skip the batched run with Kahan DFT sums and hipGraph replay")
     ap.add_argument("--dropin-shots", type=int, default=1,
                     help="gradient shots timed through the drop-in library on NR host arrays (grad276)")
     ap.add_argument("--td-shots", type=int, default=1,
@@ -791,6 +795,17 @@ def main():
                               not args.no_autotune, args.fused)
         batched["value"] = batched["shots"] / batched["elapsed"]
         log(f"batched path: {batched['value']:.4f} shots/s")
+
+    # the north star's own configuration of the batched path: compensated
+    # (Kahan) DFT accumulators and the time loops replayed from hipGraphs;
+    # Kahan doubles the accumulators, so 16 shots run as two batches of 8
+    kg = None
+    if args.path == "batched" and not args.no_kahan_graph and args.config == "grad276":
+        Sk = min(8, len(w.shots))
+        kg = run_batched(w, G, d_start, obs, max(1, len(w.shots) // Sk), 1, ig.DFT_KAHAN, Sk, args.chunk, True,
+                         autotune=False)
+        kg["value"] = kg["shots"] / kg["elapsed"]
+        log(f"batched path, Kahan DFT, hipGraph replay, {Sk} shots per batch: {kg['value']:.4f} shots/s")
 
     td = None
     if args.td_shots > 0:
@@ -876,6 +891,7 @@ def main():
         "batched_over_plain": round(batched["value"] / plain["value"], 3) if (batched and plain) else None,
         "td_shots_per_s": round(td["value"], 4) if td else None,
         "dropin_shots_per_s": round(dropin["value"], 4) if dropin else None,
+        "kahan_graph_shots_per_s": round(kg["value"], 4) if kg else None,
         "td_kernels_ms": {k: round(v[0], 1) for k, v in td["prof"].items() if v[1]} if td else None,
         "roofline": roof,
         "kernels": kernels,
