@@ -47,8 +47,8 @@ def kernel_names(path, nf, chunk, tiles=None):
     if path == "plain":
         # interior boxes at least 64 cells deep in k take the LDS marches
         return {"v_interior": "ifos::k_v_lds<4, false, false>",
-                "s_interior": "ifos::k_s_lds<4, false, false>",
-                "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs<false>",
+                "s_interior": "ifos::k_s_lds<4, false, 0>",
+                "v_cpml": "ifos::k_v_slabs", "s_cpml": "ifos::k_s_slabs<0>",
                 "dft": f"ifos::k_dft_nf<{nf}>", "dft_kahan": f"ifos::k_dft_kahan_nf<{nf}>",
                 "gradient": "ifos::k_grad_march<4, 0>"}
     tv, ts = tiles
@@ -57,8 +57,8 @@ def kernel_names(path, nf, chunk, tiles=None):
     lds = lambda t: len(t) > 7 and t[7] and t[0] >= 2 and t[5] <= 1 and t[6] == 1
     return {"v_fused": f"ifos::k_v_lds<{tv[0]}, true, false>" if lds(tv)
             else f"ifos::k_v_march<{tv[0]}, true, false, {b(tv[4])}>",
-            "s_fused": f"ifos::k_s_lds<{ts[0]}, true, false>" if lds(ts)
-            else f"ifos::k_s_march<{ts[0]}, true, false, {b(ts[4])}>",
+            "s_fused": f"ifos::k_s_lds<{ts[0]}, true, 0>" if lds(ts)
+            else f"ifos::k_s_march<{ts[0]}, true, 0, {b(ts[4])}>",
             "vs_fused": "ifos::k_vs_fused<16, false>",
             "dft": f"ifos::k_dft_chunk<{nf}, {chunk}, false>", "dft_kahan": f"ifos::k_dft_chunk<{nf}, {chunk}, true>",
             "gradient": "ifos::k_grad_march<4, 0>"}

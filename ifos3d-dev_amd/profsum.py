@@ -3,10 +3,14 @@
 
   python profsum.py stats DIR      # from --kernel-trace --stats (kernel_stats.csv)
   python profsum.py pmc DIR        # from --pmc ... (counter_collection.csv)
-  python profsum.py traffic FETCH WRITE COPY_BYTES
-                                   # HBM bytes per dispatch, calibrated by the
-                                   # copy kernel (FETCH/WRITE: a --pmc folder
-                                   # or a `pmc` summary)
+  python profsum.py traffic RD WR COPY_BYTES
+                                   # L2-to-memory bytes per dispatch from the
+                                   # request counters (RD: a --pmc folder or
+                                   # `pmc` summary of TCC_EA0_RDREQ_{32B,64B,
+                                   # 128B}_sum and TCC_EA0_RDREQ_sum; WR: of
+                                   # TCC_EA0_WRREQ_64B_sum and TCC_EA0_WRREQ_sum);
+                                   # the copy kernel, which moves COPY_BYTES
+                                   # each way, is printed as the check
 
 `stats` prints calls, average and total duration per kernel; `pmc` prints,
 per kernel and counter, the number of dispatches and the average counter
@@ -76,24 +80,37 @@ def pmc(d):
         print(f"{k:<112} {c:>12} {n:>10} {s / n:>18.1f}")
 
 
-def traffic(fetch_dir, write_dir, copy_bytes, copy_kernel="ifos::k_copy"):
-    """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of each kernel, each
-    scaled by copy_bytes / (what the counter reads for the copy kernel,
-    which reads and writes copy_bytes per dispatch)"""
-    f = {k: n and s / n for (k, c), (n, s) in _pmc_table(fetch_dir).items() if c == "FETCH_SIZE"}
-    w = {k: n and s / n for (k, c), (n, s) in _pmc_table(write_dir).items() if c == "WRITE_SIZE"}
-    fs = copy_bytes / (f[copy_kernel] * 1024.0)
-    ws = copy_bytes / (w[copy_kernel] * 1024.0)
-    print(f"# calibration: {copy_kernel} moves {copy_bytes} B each way per dispatch; "
-          f"FETCH_SIZE scale {fs:.4f}, WRITE_SIZE scale {ws:.4f}")
-    print(f"{'kernel':<60} {'fetch_KiB':>14} {'write_KiB':>14} {'hbm_bytes':>16}")
-    for k in sorted(f, key=lambda k: -(f[k] * fs + w.get(k, 0.0) * ws)):
-        hb = (f[k] * fs + w.get(k, 0.0) * ws) * 1024.0
-        print(f"{k:<60} {f[k]:>14.1f} {w.get(k, 0.0):>14.1f} {hb:>16.0f}")
+def traffic(rd_dir, wr_dir, copy_bytes, copy_kernel="ifos::k_copy"):
+    """bytes each kernel moves between L2 and memory per dispatch, from the
+    request counters: reads 32 B x RDREQ_32B + 64 B x RDREQ_64B + 128 B x
+    RDREQ_128B, writes 64 B x WRREQ_64B + 32 B x the other write requests.
+    (FETCH_SIZE, as rocprofv3 derives it on this part, counts a 128-byte read
+    as 64 bytes.)  The copy kernel must come out at copy_bytes each way."""
+    rd, wr = _pmc_table(rd_dir), _pmc_table(wr_dir)
+    avg = lambda t, k, c: (t[(k, c)][1] / t[(k, c)][0]) if (k, c) in t and t[(k, c)][0] else 0.0
+    kernels = sorted({k for (k, c) in rd if c == "TCC_EA0_RDREQ_sum"})
+    rows = []
+    for k in kernels:
+        r32, r64, r128 = (avg(rd, k, f"TCC_EA0_RDREQ_{b}_sum") for b in ("32B", "64B", "128B"))
+        rall = avg(rd, k, "TCC_EA0_RDREQ_sum")
+        w64, wall = avg(wr, k, "TCC_EA0_WRREQ_64B_sum"), avg(wr, k, "TCC_EA0_WRREQ_sum")
+        rb = 32.0 * r32 + 64.0 * r64 + 128.0 * r128
+        wb = 64.0 * w64 + 32.0 * (wall - w64)
+        other = rall - r32 - r64 - r128
+        rows.append((k, rb, wb, other))
+    cr = [r for r in rows if r[0] == copy_kernel]
+    if cr:
+        print(f"# check: {copy_kernel} moves {copy_bytes} B each way per dispatch; the counters give "
+              f"{cr[0][1] / copy_bytes:.4f} x read, {cr[0][2] / copy_bytes:.4f} x write")
+    print(f"# read requests of no counted size (RDREQ - 32B - 64B - 128B), largest: "
+          f"{max((r[3] for r in rows), default=0.0):.0f} per dispatch")
+    print(f"{'kernel':<60} {'read_bytes':>16} {'write_bytes':>16} {'hbm_bytes':>16}")
+    for k, rb, wb, _ in sorted(rows, key=lambda r: -(r[1] + r[2])):
+        print(f"{k:<60} {rb:>16.0f} {wb:>16.0f} {rb + wb:>16.0f}")
 
 
 def read_traffic(path):
-    """{kernel: calibrated HBM bytes per dispatch} from a traffic summary"""
+    """{kernel: HBM bytes per dispatch} from a traffic summary"""
     out = {}
     with open(path) as fh:
         for line in fh:
