@@ -596,7 +596,8 @@ def run_dropin(w, host_start, obs0):
     elapsed = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError(f"dd_run_shot returned {rc}")
-    return dict(elapsed=elapsed, shots=1, l2=l2.value)
+    t = (C.c_double * 3).in_dll(lib, "dd_last_times")
+    return dict(elapsed=elapsed, shots=1, l2=l2.value, shot_calls_s=t[0], setup_s=t[1])
 
 
 def copy_probe(mib=1024, reps=5):
@@ -891,6 +892,8 @@ def main():
         "batched_over_plain": round(batched["value"] / plain["value"], 3) if (batched and plain) else None,
         "td_shots_per_s": round(td["value"], 4) if td else None,
         "dropin_shots_per_s": round(dropin["value"], 4) if dropin else None,
+        "dropin_s": {"shot_calls": round(dropin["shot_calls_s"], 3), "host_setup": round(dropin["setup_s"], 3),
+                     "whole_call": round(dropin["elapsed"], 3)} if dropin else None,
         "kahan_graph_shots_per_s": round(kg["value"], 4) if kg else None,
         "td_kernels_ms": {k: round(v[0], 1) for k, v in td["prof"].items() if v[1]} if td else None,
         "roofline": roof,

@@ -384,10 +384,10 @@ static int dalloc(ifos_ctx *c, void **p, size_t bytes)
 extern "C" int ifos_ctx_destroy(ifos_ctx *c)
 {
     if (!c) return IFOS_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     td_free(c);
     adj_free(c);
-    for (void *p : c->allocs) hipFree(p);
+    for (void *p : c->allocs) (void)hipFree(p);
     delete c;
     return IFOS_OK;
 }

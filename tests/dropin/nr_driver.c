@@ -17,6 +17,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "ifos3d_dropin.h"
 #include "ifos3d_hip.h"
@@ -147,6 +148,18 @@ static void residual1(float **obs, float **syn, float **diff, int ntr, int ns, f
  */
 /* optional viscoelastic run: relaxation ratios (canonical) and eta; NULL
  * switches back to the elastic update */
+/* seconds of the last dd_run_shot: [0] the shot's library calls
+ * (zero_wavefield ... gradient_F, the per-step host copies included), [1]
+ * the set-up before them (NR tensors, model, matcopy, av_mat, CPML), [2] the
+ * whole call */
+double dd_last_times[3];
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 static const float *g_taus, *g_taup;
 static float g_eta;
 void dd_set_visco(const float *taus_c, const float *taup_c, float eta)
@@ -174,6 +187,7 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
     float *sig;
     ifos_geom geo;
     float ***rm[6] = {NULL}, ***taus = NULL, ***taup = NULL, ***tsa[3] = {NULL}, *eta = NULL;
+    double t_begin = now_s(), t_loop = t_begin;
 
     NX = nx; NY = ny; NZ = nz; FW = fw; DX = dx; DY = dy; DZ = dz; DT = dt; NT = nt;
     TIME = (NT - 1) * DT;
@@ -267,6 +281,7 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
         }                                                                                                     \
     } while (0)
     /* forward, ifos3d.c:738-814 */
+    t_loop = now_s();
     zero_wavefield(NX, NY, NZ, vx, vy, vz, sxx, syy, szz, sxy, syz, sxz, rm[0], rm[1], rm[2], rm[3], rm[4], rm[5],
                    psi[0], psi[1], psi[2], psi[3], psi[4], psi[5], psi[6], psi[7], psi[8], psi[9], psi[10],
                    psi[11], psi[12], psi[13], psi[14], psi[15], psi[16], psi[17]);
@@ -321,6 +336,8 @@ int dd_run_shot(int nx, int ny, int nz, int fw, float dx, float dy, float dz, fl
     from_canon(g3, grads + 2 * (size_t)(NY + 5) * (NX + 5) * (NZ + 5), 1, NY, 1, NX, 1, NZ);
     gradient_F(NX, NY, NZ, F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], F[11], g1, g2, g3,
                NT + 1, rho, pi, u, (float *)finv, nf, 1);
+    dd_last_times[0] = now_s() - t_loop;
+    dd_last_times[1] = t_loop - t_begin;
     to_canon(g1, grads, 1, NY, 1, NX, 1, NZ);
     to_canon(g2, grads + (size_t)(NY + 5) * (NX + 5) * (NZ + 5), 1, NY, 1, NX, 1, NZ);
     to_canon(g3, grads + 2 * (size_t)(NY + 5) * (NX + 5) * (NZ + 5), 1, NY, 1, NX, 1, NZ);
@@ -349,6 +366,7 @@ done:
         free(eta);
     }
     (void)j;
+    dd_last_times[2] = now_s() - t_begin;
     return rc;
 }
 
