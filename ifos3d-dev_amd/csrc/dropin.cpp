@@ -25,7 +25,7 @@
 // the caller's globals (IFOS3D globvar.h:27-62); defined by the host program
 extern "C" {
 extern float DT, DX, DY, DZ, TIME, ALPHA, BETA, VPPML, FPML;
-extern int NT, NX, NY, NZ, FDORDER, FDCOEFF, ABS_TYPE, FW, FREE_SURF, L, SEISMO;
+extern int NT, NX, NY, NZ, FDORDER, FDCOEFF, ABS_TYPE, FW, FREE_SURF, L, SEISMO, NDT, NDTSHIFT;
 extern int POS[4], NPROCX, NPROCY, NPROCZ;
 // the optimiser's (globvar.h:60-62)
 extern int LBFGS, NUMPAR, BFGSNUM;
@@ -59,6 +59,14 @@ struct State {
     int src_n = -1, src_back = -1;
     const void *seis_key = nullptr;
     int seis_ntr = -1;
+    // samples gathered on the device and not yet copied to the host sections
+    // (seismo), the sections they belong to and the layout [3][ntr][ns+1]
+    int seis_ns = 0, seis_pending = 0;
+    float **seis_out[3] = {};
+    // the NaN probe's pinned slots (refresh_probe)
+    float *probe_host = nullptr;
+    hipEvent_t probe_ev[2] = {};
+    bool probe_have[2] = {};
     // the L-BFGS history (bfgsgrad and bfgsscale live here; bfgsmod stays the
     // host's), keyed by the caller's bfgsgrad matrix
     ifos_lbfgs *lb = nullptr;
@@ -281,11 +289,31 @@ void check_box(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2)
         fatal("CPML slabs need the CPML_ini_elastic box, got [%d,%d]x[%d,%d]x[%d,%d]", nx1, nx2, ny1, ny2, nz1, nz2);
 }
 
-// the host NaN probe of ifos3d.c:761 reads vy[NY/2][NX/2][NZ/2]
-void refresh_probe(float ***vy, const float *dvy)
+// The host NaN probe of ifos3d.c:761 reads vy[NY/2][NX/2][NZ/2] before each
+// step.  After update_v_CPML of step nt (the step's last velocity update) the
+// cell is copied, asynchronously, into one of two pinned host slots; at step
+// nt + 1 the slot of step nt is written into the caller's array.  No call
+// waits for the device (a wait per step would leave the device idle while the
+// host issues the next launches); the host's check at the top of step nt + 2
+// sees step nt's value, so an unstable run stops one step later than the
+// reference's.  Step 1 reads no slot (the previous propagation's).
+void refresh_probe(float ***vy, const float *dvy, int nt)
 {
+    if (!S.probe_host) {
+        hck(hipHostMalloc((void **)&S.probe_host, 2 * sizeof(float), hipHostMallocDefault), "probe slots");
+        for (int q = 0; q < 2; ++q)
+            hck(hipEventCreateWithFlags(&S.probe_ev[q], hipEventDisableTiming), "probe events");
+    }
+    const int cur = nt & 1, prev = cur ^ 1;
+    if (nt > 1 && S.probe_have[prev]) {
+        hck(hipEventSynchronize(S.probe_ev[prev]), "probe wait");
+        vy[NY / 2][NX / 2][NZ / 2] = S.probe_host[prev];
+    }
+    S.probe_have[prev] = false;
     const size_t o = ifos_offset(&S.g, NY / 2, NX / 2, NZ / 2);
-    hck(hipMemcpy(&vy[NY / 2][NX / 2][NZ / 2], dvy + o, 4, hipMemcpyDeviceToHost), "NaN probe");
+    hck(hipMemcpyAsync(&S.probe_host[cur], dvy + o, 4, hipMemcpyDeviceToHost, nullptr), "NaN probe");
+    hck(hipEventRecord(S.probe_ev[cur], nullptr), "probe event");
+    S.probe_have[cur] = true;
 }
 
 ifos_fields fields(float ***vx, float ***vy, float ***vz, float ***sxx, float ***syy, float ***szz, float ***sxy,
@@ -332,7 +360,6 @@ double update_v(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2, int nt, fl
         else
             ck(ifos_add_adjoint(&g, &f, S.d_rec, S.d_chain, S.d_adj, nsrc, nt, nullptr), "adjoint traces");
     }
-    refresh_probe(vy, f.vy);
     return 0.0;
 }
 
@@ -367,7 +394,7 @@ double update_v_CPML(int nx1, int nx2, int ny1, int ny2, int nz1, int nz2, int n
         ensure_sources(false, srcpos_loc, signals, signaly, signalz, nsrc, back);
         ck(ifos_add_adjoint(&g, &f, S.d_rec, S.d_chain, S.d_adj, nsrc, nt, nullptr), "adjoint traces");
     }
-    refresh_probe(vy, f.vy);
+    refresh_probe(vy, f.vy, nt);
     return 0.0;
 }
 
@@ -499,6 +526,27 @@ void psource(int nt, float ***sxx, float ***syy, float ***szz, float **srcpos_lo
     ck(ifos_psource(&g, &f, S.d_src, S.d_sig, nsrc, nt, nullptr), "psource");
 }
 
+namespace {
+// copies samples 1..seis_pending of the device sections [3][ntr][ns+1] into
+// the host sections they were gathered for
+void flush_seis()
+{
+    if (S.seis_pending <= 0) return;
+    const int ntr = S.seis_ntr, ld = S.seis_ns + 1;
+    std::vector<float> h(3 * (size_t)ntr * ld);
+    hck(hipMemcpy(h.data(), S.d_sec, h.size() * 4, hipMemcpyDeviceToHost), "download sections");
+    for (int c = 0; c < 3; ++c)
+        for (int r = 1; r <= ntr; ++r)
+            memcpy(&S.seis_out[c][r][1], &h[((size_t)c * ntr + r - 1) * ld + 1], sizeof(float) * S.seis_pending);
+    S.seis_pending = 0;
+}
+}  // namespace
+
+// seismo_ssg.c:27-53 (SEISMO 1).  Sample lsamp of every receiver is gathered
+// on the device; the samples of a propagation reach the host sections in one
+// copy, at its last sample ns = ceil((NT - NDTSHIFT) / NDT) (ifos3d.c:178-179),
+// before saveseis and the residual read them (ifos3d.c:816, 839-858) -- and
+// before another propagation (lsamp 1) or other receivers start.
 void seismo(int lsamp, int ntr, int **recpos, float **sectionvx, float **sectionvy, float **sectionvz,
             float **sectiondiv, float **sectioncurl, float **sectionp, float ***vx, float ***vy, float ***vz,
             float ***sxx, float ***syy, float ***szz, float ***pi, float ***u)
@@ -507,31 +555,36 @@ void seismo(int lsamp, int ntr, int **recpos, float **sectionvx, float **section
     const ifos_geom &g = geom();
     if (SEISMO != 1) fatal("SEISMO %d not supported (particle velocities, 1, only)", SEISMO);
     if (ntr <= 0) return;
-    if (lsamp == 1 || S.seis_key != recpos || S.seis_ntr != ntr) {
+    const int ns = NDTSHIFT > NT || NDT < 1 ? 0 : (int)ceil((float)(NT - NDTSHIFT) / (float)NDT);
+    float **out[3] = {sectionvx, sectionvy, sectionvz};
+    if (lsamp < 1 || lsamp > ns) fatal("seismo: sample %d outside 1..%d (NT %d, NDT %d, NDTSHIFT %d)", lsamp, ns, NT,
+                                       NDT, NDTSHIFT);
+    const bool same = S.seis_key == recpos && S.seis_ntr == ntr && S.seis_ns == ns &&
+                      S.seis_out[0] == sectionvx && S.seis_out[1] == sectionvy && S.seis_out[2] == sectionvz;
+    if (lsamp == 1 || !same) {
+        flush_seis();
         std::vector<int> rec(3 * (size_t)ntr);
         for (int r = 1; r <= ntr; ++r) {
             rec[3 * (r - 1) + 0] = recpos[1][r];
             rec[3 * (r - 1) + 1] = recpos[2][r];
             rec[3 * (r - 1) + 2] = recpos[3][r];
         }
-        grow((void **)&S.d_seisrec, S.cap_seis, rec.size() * 4 + 3 * (size_t)ntr * 2 * 4);
+        grow((void **)&S.d_seisrec, S.cap_seis, rec.size() * 4 + 3 * (size_t)ntr * (ns + 1) * 4);
         S.d_sec = (float *)(S.d_seisrec + rec.size());
         hck(hipMemcpy(S.d_seisrec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice), "upload receivers");
         S.seis_key = recpos;
         S.seis_ntr = ntr;
+        S.seis_ns = ns;
+        for (int c = 0; c < 3; ++c) S.seis_out[c] = out[c];
+        S.seis_pending = 0;
     }
     ifos_fields f = {};
     f.vx = m3(vx, WAVE_TOP);
     f.vy = m3(vy, WAVE_TOP);
     f.vz = m3(vz, WAVE_TOP);
-    ck(ifos_seismo(&g, &f, S.d_seisrec, ntr, 1, 1, S.d_sec, nullptr), "seismo");
-    std::vector<float> h(3 * (size_t)ntr * 2);
-    hck(hipMemcpy(h.data(), S.d_sec, h.size() * 4, hipMemcpyDeviceToHost), "download samples");
-    for (int r = 1; r <= ntr; ++r) {
-        sectionvx[r][lsamp] = h[(0 * (size_t)ntr + r - 1) * 2 + 1];
-        sectionvy[r][lsamp] = h[(1 * (size_t)ntr + r - 1) * 2 + 1];
-        sectionvz[r][lsamp] = h[(2 * (size_t)ntr + r - 1) * 2 + 1];
-    }
+    ck(ifos_seismo(&g, &f, S.d_seisrec, ntr, lsamp, ns, S.d_sec, nullptr), "seismo");
+    if (lsamp > S.seis_pending) S.seis_pending = lsamp;
+    if (lsamp == ns) flush_seis();
 }
 
 void surface_elastic(int ndepth, float ***u, float ***pi, float ***sxx, float ***syy, float ***szz, float ***sxy,
@@ -848,10 +901,17 @@ const char *ifos_dropin_last_error(void) { return S.err.c_str(); }
 
 void ifos_dropin_release(void)
 {
+    flush_seis();
     if (S.lb) ifos_lbfgs_destroy(S.lb);
     S.lb = nullptr;
     S.lb_key = nullptr;
     (void)hipDeviceSynchronize();
+    if (S.probe_host) {
+        for (int q = 0; q < 2; ++q) (void)hipEventDestroy(S.probe_ev[q]);
+        (void)hipHostFree(S.probe_host);
+        S.probe_host = nullptr;
+        S.probe_have[0] = S.probe_have[1] = false;
+    }
     for (auto &kv : S.mir) (void)hipFree(kv.second.dev);
     S.mir.clear();
     void *bufs[] = {S.d_prof, S.d_src, S.d_rec, S.d_sig, S.d_adj, S.d_seisrec, S.d_step};
@@ -868,6 +928,8 @@ void ifos_dropin_release(void)
     S.src_n = S.src_back = -1;
     S.seis_key = nullptr;
     S.seis_ntr = -1;
+    S.seis_ns = S.seis_pending = 0;
+    for (int c = 0; c < 3; ++c) S.seis_out[c] = nullptr;
     S.have_geom = false;
 }
 

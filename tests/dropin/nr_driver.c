@@ -24,7 +24,7 @@
 
 /* the globals the drop-in routines read */
 float DT, DX, DY, DZ, TIME, ALPHA, BETA, VPPML, FPML;
-int NT, NX, NY, NZ, FDORDER, FDCOEFF, ABS_TYPE, FW, FREE_SURF, L, SEISMO;
+int NT, NX, NY, NZ, FDORDER, FDCOEFF, ABS_TYPE, FW, FREE_SURF, L, SEISMO, NDT = 1, NDTSHIFT = 0;
 int POS[4], NPROCX, NPROCY, NPROCZ;
 int LBFGS, NUMPAR, BFGSNUM;
 float VP0, VS0, RHO0, WEIGHT[3];

@@ -117,3 +117,29 @@ def test_unstable_dt_is_reported_as_nan_batched():
     assert e.value.code == ig.ENAN, str(e.value)
     ig.sync()
     b.close()
+
+
+def test_unstable_dt_stops_the_dropin_driver():
+    """the drop-in behind an IFOS3D-shaped caller (tests/dropin/nr_driver.c),
+    DT at twice the limit: the caller's own NaN check of vy at the centre cell
+    (ifos3d.c:761) stops the forward loop, from the probe value the drop-in
+    keeps in the caller's array (one step behind the device)"""
+    import ctypes as C
+    import os
+    import ifosgpu as ig
+    g, G, model, src, rec = _unstable_setup(ig)
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "lib", "libnr_driver.so"))
+    p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+    rho, pi, u = [np.ascontiguousarray(a, np.float32) for a in model]
+    si = np.array([src[0], src[1], src[2], src[6]], np.int32)
+    sf = np.array([src[3], src[4], src[5]], np.float32)
+    fv = np.array([50.0], np.float32)
+    syn = np.zeros((3, len(rec), g.nt + 1), np.float32)
+    grads = np.zeros((3,) + g.shape, np.float32)
+    l2 = C.c_float(0.0)
+    print("drop-in shot at twice the stability limit", flush=True)
+    rc = lib.dd_run_shot(g.nx, g.ny, g.nz, g.fw, C.c_float(g.dx), C.c_float(g.dy), C.c_float(g.dz),
+                         C.c_float(G.c.dt), g.nt, g.fdcoeff, g.free_surf, p(rho), p(pi), p(u), p(si), p(sf), 1,
+                         len(rec), p(np.ascontiguousarray(rec)), None, p(fv), 1, 100, C.c_float(4000.0),
+                         C.c_float(50.0), p(syn), p(grads), C.byref(l2))
+    assert rc == -1
