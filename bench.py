@@ -22,6 +22,8 @@ Two other workloads print a line of their own:
   --config fwd536   forward-only bandwidth case, 536 x 280 x 536, NT 2000 (Gcell-updates/s)
   --config levels3  three uneven grid levels x 32 shots in one graph launch (forward solves/s)
   --config aux276   the optimiser, Hessian, residual and SU steps at 276^3 (ms per call)
+  --config visco404 the viscoelastic (3 mechanisms) + 8-frequency DFT case at 404 x 212 x 404: Gcell-updates/s
+                    and the compat and Kahan DFT accumulation errors against FP64
 """
 from __future__ import annotations
 
@@ -410,6 +412,144 @@ def run_fwd(args):
     print(json.dumps(out), flush=True)
 
 
+def run_visco(args):
+    """the viscoelastic + DFT case: 404 x 212 x 404 (384 x 192 x 384 + 10-cell
+    CPML), three relaxation mechanisms fitted over 5-40 Hz, per-cell Q ~
+    U[20, 100], 8 DFT frequencies 5-40 Hz sampled every step, NT 2000.  One
+    gradient shot through the plain path with the reference's DFT rounding
+    (compat) and with Kahan sums, and through the batched path (one shot,
+    Kahan); Gcell-updates/s and the stress update against HBM peak.  The
+    DFT's accumulation error: the forward DFT fields at 4096 cells of the
+    mid-depth plane (receivers there record the velocities every step)
+    against the FP64 sum of the same terms, disc_fourier.c:36-54"""
+    import ctypes as C
+    import ifosgpu as ig
+    import workloads as wl
+    w = wl.visco404()
+    model, (taus, taup), etas = wl.visco_models(w)
+    L = len(etas)
+    G = ig.Geom(w.nx, w.ny, w.nz, w.fw, w.dh, w.dh, w.dh, w.dt, w.nt, w.fdcoeff, 0)
+    d_model = [G.to_dev(a) for a in model]
+    d_tau = [G.to_dev(a) for a in (taus, taup)]
+    del model, taus, taup
+    nf = len(w.finv)
+    jm = w.fw + 1 + (w.ny - 2 * w.fw) // 2
+    rec = np.array([[w.fw + 1 + 6 * a, jm, w.fw + 1 + 6 * b] for a in range(64) for b in range(64)], np.int32)
+    ntr = len(rec)
+    src = w.shots[0]
+    ntast = ig.ntast(w.finv[-1], w.tast, w.dt)
+    d_obs = ig.DBuf((3, ntr, w.nt + 1))
+    log(f"visco404: {w.nx}x{w.ny}x{w.nz}, L={L}, {nf} frequencies, NT {w.nt}, dt {w.dt:.4e}, ntast {ntast}")
+    # the device offsets of the sample cells within plane jm
+    lib = ig.lib()
+    base = lib.ifos_offset(G.ref, jm, -2, -2)
+    cell = np.array([lib.ifos_offset(G.ref, jm, int(r[0]), int(r[2])) - base for r in rec], np.int64)
+    plane = (w.nx + 5) * G.pitch
+    fe = G.field_elems()
+    res, fields, syn = {}, {}, None
+    for name, prec in (("compat", ig.DFT_COMPAT), ("kahan", ig.DFT_KAHAN)):
+        ctx = ig.ShotContext(G, nf, ntr, prec)
+        ctx.set_cpml(w.vppml, w.fpml)
+        ctx.set_freqs(w.finv, w.tast)
+        ctx.set_visco(*d_tau, etas)
+        ctx.set_model(*d_model)
+        d_syn = ig.DBuf((3, ntr, w.nt + 1))
+        grads = [G.empty() for _ in range(3)]
+        ig.sync()
+        _prof_start(ig)
+        t0 = time.perf_counter()
+        ctx.run_shot(src, rec, d_syn, grads, d_obs=d_obs)
+        ig.sync()
+        res[name] = (time.perf_counter() - t0, _prof_stop(ig))
+        log(f"plain gradient shot, {name} DFT: {res[name][0]:.2f} s")
+        buf = np.empty(plane, np.float32)
+        F = np.empty((6, nf, ntr), np.float64)
+        for n, ptr in enumerate(ctx.dft_ptrs(0)):
+            for l in range(nf):
+                src_p = C.c_void_p(ptr + 4 * (l * fe + base))
+                ig.check(lib.ifos_memcpy_d2h(buf.ctypes.data_as(C.c_void_p), src_p, C.c_size_t(4 * plane)), "d2h")
+                F[n, l] = buf[cell]
+        fields[name] = F
+        if syn is None:
+            syn = d_syn.numpy().astype(np.float64)
+        ctx.close()
+        del grads, d_syn
+    # FP64 DFT of the same FP32 velocities, the terms of disc_fourier.c:42-54
+    nts = np.arange(ntast, w.nt + 1, ntast)
+    t = (nts.astype(np.float32) * np.float32(w.dt)).astype(np.float64)
+    F64 = np.empty((6, nf, ntr), np.float64)
+    for l, f in enumerate(w.finv):
+        c = np.cos(2.0 * t * np.float64(np.float32(f)) * np.pi) * np.float64(np.float32(w.dt)) * ntast
+        sn = np.sin(2.0 * t * np.float64(np.float32(f)) * np.pi) * np.float64(np.float32(w.dt)) * ntast
+        for comp in range(3):
+            F64[comp, l] = syn[comp][:, nts] @ c
+            F64[3 + comp, l] = syn[comp][:, nts] @ sn
+    err = {k: float(np.linalg.norm(v - F64) / np.linalg.norm(F64)) for k, v in fields.items()}
+    log(f"DFT accumulation error against FP64: {err}")
+    # the batched path, one shot, Kahan sums
+    b = ig.BatchContext(G, 1, nf, ntr, ig.DFT_KAHAN, args.chunk)
+    b.set_cpml(w.vppml, w.fpml)
+    b.set_freqs(w.finv, w.tast)
+    b.set_visco(*d_tau, etas)
+    b.set_model(*d_model)
+    grads = [G.empty() for _ in range(3)]
+    d_b = ig.DBuf((1, 3, ntr, w.nt + 1))
+    ig.sync()
+    _prof_start(ig)
+    t0 = time.perf_counter()
+    b.run([src], rec, d_syn=d_b, grads=grads, d_obs=[d_obs])
+    ig.sync()
+    res["batched"] = (time.perf_counter() - t0, _prof_stop(ig))
+    b.close()
+    log(f"batched gradient shot (one shot, Kahan): {res['batched'][0]:.2f} s")
+    cells = w.cells
+    bi, slab, axes = slab_stats(w)
+    vis = 68 + 48 * (L - 1)   # per cell of the stress update: tau arrays, the memory variables
+    ab_p = {"v_interior": 60 * bi, "v_cpml": 60 * slab + 24 * axes,
+            "s_interior": (80 + vis) * bi, "s_cpml": (80 + vis) * slab + 24 * axes,
+            "dft": (12 + 48 * nf) * cells, "gradient": (48 * nf + 36) * cells}
+    ab_k = dict(ab_p, dft=(12 + 96 * nf) * cells)
+    # the batched DFT: a chunk of c snapshots, 12 B per cell each, folded
+    # into 2 x 48 B per cell and frequency of Kahan accumulators
+    nchunks = -(-(w.nt // ntast) // args.chunk)
+    ab_b = {"v_fused": 60 * cells + 24 * axes, "s_fused": (80 + vis) * cells + 24 * axes,
+            "dft": (12 * (w.nt // ntast) + 96 * nf * nchunks) * cells / nchunks,
+            "gradient": (48 * nf + 36) * cells}
+    out_k = {}
+    for name, ab in (("compat", ab_p), ("kahan", ab_k), ("batched", ab_b)):
+        el, prof = res[name]
+        dom = "s_interior" if name != "batched" else "s_fused"
+        avg = prof[dom][0] / prof[dom][1]
+        out_k[name] = {"gradient_shot_s": round(el, 3), "gcell_updates_per_s": round(cells * 2 * w.nt / el / 1e9, 3),
+                       "stress_kernel_ms": round(avg, 4),
+                       "stress_kernel_hbm_frac": round(ab[dom] / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "kernels": kernel_table(prof, ab)}
+    el, prof = res["batched"]
+    avg = prof["s_fused"][0] / prof["s_fused"][1]
+    out = {
+        "metric": "viscoelastic gradient shot: Gcell-updates/s and DFT error against FP64, 384x192x384 + CPML",
+        "value": round(cells * 2 * w.nt / el / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(1000.0 * el, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"visco404: {w.nx}x{w.ny}x{w.nz} (384x192x384 + {w.fw}-cell CPML), dh {w.dh} m, "
+                               f"dt {w.dt:.4e} s, NT {w.nt}, seeded random model, {L} relaxation mechanisms "
+                               f"(5-40 Hz), Q ~ U[20, 100] per cell, {nf} DFT frequencies {w.finv[0]}-{w.finv[-1]} Hz "
+                               f"every {ntast} step(s), {ntr} receivers on the mid-depth plane",
+                   "path": "batched (one shot, Kahan DFT sums); the plain path in compat and Kahan precision beside it"},
+        "dft_error_vs_fp64": {"compat": err["compat"], "kahan": err["kahan"],
+                              "what": f"rel. L2 of the forward DFT fields at {ntr} cells (6 components x {nf} "
+                                      f"frequencies) against the FP64 sum of the same FP32 velocity terms"},
+        "paths": out_k,
+        "roofline": {"kernel": "s_fused", "bound": "hbm", "achieved": round(ab_b["s_fused"] / (avg * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ab_b["s_fused"] / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(ab_b["s_fused"]), "avg_launch_ms": round(avg, 5)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_aux(args):
     """the steps on either side of the propagation at the 276^3 gradient
     workload's size, each timed over a few calls: the L-BFGS update (two
@@ -718,7 +858,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed batches (batched) or shots (plain)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad276", choices=["grad276", "toy84", "levels3", "fwd536", "aux276"])
+    ap.add_argument("--config", default="grad276",
+                    choices=["grad276", "toy84", "levels3", "fwd536", "aux276", "visco404"])
     ap.add_argument("--level-shots", type=int, default=32, help="shots per grid level (levels3)")
     ap.add_argument("--path", default="batched", choices=["batched", "plain"])
     ap.add_argument("--shots-per-batch", type=int, default=16)
@@ -770,6 +911,9 @@ def main():
         return
     if args.config == "aux276":
         run_aux(args)
+        return
+    if args.config == "visco404":
+        run_visco(args)
         return
     w = wl.grad276() if args.config == "grad276" else wl.toy84()
     if args.nt > 0:
