@@ -79,3 +79,59 @@ def f32hex(a):
 
 def from_f32hex(h):
     return np.array([int(x, 16) for x in h], np.uint32).view(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# mirror symmetry of the staggered grid.  A canonical array index idx along
+# an axis is the grid position idx - 2.  Mirrored about the cell centre c, a
+# quantity centred on that axis maps position p to 2c - p and keeps its sign;
+# one staggered half a cell up (vx and sxy, sxz along x, ...) maps p to
+# 2c - 1 - p and changes sign (an odd vector or tensor component).
+# Axes of the canonical arrays: 0 = y (j), 1 = x (i), 2 = z (k).
+FIELD_STAGGER = {"vx": (1,), "vy": (0,), "vz": (2,), "sxx": (), "syy": (), "szz": (),
+                 "sxy": (0, 1), "syz": (0, 2), "sxz": (1, 2)}
+FIELD_NAMES = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "syz", "sxz")
+
+
+def mirror_mismatch(a, axis, c, stag):
+    """cells of canonical array a (or a stack of them, the grid in its last
+    three axes) that break the mirror relation about centre c along grid
+    axis `axis`; every pair of cells inside the array is compared"""
+    ax = a.ndim - 3 + axis
+    n = a.shape[ax]
+    s = 2 * c + (3 if stag else 4)  # idx + mirror idx
+    lo, hi = max(0, s - (n - 1)), min(n - 1, s)
+    sl = [slice(None)] * a.ndim
+    sl[ax] = slice(lo, hi + 1)
+    part = a[tuple(sl)]
+    mir = np.flip(part, ax)
+    return int(np.count_nonzero(part != (-mir if stag else mir)))
+
+
+def symmetric_table_model(shape, centre, seed, vp=4000.0, rho=2000.0, pert=0.1):
+    """rho, pi, u of a heterogeneous model that is mirror-symmetric about the
+    cell centre (cy, cx, cz): each value is looked up by (|y-cy|, |x-cx|,
+    |z-cz|) in a seeded random table.  u takes the values 2^32 and 2^33
+    (vs 1.4-2.2 km/s, bulk modulus pi - 4u/3 > 0): the harmonic averages sum
+    four reciprocals in an
+    order that the mirror reverses (av_mat.c:51-53), and reciprocals of
+    powers of two sum exactly in any order, so the averages are symmetric
+    bit for bit too"""
+    rng = np.random.default_rng(seed)
+    idx = [np.abs(np.arange(n) - 2 - c) for n, c in zip(shape, centre)]
+    tshape = tuple(int(i.max()) + 1 for i in idx)
+    r = (rho * (1 + pert * rng.uniform(-1, 1, tshape))).astype(np.float32)
+    a = (vp * (1 + pert * rng.uniform(-1, 1, tshape))).astype(np.float32)
+    u = np.ldexp(np.float32(1.0), rng.integers(32, 34, tshape)).astype(np.float32)
+    r, a, u = [t[np.ix_(*idx)] for t in (r, a, u)]
+    return r, (r * a * a).astype(np.float32), u
+
+
+def symmetric_table_field(shape, centre, seed, lo, hi):
+    """a positive field (e.g. tau) mirror-symmetric about the cell centre, in
+    multiples of 2^-10, so that sums of four of them (the averages of
+    av_mat.c:56-60) are exact in any order"""
+    rng = np.random.default_rng(seed)
+    idx = [np.abs(np.arange(n) - 2 - c) for n, c in zip(shape, centre)]
+    t = rng.integers(int(lo * 1024), int(hi * 1024) + 1, tuple(int(i.max()) + 1 for i in idx))
+    return (t.astype(np.float32) / np.float32(1024.0))[np.ix_(*idx)]
