@@ -135,3 +135,16 @@ def symmetric_table_field(shape, centre, seed, lo, hi):
     idx = [np.abs(np.arange(n) - 2 - c) for n, c in zip(shape, centre)]
     t = rng.integers(int(lo * 1024), int(hi * 1024) + 1, tuple(int(i.max()) + 1 for i in idx))
     return (t.astype(np.float32) / np.float32(1024.0))[np.ix_(*idx)]
+
+
+def homogeneous_model(shape, vp=4000.0, vs=2300.0, rho=2000.0):
+    r = np.full(shape, rho, np.float32)
+    return r, (r * np.float32(vp * vp)).astype(np.float32), (r * np.float32(vs * vs)).astype(np.float32)
+
+
+def shifted_mismatch(a, b, shift, lo, hi):
+    """cells of the box lo..hi (canonical indices, inclusive, per axis) where
+    array b, shifted by `shift` cells, differs from a: a[p] vs b[p + shift]"""
+    sa = tuple(slice(l, h + 1) for l, h in zip(lo, hi))
+    sb = tuple(slice(l + s, h + 1 + s) for l, h, s in zip(lo, hi, shift))
+    return int(np.count_nonzero(a[(Ellipsis,) + sa] != b[(Ellipsis,) + sb]))

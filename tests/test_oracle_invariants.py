@@ -27,7 +27,8 @@ golden vectors (the reference ships no tests; these fix what any correct
 import numpy as np
 import pytest
 
-from helpers import FIELD_STAGGER, mirror_mismatch, random_model, symmetric_table_field, symmetric_table_model
+from helpers import (FIELD_STAGGER, homogeneous_model, mirror_mismatch, random_model, shifted_mismatch,
+                     symmetric_table_field, symmetric_table_model)
 from pyoracle import Grid, Oracle
 
 
@@ -79,6 +80,25 @@ def test_mirror_symmetry_detects_an_offset_source():
     hist = _shot(Oracle("f32"), g, c, (31, 32, 31))
     bad = _mirror_failures(hist, c)
     assert bad and all(ax == 1 for _, ax, _ in bad)
+
+
+def test_translation_invariance_in_a_homogeneous_model():
+    """the same shot moved by (5, 7, 3) cells gives the same velocities
+    moved by as much, bit for bit, at every step until the support nears the
+    CPML (61^3, 6-cell CPML, 7 steps)"""
+    g = Grid(nx=61, ny=61, nz=61, fw=6, dx=2.0, dy=2.0, dz=2.0, dt=2e-4, nt=7, fdcoeff=2)
+    o = Oracle("f32")
+    model = homogeneous_model(g.shape)
+    fc = 50.0
+    rec = np.array([[31, 31, 31]], np.int32)
+    h = [o.run_shot(g, *model, (i, j, k, -1.5 / fc, fc, 1.0e6, 1), rec, [fc], want_adjoint=False,
+                    want_history=True)["vfwd"] for (i, j, k) in ((28, 27, 30), (33, 34, 33))]
+    # the support after 7 steps spans 21 cells round each source; the box
+    # below covers it with room for the shift (canonical index = cell + 2)
+    lo, hi = (27 + 2 - 12,) * 3, (27 + 2 + 12,) * 3
+    assert np.count_nonzero(h[0][-1]) > 10000
+    assert shifted_mismatch(h[0], h[1], (7, 5, 3), lo, hi) == 0
+    assert shifted_mismatch(h[0], h[1], (7, 5, 2), lo, hi) > 0  # the check sees a wrong shift
 
 
 def _energy(f, v_next, m):
