@@ -669,7 +669,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
     constexpr int CPN = CPML ? IFOS_CPOL_NT : 0;
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const float b1 = g.b1, b2 = g.b2;
-    const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
+    const float DT = g.dt;
     const int64_t so = (int64_t)shot * S.fstride;
     const rsrc_t rvx = make_rsrc(F.vx + so, g.fbytes), rvy = make_rsrc(F.vy + so, g.fbytes),
                  rvz = make_rsrc(F.vz + so, g.fbytes), rsxx = make_rsrc(F.sxx + so, g.fbytes),
@@ -1863,7 +1863,7 @@ __global__ __launch_bounds__(64 * TI) void k_vs_fused(DevGeom g, FTiles T, Fused
     const uint32_t SJ = (uint32_t)g.sj * 4u, SI = (uint32_t)g.si * 4u;
     const uint32_t vcol = col ? (uint32_t)((int64_t)i * g.si + k - 2 + IFOS_KOFF) * 4u : 0u;
     const float b1 = g.b1, b2 = g.b2;
-    const float DX = g.dx, DY = g.dy, DZ = g.dz, DT = g.dt;
+    const float DT = g.dt;
     const int64_t so = (int64_t)shot * S.fstride;
     const rsrc_t rvx = make_rsrc(A.vx + so, g.fbytes), rvy = make_rsrc(A.vy + so, g.fbytes),
                  rvz = make_rsrc(A.vz + so, g.fbytes), rsxx = make_rsrc(A.sxx + so, g.fbytes),
