@@ -35,7 +35,7 @@
 
 namespace ifos {
 
-#define IFOS_CHUNK_MAX 16
+#define IFOS_CHUNK_MAX 32
 
 // class of a point: 1 = in a CPML slab, 0 = in the interior box
 static int point_class(const ifos_geom *g, int i, int j, int k)
@@ -634,7 +634,9 @@ static int dft_chunk(ifos_batch *b, int back, int ns, int q0, int nsnap, hipStre
     case NFV:                                                                                                   \
         if (b->chunk <= 4) IFOS_CHUNK_LAUNCH(NFV, 4);                                                           \
         else if (b->chunk <= 8) IFOS_CHUNK_LAUNCH(NFV, 8);                                                      \
-        else IFOS_CHUNK_LAUNCH(NFV, 16);                                                                        \
+        else if (b->chunk <= 16) IFOS_CHUNK_LAUNCH(NFV, 16);                                                    \
+        else if (b->chunk <= 24) IFOS_CHUNK_LAUNCH(NFV, 24);                                                    \
+        else IFOS_CHUNK_LAUNCH(NFV, 32);                                                                        \
         break;
     switch (b->nf) {
         IFOS_CHUNK_CASE(1)

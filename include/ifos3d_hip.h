@@ -368,7 +368,7 @@ typedef struct ifos_batch ifos_batch;
 /* device bytes a batch of these dimensions allocates */
 size_t ifos_batch_bytes(const ifos_geom *g, int nshots, int nf_max,
                         int ntr_max, int dft_precision, int chunk);
-/* nshots: shots per batch; nf_max <= 8; chunk in [1, 16] */
+/* nshots: shots per batch; nf_max <= 8; chunk in [1, 32] */
 int ifos_batch_create(ifos_batch **out, const ifos_geom *g, int nshots,
                       int nf_max, int ntr_max, int dft_precision, int chunk);
 int ifos_batch_destroy(ifos_batch *b);

@@ -124,7 +124,8 @@ CASES = {
 
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("prec_name,chunk,use_graph", [("compat", 4, False), ("compat", 3, True),
-                                                       ("kahan", 5, False)])
+                                                       ("kahan", 5, False), ("compat", 24, False),
+                                                       ("kahan", 29, False)])
 def test_batch_matches_plain(case, prec_name, chunk, use_graph):
     import ifosgpu as ig
     c = CASES[case]
