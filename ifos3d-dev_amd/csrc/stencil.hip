@@ -537,11 +537,16 @@ __global__ __launch_bounds__(64 * TI) void k_v_lds(DevGeom g, MTiles T, VFields 
         float sxx, sxz, szz, rip, rjp, rkp, vx0, vy0, vz0;
         float hk[3], ho[NH];  // halo words of sxz, syz, szz; outside rows
     };
+    // the cell's own operands, loaded at the top of their own iteration (as
+    // in k_s_lds: prefetching them a plane ahead costs registers, not time)
+    auto load_cell = [&](Ops &O, int D) {
+        O.rip = LB(rrip, D); O.rjp = LB(rrjp, D); O.rkp = LB(rrkp, D);
+        O.vx0 = LBN(rvx, D); O.vy0 = LBN(rvy, D); O.vz0 = LBN(rvz, D);
+    };
+    // what plane j + D stages in LDS, loaded a plane ahead
     auto load_ops = [&](Ops &O, int D) {
         const uint32_t pj = (uint32_t)(D + 2) * SJ;
         O.sxx = LB(rsxx, D); O.sxz = LB(rsxz, D); O.szz = LB(rszz, D);
-        O.rip = LB(rrip, D); O.rjp = LB(rrjp, D); O.rkp = LB(rrkp, D);
-        O.vx0 = LBN(rvx, D); O.vy0 = LBN(rvy, D); O.vz0 = LBN(rvz, D);
         O.hk[0] = bload(rsxz, vb + 8u + hd, pj + own);
         O.hk[1] = bload(rsyz, vb + 8u + hd, pj + own);
         O.hk[2] = bload(rszz, vb + 8u + hd, pj + own);
@@ -565,6 +570,7 @@ __global__ __launch_bounds__(64 * TI) void k_v_lds(DevGeom g, MTiles T, VFields 
         // j+2, syy at j+3 clamped to the last plane in memory), unconditional
         // as in k_s_lds
         load_ops(nxt, 1);
+        load_cell(cur, 0);
         const uint32_t q3 = (uint32_t)(min(j + 3, g.ny + 2) - j + 2) * SJ + own;
         const float sxy_n = LB(rsxy, 2), syz_n = LB(rsyz, 2), syy_n = bload(rsyy, vb + 8u, q3);
         // stage plane j: halo words first, then the rows over them
@@ -785,9 +791,13 @@ __global__ __launch_bounds__(64 * TI) void k_s_march(DevGeom g, MTiles T, SField
 // of the rows just outside the workgroup (vx at i0-2, i0-1, i0+TI; vy and vz
 // at i0-1, i0+TI, i0+TI+1); every i- and k-neighbour of the update is then
 // read from LDS instead of through L1.  Software-pipelined one plane ahead:
-// the halo words, the outside rows, the own cell's operands and the next
-// j-queue entries of plane j+1 are loaded while plane j is computed, so the
-// LDS writes of plane j wait only for loads issued an iteration earlier.
+// the halo words, the outside rows and the next j-queue entries of plane
+// j+1 are loaded while plane j is computed, so the LDS writes of plane j wait
+// only for loads issued an iteration earlier.  The own cell's operands (old
+// stresses, moduli) are loaded at the top of their own iteration, where the
+// staging, the barrier and the derivatives cover their latency: holding
+// them a plane ahead too would cost 11 VGPRs (90 instead of 79 for the
+// batched elastic march) and the sixth wave per SIMD that fits in 80.
 // Two LDS buffers alternate by plane, so one barrier per plane suffices (a
 // wavefront writes a buffer again only after the next barrier, which every
 // wavefront reaches after its reads of that buffer).  Every lane loads, at a
@@ -889,12 +899,18 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
         float s[6], pi, u, uipjp, ujpkp, uipkp, r[6], ts, tp, tsipjp, tsjpkp, tsipkp;
         float hk[3], ho[NH];
     };
-    auto load_ops = [&](Ops &O, int D) {
-        const uint32_t pj = (uint32_t)(D + 2) * SJ;
+    // the cell's own stresses and moduli, loaded at the top of their own
+    // iteration
+    auto load_cell = [&](Ops &O, int D) {
         O.s[0] = LBN(rsxx, D); O.s[1] = LBN(rsyy, D); O.s[2] = LBN(rszz, D);
         O.s[3] = LBN(rsxy, D); O.s[4] = LBN(rsyz, D); O.s[5] = LBN(rsxz, D);
         O.uipjp = LB(ruipjp, D); O.ujpkp = LB(rujpkp, D); O.uipkp = LB(ruipkp, D);
         O.pi = LB(rpi, D); O.u = LB(ru, D);
+    };
+    // a plane ahead: what plane j + D stages in LDS, and the viscoelastic
+    // operands
+    auto load_ops = [&](Ops &O, int D) {
+        const uint32_t pj = (uint32_t)(D + 2) * SJ;
         if (VM) {
 #pragma unroll
             for (int n = 0; n < 6; ++n) O.r[n] = LBN(rr[n], D);
@@ -926,6 +942,7 @@ __global__ __launch_bounds__(64 * TI) void k_s_lds(DevGeom g, MTiles T, SFields 
         // j = je they are unused, plane je+1 <= ny+1 exists and j+3 is
         // clamped to the last plane in memory, ny+2
         load_ops(nxt, 1);
+        load_cell(cur, 0);
         const uint32_t q3 = (uint32_t)(min(j + 3, g.ny + 2) - j + 2) * SJ + own;
         const float vx_n = bload(rvx, vb + 8u, q3), vz_n = bload(rvz, vb + 8u, q3), vy_n = LB(rvy, 2);
         // stage plane j: halo words first, then the row over them
