@@ -120,6 +120,12 @@ def test_entry_points_reject_bad_arguments_before_any_device_call():
     # time-domain gradient: no context / no buffers
     assert lib.ifos_ctx_run_shot_td(None, None, 0, None, None, None, None) == EINVAL
     assert "ifos_ctx_run_shot_td" in last()
+    # batched path: DFT chunks of 1..32 snapshots, 1..1024 shots, 1..8 frequencies
+    G = ig.Geom(20, 20, 20, 4, 2.0, 2.0, 2.0, 2e-4, 10)
+    h = C.c_void_p()
+    for shots, nf, chunk in ((1, 1, 33), (1, 1, 0), (0, 1, 8), (1025, 1, 8), (1, 9, 8)):
+        assert lib.ifos_batch_create(C.byref(h), G.ref, shots, nf, 1, 0, chunk) == EINVAL, (shots, nf, chunk)
+        assert "chunk in [1,32]" in last()
 
 
 def test_plain_tile_shapes():
